@@ -1,0 +1,329 @@
+"""ctypes binding of the CPU oracle (oracle/ckks_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the CPU-baseline
+leg of bench.py, always as the checker. The product library never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libckks_oracle.so")
+_lib = None
+
+u64 = C.c_uint64
+P64 = C.POINTER(C.c_uint64)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.or_params_new.restype = C.c_void_p
+        L.or_params_new.argtypes = [C.c_uint, C.POINTER(C.c_uint), u64, C.POINTER(C.c_uint), u64, u64]
+        L.or_params_from_primes.restype = C.c_void_p
+        L.or_params_from_primes.argtypes = [C.c_uint, P64, u64, P64, u64, u64]
+        L.or_params_free.argtypes = [C.c_void_p]
+        L.or_params_q.restype = u64
+        L.or_params_q.argtypes = [C.c_void_p, u64]
+        L.or_params_p.restype = u64
+        L.or_params_p.argtypes = [C.c_void_p, u64]
+        for name in ("or_mod_add", "or_mod_sub", "or_mod_mul", "or_mod_pow"):
+            f = getattr(L, name)
+            f.restype = u64
+            f.argtypes = [C.c_void_p, u64, u64]
+        L.or_mod_new.argtypes = [u64, C.c_void_p]
+        L.or_mod_inv.argtypes = [C.c_void_p, u64, P64]
+        L.or_shoup.restype = u64
+        L.or_shoup.argtypes = [C.c_void_p, u64]
+        L.or_mul_shoup.restype = u64
+        L.or_mul_shoup.argtypes = [C.c_void_p, u64, u64, u64]
+        L.or_mul_shoup_lazy.restype = u64
+        L.or_mul_shoup_lazy.argtypes = [C.c_void_p, u64, u64, u64]
+        L.or_mod_barrett.argtypes = [u64, P64, P64]
+        L.or_is_prime.argtypes = [u64]
+        L.or_ntt_primes_skip.argtypes = [C.c_uint, u64, u64, u64, P64]
+        L.or_ntt_forward_limbs.argtypes = [u64, P64, u64, P64]
+        L.or_ntt_inverse_limbs.argtypes = [u64, P64, u64, P64]
+        L.or_ntt_psi.restype = u64
+        L.or_ntt_psi.argtypes = [u64, u64]
+        L.or_galois_elt.restype = u64
+        L.or_galois_elt.argtypes = [u64, C.c_int]
+        L.or_conjugation_elt.restype = u64
+        L.or_conjugation_elt.argtypes = [u64]
+        L.or_galois_perm_ntt.argtypes = [u64, u64, C.POINTER(C.c_uint32)]
+        L.or_automorphism_ntt.argtypes = [u64, u64, P64, P64, u64]
+        L.or_automorphism_coeff.argtypes = [u64, u64, P64, P64, P64, u64]
+        L.or_convert.argtypes = [P64, u64, P64, u64, u64, P64, P64, C.c_int]
+        L.or_rescale_by_last.argtypes = [C.c_void_p, P64, u64, C.c_int]
+        L.or_mod_down.argtypes = [C.c_void_p, P64, u64, P64, u64, C.c_int]
+        L.or_digit_ranges.restype = u64
+        L.or_digit_ranges.argtypes = [C.c_void_p, u64, P64, P64]
+        L.or_decompose.argtypes = [C.c_void_p, P64, u64, P64]
+        L.or_apply_key.argtypes = [C.c_void_p, P64, u64, u64, P64, u64, P64, P64]
+        L.or_key_switch.argtypes = [C.c_void_p, P64, u64, P64, u64, P64, P64]
+        L.or_relinearize.argtypes = [C.c_void_p, P64, P64, P64, u64, P64, u64, P64, P64]
+        L.or_apply_galois.argtypes = [C.c_void_p, P64, P64, u64, u64, P64, u64, P64, P64]
+        L.or_hoisted_galois.argtypes = [C.c_void_p, P64, P64, u64, u64, P64, C.POINTER(P64), u64, P64]
+        L.or_rescale_ct.argtypes = [C.c_void_p, P64, u64, u64]
+        L.or_mul_no_relin.argtypes = [C.c_void_p, P64, P64, P64, P64, u64, P64, P64, P64]
+        L.or_poly_add.argtypes = [C.c_void_p, P64, P64, u64]
+        L.or_poly_mul.argtypes = [C.c_void_p, P64, P64, u64]
+        L.or_uniform_word.restype = u64
+        L.or_uniform_word.argtypes = [u64, u64, u64, u64]
+        L.or_fill_uniform.argtypes = [P64, u64, u64, u64, u64]
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_get_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(P64)
+
+
+def arr(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.uint64))
+
+
+class Mod:
+    """A Modulus (q < 2^60) with the oracle's Barrett / Shoup routines."""
+
+    def __init__(self, q):
+        self.buf = (u64 * 3)()
+        if lib().or_mod_new(q, C.byref(self.buf)) != 0:
+            raise ValueError(f"bad modulus {q}")
+        self.q = q
+
+    def mul(self, a, b):
+        return lib().or_mod_mul(C.byref(self.buf), a, b)
+
+    def inv(self, a):
+        out = u64(0)
+        ok = lib().or_mod_inv(C.byref(self.buf), a, C.byref(out))
+        return out.value if ok else None
+
+    def shoup(self, w):
+        return lib().or_shoup(C.byref(self.buf), w)
+
+    def mul_shoup(self, a, w):
+        return lib().or_mul_shoup(C.byref(self.buf), a, w % self.q, self.shoup(w))
+
+    def mul_shoup_lazy(self, a, w):
+        return lib().or_mul_shoup_lazy(C.byref(self.buf), a, w % self.q, self.shoup(w))
+
+
+def barrett(q):
+    lo, hi = u64(0), u64(0)
+    lib().or_mod_barrett(q, C.byref(lo), C.byref(hi))
+    return lo.value | (hi.value << 64)
+
+
+def is_prime(n):
+    return bool(lib().or_is_prime(n))
+
+
+def ntt_primes(bits, n, count, skip=0):
+    out = np.zeros(max(count, 1), dtype=np.uint64)
+    if lib().or_ntt_primes_skip(bits, n, skip, count, ptr(out)) != 0:
+        raise ValueError("InvalidParams")
+    return [int(x) for x in out[:count]]
+
+
+def ntt_forward(n, primes, a):
+    a = arr(a).copy()
+    q = arr(primes)
+    assert lib().or_ntt_forward_limbs(n, ptr(q), len(primes), ptr(a)) == 0
+    return a
+
+
+def ntt_inverse(n, primes, a):
+    a = arr(a).copy()
+    q = arr(primes)
+    assert lib().or_ntt_inverse_limbs(n, ptr(q), len(primes), ptr(a)) == 0
+    return a
+
+
+def psi(n, q):
+    return lib().or_ntt_psi(n, q)
+
+
+def galois_elt(n, step):
+    return lib().or_galois_elt(n, step)
+
+
+def conjugation_elt(n):
+    return lib().or_conjugation_elt(n)
+
+
+def galois_perm(n, g):
+    out = np.zeros(n, dtype=np.uint32)
+    lib().or_galois_perm_ntt(n, g, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
+
+
+def automorphism_ntt(n, g, a):
+    a = arr(a)
+    out = np.zeros_like(a)
+    lib().or_automorphism_ntt(n, g, ptr(a), ptr(out), a.size // n)
+    return out
+
+
+def automorphism_coeff(n, g, primes, a):
+    a = arr(a)
+    out = np.zeros_like(a)
+    q = arr(primes)
+    lib().or_automorphism_coeff(n, g, ptr(q), ptr(a), ptr(out), a.size // n)
+    return out
+
+
+def convert(src_primes, dst_primes, src, exact):
+    src = arr(src)
+    k, t = len(src_primes), len(dst_primes)
+    n = src.size // k
+    out = np.zeros(t * n, dtype=np.uint64)
+    lib().or_convert(ptr(arr(src_primes)), k, ptr(arr(dst_primes)), t, n, ptr(src), ptr(out), int(exact))
+    return out.reshape(t, n)
+
+
+def uniform(n, q, seed, stream):
+    out = np.zeros(n, dtype=np.uint64)
+    lib().or_fill_uniform(ptr(out), n, q, seed, stream)
+    return out
+
+
+def uniform_poly(primes, n, seed, stream0=0):
+    """Limb i uniform mod primes[i] from stream stream0 + i."""
+    return np.stack([uniform(n, q, seed, stream0 + i) for i, q in enumerate(primes)])
+
+
+def set_threads(n):
+    lib().or_set_threads(n)
+
+
+def get_threads():
+    return lib().or_get_threads()
+
+
+class Params:
+    """or_params: a CkksParams with its Q and P bases."""
+
+    def __init__(self, log_n, q_bits=None, p_bits=None, dnum=1, q=None, p=None):
+        L = lib()
+        if q is not None:
+            self.h = L.or_params_from_primes(log_n, ptr(arr(q)), len(q), ptr(arr(p)), len(p), dnum)
+            nq, np_ = len(q), len(p)
+        else:
+            qb = (C.c_uint * len(q_bits))(*q_bits)
+            pb = (C.c_uint * len(p_bits))(*p_bits)
+            self.h = L.or_params_new(log_n, qb, len(q_bits), pb, len(p_bits), dnum)
+            nq, np_ = len(q_bits), len(p_bits)
+        if not self.h:
+            raise ValueError("InvalidParams")
+        self.log_n, self.n, self.dnum = log_n, 1 << log_n, dnum
+        self.q = [L.or_params_q(self.h, i) for i in range(nq)]
+        self.p = [L.or_params_p(self.h, i) for i in range(np_)]
+        self.alpha = nq // dnum
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_params_free(self.h)
+            self.h = None
+
+    @property
+    def k(self):
+        return len(self.p)
+
+    def digit_ranges(self, level):
+        s = np.zeros(64, dtype=np.uint64)
+        e = np.zeros(64, dtype=np.uint64)
+        cnt = lib().or_digit_ranges(self.h, level, ptr(s), ptr(e))
+        return [(int(s[i]), int(e[i])) for i in range(cnt)]
+
+    def rescale_by_last(self, poly, level, ntt=True):
+        a = arr(poly).copy()
+        lib().or_rescale_by_last(self.h, ptr(a), level, int(ntt))
+        return a.reshape(level + 1, self.n)[:level].copy()
+
+    def mod_down(self, qpart, level, ppart, ntt=True):
+        a = arr(qpart).copy()
+        pp = arr(ppart)
+        lib().or_mod_down(self.h, ptr(a), level, ptr(pp), pp.size // self.n, int(ntt))
+        return a.reshape(level + 1, self.n)
+
+    def decompose(self, poly, level):
+        nd = len(self.digit_ranges(level))
+        out = np.zeros((nd, level + 1 + self.k, self.n), dtype=np.uint64)
+        lib().or_decompose(self.h, ptr(arr(poly)), level, ptr(out))
+        return out
+
+    def apply_key(self, digits, level, key, key_max_level):
+        digits = arr(digits)
+        nd = digits.size // ((level + 1 + self.k) * self.n)
+        kb = np.zeros((level + 1, self.n), dtype=np.uint64)
+        ka = np.zeros_like(kb)
+        lib().or_apply_key(self.h, ptr(digits), nd, level, ptr(arr(key)), key_max_level, ptr(kb), ptr(ka))
+        return kb, ka
+
+    def key_switch(self, poly, level, key, key_max_level):
+        kb = np.zeros((level + 1, self.n), dtype=np.uint64)
+        ka = np.zeros_like(kb)
+        lib().or_key_switch(self.h, ptr(arr(poly)), level, ptr(arr(key)), key_max_level, ptr(kb), ptr(ka))
+        return kb, ka
+
+    def relinearize(self, c0, c1, c2, level, key, key_max_level):
+        o0 = np.zeros((level + 1, self.n), dtype=np.uint64)
+        o1 = np.zeros_like(o0)
+        lib().or_relinearize(self.h, ptr(arr(c0)), ptr(arr(c1)), ptr(arr(c2)), level, ptr(arr(key)),
+                             key_max_level, ptr(o0), ptr(o1))
+        return o0, o1
+
+    def apply_galois(self, c0, c1, level, g, key, key_max_level):
+        o0 = np.zeros((level + 1, self.n), dtype=np.uint64)
+        o1 = np.zeros_like(o0)
+        lib().or_apply_galois(self.h, ptr(arr(c0)), ptr(arr(c1)), level, g, ptr(arr(key)), key_max_level,
+                              ptr(o0), ptr(o1))
+        return o0, o1
+
+    def hoisted_galois(self, c0, c1, level, elts, keys, key_max_level):
+        keys = [arr(k) for k in keys]
+        kp = (P64 * len(keys))(*[ptr(k) for k in keys])
+        outs = np.zeros((len(elts), 2, level + 1, self.n), dtype=np.uint64)
+        lib().or_hoisted_galois(self.h, ptr(arr(c0)), ptr(arr(c1)), level, len(elts), ptr(arr(elts)), kp,
+                                key_max_level, ptr(outs))
+        return outs
+
+    def rescale_ct(self, ct, level):
+        a = arr(ct).copy()
+        ncomp = a.size // ((level + 1) * self.n)
+        lib().or_rescale_ct(self.h, ptr(a), ncomp, level)
+        return a.reshape(ncomp, level + 1, self.n)[:, :level].copy()
+
+    def mul_no_relin(self, a0, a1, b0, b1, level):
+        d = [np.zeros((level + 1, self.n), dtype=np.uint64) for _ in range(3)]
+        lib().or_mul_no_relin(self.h, ptr(arr(a0)), ptr(arr(a1)), ptr(arr(b0)), ptr(arr(b1)), level,
+                              ptr(d[0]), ptr(d[1]), ptr(d[2]))
+        return d
+
+    def key_shape(self, key_max_level):
+        """(parts, limbs per key polynomial) of a key serving levels 0..=key_max_level."""
+        return len(self.digit_ranges(key_max_level)), key_max_level + 1 + self.k
+
+    def uniform_key(self, key_max_level, seed):
+        """A key in the device layout with uniform residues (keyswitch arithmetic is
+        key-agnostic): per part, b_j then a_j, each Q limbs 0..=key_max_level then P limbs."""
+        parts, lp = self.key_shape(key_max_level)
+        primes = list(self.q[: key_max_level + 1]) + list(self.p)
+        polys = []
+        for j in range(2 * parts):
+            polys.append(uniform_poly(primes, self.n, seed, stream0=1000 * (j + 1)))
+        return np.stack(polys)
