@@ -1,0 +1,69 @@
+// Centered exact RNS base conversion (ModUp, ModDown, rescale) and the division finish.
+#pragma once
+#include "common.h"
+
+namespace ckh {
+
+constexpr u32 MAX_SRC = 16;
+
+// One source prime q_j of a conversion from Q = prod q_j.
+struct ConvSrc {
+    u64 q;
+    u64 h;               // q_j >> 1: [x + (Q-1)/2]_Q has residue x + h mod q_j
+    u64 qhinv, qhinv_s;  // (Q / q_j)^-1 mod q_j
+    double inv;          // 1.0 / f64(q_j), round to nearest (host computed)
+    u64 pad;
+};
+
+// One target prime r of a conversion.
+struct ConvTgt {
+    u32 prime;  // index into the context's prime constants
+    u32 limb;   // output limb index within the job's output polynomial
+    u64 qmod, qmod_s;  // Q mod r
+    u64 hmod;          // (Q - 1)/2 mod r
+    u64 qhat[MAX_SRC], qhat_s[MAX_SRC];  // (Q / q_j) mod r
+};
+
+// A conversion: sources [src_first, src_first + nsrc) of the ConvSrc table (their input limbs
+// are src_limb0 + j of the job's input), targets [tgt_first, tgt_first + ntgt) of the ConvTgt
+// table. exact = use the f64 overflow estimate (nsrc > 1).
+struct ConvPlan {
+    u32 src_first, nsrc, tgt_first, ntgt, src_limb0, exact;
+};
+
+// One launch row: plan `plan`, input polynomial (its limb 0) at `in`, output at `out`.
+struct ConvJob {
+    const u64* in;
+    u64* out;
+    u32 plan, pad;
+};
+
+// out <- (a - t) * inv mod q (+ add when add != null), one limb: finishes rescale and mod_down.
+struct FinishJob {
+    const u64* a;
+    const u64* t;
+    u64* out;
+    const u64* add;
+    u64 inv, inv_s;
+    u32 prime, pad;
+};
+
+// Conversion tables (device pointers).
+struct ConvTables {
+    const ConvSrc* src;
+    const ConvTgt* tgt;
+    const ConvPlan* plans;
+    const PrimeConst* pc;
+    u32 log_n;
+};
+
+// For each job: coefficient-form input limbs (reduced) -> converted output limbs (reduced,
+// coefficient form): out_t = [x_c]_r with x_c the centered value of the source residues
+// (BaseConverter::convert_exact of [x + h]_Q minus h mod r).
+void conv_launch(const ConvTables& t, const ConvJob* jobs, u32 njobs, u32 max_tgt, u32 max_src,
+                 hipStream_t stream);
+
+void finish_launch(const FinishJob* jobs, u32 njobs, const PrimeConst* pc, u32 log_n,
+                   hipStream_t stream);
+
+}  // namespace ckh

@@ -1,0 +1,88 @@
+// Shared definitions of the gfx950 RNS-CKKS library: device-side types, limb descriptors,
+// per-prime constants and the modular arithmetic used by every kernel.
+//
+// Residues are u64 in HBM, limb-major: limb i of a polynomial is N contiguous words. All
+// arithmetic is exact modular integer arithmetic, so results are unique residues whatever the
+// evaluation order; the one floating-point step (the overflow estimate of the exact base
+// conversion) lives in basis.hip and is compiled without FMA contraction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+namespace ckh {
+
+// One limb of work: where to read, where to write, and which prime (index into the context's
+// prime list: Q primes first, then P primes).
+struct LimbRef {
+    const u64* src;
+    u64* dst;
+    u32 prime;
+    u32 aux;
+};
+
+// Per-prime constants (device).
+struct PrimeConst {
+    u64 q;
+    u64 n_inv, n_inv_s;     // N^-1 and its Shoup companion
+    u64 w_n_inv, w_n_inv_s; // psi^-1 * N^-1 (fused last inverse stage)
+    u64 one_s;              // floor(2^64 / q): Shoup companion of 1
+    u64 two64, two64_s;     // 2^64 mod q and companion
+};
+
+// A twiddle and its Shoup companion, loaded as one 16-byte word.
+struct Tw {
+    u64 w, ws;
+};
+
+#define CKH_DEV __device__ __forceinline__
+
+// High word of a 64x64 product (the compiler builds it from 32-bit multiplies on gfx950).
+CKH_DEV u64 mulhi(u64 a, u64 b) { return __umul64hi(a, b); }
+
+// Shoup product in [0, 2q) for any u64 a (w < q fixed, ws = floor(w 2^64 / q)).
+CKH_DEV u64 shoup_lazy(u64 a, u64 w, u64 ws, u64 q) { return a * w - mulhi(a, ws) * q; }
+
+CKH_DEV u64 csub(u64 a, u64 q) { return a >= q ? a - q : a; }
+
+CKH_DEV u64 shoup(u64 a, u64 w, u64 ws, u64 q) { return csub(shoup_lazy(a, w, ws, q), q); }
+
+CKH_DEV u64 add_mod(u64 a, u64 b, u64 q) { return csub(a + b, q); }
+
+CKH_DEV u64 sub_mod(u64 a, u64 b, u64 q) { return a >= b ? a - b : a + q - b; }
+
+// Any u64 reduced mod q.
+CKH_DEV u64 reduce64(u64 a, const PrimeConst& c) { return csub(a - mulhi(a, c.one_s) * c.q, c.q); }
+
+// (hi 2^64 + lo) mod q for any hi, lo.
+CKH_DEV u64 reduce128(u64 hi, u64 lo, const PrimeConst& c) {
+    return add_mod(shoup(hi, c.two64, c.two64_s, c.q), reduce64(lo, c), c.q);
+}
+
+// a * b mod q for reduced a, b.
+CKH_DEV u64 mul_mod(u64 a, u64 b, const PrimeConst& c) { return reduce128(mulhi(a, b), a * b, c); }
+
+// 128-bit accumulator.
+struct Acc128 {
+    u64 lo, hi;
+};
+
+CKH_DEV void mac(Acc128& acc, u64 a, u64 b) {
+    u64 lo = a * b;
+    u64 hi = mulhi(a, b);
+    acc.lo += lo;
+    acc.hi += hi + (acc.lo < lo ? 1 : 0);
+}
+
+// blockIdx.x remapped so that consecutive logical tiles of one launch land on one XCD
+// (workgroups are dealt round-robin over the 8 XCDs, each with its own L2).
+CKH_DEV u32 xcd_tile(u32 bid, u32 nblocks) {
+    const u32 nx = 8;
+    if (nblocks % nx != 0) return bid;
+    return (bid % nx) * (nblocks / nx) + bid / nx;
+}
+
+}  // namespace ckh

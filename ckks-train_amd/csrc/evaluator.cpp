@@ -1,0 +1,595 @@
+// Polynomial operations and the HeOps evaluator over the device engine.
+#include "evaluator.h"
+
+#include <cmath>
+#include <sstream>
+
+#include "hostmath.h"
+
+namespace ckh {
+
+// ------------------------------------------------------------------ polynomials
+
+PolyPtr poly_alloc(Context& c, u32 level, u32 p_limbs, bool ntt) {
+    if (level >= c.nq) fail(CKH_ERR_INVALID_PARAMS, "level above max_level");
+    if (p_limbs != 0 && p_limbs != c.np) fail(CKH_ERR_INVALID_PARAMS, "p_limbs must be 0 or the special prime count");
+    auto p = std::make_shared<Poly>();
+    p->buf = c.alloc((u64)(level + 1 + p_limbs) * c.n);
+    p->off = 0;
+    p->level = level;
+    p->p_limbs = p_limbs;
+    p->ntt = ntt;
+    return p;
+}
+
+static PolyPtr view(const BufPtr& b, u64 off, u32 level, u32 p_limbs, bool ntt) {
+    auto p = std::make_shared<Poly>();
+    p->buf = b;
+    p->off = off;
+    p->level = level;
+    p->p_limbs = p_limbs;
+    p->ntt = ntt;
+    return p;
+}
+
+static u64 prime_of_limb(const Context& c, const Poly& p, u32 i) {
+    return c.primes[i <= p.level ? i : c.nq + (i - p.level - 1)];
+}
+
+PolyPtr poly_upload(Context& c, const u64* host, u32 level, u32 p_limbs, bool ntt) {
+    if (!host) fail(CKH_ERR_INVALID_PARAMS, "null limbs");
+    PolyPtr p = poly_alloc(c, level, p_limbs, ntt);
+    for (u32 i = 0; i < p->limbs(); ++i) {
+        const u64 q = prime_of_limb(c, *p, i);
+        for (u64 k = 0; k < c.n; ++k)
+            if (host[i * c.n + k] >= q) fail(CKH_ERR_INVALID_PARAMS, "limb entry not reduced");
+    }
+    hipStream_t s = c.stream();
+    hip_check(hipMemcpyAsync(p->data(), host, (u64)p->limbs() * c.n * 8, hipMemcpyHostToDevice, s), "upload");
+    hip_check(hipStreamSynchronize(s), "upload");
+    return p;
+}
+
+void poly_download(Context& c, const Poly& p, u64* host) {
+    if (!host) fail(CKH_ERR_INVALID_PARAMS, "null output");
+    hipStream_t s = c.stream();
+    hip_check(hipMemcpyAsync(host, p.data(), (u64)p.limbs() * c.n * 8, hipMemcpyDeviceToHost, s), "download");
+    hip_check(hipStreamSynchronize(s), "download");
+}
+
+PolyPtr poly_uniform(Context& c, u32 level, u32 p_limbs, bool ntt, u64 seed, u64 stream0) {
+    PolyPtr p = poly_alloc(c, level, p_limbs, ntt);
+    fill_uniform_launch(p->data(), level + 1, p_limbs, c.nq, seed, stream0, c.d_pc, c.log_n, c.stream());
+    hip_check(hipGetLastError(), "fill");
+    return p;
+}
+
+PolyPtr poly_transform(Context& c, const Poly& in, bool forward) {
+    if (in.ntt == forward) fail(CKH_ERR_INVALID_PARAMS, forward ? "already in NTT form" : "already in coefficient form");
+    PolyPtr out = poly_alloc(c, in.level, in.p_limbs, forward);
+    c.ntt_poly(in.data(), out->data(), in.limbs(), in.level, in.p_limbs, forward, c.stream());
+    hip_check(hipGetLastError(), "ntt");
+    return out;
+}
+
+PolyPtr poly_automorphism(Context& c, const Poly& in, u64 g) {
+    if (!in.ntt) fail(CKH_ERR_INVALID_PARAMS, "automorphism needs NTT form");
+    if (g % 2 == 0 || g >= 2 * c.n) fail(CKH_ERR_INVALID_PARAMS, "Galois element must be odd and < 2N");
+    PolyPtr out = poly_alloc(c, in.level, in.p_limbs, true);
+    automorphism_ntt_launch(in.data(), out->data(), in.limbs(), g, c.log_n, c.stream());
+    hip_check(hipGetLastError(), "automorphism");
+    return out;
+}
+
+PolyPtr poly_rescale(Context& c, const Poly& in) {
+    if (!in.ntt || in.p_limbs) fail(CKH_ERR_INVALID_PARAMS, "rescale needs an NTT-form Q polynomial");
+    if (in.level == 0) fail(CKH_ERR_NO_LEVELS_LEFT, "cannot rescale at level 0");
+    PolyPtr out = poly_alloc(c, in.level - 1, 0, true);
+    BufPtr scratch = c.alloc((u64)(1 + in.level) * c.n);
+    c.rescale({{in.data(), in.level, out->data(), scratch->ptr, scratch->ptr + c.n}}, c.stream());
+    hip_check(hipGetLastError(), "rescale");
+    return out;
+}
+
+PolyPtr poly_pointwise(Context& c, PwOp op, const Poly& a, const Poly* b) {
+    if (b && (b->level != a.level || b->p_limbs != a.p_limbs || b->ntt != a.ntt))
+        fail(CKH_ERR_LEVEL_MISMATCH, "operands at different levels or forms");
+    if (a.p_limbs) fail(CKH_ERR_INVALID_PARAMS, "pointwise ops take Q polynomials");
+    PolyPtr out = poly_alloc(c, a.level, 0, a.ntt);
+    pointwise_launch(op, a.data(), b ? b->data() : nullptr, out->data(), a.level + 1, 0, c.d_pc, c.log_n, c.stream());
+    hip_check(hipGetLastError(), "pointwise");
+    return out;
+}
+
+PolyPtr poly_mod_down(Context& c, const Poly& qp) {
+    if (!qp.ntt || qp.p_limbs != c.np) fail(CKH_ERR_INVALID_PARAMS, "mod_down needs an NTT-form QpPoly");
+    const u64 L = qp.level + 1;
+    PolyPtr out = poly_alloc(c, qp.level, 0, true);
+    BufPtr scratch = c.alloc((c.np + L) * c.n);
+    c.mod_down({{qp.data(), qp.level, out->data(), nullptr, scratch->ptr, scratch->ptr + c.np * c.n}}, c.stream());
+    hip_check(hipGetLastError(), "mod_down");
+    return out;
+}
+
+// ------------------------------------------------------------------ key switching
+
+KskPtr ksk_upload(Context& c, const u64* host, u32 max_level, u64 galois) {
+    if (max_level >= c.nq) fail(CKH_ERR_INVALID_PARAMS, "key level above max_level");
+    if (galois != 0 && (galois % 2 == 0 || galois >= 2 * c.n))
+        fail(CKH_ERR_INVALID_PARAMS, "Galois element must be odd and < 2N");
+    auto k = std::make_shared<Ksk>();
+    k->ctx = &c;
+    k->max_level = max_level;
+    k->parts = c.ndigits(max_level);
+    k->galois = galois;
+    const u64 words = (u64)k->parts * 2 * k->poly_words();
+    k->buf = c.alloc(words);
+    if (host) {
+        const u32 lp = max_level + 1 + c.np;
+        for (u32 pj = 0; pj < 2 * k->parts; ++pj)
+            for (u32 i = 0; i < lp; ++i) {
+                const u64 q = c.primes[i <= max_level ? i : c.nq + (i - max_level - 1)];
+                const u64* row = host + ((u64)pj * lp + i) * c.n;
+                for (u64 x = 0; x < c.n; ++x)
+                    if (row[x] >= q) fail(CKH_ERR_INVALID_PARAMS, "key entry not reduced");
+            }
+        hipStream_t s = c.stream();
+        hip_check(hipMemcpyAsync(k->buf->ptr, host, words * 8, hipMemcpyHostToDevice, s), "key upload");
+        hip_check(hipStreamSynchronize(s), "key upload");
+    }
+    return k;
+}
+
+// Uniform key in the device order: polynomial j (b_0, a_0, b_1, ...) uses generator streams
+// 1000 (j + 1) + limb, so a host copy is reproducible.
+KskPtr ksk_uniform(Context& c, u32 max_level, u64 galois, u64 seed) {
+    KskPtr k = ksk_upload(c, nullptr, max_level, galois);
+    for (u32 j = 0; j < 2 * k->parts; ++j)
+        fill_uniform_launch(k->buf->ptr + (u64)j * k->poly_words(), max_level + 1, c.np, c.nq, seed, 1000ull * (j + 1),
+                            c.d_pc, c.log_n, c.stream());
+    hip_check(hipGetLastError(), "fill");
+    return k;
+}
+
+namespace {
+// Scratch words of one key switch at level l: x (L), digits, acc (2 (L + k)), y (2k), z (2L).
+u64 ks_scratch_words(const Context& c, u32 level, bool with_digits) {
+    const u64 L = level + 1, k = c.np;
+    return c.n * (L + (with_digits ? c.ndigits(level) * (L + k) : 0) + 2 * (L + k) + 2 * k + 2 * L);
+}
+
+struct KsReq {
+    const Poly* d;
+    const Ksk* key;
+    const u64* add_b;
+    const u64* add_a;
+    u64* out_b;
+    u64* out_a;
+};
+
+void check_switch(const Context& c, const Poly& d, const Ksk& key) {
+    if (!d.ntt || d.p_limbs) fail(CKH_ERR_INVALID_PARAMS, "key switching needs an NTT-form Q polynomial");
+    if (key.ctx != &c) fail(CKH_ERR_INVALID_PARAMS, "key is not over this context");
+    if (d.level > key.max_level) fail(CKH_ERR_INVALID_PARAMS, "key does not serve this level");
+}
+
+// decompose + apply for a batch of requests, in one launch per stage.
+void run_ks(Context& c, const std::vector<KsReq>& reqs) {
+    hipStream_t s = c.stream();
+    u64 total = 0;
+    for (auto& r : reqs) total += ks_scratch_words(c, r.d->level, true);
+    BufPtr scratch = c.alloc(total, s);
+    std::vector<DecTask> dec;
+    std::vector<ApplyTask> app;
+    u64* p = scratch->ptr;
+    for (auto& r : reqs) {
+        const u64 L = r.d->level + 1, k = c.np;
+        u64* x = p;
+        u64* dig = x + L * c.n;
+        u64* acc = dig + (u64)c.ndigits(r.d->level) * (L + k) * c.n;
+        u64* y = acc + 2 * (L + k) * c.n;
+        u64* z = y + 2 * k * c.n;
+        p = z + 2 * L * c.n;
+        dec.push_back({r.d->data(), r.d->level, dig, x});
+        app.push_back({dig, r.d->level, 0, r.key, r.out_b, r.out_a, r.add_b, r.add_a, acc, y, z});
+    }
+    c.decompose(dec, s);
+    c.apply(app, s);
+    hip_check(hipGetLastError(), "key switch");
+}
+}  // namespace
+
+std::vector<PolyPtr> ks_decompose(Context& c, const Poly& d) {
+    if (!d.ntt || d.p_limbs) fail(CKH_ERR_INVALID_PARAMS, "key switching needs an NTT-form Q polynomial");
+    const u32 nd = c.ndigits(d.level);
+    const u64 dw = c.digit_words(d.level);
+    BufPtr digs = c.alloc(nd * dw);
+    BufPtr x = c.alloc((u64)(d.level + 1) * c.n);
+    c.decompose({{d.data(), d.level, digs->ptr, x->ptr}}, c.stream());
+    hip_check(hipGetLastError(), "decompose");
+    std::vector<PolyPtr> out;
+    for (u32 j = 0; j < nd; ++j) out.push_back(view(digs, j * dw, d.level, c.np, true));
+    return out;
+}
+
+std::pair<PolyPtr, PolyPtr> ks_apply(Context& c, const std::vector<PolyPtr>& digits, const Ksk& key) {
+    if (digits.empty()) fail(CKH_ERR_INVALID_PARAMS, "no digits to switch");
+    const u32 level = digits[0]->level;
+    const u32 nd = c.ndigits(level);
+    if (digits.size() != nd) fail(CKH_ERR_INVALID_PARAMS, "digit count does not match the level");
+    if (nd > key.parts || level > key.max_level) fail(CKH_ERR_INVALID_PARAMS, "key does not serve this level");
+    if (key.ctx != &c) fail(CKH_ERR_INVALID_PARAMS, "key is not over this context");
+    const u64 dw = c.digit_words(level);
+    bool contiguous = true;
+    for (u32 j = 0; j < nd; ++j) {
+        const Poly& d = *digits[j];
+        if (d.level != level || d.p_limbs != c.np || !d.ntt)
+            fail(CKH_ERR_INVALID_PARAMS, "digits must share one level and be NTT-form QpPolys");
+        if (d.data() != digits[0]->data() + j * dw) contiguous = false;
+    }
+    hipStream_t s = c.stream();
+    BufPtr packed;
+    const u64* dig = digits[0]->data();
+    if (!contiguous) {
+        packed = c.alloc(nd * dw);
+        for (u32 j = 0; j < nd; ++j)
+            hip_check(hipMemcpyAsync(packed->ptr + j * dw, digits[j]->data(), dw * 8, hipMemcpyDeviceToDevice, s),
+                      "copy");
+        dig = packed->ptr;
+    }
+    const u64 L = level + 1;
+    BufPtr out = c.alloc(2 * L * c.n);
+    BufPtr scratch = c.alloc(ks_scratch_words(c, level, false));
+    u64* acc = scratch->ptr + L * c.n;
+    u64* y = acc + 2 * (L + c.np) * c.n;
+    u64* z = y + 2 * c.np * c.n;
+    c.apply({{dig, level, 0, &key, out->ptr, out->ptr + L * c.n, nullptr, nullptr, acc, y, z}}, s);
+    hip_check(hipGetLastError(), "apply_key");
+    return {view(out, 0, level, 0, true), view(out, L * c.n, level, 0, true)};
+}
+
+std::pair<PolyPtr, PolyPtr> ks_switch(Context& c, const Poly& d, const Ksk& key) {
+    check_switch(c, d, key);
+    const u64 L = d.level + 1;
+    BufPtr out = c.alloc(2 * L * c.n);
+    run_ks(c, {{&d, &key, nullptr, nullptr, out->ptr, out->ptr + L * c.n}});
+    return {view(out, 0, d.level, 0, true), view(out, L * c.n, d.level, 0, true)};
+}
+
+std::vector<std::pair<PolyPtr, PolyPtr>> ks_switch_batch(Context& c, const std::vector<const Poly*>& ds,
+                                                         const Ksk& key) {
+    std::vector<KsReq> reqs;
+    std::vector<std::pair<PolyPtr, PolyPtr>> res;
+    u64 total = 0;
+    for (const Poly* d : ds) {
+        check_switch(c, *d, key);
+        total += 2 * (u64)(d->level + 1) * c.n;
+    }
+    if (ds.empty()) return res;
+    BufPtr out = c.alloc(total);
+    u64 off = 0;
+    for (const Poly* d : ds) {
+        const u64 L = d->level + 1;
+        reqs.push_back({d, &key, nullptr, nullptr, out->ptr + off, out->ptr + off + L * c.n});
+        res.push_back({view(out, off, d->level, 0, true), view(out, off + L * c.n, d->level, 0, true)});
+        off += 2 * L * c.n;
+    }
+    run_ks(c, reqs);
+    return res;
+}
+
+u64 galois_elt(u64 n, int32_t step) {
+    const u64 m = 2 * n;
+    const int64_t slots = (int64_t)std::max<u64>(n / 2, 1);
+    int64_t k0 = (int64_t)step % slots;
+    if (k0 < 0) k0 += slots;
+    u64 k = (u64)k0, base = 3 % m, acc = 1 % m;
+    while (k) {
+        if (k & 1) acc = acc * base % m;
+        base = base * base % m;
+        k >>= 1;
+    }
+    return acc;
+}
+
+// ------------------------------------------------------------------ evaluator
+
+Evaluator::Evaluator(Context& c, KskPtr r, std::vector<KskPtr> g) : ctx(c), relin(std::move(r)) {
+    for (auto& k : g) galois[k->galois] = k;
+}
+
+void Evaluator::count(u64 ckh_op_counter::*field, u64 v) {
+    std::lock_guard<std::mutex> lk(mu_);
+    cnt_.*field += v;
+}
+
+ckh_op_counter Evaluator::counter() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return cnt_;
+}
+
+void Evaluator::reset_counter() {
+    std::lock_guard<std::mutex> lk(mu_);
+    cnt_ = ckh_op_counter{};
+}
+
+void Evaluator::add_counts(const ckh_op_counter& c) {
+    std::lock_guard<std::mutex> lk(mu_);
+    cnt_.rotations += c.rotations;
+    cnt_.relins += c.relins;
+    cnt_.ct_ct_tensor += c.ct_ct_tensor;
+    cnt_.ct_pt_mults += c.ct_pt_mults;
+    cnt_.rescales += c.rescales;
+    cnt_.sse_switches += c.sse_switches;
+    cnt_.bootstraps += c.bootstraps;
+}
+
+bool Evaluator::owns(const Ct& ct) const {
+    for (auto& p : ct.comps)
+        if (!p || p->buf->ctx != &ctx || p->p_limbs || !p->ntt) return false;
+    return !ct.comps.empty();
+}
+
+void Evaluator::check_basis(const Ct& ct) const {
+    if (!owns(ct)) fail(CKH_ERR_INVALID_PARAMS, "operand is not over this evaluator's data basis");
+}
+
+static void check_level(u32 a, u32 b) {
+    if (a != b) fail(CKH_ERR_LEVEL_MISMATCH, "level mismatch");
+}
+
+// SCALE_TOLERANCE = 2^-10 (evaluator.rs:96).
+static void check_scale(double a, double b) {
+    if (!(std::fabs(a - b) < (1.0 / 1024.0) * std::min(a, b))) fail(CKH_ERR_SCALE_MISMATCH, "scale mismatch");
+}
+
+static CtPtr make_ct(std::vector<PolyPtr> comps, double scale) {
+    auto c = std::make_shared<Ct>();
+    c->comps = std::move(comps);
+    c->scale = scale;
+    return c;
+}
+
+bool Evaluator::has_galois_key(int32_t step) const {
+    const u64 e = rotation_elt(step);
+    return e != 0 && galois.count(e);
+}
+
+u64 Evaluator::rotation_elt(int32_t steps) const {
+    const int64_t slots = (int64_t)(ctx.n / 2);
+    int64_t k = (int64_t)steps % slots;
+    if (k < 0) k += slots;
+    return k == 0 ? 0 : galois_elt(ctx.n, (int32_t)k);
+}
+
+const Ksk& Evaluator::relin_key_checked(u32 level) const {
+    if (!relin) fail(CKH_ERR_INVALID_PARAMS, "evaluator has no relinearization key");
+    if (level > relin->max_level) fail(CKH_ERR_INVALID_PARAMS, "relinearization key does not serve this level");
+    return *relin;
+}
+
+const Ksk& Evaluator::galois_key(u64 elt, u32 level) const {
+    if (elt % 2 == 0 || elt >= 2 * ctx.n) fail(CKH_ERR_INVALID_PARAMS, "Galois element must be odd and < 2N");
+    auto it = galois.find(elt);
+    if (it == galois.end()) fail(CKH_ERR_INVALID_PARAMS, "missing galois key");
+    if (level > it->second->max_level) fail(CKH_ERR_INVALID_PARAMS, "galois key does not serve this level");
+    return *it->second;
+}
+
+// Evaluator::add_sub (evaluator.rs:694-716): the shorter operand is padded with zeros.
+CtPtr Evaluator::add(const Ct& a, const Ct& b, bool sub) {
+    check_basis(a);
+    check_basis(b);
+    check_level(a.level(), b.level());
+    check_scale(a.scale, b.scale);
+    std::vector<PolyPtr> out;
+    for (size_t i = 0; i < std::max(a.comps.size(), b.comps.size()); ++i) {
+        if (i < a.comps.size() && i < b.comps.size())
+            out.push_back(poly_pointwise(ctx, sub ? PW_SUB : PW_ADD, *a.comps[i], b.comps[i].get()));
+        else if (i < a.comps.size())
+            out.push_back(a.comps[i]);
+        else
+            out.push_back(sub ? poly_pointwise(ctx, PW_NEG, *b.comps[i], nullptr) : b.comps[i]);
+    }
+    return make_ct(std::move(out), a.scale);
+}
+
+CtPtr Evaluator::negate(const Ct& a) {
+    check_basis(a);
+    std::vector<PolyPtr> out;
+    for (auto& p : a.comps) out.push_back(poly_pointwise(ctx, PW_NEG, *p, nullptr));
+    return make_ct(std::move(out), a.scale);
+}
+
+void Evaluator::check_mul_operands(const Ct& a, const Ct& b) const {
+    check_basis(a);
+    check_basis(b);
+    if (a.comps.size() != 2 || b.comps.size() != 2)
+        fail(CKH_ERR_INVALID_PARAMS, "multiplication needs 2-component ciphertexts; relinearize first");
+    check_level(a.level(), b.level());
+    if (a.level() == 0) fail(CKH_ERR_NO_LEVELS_LEFT, "no levels left");
+}
+
+// Evaluator::mul_no_relin (evaluator.rs:422-438).
+CtPtr Evaluator::mul_no_relin(const Ct& a, const Ct& b) {
+    check_mul_operands(a, b);
+    auto& x = a.comps;
+    auto& y = b.comps;
+    PolyPtr d0 = poly_pointwise(ctx, PW_MUL, *x[0], y[0].get());
+    PolyPtr t0 = poly_pointwise(ctx, PW_MUL, *x[0], y[1].get());
+    PolyPtr t1 = poly_pointwise(ctx, PW_MUL, *x[1], y[0].get());
+    PolyPtr d1 = poly_pointwise(ctx, PW_ADD, *t0, t1.get());
+    PolyPtr d2 = poly_pointwise(ctx, PW_MUL, *x[1], y[1].get());
+    CtPtr r = make_ct({d0, d1, d2}, a.scale * b.scale);
+    count(&ckh_op_counter::ct_ct_tensor);
+    return r;
+}
+
+// Evaluator::relinearize (evaluator.rs:441-458): (c0 + kb, c1 + ka), the additions fused into
+// the mod_down finish.
+CtPtr Evaluator::relinearize(const Ct& ct) {
+    check_basis(ct);
+    if (ct.comps.size() != 3) fail(CKH_ERR_INVALID_PARAMS, "relinearize needs a 3-component ciphertext");
+    const Ksk& key = relin_key_checked(ct.level());
+    const u32 level = ct.level();
+    const u64 L = level + 1;
+    BufPtr out = ctx.alloc(2 * L * ctx.n);
+    run_ks(ctx, {{ct.comps[2].get(), &key, ct.comps[0]->data(), ct.comps[1]->data(), out->ptr, out->ptr + L * ctx.n}});
+    CtPtr r = make_ct({view(out, 0, level, 0, true), view(out, L * ctx.n, level, 0, true)}, ct.scale);
+    count(&ckh_op_counter::relins);
+    return r;
+}
+
+std::vector<CtPtr> Evaluator::relinearize_batch(const std::vector<const Ct*>& cts) {
+    u64 total = 0;
+    for (const Ct* ct : cts) {
+        check_basis(*ct);
+        if (ct->comps.size() != 3) fail(CKH_ERR_INVALID_PARAMS, "relinearize needs a 3-component ciphertext");
+        relin_key_checked(ct->level());
+        total += 2 * (u64)(ct->level() + 1) * ctx.n;
+    }
+    std::vector<CtPtr> res;
+    if (cts.empty()) return res;
+    BufPtr out = ctx.alloc(total);
+    std::vector<KsReq> reqs;
+    u64 off = 0;
+    for (const Ct* ct : cts) {
+        const u32 level = ct->level();
+        const u64 L = level + 1;
+        reqs.push_back({ct->comps[2].get(), relin.get(), ct->comps[0]->data(), ct->comps[1]->data(), out->ptr + off,
+                        out->ptr + off + L * ctx.n});
+        res.push_back(make_ct({view(out, off, level, 0, true), view(out, off + L * ctx.n, level, 0, true)}, ct->scale));
+        off += 2 * L * ctx.n;
+    }
+    run_ks(ctx, reqs);
+    count(&ckh_op_counter::relins, cts.size());
+    return res;
+}
+
+CtPtr Evaluator::mul(const Ct& a, const Ct& b) {
+    check_mul_operands(a, b);
+    relin_key_checked(a.level());
+    return relinearize(*mul_no_relin(a, b));
+}
+
+// Evaluator::rescale (evaluator.rs:498-509): scale / q_l as f64.
+CtPtr Evaluator::rescale(const Ct& ct) {
+    check_basis(ct);
+    const u32 l = ct.level();
+    if (l == 0) fail(CKH_ERR_NO_LEVELS_LEFT, "no levels left");
+    const u64 L = l;  // output limbs
+    BufPtr out = ctx.alloc((u64)ct.comps.size() * L * ctx.n);
+    BufPtr scratch = ctx.alloc((u64)ct.comps.size() * (1 + L) * ctx.n);
+    std::vector<RescaleTask> tasks;
+    std::vector<PolyPtr> comps;
+    for (size_t i = 0; i < ct.comps.size(); ++i) {
+        u64* x = scratch->ptr + i * (1 + L) * ctx.n;
+        tasks.push_back({ct.comps[i]->data(), l, out->ptr + i * L * ctx.n, x, x + ctx.n});
+        comps.push_back(view(out, i * L * ctx.n, l - 1, 0, true));
+    }
+    ctx.rescale(tasks, ctx.stream());
+    hip_check(hipGetLastError(), "rescale");
+    CtPtr r = make_ct(std::move(comps), ct.scale / (double)ctx.primes[l]);
+    count(&ckh_op_counter::rescales);
+    return r;
+}
+
+CtPtr Evaluator::mod_switch_to_level(const Ct& ct, u32 level) {
+    check_basis(ct);
+    if (level > ct.level()) fail(CKH_ERR_INVALID_PARAMS, "cannot switch a ciphertext up to a higher level");
+    std::vector<PolyPtr> comps;
+    for (auto& p : ct.comps) comps.push_back(view(p->buf, p->off, level, 0, true));
+    return make_ct(std::move(comps), ct.scale);
+}
+
+CtPtr Evaluator::relabel(const Ct& ct, double scale) {
+    if (!(std::isfinite(scale) && scale > 0)) fail(CKH_ERR_INVALID_PARAMS, "scale must be positive and finite");
+    return make_ct(ct.comps, scale);
+}
+
+// Evaluator::apply_galois (evaluator.rs:640-647) + finish_galois (800-804).
+CtPtr Evaluator::apply_galois(const Ct& ct, u64 elt) {
+    const Ksk& key = galois_key(elt, ct.level());
+    const u32 level = ct.level();
+    const u64 L = level + 1;
+    PolyPtr d = poly_automorphism(ctx, *ct.comps[1], elt);
+    PolyPtr r0 = poly_automorphism(ctx, *ct.comps[0], elt);
+    BufPtr out = ctx.alloc(2 * L * ctx.n);
+    run_ks(ctx, {{d.get(), &key, r0->data(), nullptr, out->ptr, out->ptr + L * ctx.n}});
+    CtPtr r = make_ct({view(out, 0, level, 0, true), view(out, L * ctx.n, level, 0, true)}, ct.scale);
+    count(&ckh_op_counter::rotations);
+    return r;
+}
+
+static void check_galois_input(const Ct& ct) {
+    if (ct.comps.size() != 2)
+        fail(CKH_ERR_INVALID_PARAMS, "rotation needs a 2-component ciphertext; relinearize first");
+}
+
+CtPtr Evaluator::rotate(const Ct& ct, int32_t steps) {
+    check_basis(ct);
+    check_galois_input(ct);
+    const u64 elt = rotation_elt(steps);
+    if (elt == 0) return make_ct(ct.comps, ct.scale);
+    return apply_galois(ct, elt);
+}
+
+CtPtr Evaluator::conjugate(const Ct& ct) {
+    check_basis(ct);
+    check_galois_input(ct);
+    return apply_galois(ct, 2 * ctx.n - 1);
+}
+
+// Evaluator::hoisted_rotate (evaluator.rs:572-611): every key looked up first, one
+// decomposition of c1, then every step's inner product (digits read through sigma_g) and
+// mod_down in one batch; sigma_g(c0) is added in the mod_down finish.
+std::vector<CtPtr> Evaluator::hoisted_rotate(const Ct& ct, const std::vector<int32_t>& steps) {
+    check_basis(ct);
+    check_galois_input(ct);
+    const u32 level = ct.level();
+    std::vector<std::pair<u64, const Ksk*>> keys;
+    for (int32_t s : steps) {
+        const u64 e = rotation_elt(s);
+        keys.push_back({e, e ? &galois_key(e, level) : nullptr});
+    }
+    std::vector<CtPtr> res(steps.size());
+    u32 nontrivial = 0;
+    for (auto& k : keys) nontrivial += k.first != 0;
+    if (nontrivial == 0) {
+        for (auto& r : res) r = make_ct(ct.comps, ct.scale);
+        return res;
+    }
+    const u64 L = level + 1, n = ctx.n, k = ctx.np;
+    const u32 nd = ctx.ndigits(level);
+    const u64 dw = ctx.digit_words(level);
+    hipStream_t s = ctx.stream();
+    BufPtr digs = ctx.alloc(nd * dw + L * n);
+    ctx.decompose({{ct.comps[1]->data(), level, digs->ptr, digs->ptr + nd * dw}}, s);
+    const u64 per = (2 * (L + k) + 2 * k + 2 * L + L) * n;  // acc, y, z, sigma(c0)
+    BufPtr scratch = ctx.alloc(per * nontrivial);
+    BufPtr out = ctx.alloc(2 * L * n * nontrivial);
+    std::vector<ApplyTask> tasks;
+    u32 t = 0;
+    for (size_t i = 0; i < steps.size(); ++i) {
+        if (!keys[i].first) {
+            res[i] = make_ct(ct.comps, ct.scale);
+            continue;
+        }
+        u64* acc = scratch->ptr + t * per;
+        u64* y = acc + 2 * (L + k) * n;
+        u64* z = y + 2 * k * n;
+        u64* r0 = z + 2 * L * n;
+        automorphism_ntt_launch(ct.comps[0]->data(), r0, level + 1, keys[i].first, ctx.log_n, s);
+        u64* ob = out->ptr + 2 * L * n * t;
+        tasks.push_back({digs->ptr, level, keys[i].first, keys[i].second, ob, ob + L * n, r0, nullptr, acc, y, z});
+        res[i] = make_ct({view(out, 2 * L * n * t, level, 0, true), view(out, 2 * L * n * t + L * n, level, 0, true)},
+                         ct.scale);
+        ++t;
+    }
+    ctx.apply(tasks, s);
+    hip_check(hipGetLastError(), "hoisted rotate");
+    count(&ckh_op_counter::rotations, nontrivial);
+    return res;
+}
+
+}  // namespace ckh

@@ -1,0 +1,73 @@
+// The HeOps surface (crates/ckks-train-ckks/src/ops.rs:198-356) over the device engine, with
+// the checks, errors and counts of the CPU Evaluator (crates/ckks-train-ckks/src/evaluator.rs).
+#pragma once
+
+#include <map>
+#include <mutex>
+
+#include "runtime.h"
+
+namespace ckh {
+
+// Polynomial-level operations used by the evaluator and exported through the C ABI.
+PolyPtr poly_alloc(Context& c, u32 level, u32 p_limbs, bool ntt);
+PolyPtr poly_upload(Context& c, const u64* host, u32 level, u32 p_limbs, bool ntt);
+void poly_download(Context& c, const Poly& p, u64* host);
+PolyPtr poly_uniform(Context& c, u32 level, u32 p_limbs, bool ntt, u64 seed, u64 stream0);
+PolyPtr poly_transform(Context& c, const Poly& in, bool forward);
+PolyPtr poly_automorphism(Context& c, const Poly& in, u64 g);
+PolyPtr poly_rescale(Context& c, const Poly& in);
+PolyPtr poly_mod_down(Context& c, const Poly& qp);
+PolyPtr poly_pointwise(Context& c, PwOp op, const Poly& a, const Poly* b);
+// keyswitch::decompose: one buffer holding all digits; returns one QpPoly handle per digit.
+std::vector<PolyPtr> ks_decompose(Context& c, const Poly& d);
+// keyswitch::apply_key for digits laid out as ks_decompose returns them (or copied so).
+std::pair<PolyPtr, PolyPtr> ks_apply(Context& c, const std::vector<PolyPtr>& digits, const Ksk& key);
+std::pair<PolyPtr, PolyPtr> ks_switch(Context& c, const Poly& d, const Ksk& key);
+// n switches under one key in batched launches.
+std::vector<std::pair<PolyPtr, PolyPtr>> ks_switch_batch(Context& c, const std::vector<const Poly*>& d,
+                                                         const Ksk& key);
+KskPtr ksk_upload(Context& c, const u64* host, u32 max_level, u64 galois);
+KskPtr ksk_uniform(Context& c, u32 max_level, u64 galois, u64 seed);
+
+u64 galois_elt(u64 n, int32_t step);
+
+class Evaluator {
+   public:
+    Evaluator(Context& c, KskPtr relin, std::vector<KskPtr> galois);
+    Context& ctx;
+    KskPtr relin;
+    std::map<u64, KskPtr> galois;
+
+    ckh_op_counter counter();
+    void reset_counter();
+    void add_counts(const ckh_op_counter& c);
+    bool has_galois_key(int32_t step) const;
+    bool owns(const Ct& ct) const;
+
+    CtPtr add(const Ct& a, const Ct& b, bool sub);
+    CtPtr negate(const Ct& a);
+    CtPtr mul_no_relin(const Ct& a, const Ct& b);
+    CtPtr relinearize(const Ct& ct);
+    std::vector<CtPtr> relinearize_batch(const std::vector<const Ct*>& cts);
+    CtPtr mul(const Ct& a, const Ct& b);
+    CtPtr rescale(const Ct& ct);
+    CtPtr mod_switch_to_level(const Ct& ct, u32 level);
+    CtPtr relabel(const Ct& ct, double scale);
+    CtPtr rotate(const Ct& ct, int32_t steps);
+    CtPtr conjugate(const Ct& ct);
+    std::vector<CtPtr> hoisted_rotate(const Ct& ct, const std::vector<int32_t>& steps);
+
+   private:
+    std::mutex mu_;
+    ckh_op_counter cnt_{};
+    void count(u64 ckh_op_counter::*field, u64 v = 1);
+    void check_basis(const Ct& ct) const;
+    void check_mul_operands(const Ct& a, const Ct& b) const;
+    const Ksk& relin_key_checked(u32 level) const;
+    const Ksk& galois_key(u64 elt, u32 level) const;
+    u64 rotation_elt(int32_t steps) const;  // 0 for a multiple of the slot count
+    CtPtr apply_galois(const Ct& ct, u64 elt);
+};
+
+}  // namespace ckh

@@ -1,0 +1,21 @@
+// NTT launchers (see ntt.hip for the schedule).
+#pragma once
+#include "common.h"
+
+namespace ckh {
+
+// Device tables shared by every transform of a context: per-prime constants and, per prime,
+// N (w, w') pairs for the forward (psi^i at bitrev(i)) and inverse (psi^-i) directions.
+struct NttTables {
+    u32 log_n;
+    const PrimeConst* pc;
+    const Tw* fwd;  // [prime][N]
+    const Tw* inv;  // [prime][N]
+};
+
+// ref.dst <- NTT(ref.src) for each of the nrefs descriptors (device array). src may equal
+// dst (in place). Inputs reduced, outputs reduced.
+void ntt_forward(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream);
+void ntt_inverse(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream);
+
+}  // namespace ckh

@@ -1,0 +1,42 @@
+// Inner product, pointwise, automorphism and seeded-fill launchers.
+#pragma once
+#include "common.h"
+
+namespace ckh {
+
+// One limb of a key-switch inner product: acc_b = sum_j dig_j * key_b_j, acc_a = sum_j dig_j *
+// key_a_j over ndig digits (digit j's limb at dig + j * dig_stride). The key limb of part j is
+// at key + (2 j) * key_poly (b) and key + (2 j + 1) * key_poly (a). galois != 0 reads the digit
+// through the NTT-form automorphism permutation (hoisted rotations).
+struct IpJob {
+    const u64* dig;
+    u64 dig_stride;
+    const u64* key;
+    u64 key_poly;
+    u64* acc_b;
+    u64* acc_a;
+    u64 galois;
+    u32 ndig, prime;
+};
+
+void inner_product_launch(const IpJob* jobs, u32 njobs, const PrimeConst* pc, u32 log_n,
+                          hipStream_t stream);
+
+// ref.dst <- ref.src (one limb per descriptor).
+void copy_launch(const LimbRef* refs, u32 nrefs, u32 log_n, hipStream_t stream);
+
+enum PwOp : u32 { PW_ADD = 0, PW_SUB = 1, PW_NEG = 2, PW_MUL = 3, PW_COPY = 4 };
+
+// out limb i <- a op b over limbs [0, limbs) with prime index prime0 + i (Q primes).
+void pointwise_launch(PwOp op, const u64* a, const u64* b, u64* out, u32 limbs, u32 prime0,
+                      const PrimeConst* pc, u32 log_n, hipStream_t stream);
+
+// out limb i <- sigma_g(in limb i) in NTT form (index permutation, prime independent).
+void automorphism_ntt_launch(const u64* in, u64* out, u32 limbs, u64 g, u32 log_n, hipStream_t stream);
+
+// Seeded uniform residues: limb i of the layout gets prime primes_map(i) and generator stream
+// stream0 + i; q_limbs limbs use Q primes 0.., the rest P primes (QpPoly layout).
+void fill_uniform_launch(u64* out, u32 q_limbs, u32 p_limbs, u32 nq, u64 seed, u64 stream0,
+                         const PrimeConst* pc, u32 log_n, hipStream_t stream);
+
+}  // namespace ckh

@@ -1,0 +1,139 @@
+// Key-switch inner product, pointwise arithmetic, NTT-form automorphism and the seeded
+// input generator, for gfx950.
+#include "ops.h"
+
+namespace ckh {
+
+// NTT-form Galois permutation: sigma_g(a)[i] = a[perm(i)],
+// perm(i) = bitrev(((g * (2 bitrev(i) + 1)) mod 2N - 1) / 2)
+// (crates/ckks-train-core/src/rns.rs:5-17,147-157), computed inline instead of read from a table.
+CKH_DEV u32 galois_index(u32 i, u64 g, u32 log_n) {
+    const u32 rev = (__brev(i) >> (32 - log_n)) * 2 + 1;
+    const u32 raw = (u32)(((g * rev) >> 1) & ((1u << log_n) - 1));
+    return __brev(raw) >> (32 - log_n);
+}
+
+// acc_b = sum_j d_j * b_j and acc_a = sum_j d_j * a_j per limb of Q_l * P, exact 128-bit sums
+// reduced once (crates/ckks-train-ckks/src/keyswitch.rs:254-303). Digit limbs may be read
+// through the Galois permutation (hoisted rotations, evaluator.rs:596-604).
+__global__ __launch_bounds__(256) void k_inner_product(const IpJob* __restrict__ jobs,
+                                                       const PrimeConst* __restrict__ pc, u32 log_n) {
+    const u32 n = 1u << log_n;
+    const u32 c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const IpJob J = jobs[blockIdx.y];
+    const PrimeConst C = pc[J.prime];
+    const u32 cs = J.galois ? galois_index(c, J.galois, log_n) : c;
+    Acc128 sb = {0, 0}, sa = {0, 0};
+    for (u32 j = 0; j < J.ndig; ++j) {
+        const u64 d = J.dig[j * J.dig_stride + cs];
+        const u64 b = J.key[(2 * j) * J.key_poly + c];
+        const u64 k = J.key[(2 * j + 1) * J.key_poly + c];
+        mac(sb, d, b);
+        mac(sa, d, k);
+    }
+    J.acc_b[c] = reduce128(sb.hi, sb.lo, C);
+    J.acc_a[c] = reduce128(sa.hi, sa.lo, C);
+}
+
+void inner_product_launch(const IpJob* jobs, u32 njobs, const PrimeConst* pc, u32 log_n, hipStream_t stream) {
+    if (njobs == 0) return;
+    const u32 n = 1u << log_n;
+    hipLaunchKernelGGL(k_inner_product, dim3((n + 255) / 256, njobs), dim3(256), 0, stream, jobs, pc, log_n);
+}
+
+__global__ __launch_bounds__(256) void k_copy(const LimbRef* __restrict__ refs, u32 log_n) {
+    const u32 n = 1u << log_n;
+    const u32 c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const LimbRef r = refs[blockIdx.y];
+    r.dst[c] = r.src[c];
+}
+
+void copy_launch(const LimbRef* refs, u32 nrefs, u32 log_n, hipStream_t stream) {
+    if (nrefs == 0) return;
+    const u32 n = 1u << log_n;
+    hipLaunchKernelGGL(k_copy, dim3((n + 255) / 256, nrefs), dim3(256), 0, stream, refs, log_n);
+}
+
+__global__ __launch_bounds__(256) void k_pointwise(u32 op, const u64* __restrict__ a, const u64* __restrict__ b,
+                                                   u64* __restrict__ out, u32 prime0,
+                                                   const PrimeConst* __restrict__ pc, u32 log_n) {
+    const u32 n = 1u << log_n;
+    const u32 c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const u64 i = (u64)blockIdx.y * n + c;
+    const PrimeConst C = pc[prime0 + blockIdx.y];
+    u64 v;
+    switch (op) {
+        case PW_ADD: v = add_mod(a[i], b[i], C.q); break;
+        case PW_SUB: v = sub_mod(a[i], b[i], C.q); break;
+        case PW_NEG: v = sub_mod(0, a[i], C.q); break;
+        case PW_MUL: v = mul_mod(a[i], b[i], C); break;
+        default: v = a[i]; break;
+    }
+    out[i] = v;
+}
+
+void pointwise_launch(PwOp op, const u64* a, const u64* b, u64* out, u32 limbs, u32 prime0, const PrimeConst* pc,
+                      u32 log_n, hipStream_t stream) {
+    if (limbs == 0) return;
+    const u32 n = 1u << log_n;
+    hipLaunchKernelGGL(k_pointwise, dim3((n + 255) / 256, limbs), dim3(256), 0, stream, (u32)op, a, b, out, prime0,
+                       pc, log_n);
+}
+
+__global__ __launch_bounds__(256) void k_automorphism_ntt(const u64* __restrict__ in, u64* __restrict__ out, u64 g,
+                                                          u32 log_n) {
+    const u32 n = 1u << log_n;
+    const u32 c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const u64 base = (u64)blockIdx.y * n;
+    out[base + c] = in[base + galois_index(c, g, log_n)];
+}
+
+void automorphism_ntt_launch(const u64* in, u64* out, u32 limbs, u64 g, u32 log_n, hipStream_t stream) {
+    if (limbs == 0) return;
+    const u32 n = 1u << log_n;
+    hipLaunchKernelGGL(k_automorphism_ntt, dim3((n + 255) / 256, limbs), dim3(256), 0, stream, in, out, g, log_n);
+}
+
+// Stateless seeded generator: word (seed, stream, index) is uniform below q by rejection on
+// a splitmix64-style mix of a counter (no modulo bias, as Rng::below in
+// crates/ckks-train-core/src/sampler.rs:73-82).
+CKH_DEV u64 mix64(u64 z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+CKH_DEV u64 uniform_word(u64 seed, u64 stream, u64 index, u64 q) {
+    const u64 key = mix64(seed ^ mix64(stream + 0x9e3779b97f4a7c15ULL));
+    const u64 zone = (~0ULL / q) * q;
+    u64 x = 0;
+    for (u64 a = 0; a < 64; ++a) {
+        x = mix64(key + (index * 64 + a) * 0x9e3779b97f4a7c15ULL);
+        if (x < zone) break;
+    }
+    return x % q;
+}
+
+__global__ __launch_bounds__(256) void k_fill_uniform(u64* __restrict__ out, u32 q_limbs, u32 nq, u64 seed,
+                                                      u64 stream0, const PrimeConst* __restrict__ pc, u32 log_n) {
+    const u32 n = 1u << log_n;
+    const u32 c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const u32 li = blockIdx.y;
+    const u32 prime = li < q_limbs ? li : nq + (li - q_limbs);
+    out[(u64)li * n + c] = uniform_word(seed, stream0 + li, c, pc[prime].q);
+}
+
+void fill_uniform_launch(u64* out, u32 q_limbs, u32 p_limbs, u32 nq, u64 seed, u64 stream0, const PrimeConst* pc,
+                         u32 log_n, hipStream_t stream) {
+    const u32 n = 1u << log_n;
+    if (q_limbs + p_limbs == 0) return;
+    hipLaunchKernelGGL(k_fill_uniform, dim3((n + 255) / 256, q_limbs + p_limbs), dim3(256), 0, stream, out, q_limbs,
+                       nq, seed, stream0, pc, log_n);
+}
+
+}  // namespace ckh

@@ -1,0 +1,478 @@
+// Host runtime: context construction (device tables), stream-ordered memory, descriptor
+// upload, and the batched key-switching engine.
+#include "runtime.h"
+
+#include <algorithm>
+#include <cstring>
+#include <sstream>
+#include <unordered_map>
+
+#include "hostmath.h"
+
+namespace ckh {
+
+[[noreturn]] void fail(ckh_status st, const std::string& msg) { throw Error{st, msg}; }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        std::ostringstream os;
+        os << what << ": " << hipGetErrorString(e);
+        fail(CKH_ERR_DEVICE, os.str());
+    }
+}
+
+Buffer::~Buffer() {
+    if (ctx) ctx->free_buffer(this);
+    // `keep` is released after the free has been queued on the context's stream.
+}
+
+u64 Ksk::poly_words() const { return (u64)(max_level + 1 + ctx->np) * ctx->n; }
+
+// ------------------------------------------------------------------ context
+
+namespace {
+thread_local std::unordered_map<const Context*, u32> tl_stream;
+}
+
+Context::Context(int ordinal, const ckh_params& p, u32 n_streams) {
+    if (p.log_n < 1 || p.log_n > 17) fail(CKH_ERR_INVALID_PARAMS, "log_n must be in 1..=17");
+    if (p.nq == 0 || p.np == 0 || !p.q || !p.p) fail(CKH_ERR_INVALID_PARAMS, "q and p must be non-empty");
+    if (p.dnum == 0 || p.nq % p.dnum != 0)
+        fail(CKH_ERR_INVALID_PARAMS, "dnum must divide the number of q primes");
+    log_n = p.log_n;
+    n = 1ull << log_n;
+    nq = p.nq;
+    np = p.np;
+    dnum = p.dnum;
+    alpha = nq / dnum;
+    if (alpha > MAX_SRC || np > MAX_SRC)
+        fail(CKH_ERR_INVALID_PARAMS, "at most 16 primes per digit and 16 special primes");
+    primes.assign(p.q, p.q + nq);
+    primes.insert(primes.end(), p.p, p.p + np);
+    for (size_t i = 0; i < primes.size(); ++i) {
+        u64 q = primes[i];
+        if (q < 2 || q >= (1ull << 60) || (q - 1) % (2 * n) != 0 || !hm::is_prime(q))
+            fail(CKH_ERR_INVALID_PARAMS, "every modulus must be a prime < 2^60 with q = 1 mod 2N");
+        for (size_t j = 0; j < i; ++j)
+            if (primes[j] == q) fail(CKH_ERR_INVALID_PARAMS, "duplicate modulus");
+    }
+    int count = 0;
+    hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    if (ordinal < 0 || ordinal >= count) fail(CKH_ERR_DEVICE, "no such device ordinal");
+    device = ordinal;
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+        uint64_t thr = ~0ull;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+    if (n_streams == 0) n_streams = 1;
+    for (u32 i = 0; i < n_streams; ++i) {
+        hipStream_t s;
+        hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        streams_.push_back(s);
+    }
+    build_tables();
+}
+
+Context::~Context() {
+    (void)hipSetDevice(device);
+    for (auto s : streams_) (void)hipStreamSynchronize(s);
+    for (void* a : perm_allocs_) (void)hipFree(a);
+    for (auto s : streams_) (void)hipStreamDestroy(s);
+}
+
+hipStream_t Context::stream() const {
+    auto it = tl_stream.find(this);
+    return streams_[it == tl_stream.end() ? 0 : it->second];
+}
+
+void Context::set_stream(u32 index) {
+    if (index >= streams_.size()) fail(CKH_ERR_INVALID_PARAMS, "stream index out of range");
+    tl_stream[this] = index;
+}
+
+BufPtr Context::alloc(size_t words, hipStream_t s) {
+    void* p = nullptr;
+    size_t bytes = std::max<size_t>(words, 1) * sizeof(u64);
+    hipError_t e = hipMallocAsync(&p, bytes, s);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        std::ostringstream os;
+        os << "out of device memory allocating " << bytes << " bytes";
+        fail(CKH_ERR_DEVICE, os.str());
+    }
+    live_bytes += bytes;
+    auto b = std::make_shared<Buffer>();
+    b->ctx = this;
+    b->keep = shared_from_this();
+    b->ptr = (u64*)p;
+    b->words = words;
+    b->stream = s;
+    return b;
+}
+
+void Context::free_buffer(Buffer* b) {
+    if (!b->ptr) return;
+    (void)hipSetDevice(device);
+    (void)hipFreeAsync(b->ptr, b->stream);
+    live_bytes -= std::max<size_t>(b->words, 1) * sizeof(u64);
+    b->ptr = nullptr;
+}
+
+BufPtr Context::upload_blob(const void* host, size_t bytes, hipStream_t s) {
+    BufPtr b = alloc((bytes + 7) / 8, s);
+    if (bytes) hip_check(hipMemcpyAsync(b->ptr, host, bytes, hipMemcpyHostToDevice, s), "descriptor upload");
+    return b;
+}
+
+template <class T>
+T* Context::dev_copy(const std::vector<T>& v) {
+    void* p = nullptr;
+    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+    hip_check(hipMalloc(&p, bytes), "hipMalloc(tables)");
+    perm_allocs_.push_back(p);
+    if (!v.empty()) hip_check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "table upload");
+    return (T*)p;
+}
+
+std::vector<std::pair<u32, u32>> Context::digit_ranges(u32 level) const {
+    std::vector<std::pair<u32, u32>> r;
+    for (u32 j = 0; j < dnum; ++j) {
+        u32 s = j * alpha, e = std::min((j + 1) * alpha, level + 1);
+        if (s > level) break;
+        r.push_back({s, e});
+    }
+    return r;
+}
+
+u32 Context::ndigits(u32 level) const { return (u32)digit_ranges(level).size(); }
+
+// Device tables: per-prime constants and twiddles (NttTable::new, ntt.rs:89-122), and the
+// conversion plans of ModUp (per level and digit), ModDown (per level) and rescale (per level).
+void Context::build_tables() {
+    const u32 np_all = nq + np;
+    std::vector<PrimeConst> pc(np_all);
+    std::vector<Tw> fwd((size_t)np_all * n), inv((size_t)np_all * n);
+    for (u32 k = 0; k < np_all; ++k) {
+        const u64 q = primes[k];
+        const u64 psi = hm::min_root(n, q), psi_inv = hm::inv(psi, q), n_inv = hm::inv(n % q, q);
+        u64 p1 = 1, p2 = 1;
+        for (u64 i = 0; i < n; ++i) {
+            const u32 j = hm::bitrev((u32)i, log_n);
+            fwd[(size_t)k * n + j] = {p1, hm::shoup(p1, q)};
+            inv[(size_t)k * n + j] = {p2, hm::shoup(p2, q)};
+            p1 = hm::mul(p1, psi, q);
+            p2 = hm::mul(p2, psi_inv, q);
+        }
+        const u64 iroot1 = n >= 2 ? inv[(size_t)k * n + 1].w : 1;
+        PrimeConst& c = pc[k];
+        c.q = q;
+        c.n_inv = n_inv;
+        c.n_inv_s = hm::shoup(n_inv, q);
+        c.w_n_inv = hm::mul(iroot1, n_inv, q);
+        c.w_n_inv_s = hm::shoup(c.w_n_inv, q);
+        c.one_s = hm::shoup(1, q);
+        c.two64 = (u64)(((hm::u128)1 << 64) % q);
+        c.two64_s = hm::shoup(c.two64, q);
+    }
+    d_pc = dev_copy(pc);
+    ntt.log_n = log_n;
+    ntt.pc = d_pc;
+    ntt.fwd = dev_copy(fwd);
+    ntt.inv = dev_copy(inv);
+
+    std::vector<ConvSrc> srcs;
+    std::vector<ConvTgt> tgts;
+    std::vector<ConvPlan> plans;
+    // sources s (prime indices) -> targets (prime index, output limb)
+    auto add_plan = [&](const std::vector<u32>& src, const std::vector<std::pair<u32, u32>>& tg, u32 limb0,
+                        bool share_tgt_first, u32 shared_first) {
+        ConvPlan pl{};
+        pl.src_first = (u32)srcs.size();
+        pl.nsrc = (u32)src.size();
+        pl.src_limb0 = limb0;
+        pl.exact = src.size() > 1;
+        std::vector<u64> sq;
+        for (u32 s : src) sq.push_back(primes[s]);
+        for (size_t j = 0; j < src.size(); ++j) {
+            const u64 q = sq[j];
+            u64 qh = 1 % q;
+            for (size_t i = 0; i < src.size(); ++i)
+                if (i != j) qh = hm::mul(qh, sq[i] % q, q);
+            ConvSrc cs{};
+            cs.q = q;
+            cs.h = q >> 1;
+            cs.qhinv = hm::inv(qh, q);
+            cs.qhinv_s = hm::shoup(cs.qhinv, q);
+            cs.inv = 1.0 / (double)q;
+            srcs.push_back(cs);
+        }
+        if (share_tgt_first) {
+            pl.tgt_first = shared_first;
+        } else {
+            pl.tgt_first = (u32)tgts.size();
+            for (auto [prime, limb] : tg) {
+                const u64 r = primes[prime];
+                ConvTgt t{};
+                t.prime = prime;
+                t.limb = limb;
+                t.qmod = hm::prod_mod(sq.data(), sq.size(), r);
+                t.qmod_s = hm::shoup(t.qmod, r);
+                t.hmod = hm::half_mod(sq.data(), sq.size(), r);
+                for (size_t j = 0; j < src.size(); ++j) {
+                    u64 qh = 1 % r;
+                    for (size_t i = 0; i < src.size(); ++i)
+                        if (i != j) qh = hm::mul(qh, sq[i] % r, r);
+                    t.qhat[j] = qh;
+                    t.qhat_s[j] = hm::shoup(qh, r);
+                }
+                tgts.push_back(t);
+            }
+        }
+        pl.ntgt = (u32)tg.size();
+        plans.push_back(pl);
+        return (u32)plans.size() - 1;
+    };
+
+    modup_plan_.assign(nq, {});
+    modup_maxtgt_.assign(nq, 0);
+    for (u32 l = 0; l < nq; ++l) {
+        for (auto [s, e] : digit_ranges(l)) {
+            std::vector<u32> src;
+            for (u32 i = s; i < e; ++i) src.push_back(i);
+            std::vector<std::pair<u32, u32>> tg;
+            for (u32 i = 0; i <= l; ++i)
+                if (i < s || i >= e) tg.push_back({i, i});
+            for (u32 t = 0; t < np; ++t) tg.push_back({nq + t, l + 1 + t});
+            modup_maxtgt_[l] = std::max(modup_maxtgt_[l], (u32)tg.size());
+            modup_plan_[l].push_back(add_plan(src, tg, s, false, 0));
+        }
+    }
+    // ModDown: P -> Q_l; the targets of level l are the first l + 1 of one shared list.
+    {
+        std::vector<u32> src;
+        for (u32 t = 0; t < np; ++t) src.push_back(nq + t);
+        std::vector<std::pair<u32, u32>> all;
+        for (u32 i = 0; i < nq; ++i) all.push_back({i, i});
+        u32 top = add_plan(src, all, 0, false, 0);
+        u32 first = plans[top].tgt_first;
+        moddown_plan_.assign(nq, 0);
+        for (u32 l = 0; l < nq; ++l) {
+            std::vector<std::pair<u32, u32>> tg(all.begin(), all.begin() + l + 1);
+            moddown_plan_[l] = add_plan(src, tg, 0, true, first);
+        }
+        pinv_.resize(nq);
+        pinv_s_.resize(nq);
+        for (u32 i = 0; i < nq; ++i) {
+            u64 pm = hm::prod_mod(primes.data() + nq, np, primes[i]);
+            pinv_[i] = hm::inv(pm, primes[i]);
+            pinv_s_[i] = hm::shoup(pinv_[i], primes[i]);
+        }
+    }
+    // Rescale: q_l -> q_0..q_{l-1}.
+    rescale_plan_.assign(nq, 0);
+    qlinv_.assign(nq, {});
+    qlinv_s_.assign(nq, {});
+    for (u32 l = 1; l < nq; ++l) {
+        std::vector<std::pair<u32, u32>> tg;
+        for (u32 i = 0; i < l; ++i) tg.push_back({i, i});
+        rescale_plan_[l] = add_plan({l}, tg, 0, false, 0);
+        for (u32 i = 0; i < l; ++i) {
+            u64 v = hm::inv(primes[l] % primes[i], primes[i]);
+            qlinv_[l].push_back(v);
+            qlinv_s_[l].push_back(hm::shoup(v, primes[i]));
+        }
+    }
+    conv.src = dev_copy(srcs);
+    conv.tgt = dev_copy(tgts);
+    conv.plans = dev_copy(plans);
+    conv.pc = d_pc;
+    conv.log_n = log_n;
+    hip_check(hipDeviceSynchronize(), "table upload");
+}
+
+// ------------------------------------------------------------------ descriptor blobs
+
+namespace {
+struct Blob {
+    std::vector<uint8_t> bytes;
+    template <class T>
+    size_t add(const std::vector<T>& v) {
+        size_t off = (bytes.size() + 15) & ~(size_t)15;
+        bytes.resize(off + v.size() * sizeof(T));
+        if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+        return off;
+    }
+};
+template <class T>
+const T* at(const BufPtr& b, size_t off) {
+    return (const T*)((const uint8_t*)b->ptr + off);
+}
+constexpr u32 MAX_ROWS = 65535;
+}  // namespace
+
+void launch_ntt(const Context& c, const LimbRef* d_refs, u32 nrefs, bool forward, hipStream_t s) {
+    for (u32 i = 0; i < nrefs; i += MAX_ROWS) {
+        u32 cnt = std::min(MAX_ROWS, nrefs - i);
+        if (forward)
+            ntt_forward(c.ntt, d_refs + i, cnt, s);
+        else
+            ntt_inverse(c.ntt, d_refs + i, cnt, s);
+    }
+}
+
+static void launch_copy(const LimbRef* d, u32 nrefs, u32 log_n, hipStream_t s) {
+    for (u32 i = 0; i < nrefs; i += MAX_ROWS) copy_launch(d + i, std::min(MAX_ROWS, nrefs - i), log_n, s);
+}
+
+static void launch_conv(const ConvTables& t, const ConvJob* d, u32 njobs, u32 max_tgt, u32 max_src, hipStream_t s) {
+    for (u32 i = 0; i < njobs; i += MAX_ROWS) conv_launch(t, d + i, std::min(MAX_ROWS, njobs - i), max_tgt, max_src, s);
+}
+
+static void launch_ip(const IpJob* d, u32 njobs, const PrimeConst* pc, u32 log_n, hipStream_t s) {
+    for (u32 i = 0; i < njobs; i += MAX_ROWS) inner_product_launch(d + i, std::min(MAX_ROWS, njobs - i), pc, log_n, s);
+}
+
+static void launch_finish(const FinishJob* d, u32 njobs, const PrimeConst* pc, u32 log_n, hipStream_t s) {
+    for (u32 i = 0; i < njobs; i += MAX_ROWS) finish_launch(d + i, std::min(MAX_ROWS, njobs - i), pc, log_n, s);
+}
+
+// ------------------------------------------------------------------ engine
+
+void Context::ntt_poly(const u64* in, u64* out, u32 limbs, u32 level, u32 p_limbs, bool forward, hipStream_t s) {
+    std::vector<LimbRef> refs;
+    for (u32 i = 0; i < limbs; ++i) {
+        u32 prime = i <= level ? i : nq + (i - level - 1);
+        refs.push_back({in + i * n, out + i * n, prime, 0});
+    }
+    (void)p_limbs;
+    Blob b;
+    size_t o = b.add(refs);
+    BufPtr d = upload_blob(b.bytes.data(), b.bytes.size(), s);
+    launch_ntt(*this, at<LimbRef>(d, o), (u32)refs.size(), forward, s);
+}
+
+// keyswitch::decompose (keyswitch.rs:82-124) for a batch: INTT of every input limb, the
+// centered exact conversion of every digit to its targets, the digit limbs copied, then the
+// NTT of every converted limb (lift_digit_exact, keyswitch.rs:140-210).
+void Context::decompose(const std::vector<DecTask>& tasks, hipStream_t s) {
+    std::vector<LimbRef> intt, copies, nttr;
+    std::vector<ConvJob> jobs;
+    u32 max_tgt = 0;
+    for (const DecTask& t : tasks) {
+        const u32 L = t.level + 1;
+        const u64 ds = digit_words(t.level);
+        for (u32 i = 0; i < L; ++i) intt.push_back({t.d + i * n, t.x + i * n, i, 0});
+        auto dr = digit_ranges(t.level);
+        for (u32 j = 0; j < dr.size(); ++j) {
+            u64* dj = t.dig + j * ds;
+            jobs.push_back({t.x, dj, modup_plan_[t.level][j], 0});
+            for (u32 i = dr[j].first; i < dr[j].second; ++i) copies.push_back({t.d + i * n, dj + i * n, i, 0});
+            for (u32 i = 0; i < L; ++i)
+                if (i < dr[j].first || i >= dr[j].second) nttr.push_back({dj + i * n, dj + i * n, i, 0});
+            for (u32 k = 0; k < np; ++k) nttr.push_back({dj + (L + k) * n, dj + (L + k) * n, nq + k, 0});
+        }
+        max_tgt = std::max(max_tgt, modup_maxtgt_[t.level]);
+    }
+    Blob b;
+    size_t o_intt = b.add(intt), o_jobs = b.add(jobs), o_copy = b.add(copies), o_ntt = b.add(nttr);
+    BufPtr d = upload_blob(b.bytes.data(), b.bytes.size(), s);
+    launch_ntt(*this, at<LimbRef>(d, o_intt), (u32)intt.size(), false, s);
+    launch_conv(conv, at<ConvJob>(d, o_jobs), (u32)jobs.size(), max_tgt, alpha, s);
+    launch_copy(at<LimbRef>(d, o_copy), (u32)copies.size(), log_n, s);
+    launch_ntt(*this, at<LimbRef>(d, o_ntt), (u32)nttr.size(), true, s);
+}
+
+// keyswitch::apply_key (keyswitch.rs:230-309) for a batch: the inner product over Q_l * P, then
+// RnsPoly::mod_down of both sums (baseconv.rs:459-511): INTT of the P limbs, the centered
+// exact conversion P -> Q_l, NTT, and (x - t) * P^-1 (+ the optional addend).
+void Context::apply(const std::vector<ApplyTask>& tasks, hipStream_t s) {
+    std::vector<IpJob> ip;
+    u32 max_l = 0;
+    for (const ApplyTask& t : tasks) max_l = std::max(max_l, t.level + 1);
+    // Inner-product rows ordered by key limb across tasks, so rows reading the same key limb
+    // run back to back (the shared key is re-read from L2, not HBM).
+    for (u32 k = 0; k < np; ++k)
+        for (const ApplyTask& t : tasks) {
+            const u32 L = t.level + 1, li = L + k;
+            const u64 aw = (u64)(L + np) * n;
+            ip.push_back({t.dig + li * n, digit_words(t.level), t.key->buf->ptr + (t.key->max_level + 1 + k) * n,
+                          t.key->poly_words(), t.acc + li * n, t.acc + aw + li * n, t.galois, ndigits(t.level),
+                          nq + k});
+        }
+    for (u32 i = 0; i < max_l; ++i)
+        for (const ApplyTask& t : tasks) {
+            const u32 L = t.level + 1;
+            if (i >= L) continue;
+            const u64 aw = (u64)(L + np) * n;
+            ip.push_back({t.dig + i * n, digit_words(t.level), t.key->buf->ptr + i * n, t.key->poly_words(),
+                          t.acc + i * n, t.acc + aw + i * n, t.galois, ndigits(t.level), i});
+        }
+    Blob b;
+    size_t o_ip = b.add(ip);
+    BufPtr d = upload_blob(b.bytes.data(), b.bytes.size(), s);
+    launch_ip(at<IpJob>(d, o_ip), (u32)ip.size(), d_pc, log_n, s);
+    std::vector<MdTask> md;
+    for (const ApplyTask& t : tasks) {
+        const u32 L = t.level + 1;
+        const u64 aw = (u64)(L + np) * n;
+        md.push_back({t.acc, t.level, t.out_b, t.add_b, t.y, t.z});
+        md.push_back({t.acc + aw, t.level, t.out_a, t.add_a, t.y + (u64)np * n, t.z + (u64)L * n});
+    }
+    mod_down(md, s);
+}
+
+// RnsPoly::mod_down (baseconv.rs:459-511) for a batch: INTT of the P limbs, the centered exact
+// conversion P -> Q_l, NTT, and (x - t) * P^-1 (+ the optional addend).
+void Context::mod_down(const std::vector<MdTask>& tasks, hipStream_t s) {
+    std::vector<LimbRef> intt, nttr;
+    std::vector<ConvJob> jobs;
+    std::vector<FinishJob> fin;
+    u32 max_l = 0;
+    for (const MdTask& t : tasks) {
+        const u32 L = t.level + 1;
+        max_l = std::max(max_l, L);
+        for (u32 k = 0; k < np; ++k) intt.push_back({t.acc + (L + k) * n, t.y + k * n, nq + k, 0});
+        jobs.push_back({t.y, t.z, moddown_plan_[t.level], 0});
+        for (u32 i = 0; i < L; ++i) {
+            u64* zi = t.z + i * n;
+            nttr.push_back({zi, zi, i, 0});
+            fin.push_back({t.acc + i * n, zi, t.out + i * n, t.add ? t.add + i * n : nullptr, pinv_[i], pinv_s_[i], i, 0});
+        }
+    }
+    Blob b;
+    size_t o_intt = b.add(intt), o_jobs = b.add(jobs), o_ntt = b.add(nttr), o_fin = b.add(fin);
+    BufPtr d = upload_blob(b.bytes.data(), b.bytes.size(), s);
+    launch_ntt(*this, at<LimbRef>(d, o_intt), (u32)intt.size(), false, s);
+    launch_conv(conv, at<ConvJob>(d, o_jobs), (u32)jobs.size(), max_l, np, s);
+    launch_ntt(*this, at<LimbRef>(d, o_ntt), (u32)nttr.size(), true, s);
+    launch_finish(at<FinishJob>(d, o_fin), (u32)fin.size(), d_pc, log_n, s);
+}
+
+// RnsPoly::rescale_by_last in NTT form (baseconv.rs:258-291) for a batch.
+void Context::rescale(const std::vector<RescaleTask>& tasks, hipStream_t s) {
+    std::vector<LimbRef> intt, nttr;
+    std::vector<ConvJob> jobs;
+    std::vector<FinishJob> fin;
+    u32 max_t = 0;
+    for (const RescaleTask& t : tasks) {
+        const u32 l = t.level;
+        intt.push_back({t.in + l * n, t.x, l, 0});
+        jobs.push_back({t.x, t.z, rescale_plan_[l], 0});
+        for (u32 i = 0; i < l; ++i) {
+            nttr.push_back({t.z + i * n, t.z + i * n, i, 0});
+            fin.push_back({t.in + i * n, t.z + i * n, t.out + i * n, nullptr, qlinv_[l][i], qlinv_s_[l][i], i, 0});
+        }
+        max_t = std::max(max_t, l);
+    }
+    Blob b;
+    size_t o_intt = b.add(intt), o_jobs = b.add(jobs), o_ntt = b.add(nttr), o_fin = b.add(fin);
+    BufPtr d = upload_blob(b.bytes.data(), b.bytes.size(), s);
+    launch_ntt(*this, at<LimbRef>(d, o_intt), (u32)intt.size(), false, s);
+    launch_conv(conv, at<ConvJob>(d, o_jobs), (u32)jobs.size(), max_t, 1, s);
+    launch_ntt(*this, at<LimbRef>(d, o_ntt), (u32)nttr.size(), true, s);
+    launch_finish(at<FinishJob>(d, o_fin), (u32)fin.size(), d_pc, log_n, s);
+}
+
+}  // namespace ckh

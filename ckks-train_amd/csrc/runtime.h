@@ -1,0 +1,177 @@
+// Host runtime of the gfx950 RNS-CKKS library: device context, tables, stream-ordered memory,
+// and the key-switching engine (decompose -> inner product -> mod_down) over batches of tasks.
+#pragma once
+
+#include <atomic>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/ckh.h"
+#include "basis.h"
+#include "common.h"
+#include "ntt.h"
+#include "ops.h"
+
+namespace ckh {
+
+// Internal error, mapped to a ckh_status at the C boundary.
+struct Error {
+    ckh_status status;
+    std::string msg;
+};
+
+[[noreturn]] void fail(ckh_status st, const std::string& msg);
+void hip_check(hipError_t e, const char* what);
+
+class Context;
+
+// A device allocation from the context's stream-ordered pool; freed on its stream when the
+// last reference goes.
+struct Buffer {
+    Context* ctx;
+    std::shared_ptr<Context> keep;  // a buffer keeps its context (and streams) alive
+    u64* ptr;
+    size_t words;
+    hipStream_t stream;
+    ~Buffer();
+};
+using BufPtr = std::shared_ptr<Buffer>;
+
+// A polynomial: Q limbs 0..=level, then p_limbs P limbs, inside a buffer.
+struct Poly {
+    BufPtr buf;
+    u64 off;  // word offset in buf
+    u32 level;
+    u32 p_limbs;
+    bool ntt;
+    u64* data() const { return buf->ptr + off; }
+    u32 limbs() const { return level + 1 + p_limbs; }
+};
+using PolyPtr = std::shared_ptr<Poly>;
+
+// A key-switching key in the reference's device order (parts x {b, a} x (kml + 1 + k) limbs).
+struct Ksk {
+    BufPtr buf;
+    u32 max_level;
+    u32 parts;
+    u64 galois;  // 0 = relinearization
+    Context* ctx;
+    u64 poly_words() const;
+};
+using KskPtr = std::shared_ptr<Ksk>;
+
+struct Ct {
+    std::vector<PolyPtr> comps;
+    double scale;
+    u32 level() const { return comps.empty() ? 0 : comps[0]->level; }
+};
+using CtPtr = std::shared_ptr<Ct>;
+
+// One decomposition: NTT-form d (level l) -> digits (ndig QpPolys of l + 1 + k limbs, stride
+// ds words) using x (l + 1 limbs) as scratch.
+struct DecTask {
+    const u64* d;
+    u32 level;
+    u64* dig;
+    u64* x;
+};
+
+// One inner product + mod_down: digits (from a DecTask) times key (galois != 0: digits read
+// through sigma_galois) -> (out_b, out_a) at level l, plus optional addends. acc (2 (l+1+k)
+// limbs), y (2k limbs) and z (2 (l+1) limbs) are scratch.
+struct ApplyTask {
+    const u64* dig;
+    u32 level;
+    u64 galois;
+    const Ksk* key;
+    u64* out_b;
+    u64* out_a;
+    const u64* add_b;
+    const u64* add_a;
+    u64* acc;
+    u64* y;
+    u64* z;
+};
+
+// One mod_down of a QpPoly sum (level l, l + 1 + k limbs, NTT form) -> out (l + 1 limbs) plus an
+// optional addend; y (k limbs) and z (l + 1 limbs) are scratch.
+struct MdTask {
+    const u64* acc;
+    u32 level;
+    u64* out;
+    const u64* add;
+    u64* y;
+    u64* z;
+};
+
+// One rescale of a polynomial (level l -> l - 1): out (l limbs), scratch x (1 limb), z (l limbs).
+struct RescaleTask {
+    const u64* in;
+    u32 level;
+    u64* out;
+    u64* x;
+    u64* z;
+};
+
+class Context : public std::enable_shared_from_this<Context> {
+   public:
+    Context(int ordinal, const ckh_params& p, u32 n_streams);
+    ~Context();
+
+    int device = 0;
+    u32 log_n = 0;
+    u64 n = 0;
+    u32 nq = 0, np = 0, dnum = 0, alpha = 0;
+    std::vector<u64> primes;  // Q then P
+
+    NttTables ntt{};
+    ConvTables conv{};
+    PrimeConst* d_pc = nullptr;
+
+    // Stream of the calling thread.
+    hipStream_t stream() const;
+    void set_stream(u32 index);
+    u32 n_streams() const { return (u32)streams_.size(); }
+    hipStream_t stream_at(u32 i) const { return streams_[i]; }
+
+    BufPtr alloc(size_t words, hipStream_t s);
+    BufPtr alloc(size_t words) { return alloc(words, stream()); }
+    void free_buffer(Buffer* b);
+    std::atomic<u64> live_bytes{0};
+
+    // Digit ranges of a level-l polynomial (keyswitch.rs:68-74).
+    std::vector<std::pair<u32, u32>> digit_ranges(u32 level) const;
+    u32 ndigits(u32 level) const;
+    u64 digit_words(u32 level) const { return (u64)(level + 1 + np) * n; }
+
+    // The engine.
+    void decompose(const std::vector<DecTask>& tasks, hipStream_t s);
+    void apply(const std::vector<ApplyTask>& tasks, hipStream_t s);
+    void mod_down(const std::vector<MdTask>& tasks, hipStream_t s);
+    void rescale(const std::vector<RescaleTask>& tasks, hipStream_t s);
+    void ntt_poly(const u64* in, u64* out, u32 limbs, u32 level, u32 p_limbs, bool forward, hipStream_t s);
+
+    // Copies a host blob to a fresh device buffer on stream s.
+    BufPtr upload_blob(const void* host, size_t bytes, hipStream_t s);
+
+   private:
+    std::vector<hipStream_t> streams_;
+    std::vector<void*> perm_allocs_;
+    std::vector<std::vector<u32>> modup_plan_;  // [level][digit]
+    std::vector<u32> modup_maxtgt_;            // [level]
+    std::vector<u32> moddown_plan_;            // [level]
+    std::vector<u32> rescale_plan_;            // [level], level >= 1
+    std::vector<u64> pinv_, pinv_s_;           // P^-1 mod q_i
+    std::vector<std::vector<u64>> qlinv_, qlinv_s_;  // [l][i]: q_l^-1 mod q_i
+    void build_tables();
+    template <class T>
+    T* dev_copy(const std::vector<T>& v);
+};
+
+// Launch helpers that split grids above the 65535 row limit.
+void launch_ntt(const Context& c, const LimbRef* d_refs, u32 nrefs, bool forward, hipStream_t s);
+
+}  // namespace ckh

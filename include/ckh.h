@@ -1,0 +1,202 @@
+/*
+ * ckh: RNS-CKKS key-switching backend for AMD Instinct MI355X (gfx950).
+ *
+ * A C ABI over a C++ host runtime and offline-compiled HIP kernels. It replaces the device
+ * backend behind the reference's `HeOps` trait (crates/ckks-train-ckks/src/ops.rs:198-356),
+ * i.e. the role of `CudaEvaluator` / `CudaDevice` (crates/ckks-train-cuda/src/evaluator.rs,
+ * src/device.rs). Plain pointers and sizes only; every residue array is u64, limb-major
+ * (limb i of a polynomial at offset i * N), NTT form unless stated.
+ *
+ * Semantics, checks, errors and operation counts follow the CPU `Evaluator`
+ * (crates/ckks-train-ckks/src/evaluator.rs); results are bit-identical to the CPU path.
+ *
+ * Threading: a context may be used from several threads; each thread works on its selected
+ * stream (ckh_set_stream). Operations are asynchronous on that stream: a returned handle may
+ * be used at once by later calls; downloads and ckh_synchronize wait.
+ * Ownership: handles are reference counted (`retain` = the reference's cheap `Clone`,
+ * ops.rs:17-20); device memory is returned to a stream-ordered pool on the last release.
+ */
+#ifndef CKH_H
+#define CKH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: OK plus one per HekitError variant (crates/ckks-train-core/src/error.rs:7-21). */
+typedef enum ckh_status {
+    CKH_OK = 0,
+    CKH_ERR_INVALID_PARAMS = 1, /* HekitError::InvalidParams */
+    CKH_ERR_SCALE_MISMATCH = 2, /* HekitError::ScaleMismatch */
+    CKH_ERR_LEVEL_MISMATCH = 3, /* HekitError::LevelMismatch */
+    CKH_ERR_NO_LEVELS_LEFT = 4, /* HekitError::NoLevelsLeft */
+    CKH_ERR_DECODE = 5,         /* HekitError::Decode */
+    CKH_ERR_DEVICE = 6          /* HekitError::Device: never a crash (device.rs:29-50) */
+} ckh_status;
+
+typedef struct ckh_context ckh_context;
+typedef struct ckh_poly ckh_poly;
+typedef struct ckh_ksk ckh_ksk;
+typedef struct ckh_ct ckh_ct;
+typedef struct ckh_evaluator ckh_evaluator;
+typedef struct ckh_server ckh_server;
+
+/* Message of the calling thread's last error (the String payload of HekitError). */
+const char *ckh_last_error(void);
+
+/* ------------------------------------------------------------------ parameters */
+
+/* SEAL-style prime hand-out of CkksParams::new (crates/ckks-train-ckks/src/params.rs:87-157):
+ * per bit size the c largest NTT primes (descending, primes.rs:67-112) handed out from the
+ * smallest in order of appearance. Writes nq primes to q_out and np to p_out. */
+ckh_status ckh_params_primes(uint32_t log_n, const uint32_t *q_bits, uint32_t nq,
+                             const uint32_t *p_bits, uint32_t np, uint64_t *q_out,
+                             uint64_t *p_out);
+
+/* Galois elements (crates/ckks-train-ckks/src/keys.rs:89-107). */
+uint64_t ckh_galois_elt(uint32_t log_n, int32_t step);
+uint64_t ckh_conjugation_elt(uint32_t log_n);
+
+typedef struct ckh_params {
+    uint32_t log_n;     /* 6 ..= 17 */
+    const uint64_t *q;  /* data primes q_0..q_L (< 2^60, = 1 mod 2N) */
+    uint32_t nq;
+    const uint64_t *p;  /* special primes */
+    uint32_t np;
+    uint32_t dnum;      /* key-switch digits; alpha = nq / dnum, must divide */
+} ckh_params;
+
+/* ------------------------------------------------------------------ context */
+
+/* Opens device `ordinal` for one parameter set and builds its tables (CudaDevice::new,
+ * crates/ckks-train-cuda/src/device.rs:262-330). n_streams >= 1 HIP streams (default priority). */
+ckh_status ckh_device_open(int ordinal, const ckh_params *params, uint32_t n_streams,
+                           ckh_context **out);
+void ckh_device_close(ckh_context *ctx);
+/* Selects the calling thread's stream (device.rs:131-136,539-574). */
+ckh_status ckh_set_stream(ckh_context *ctx, uint32_t index);
+/* The calling thread's hipStream_t (for event timing by callers). */
+void *ckh_stream_handle(ckh_context *ctx);
+/* Waits for all work of the context. */
+ckh_status ckh_synchronize(ckh_context *ctx);
+/* Device memory: bytes in use by live handles, and free device bytes (evaluator.rs:496-519). */
+ckh_status ckh_memory_info(ckh_context *ctx, uint64_t *live_bytes, uint64_t *free_bytes);
+
+/* ------------------------------------------------------------------ polynomials */
+
+/* A polynomial over Q_level (level + 1 limbs) followed by p_limbs special-prime limbs
+ * (0 or np: a QpPoly, crates/ckks-train-ckks/src/keys.rs:109-150). */
+ckh_status ckh_poly_upload(ckh_context *ctx, const uint64_t *limbs, uint32_t level,
+                           uint32_t p_limbs, int ntt_form, ckh_poly **out);
+ckh_status ckh_poly_download(ckh_context *ctx, const ckh_poly *poly, uint64_t *out);
+ckh_status ckh_poly_info(const ckh_poly *poly, uint32_t *level, uint32_t *p_limbs,
+                         int *ntt_form);
+ckh_poly *ckh_poly_retain(ckh_poly *poly);
+void ckh_poly_release(ckh_poly *poly);
+/* Seeded uniform residues on the device: limb i from generator stream stream0 + i. */
+ckh_status ckh_poly_uniform(ckh_context *ctx, uint32_t level, uint32_t p_limbs, int ntt_form,
+                            uint64_t seed, uint64_t stream0, ckh_poly **out);
+/* RnsPoly::to_ntt / to_coeff (crates/ckks-train-core/src/poly.rs:192-211). */
+ckh_status ckh_ntt(ckh_context *ctx, const ckh_poly *in, ckh_poly **out);
+ckh_status ckh_intt(ckh_context *ctx, const ckh_poly *in, ckh_poly **out);
+/* RnsPoly::automorphism in NTT form (poly.rs:242-283). */
+ckh_status ckh_automorphism(ckh_context *ctx, const ckh_poly *in, uint64_t galois_elt,
+                            ckh_poly **out);
+/* RnsPoly::rescale_by_last (crates/ckks-train-core/src/baseconv.rs:258-291). */
+ckh_status ckh_rescale_poly(ckh_context *ctx, const ckh_poly *in, ckh_poly **out);
+/* RnsPoly::mod_down of a QpPoly (baseconv.rs:459-511): Q part level l -> level l. */
+ckh_status ckh_mod_down(ckh_context *ctx, const ckh_poly *qp, ckh_poly **out);
+
+/* ------------------------------------------------------------------ key switching */
+
+/* A key-switching key in the reference's device order (crates/ckks-train-cuda/src/ks.rs:76-116):
+ * parts j < digits(max_level), each b_j then a_j, each as Q limbs 0..=max_level then the np P
+ * limbs, NTT form. galois_elt 0 = relinearization key. */
+ckh_status ckh_ksk_upload(ckh_context *ctx, const uint64_t *limbs, uint32_t max_level,
+                          uint64_t galois_elt, ckh_ksk **out);
+ckh_status ckh_ksk_uniform(ckh_context *ctx, uint32_t max_level, uint64_t galois_elt,
+                           uint64_t seed, ckh_ksk **out);
+ckh_status ckh_ksk_info(const ckh_ksk *key, uint32_t *max_level, uint64_t *galois_elt,
+                        uint64_t *bytes);
+void ckh_ksk_release(ckh_ksk *key);
+
+/* keyswitch::decompose (crates/ckks-train-ckks/src/keyswitch.rs:82-124): the exact centered
+ * lift of every digit of d (NTT form, level l) to Q_l * P. digits must hold dnum handles. */
+ckh_status ckh_decompose(ckh_context *ctx, const ckh_poly *d, ckh_poly **digits,
+                         uint32_t *n_digits);
+/* keyswitch::apply_key (keyswitch.rs:230-309): inner product and mod_down of both sums. */
+ckh_status ckh_apply_key(ckh_context *ctx, ckh_poly *const *digits, uint32_t n_digits,
+                         const ckh_ksk *key, ckh_poly **kb, ckh_poly **ka);
+/* keyswitch::key_switch (keyswitch.rs:317-319): one ciphertext per launch sequence. */
+ckh_status ckh_key_switch(ckh_context *ctx, const ckh_poly *d, const ckh_ksk *key,
+                          ckh_poly **kb, ckh_poly **ka);
+/* n key switches under one shared key, polynomials at any levels, in batched launches
+ * (no counterpart in the reference; bit-identical to n calls of ckh_key_switch). */
+ckh_status ckh_key_switch_batch(ckh_context *ctx, uint32_t n, const ckh_poly *const *d,
+                                const ckh_ksk *key, ckh_poly **kb, ckh_poly **ka);
+
+/* ------------------------------------------------------------------ ciphertexts */
+
+/* n_comp polynomials of level + 1 limbs each, back to back, NTT form; scale is the f64 label. */
+ckh_status ckh_ct_upload(ckh_context *ctx, const uint64_t *polys, uint32_t n_comp,
+                         uint32_t level, double scale, ckh_ct **out);
+ckh_status ckh_ct_from_polys(ckh_context *ctx, ckh_poly *const *polys, uint32_t n_comp,
+                             double scale, ckh_ct **out);
+ckh_status ckh_ct_download(ckh_context *ctx, const ckh_ct *ct, uint64_t *out);
+ckh_status ckh_ct_info(const ckh_ct *ct, uint32_t *n_comp, uint32_t *level, double *scale);
+/* Borrowed component handle (retain it to keep it beyond the ciphertext). */
+ckh_poly *ckh_ct_component(const ckh_ct *ct, uint32_t index);
+ckh_ct *ckh_ct_retain(ckh_ct *ct);
+void ckh_ct_release(ckh_ct *ct);
+
+/* ------------------------------------------------------------------ evaluator (HeOps) */
+
+/* OpCounter (crates/ckks-train-ckks/src/counter.rs:42-79). */
+typedef struct ckh_op_counter {
+    uint64_t rotations, relins, ct_ct_tensor, ct_pt_mults, rescales, sse_switches, bootstraps;
+} ckh_op_counter;
+
+/* Evaluator::new(params, relin, galois) (evaluator.rs:179-186); keys are shared, not copied. */
+ckh_status ckh_evaluator_create(ckh_context *ctx, const ckh_ksk *relin,
+                                const ckh_ksk *const *galois, uint32_t n_galois,
+                                ckh_evaluator **out);
+/* HeOps::fork (ops.rs:243-246): same keys, own counter from zero. */
+ckh_status ckh_evaluator_fork(ckh_evaluator *ev, ckh_evaluator **out);
+void ckh_evaluator_destroy(ckh_evaluator *ev);
+
+ckh_status ckh_counter(ckh_evaluator *ev, ckh_op_counter *out);
+ckh_status ckh_reset_counter(ckh_evaluator *ev);
+ckh_status ckh_add_counts(ckh_evaluator *ev, const ckh_op_counter *counts);
+int ckh_has_relin_key(ckh_evaluator *ev);
+int ckh_has_galois_key(ckh_evaluator *ev, int32_t step);
+int ckh_owns(ckh_evaluator *ev, const ckh_ct *ct);
+
+/* HeOps::{add, sub, negate} (evaluator.rs add/sub/negate). */
+ckh_status ckh_add(ckh_evaluator *ev, const ckh_ct *a, const ckh_ct *b, ckh_ct **out);
+ckh_status ckh_sub(ckh_evaluator *ev, const ckh_ct *a, const ckh_ct *b, ckh_ct **out);
+ckh_status ckh_negate(ckh_evaluator *ev, const ckh_ct *a, ckh_ct **out);
+/* HeOps::mul_no_relin (evaluator.rs:422-438), relinearize (441-458), mul (463-467). */
+ckh_status ckh_mul_no_relin(ckh_evaluator *ev, const ckh_ct *a, const ckh_ct *b, ckh_ct **out);
+ckh_status ckh_relinearize(ckh_evaluator *ev, const ckh_ct *ct, ckh_ct **out);
+ckh_status ckh_mul(ckh_evaluator *ev, const ckh_ct *a, const ckh_ct *b, ckh_ct **out);
+/* HeOps::rescale (evaluator.rs:498-509), mod_switch_to_level (513-525), relabel (ops.rs:300-306). */
+ckh_status ckh_rescale(ckh_evaluator *ev, const ckh_ct *ct, ckh_ct **out);
+ckh_status ckh_mod_switch_to_level(ckh_evaluator *ev, const ckh_ct *ct, uint32_t level,
+                                   ckh_ct **out);
+ckh_status ckh_relabel(ckh_evaluator *ev, const ckh_ct *ct, double scale, ckh_ct **out);
+/* HeOps::rotate (evaluator.rs:534-540), conjugate (547-550), hoisted_rotate (572-611). */
+ckh_status ckh_rotate(ckh_evaluator *ev, const ckh_ct *ct, int32_t steps, ckh_ct **out);
+ckh_status ckh_conjugate(ckh_evaluator *ev, const ckh_ct *ct, ckh_ct **out);
+ckh_status ckh_hoisted_rotate(ckh_evaluator *ev, const ckh_ct *ct, const int32_t *steps,
+                              uint32_t n_steps, ckh_ct **outs);
+/* n relinearizations in batched launches (bit-identical to n ckh_relinearize calls). */
+ckh_status ckh_relinearize_batch(ckh_evaluator *ev, uint32_t n, const ckh_ct *const *in,
+                                 ckh_ct **out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CKH_H */

@@ -1,0 +1,69 @@
+"""Shared parameter sets and seeded inputs for the device parity tests: the device library and
+the CPU oracle get the same primes and the same seeded residues, and results are compared word
+for word."""
+import numpy as np
+
+import ckh
+import oracle as O
+
+# (name, log_n, q_bits, p_bits, dnum)
+SMALL = ("small_a2", 10, [60, 40, 40, 40, 40, 40], [60, 60], 3)     # alpha 2, two special primes
+ALPHA1 = ("alpha1", 10, [60, 40, 40, 40], [60], 4)                  # one-prime digits, k = 1
+C2_SHAPE_SMALL_N = ("c2_shape_n1024", 10, [50] * 24, [60] * 6, 4)   # C2 prime shape at N = 2^10
+C4_SHAPE_SMALL_N = ("c4_shape_n1024", 10, [50] * 20, [60] * 4, 5)   # C4 prime shape at N = 2^10
+C2 = ("c2", 16, [50] * 24, [60] * 6, 4)                             # the headline key switch
+C4 = ("c4", 15, [50] * 20, [60] * 4, 5)
+
+
+def make(case, n_streams=1):
+    name, log_n, qb, pb, dnum = case
+    q, p = ckh.params_primes(log_n, qb, pb)
+    ctx = ckh.Context(log_n, q, p, dnum, n_streams=n_streams)
+    ref = O.Params(log_n, q=q, p=p, dnum=dnum)
+    assert ref.q == q and ref.p == p
+    return ctx, ref
+
+
+def levels_for(nq):
+    """Levels spread over top / top-1 / 3L/4 / L/2 / L/3 / 2 / 1 / 0 (clipped digits included)."""
+    L = nq - 1
+    return sorted({L, max(L - 1, 0), 3 * L // 4, L // 2, L // 3, min(2, L), min(1, L), 0}, reverse=True)
+
+
+def host_uniform(ref, level, seed, stream0=0, p_limbs=0):
+    primes = ref.q[: level + 1] + (ref.p if p_limbs else [])
+    return O.uniform_poly(primes, ref.n, seed, stream0)
+
+
+def near_tie_digits(ref, level):
+    """Coefficient-form limbs where every digit D puts [x + h]_D next to 0 or Q_D (the f64
+    estimate of the exact conversion is then decided by its last bits), transformed to NTT form
+    by the oracle."""
+    n = ref.n
+    limbs = np.zeros((level + 1, n), dtype=np.uint64)
+    for s, e in ref.digit_ranges(level):
+        D = 1
+        for q in ref.q[s:e]:
+            D *= q
+        for i in range(s, e):
+            r = ref.q[i]
+            for c in range(n):
+                m = c // 2 + 1
+                sv = m % r if c % 2 == 0 else (r - m % r) % r
+                limbs[i, c] = (sv - (r >> 1)) % r
+    return O.ntt_forward(n, ref.q[: level + 1], limbs.ravel()).reshape(level + 1, n)
+
+
+def near_tie_p_part(ref):
+    """P limbs (coefficient form) with [X + h]_P next to 0 or P in every coefficient."""
+    n = ref.n
+    P = 1
+    for p in ref.p:
+        P *= p
+    limbs = np.zeros((len(ref.p), n), dtype=np.uint64)
+    for t, r in enumerate(ref.p):
+        for c in range(n):
+            m = c // 2 + 1
+            sv = m % r if c % 2 == 0 else (r - m % r) % r
+            limbs[t, c] = (sv - (r >> 1)) % r
+    return O.ntt_forward(n, ref.p, limbs.ravel()).reshape(len(ref.p), n)

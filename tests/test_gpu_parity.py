@@ -1,0 +1,320 @@
+"""Device parity: every HIP stage against the CPU oracle on identical seeded inputs, bit for
+bit, through the C ABI (mirrors the reference's GPU case suite,
+crates/ckks-train-cuda/tests/cases/{mod,ks}.rs: seeded uniform residues, all-(q-1) extremes,
+levels spread so clipped digits occur, near-tie inputs for the f64 overflow estimate)."""
+import sys
+
+import numpy as np
+import pytest
+
+import ckh
+import oracle as O
+from gpu_cases import (ALPHA1, C2, C2_SHAPE_SMALL_N, C4_SHAPE_SMALL_N, SMALL, host_uniform, levels_for, make,
+                       near_tie_digits, near_tie_p_part)
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", params=[SMALL, ALPHA1, C2_SHAPE_SMALL_N, C4_SHAPE_SMALL_N], ids=lambda c: c[0])
+def env(request):
+    say("case", request.param[0])
+    ctx, ref = make(request.param)
+    yield ctx, ref
+    ctx.synchronize()
+
+
+# ---------------------------------------------------------------- NTT
+
+@pytest.mark.parametrize("log_n,bits,limbs", [(1, 20, 1), (4, 30, 2), (10, 60, 3), (12, 50, 2), (13, 50, 4),
+                                              (14, 60, 2), (15, 50, 3), (16, 50, 3), (16, 60, 2), (17, 60, 1)])
+def test_ntt_matches_oracle(log_n, bits, limbs):
+    say("ntt", log_n, bits, limbs)
+    n = 1 << log_n
+    q = O.ntt_primes(bits, n, limbs)
+    p = O.ntt_primes(bits, n, limbs + 1)[limbs:]
+    ctx = ckh.Context(log_n, q, p, 1)
+    a = O.uniform_poly(q, n, 7 + log_n)
+    d = ctx.upload(a, limbs - 1, ntt=False)
+    f = d.ntt()
+    want = O.ntt_forward(n, q, a.ravel()).reshape(limbs, n)
+    assert (f.download() == want).all()
+    assert (f.intt().download() == a).all()
+    # extremes: all q - 1 and zero
+    mx = np.stack([np.full(n, qi - 1, dtype=np.uint64) for qi in q])
+    fm = ctx.upload(mx, limbs - 1, ntt=False).ntt()
+    assert (fm.download() == O.ntt_forward(n, q, mx.ravel()).reshape(limbs, n)).all()
+    assert (fm.intt().download() == mx).all()
+    z = ctx.upload(np.zeros((limbs, n), dtype=np.uint64), limbs - 1, ntt=False).ntt()
+    assert (z.download() == 0).all()
+
+
+def test_c1_plain_ntt_round_trip():
+    """N = 2^13, the 4 largest 50-bit primes = 1 mod 2N, forward then inverse, bit-exact."""
+    log_n, n = 13, 1 << 13
+    q = O.ntt_primes(50, n, 4)
+    p = O.ntt_primes(60, n, 1)
+    ctx = ckh.Context(log_n, q, p, 1)
+    x = ctx.uniform(3, seed=2024, ntt=False)
+    a = O.uniform_poly(q, n, 2024)
+    assert (x.download() == a).all()
+    f = x.ntt()
+    assert (f.download() == O.ntt_forward(n, q, a.ravel()).reshape(4, n)).all()
+    assert (f.intt().download() == a).all()
+
+
+# ---------------------------------------------------------------- pointwise stages
+
+def test_uniform_generator_matches_oracle(env):
+    ctx, ref = env
+    level = len(ref.q) - 1
+    x = ctx.uniform(level, seed=99, stream0=5, p_limbs=ref.k)
+    assert (x.download() == host_uniform(ref, level, 99, 5, p_limbs=1)).all()
+
+
+def test_automorphism(env):
+    ctx, ref = env
+    level = len(ref.q) - 1
+    a = host_uniform(ref, level, 3)
+    d = ctx.upload(a, level)
+    for g in [ckh.galois_elt(ref.log_n, 1), ckh.galois_elt(ref.log_n, -3), ckh.conjugation_elt(ref.log_n)]:
+        got = d.automorphism(g).download()
+        assert (got == O.automorphism_ntt(ref.n, g, a.ravel()).reshape(a.shape)).all()
+
+
+def test_rescale_poly(env):
+    ctx, ref = env
+    for level in levels_for(len(ref.q)):
+        if level == 0:
+            with pytest.raises(ckh.NoLevelsLeft):
+                ctx.upload(host_uniform(ref, 0, 1), 0).rescale_by_last()
+            continue
+        a = host_uniform(ref, level, 11 + level)
+        got = ctx.upload(a, level).rescale_by_last().download()
+        assert (got == ref.rescale_by_last(a.ravel(), level)).all(), level
+
+
+def test_mod_down_random_and_near_ties(env):
+    ctx, ref = env
+    for level in levels_for(len(ref.q)):
+        qpart = host_uniform(ref, level, 21 + level)
+        for ppart in (O.uniform_poly(ref.p, ref.n, 31 + level), near_tie_p_part(ref)):
+            qp = np.concatenate([qpart, ppart])
+            got = ctx.upload(qp, level, p_limbs=ref.k).mod_down().download()
+            want = ref.mod_down(qpart.ravel(), level, ppart.ravel())
+            assert (got == want).all(), level
+
+
+# ---------------------------------------------------------------- key switching
+
+def test_decompose_random_extreme_near_tie(env):
+    ctx, ref = env
+    for level in levels_for(len(ref.q)):
+        inputs = [host_uniform(ref, level, 41 + level),
+                  np.stack([np.full(ref.n, q - 1, dtype=np.uint64) for q in ref.q[: level + 1]])]
+        if level >= 1:
+            inputs.append(near_tie_digits(ref, level))
+        for a in inputs:
+            digs = ctx.decompose(ctx.upload(a, level))
+            want = ref.decompose(a.ravel(), level)
+            assert len(digs) == len(want)
+            for j, d in enumerate(digs):
+                assert (d.download() == want[j]).all(), (level, j)
+
+
+def test_key_switch_levels_and_capped_key(env):
+    ctx, ref = env
+    L = len(ref.q) - 1
+    key = ctx.ksk_uniform(L, seed=5)
+    hkey = ref.uniform_key(L, 5)
+    for level in levels_for(len(ref.q)):
+        a = host_uniform(ref, level, 51 + level)
+        kb, ka = ctx.key_switch(ctx.upload(a, level), key)
+        wb, wa = ref.key_switch(a.ravel(), level, hkey, L)
+        assert (kb.download() == wb).all() and (ka.download() == wa).all(), level
+    # a key uploaded for levels <= L // 2 switches those levels identically
+    cap = L // 2
+    ckey = ctx.ksk_uniform(cap, seed=6)
+    hcap = ref.uniform_key(cap, 6)
+    a = host_uniform(ref, cap, 77)
+    kb, ka = ctx.key_switch(ctx.upload(a, cap), ckey)
+    wb, wa = ref.key_switch(a.ravel(), cap, hcap, cap)
+    assert (kb.download() == wb).all() and (ka.download() == wa).all()
+    with pytest.raises(ckh.InvalidParams):
+        ctx.key_switch(ctx.upload(host_uniform(ref, cap + 1, 78), cap + 1), ckey)
+
+
+def test_key_switch_worst_case_inputs(env):
+    ctx, ref = env
+    L = len(ref.q) - 1
+    parts, lp = ref.key_shape(L)
+    primes = ref.q + ref.p
+    hkey = np.stack([np.stack([np.full(ref.n, q - 1, dtype=np.uint64) for q in primes])
+                     for _ in range(2 * parts)])
+    key = ctx.ksk_upload(hkey, L)
+    a = np.stack([np.full(ref.n, q - 1, dtype=np.uint64) for q in ref.q])
+    kb, ka = ctx.key_switch(ctx.upload(a, L), key)
+    wb, wa = ref.key_switch(a.ravel(), L, hkey, L)
+    assert (kb.download() == wb).all() and (ka.download() == wa).all()
+
+
+def test_decompose_apply_equals_key_switch(env):
+    ctx, ref = env
+    L = len(ref.q) - 1
+    key = ctx.ksk_uniform(L, seed=8)
+    a = ctx.uniform(L, seed=61)
+    digs = ctx.decompose(a)
+    kb, ka = ctx.apply_key(digs, key)
+    kb2, ka2 = ctx.key_switch(a, key)
+    assert (kb.download() == kb2.download()).all() and (ka.download() == ka2.download()).all()
+
+
+def test_batch_equals_individual_switches(env):
+    ctx, ref = env
+    L = len(ref.q) - 1
+    key = ctx.ksk_uniform(L, seed=9)
+    rng = np.random.default_rng(3)
+    lv = [int(x) for x in rng.integers(0, L + 1, size=9)]
+    polys = [ctx.uniform(l, seed=100 + i) for i, l in enumerate(lv)]
+    batch = ctx.key_switch_batch(polys, key)
+    for p, (kb, ka) in zip(polys, batch):
+        sb, sa = ctx.key_switch(p, key)
+        assert (kb.download() == sb.download()).all() and (ka.download() == sa.download()).all()
+
+
+# ---------------------------------------------------------------- evaluator (HeOps)
+
+def _ct(ctx, ref, level, seed, ncomp=2):
+    polys = np.stack([host_uniform(ref, level, seed + i) for i in range(ncomp)])
+    return ctx.ct_upload(polys, level, 2.0 ** 40), polys
+
+
+def test_relinearize_rotate_conjugate_rescale(env):
+    ctx, ref = env
+    L = len(ref.q) - 1
+    n = ref.n
+    relin = ctx.ksk_uniform(L, seed=12)
+    steps = [1, 5, -2]
+    elts = [ckh.galois_elt(ref.log_n, s) for s in steps] + [ckh.conjugation_elt(ref.log_n)]
+    gkeys = [ctx.ksk_uniform(L, seed=20 + i, galois=g) for i, g in enumerate(elts)]
+    ev = ckh.Evaluator(ctx, relin, gkeys)
+    for level in (L, L // 2, 0):
+        ct3, h3 = _ct(ctx, ref, level, 300 + level, ncomp=3)
+        r = ev.relinearize(ct3)
+        w0, w1 = ref.relinearize(h3[0], h3[1], h3[2], level, ref.uniform_key(L, 12), L)
+        got = r.download()
+        assert (got[0] == w0).all() and (got[1] == w1).all(), level
+        assert r.scale == 2.0 ** 40 and r.level == level
+        ct2, h2 = _ct(ctx, ref, level, 400 + level)
+        for i, s in enumerate(steps):
+            rr = ev.rotate(ct2, s).download()
+            w0, w1 = ref.apply_galois(h2[0], h2[1], level, elts[i], ref.uniform_key(L, 20 + i), L)
+            assert (rr[0] == w0).all() and (rr[1] == w1).all(), (level, s)
+        cc = ev.conjugate(ct2).download()
+        w0, w1 = ref.apply_galois(h2[0], h2[1], level, elts[-1], ref.uniform_key(L, 23), L)
+        assert (cc[0] == w0).all() and (cc[1] == w1).all()
+        if level > 0:
+            rs = ev.rescale(ct2)
+            assert (rs.download() == ref.rescale_ct(h2.ravel(), level)).all()
+            assert rs.scale == 2.0 ** 40 / float(ref.q[level])
+    c = ev.counter()
+    assert c.relins == 3 and c.rotations == 12 and c.rescales == 2
+    assert c.key_switches() == 15
+
+
+def test_hoisted_rotate_matches_oracle_and_direct(env):
+    ctx, ref = env
+    L = len(ref.q) - 1
+    steps = [1, 2, 3, 7, -1, 0, ref.n // 2]
+    nontriv = [s for s in steps if s % (ref.n // 2)]
+    elts = {s: ckh.galois_elt(ref.log_n, s) for s in nontriv}
+    gkeys = [ctx.ksk_uniform(L, seed=50 + i, galois=elts[s]) for i, s in enumerate(nontriv)]
+    ev = ckh.Evaluator(ctx, None, gkeys)
+    for level in (L, L // 2):
+        ct, h = _ct(ctx, ref, level, 500 + level)
+        outs = ev.hoisted_rotate(ct, steps)
+        want = ref.hoisted_galois(h[0], h[1], level, [elts[s] for s in nontriv],
+                                  [ref.uniform_key(L, 50 + i) for i in range(len(nontriv))], L)
+        k = 0
+        for s, o in zip(steps, outs):
+            got = o.download()
+            if s in elts:
+                assert (got[0] == want[k, 0]).all() and (got[1] == want[k, 1]).all(), (level, s)
+                direct = ev.rotate(ct, s).download()
+                assert (got == direct).all()
+                k += 1
+            else:
+                assert (got == h).all()
+
+
+def test_mul_add_sub_negate(env):
+    ctx, ref = env
+    L = len(ref.q) - 1
+    relin = ctx.ksk_uniform(L, seed=13)
+    ev = ckh.Evaluator(ctx, relin)
+    level = max(L // 2, 1)
+    a, ha = _ct(ctx, ref, level, 600)
+    b, hb = _ct(ctx, ref, level, 610)
+    d = ev.mul_no_relin(a, b).download()
+    w = ref.mul_no_relin(ha[0], ha[1], hb[0], hb[1], level)
+    for i in range(3):
+        assert (d[i] == w[i]).all()
+    m = ev.mul(a, b).download()
+    w0, w1 = ref.relinearize(w[0], w[1], w[2], level, ref.uniform_key(L, 13), L)
+    assert (m[0] == w0).all() and (m[1] == w1).all()
+    s = ev.add(a, b).download()
+    df = ev.sub(a, b).download()
+    ng = ev.negate(a).download()
+    for i, q in enumerate(ref.q[: level + 1]):
+        for c in range(2):
+            x, y = ha[c, i].astype(object), hb[c, i].astype(object)
+            assert s[c, i].tolist() == list((x + y) % q)
+            assert df[c, i].tolist() == list((x - y) % q)
+            assert ng[c, i].tolist() == list((-x) % q)
+    assert ev.counter().ct_ct_tensor == 2 and ev.counter().relins == 1
+
+
+def test_evaluator_errors(env):
+    ctx, ref = env
+    L = len(ref.q) - 1
+    ev = ckh.Evaluator(ctx)
+    a, _ = _ct(ctx, ref, L, 700)
+    b, _ = _ct(ctx, ref, max(L - 1, 0), 710)
+    if L > 0:
+        with pytest.raises(ckh.LevelMismatch):
+            ev.add(a, b)
+    with pytest.raises(ckh.InvalidParams, match="relinearization key"):
+        ev.relinearize(_ct(ctx, ref, L, 720, ncomp=3)[0])
+    with pytest.raises(ckh.InvalidParams, match="missing galois key"):
+        ev.rotate(a, 1)
+    with pytest.raises(ckh.InvalidParams):
+        ev.mul_no_relin(_ct(ctx, ref, L, 730, ncomp=3)[0], a)
+    z, _ = _ct(ctx, ref, 0, 740)
+    with pytest.raises(ckh.NoLevelsLeft):
+        ev.rescale(z)
+    with pytest.raises(ckh.NoLevelsLeft):
+        ev.mul_no_relin(z, z)
+    c2 = ctx.ct_upload(np.stack([host_uniform(ref, L, 1), host_uniform(ref, L, 2)]), L, 2.0 ** 30)
+    with pytest.raises(ckh.ScaleMismatch):
+        ev.add(a, c2)
+    assert ev.rotate(a, ref.n // 2).download().tolist() == a.download().tolist()
+    assert ev.counter().key_switches() == 0
+
+
+# ---------------------------------------------------------------- full size (C2)
+
+def test_c2_full_size_key_switch():
+    """N = 2^16, 24 x 50-bit q, 6 x 60-bit p, dnum 4, level 23: bit-exact against the oracle."""
+    say("c2 full size")
+    ctx, ref = make(C2)
+    L = 23
+    key = ctx.ksk_uniform(L, seed=424242)
+    d = ctx.uniform(L, seed=4242)
+    kb, ka = ctx.key_switch(d, key)
+    O.set_threads(16)
+    wb, wa = ref.key_switch(host_uniform(ref, L, 4242).ravel(), L, ref.uniform_key(L, 424242), L)
+    assert (kb.download() == wb).all() and (ka.download() == wa).all()
