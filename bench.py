@@ -1,0 +1,235 @@
+"""Benchmark: bit-exact hybrid key switches per second at N = 2^16, batched vs plain.
+
+Workload (one "step"): 256 ciphertext polynomials at N = 2^16 with limb counts drawn
+uniformly from 4..24 (seeded), 24 x 50-bit q primes, 6 x 60-bit special primes, dnum 4, all
+switched under one shared key in batched launches (ckh_key_switch_batch). Inputs and the key
+are generated on the device before timing (resident in HBM).
+
+Also reported: the plain path (the same 256 switches, one ciphertext per launch sequence),
+the p50/p99 latency of one key switch at the full level (24 limbs), the HIP-event timing of
+the dominant kernel against the HBM roofline, and the CPU oracle on a bounded sample of the
+same workload. The batched results are checked word for word against the plain path, and one
+switch against the CPU oracle.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for sub in ("ckks-train_amd", "oracle", "tests"):
+    sys.path.insert(0, os.path.join(ROOT, sub))
+
+LOG_N = 16
+N = 1 << LOG_N
+Q_BITS = [50] * 24
+P_BITS = [60] * 6
+DNUM = 4
+N_CT = 256
+SEED = 20261202
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_levels():
+    rng = np.random.default_rng(SEED)
+    limbs = rng.integers(4, 25, size=N_CT)  # limb counts 4..24
+    return [int(x) - 1 for x in limbs]
+
+
+def ks_bytes(level, k=6, alpha=6):
+    """Algorithmic HBM bytes of one key switch: read d, read the key parts of its digits,
+    write kb and ka (8N [L + 2 d_L (L + k) + 2L])."""
+    L = level + 1
+    d = -(-L // alpha)
+    return 8 * N * (L + 2 * d * (L + k) + 2 * L)
+
+
+class HipEvents:
+    def __init__(self):
+        self.h = C.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.h.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.h.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.h.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.h.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+
+def cpu_baseline(q, p, levels, budget_s=12.0):
+    """The CPU oracle (C restatement of the ckks-train CPU path, OpenMP over digits and limbs as
+    the reference's rayon feature) on the first ciphertexts of the same workload until the time
+    budget is spent (at least two switches)."""
+    import oracle as O
+
+    threads = min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    ref = O.Params(LOG_N, q=q, p=p, dnum=DNUM)
+    key = ref.uniform_key(23, SEED + 1)
+    done, t0 = 0, time.perf_counter()
+    for i, lv in enumerate(levels):
+        d = O.uniform_poly(q[: lv + 1], N, SEED + 100 + i)
+        ref.key_switch(d.ravel(), lv, key, 23)
+        done += 1
+        if done >= 2 and time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    return {"value": done / el, "unit": "key_switches/s", "cores": threads, "kind": "port",
+            "sample": f"first {done} ciphertexts of the workload (levels {levels[:done]}), oracle "
+                      f"key_switch with {threads} OpenMP threads, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: the key-switch backend shards by ciphertext with no "
+              "collectives, and multi-GPU runs are not part of this build (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import ckh
+    import oracle as O
+
+    q, p = ckh.params_primes(LOG_N, Q_BITS, P_BITS)
+    ctx = ckh.Context(LOG_N, q, p, DNUM)
+    stream = ctx.stream_handle()
+    ev = HipEvents()
+    levels = workload_levels()
+    key = ctx.ksk_uniform(23, seed=SEED + 1)
+    polys = [ctx.uniform(lv, seed=SEED + 100 + i) for i, lv in enumerate(levels)]
+    ctx.synchronize()
+    print(f"# workload: {N_CT} polys, limbs 4..24 (mean {np.mean(levels) + 1:.1f}), key {key.info()[2] / 2**20:.0f} MiB",
+          file=sys.stderr, flush=True)
+
+    # ---- batched (the measured step)
+    for _ in range(args.warmup):
+        out = ctx.key_switch_batch(polys, key)
+        del out
+    ctx.synchronize()
+    e0, e1 = ev.event(), ev.event()
+    t0 = time.perf_counter()
+    ev.record(e0, stream)
+    for _ in range(args.steps):
+        out = ctx.key_switch_batch(polys, key)
+    ev.record(e1, stream)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    batched_ms = ev.elapsed_ms(e0, e1)
+    ms_step = batched_ms / args.steps
+    batched = N_CT / (ms_step * 1e-3)
+    print(f"# batched: {ms_step:.2f} ms/step ({wall / args.steps * 1e3:.2f} ms wall)", file=sys.stderr, flush=True)
+
+    # ---- plain: one ciphertext per launch sequence, same switches
+    for i in range(8):
+        ctx.key_switch(polys[i], key)
+    ctx.synchronize()
+    ev.record(e0, stream)
+    plain_out = [ctx.key_switch(pl, key) for pl in polys]
+    ev.record(e1, stream)
+    ctx.synchronize()
+    plain_ms = ev.elapsed_ms(e0, e1)
+    plain = N_CT / (plain_ms * 1e-3)
+    print(f"# plain: {plain_ms:.2f} ms for {N_CT}", file=sys.stderr, flush=True)
+
+    # ---- bit-exactness: batched == plain on every 8th ciphertext; plain == oracle on one
+    exact = True
+    for i in range(0, N_CT, 8):
+        for a, b in zip(out[i], plain_out[i]):
+            exact &= bool((a.download() == b.download()).all())
+    i0 = int(np.argmin(levels))
+    ref = O.Params(LOG_N, q=q, p=p, dnum=DNUM)
+    O.set_threads(min(16, os.cpu_count() or 1))
+    wb, wa = ref.key_switch(O.uniform_poly(q[: levels[i0] + 1], N, SEED + 100 + i0).ravel(), levels[i0],
+                            ref.uniform_key(23, SEED + 1), 23)
+    exact &= bool((plain_out[i0][0].download() == wb).all() and (plain_out[i0][1].download() == wa).all())
+    del plain_out, out
+
+    # ---- latency of one plain key switch at 24 limbs
+    top = ctx.uniform(23, seed=SEED + 7)
+    lat = []
+    for i in range(40):
+        ctx.synchronize()
+        ev.record(e0, stream)
+        r = ctx.key_switch(top, key)
+        ev.record(e1, stream)
+        lat.append(ev.elapsed_ms(e0, e1))
+        del r
+    lat = np.array(lat[5:])
+
+    # ---- dominant kernel: the forward NTT over the 96 converted limbs of one C2 ModUp
+    # (24 limbs per digit x 4 digits, 50- and 60-bit primes in the C2 proportion is
+    # approximated by a 72 x 50-bit + 24 x 60-bit batch), timed with HIP events.
+    big_q = ctx.uniform(23, seed=SEED + 9, p_limbs=6, ntt=False)
+    reps = 20
+    x = big_q.ntt()
+    ctx.synchronize()
+    ev.record(e0, stream)
+    for _ in range(reps):
+        x = big_q.ntt()
+    ev.record(e1, stream)
+    ntt_ms = ev.elapsed_ms(e0, e1) / reps
+    limbs = 24 + 6
+    ntt_bytes = 16 * N * limbs
+    achieved = ntt_bytes / (ntt_ms * 1e-3) / 1e9
+
+    cpu = None if args.no_cpu else cpu_baseline(q, p, levels)
+
+    mean_bytes = float(np.mean([ks_bytes(lv) for lv in levels]))
+    line = {
+        "metric": "bit-exact key switches/sec at N=2^16 (batched vs plain) + p50/p99 op latency",
+        "value": batched,
+        "unit": "key_switches/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched uneven-level hybrid key switch: 256 polynomials at N=2^16, limbs U{4..24} "
+                        "(seeded), 24x50-bit q + 6x60-bit p, dnum 4, one shared key",
+            "n": N, "ciphertexts_per_step": N_CT, "q_bits": 50, "nq": 24, "p_bits": 60, "np": 6, "dnum": DNUM,
+            "seed": SEED, "parallelism": "single GPU",
+        },
+        "plain_key_switches_per_s": plain,
+        "batched_over_plain": batched / plain,
+        "latency_ms_24_limbs": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                                "mean": float(lat.mean())},
+        "bit_exact": exact,
+        "effective_ks_GBps": batched * mean_bytes / 1e9,
+        "roofline": {
+            "kernel": "forward NTT (column pass + row pass) over 30 limbs at N=2^16, 24x50-bit + 6x60-bit",
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "ms_per_launch": ntt_ms, "bytes_per_launch": ntt_bytes,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -24,6 +24,20 @@ struct LimbRef {
     u32 aux;
 };
 
+// A twiddle and its Shoup companion, loaded as one 16-byte word.
+struct Tw {
+    u64 w, ws;
+};
+
+// A twiddle for the f64 path (primes below 2^50): w exactly, and w / q rounded.
+struct FTw {
+    double w, wq;
+};
+
+// Primes below this bound use the f64 arithmetic (every intermediate stays an exact integer
+// below 2^53); larger primes use 64-bit integer Shoup arithmetic.
+constexpr u64 FP_PRIME_BOUND = 1ull << 50;
+
 // Per-prime constants (device).
 struct PrimeConst {
     u64 q;
@@ -31,11 +45,8 @@ struct PrimeConst {
     u64 w_n_inv, w_n_inv_s; // psi^-1 * N^-1 (fused last inverse stage)
     u64 one_s;              // floor(2^64 / q): Shoup companion of 1
     u64 two64, two64_s;     // 2^64 mod q and companion
-};
-
-// A twiddle and its Shoup companion, loaded as one 16-byte word.
-struct Tw {
-    u64 w, ws;
+    double qf, qinv;        // q and 1/q as f64 (f64 path)
+    FTw n_inv_f, w_n_inv_f; // the last inverse stage constants, f64 path
 };
 
 #define CKH_DEV __device__ __forceinline__

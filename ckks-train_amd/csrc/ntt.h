@@ -9,9 +9,16 @@ namespace ckh {
 struct NttTables {
     u32 log_n;
     const PrimeConst* pc;
-    const Tw* fwd;  // [prime][N]
-    const Tw* inv;  // [prime][N]
+    const Tw* fwd;     // [prime][N]
+    const Tw* inv;     // [prime][N]
+    const FTw* fwd_f;  // [prime][N], f64 path (primes below 2^50)
+    const FTw* inv_f;
 };
+
+// Register-radix passes (ntt_fast.hip) for 2^12 <= N <= 2^16.
+bool ntt_fast_supported(u32 log_n);
+void ntt_fast_forward(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream);
+void ntt_fast_inverse(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream);
 
 // ref.dst <- NTT(ref.src) for each of the nrefs descriptors (device array). src may equal
 // dst (in place). Inputs reduced, outputs reduced.

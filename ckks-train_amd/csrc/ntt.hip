@@ -229,6 +229,7 @@ __global__ __launch_bounds__(NTHREADS) void k_ntt_inv_cols(const LimbRef* __rest
 
 void ntt_forward(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream) {
     if (nrefs == 0) return;
+    if (ntt_fast_supported(t.log_n)) return ntt_fast_forward(t, refs, nrefs, stream);
     u32 A, B;
     split_log(t.log_n, A, B);
     if (A > 0) {
@@ -243,6 +244,7 @@ void ntt_forward(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t
 
 void ntt_inverse(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream) {
     if (nrefs == 0) return;
+    if (ntt_fast_supported(t.log_n)) return ntt_fast_inverse(t, refs, nrefs, stream);
     u32 A, B;
     split_log(t.log_n, A, B);
     if (A > 0) {

@@ -1,0 +1,409 @@
+// Register-radix NTT passes for 2^12 <= N <= 2^16 (gfx950).
+//
+// Same transform as ntt.hip (crates/ckks-train-core/src/ntt.rs:141-232; outputs are unique
+// residues), scheduled for CDNA: a limb of N = 2^(A+B) words is 2^A rows x 2^B columns; the
+// column pass runs the first A forward stages, the row pass the last B. A pass works on a tile
+// of 4096 words with 256 threads holding 16 words each in registers: four stages run in
+// registers (radix 16), one LDS exchange regroups the words, and the remaining stages of the
+// pass run in registers again, so a pass has one barrier for its exchange instead of one per
+// stage. The row pass stages its output through LDS so every global store is coalesced; the
+// inverse row pass stages its input likewise.
+//
+// Arithmetic, chosen per limb (uniform per workgroup):
+//  * q < 2^50 (FpAr): residues as f64 integers. Twiddle product t = y w mod q via
+//    h = RN(y w), l = fma(y, w, -h) (exact: y w = h + l), k = rint(y RN(w/q)),
+//    t = fma(-k, q, h) + l. With |y| <= 2^51 the quotient error is below 1, |t| <= q, and
+//    h - k q, t are integers below 2^53, so every step is exact. Forward: x is reduced to
+//    |x| <= q/2 by x - rint(x/q) q before x +- t, keeping every value within 1.5 q < 2^51.
+//    Inverse: |x|, |y| <= q throughout. Between the two passes a limb holds these signed
+//    values as f64 bits; the last stage writes canonical residues.
+//  * q >= 2^50 (IntAr): Harvey butterflies with 64-bit Shoup products, values in [0, 4q)
+//    (forward) or [0, 2q) (inverse), as in ntt.hip.
+#include "common.h"
+#include "ntt.h"
+
+namespace ckh {
+namespace fast {
+
+constexpr u32 TL = 12;
+constexpr u32 TILE = 1u << TL;
+constexpr u32 NT = 256;
+constexpr u32 LDS_WORDS = TILE + TILE / 16;
+
+// Padded LDS index: one spare word per 16 spreads strided accesses over the banks.
+CKH_DEV u32 pad(u32 a) { return a + (a >> 4); }
+
+// ---------------------------------------------------------------- f64 arithmetic (q < 2^50)
+
+// u64 < 2^53 -> f64, exact.
+CKH_DEV double u2d(u64 v) { return __builtin_fma((double)(u32)(v >> 32), 4294967296.0, (double)(u32)v); }
+
+// Non-negative integer-valued f64 below 2^52 -> u64, exact.
+CKH_DEV u64 d2u(double d) {
+    const u32 hi = (u32)(d * (1.0 / 4294967296.0));
+    const double lo = __builtin_fma(-(double)hi, 4294967296.0, d);
+    return ((u64)hi << 32) | (u32)lo;
+}
+
+CKH_DEV double fmulmod(double y, double w, double wq, double q) {
+    const double h = y * w;
+    const double l = __builtin_fma(y, w, -h);
+    const double k = __builtin_rint(y * wq);
+    return __builtin_fma(-k, q, h) + l;
+}
+
+CKH_DEV double freduce(double x, double q, double qinv) { return __builtin_fma(-__builtin_rint(x * qinv), q, x); }
+
+CKH_DEV double fcanon(double x, double q, double qinv) {
+    const double r = freduce(x, q, qinv);
+    return r < 0.0 ? r + q : r;
+}
+
+struct FpAr {
+    typedef double V;
+    const FTw* tab;
+    double q, qinv;
+    FTw ninv, wninv;
+    CKH_DEV V from_canon(u64 x) const { return u2d(x); }
+    CKH_DEV V from_mid(u64 x) const { return __longlong_as_double((long long)x); }
+    CKH_DEV u64 to_mid(V v) const { return (u64)__double_as_longlong(v); }
+    CKH_DEV void ct(V& x, V& y, u32 idx) const {
+        const FTw w = tab[idx];
+        const double xr = freduce(x, q, qinv);
+        const double t = fmulmod(y, w.w, w.wq, q);
+        x = xr + t;
+        y = xr - t;
+    }
+    CKH_DEV u64 fwd_final(V v) const { return d2u(fcanon(v, q, qinv)); }
+    CKH_DEV void gs(V& x, V& y, u32 idx) const {
+        const FTw w = tab[idx];
+        const double s = x + y, d = x - y;
+        x = freduce(s, q, qinv);
+        y = fmulmod(d, w.w, w.wq, q);
+    }
+    CKH_DEV void gs_final(V& x, V& y) const {
+        const double s = x + y, d = x - y;
+        x = fcanon(fmulmod(s, ninv.w, ninv.wq, q), q, qinv);
+        y = fcanon(fmulmod(d, wninv.w, wninv.wq, q), q, qinv);
+    }
+    CKH_DEV u64 canon_out(V v) const { return d2u(v); }
+};
+
+// ---------------------------------------------------------------- integer arithmetic
+
+struct IntAr {
+    typedef u64 V;
+    const Tw* tab;
+    u64 q;
+    u64 n_inv, n_inv_s, w_n_inv, w_n_inv_s;
+    CKH_DEV V from_canon(u64 x) const { return x; }
+    CKH_DEV V from_mid(u64 x) const { return x; }
+    CKH_DEV u64 to_mid(V v) const { return v; }
+    CKH_DEV void ct(V& x, V& y, u32 idx) const {
+        const Tw w = tab[idx];
+        const u64 two_q = 2 * q;
+        const u64 xr = x >= two_q ? x - two_q : x;
+        const u64 t = shoup_lazy(y, w.w, w.ws, q);
+        x = xr + t;
+        y = xr + two_q - t;
+    }
+    CKH_DEV u64 fwd_final(V v) const {
+        v = v >= 2 * q ? v - 2 * q : v;
+        return v >= q ? v - q : v;
+    }
+    CKH_DEV void gs(V& x, V& y, u32 idx) const {
+        const Tw w = tab[idx];
+        const u64 two_q = 2 * q;
+        const u64 s = x + y, d = x + two_q - y;
+        x = s >= two_q ? s - two_q : s;
+        y = shoup_lazy(d, w.w, w.ws, q);
+    }
+    CKH_DEV void gs_final(V& x, V& y) const {
+        const u64 s = x + y, d = x + 2 * q - y;
+        x = csub(shoup_lazy(s, n_inv, n_inv_s, q), q);
+        y = csub(shoup_lazy(d, w_n_inv, w_n_inv_s, q), q);
+    }
+    CKH_DEV u64 canon_out(V v) const { return v; }
+};
+
+// ---------------------------------------------------------------- passes
+//
+// Thread layout (M = log2 of the sub-transform, 6..8; R2 = M - 4 bits handled in round 2;
+// U = 4 - R2). "k-layout": a thread holds sub-transform indices o + (k << R2), k < 16 (the
+// four top bits vary). "e-layout": it holds ((hh << U | u) << R2) | j, e = (u << R2) | j (the
+// R2 low bits vary, plus U of the top bits).
+
+template <u32 M, class Ar>
+CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out, u64* lds, u32 log_n) {
+    constexpr u32 R2 = M - 4, U = 4 - R2, TCL = TL - M, TC = 1u << TCL, JM = (1u << R2) - 1;
+    const u32 B = log_n - M;
+    const u32 tid = threadIdx.x, cc = tid & (TC - 1), o = tid >> TCL;
+    const u32 c0 = blockIdx.x * TC;
+    typename Ar::V v[16];
+#pragma unroll
+    for (u32 k = 0; k < 16; ++k) v[k] = ar.from_canon(in[((u64)(o + (k << R2)) << B) + c0 + cc]);
+#pragma unroll
+    for (u32 s = 0; s < 4; ++s) {
+#pragma unroll
+        for (u32 k = 0; k < 16; ++k) {
+            const u32 d = 1u << (3 - s);
+            if (k & d) continue;
+            ar.ct(v[k], v[k + d], (1u << s) + (k >> (4 - s)));
+        }
+    }
+#pragma unroll
+    for (u32 k = 0; k < 16; ++k) lds[pad(((o + (k << R2)) << TCL) + cc)] = ar.to_mid(v[k]);
+    __syncthreads();
+#pragma unroll
+    for (u32 e = 0; e < 16; ++e) {
+        const u32 r = ((((o << U) | (e >> R2)) << R2) | (e & JM));
+        v[e] = ar.from_mid(lds[pad((r << TCL) + cc)]);
+    }
+#pragma unroll
+    for (u32 s = 4; s < M; ++s) {
+        const u32 p = M - 1 - s;
+#pragma unroll
+        for (u32 e = 0; e < 16; ++e) {
+            if (e & (1u << p)) continue;
+            const u32 r = ((((o << U) | (e >> R2)) << R2) | (e & JM));
+            ar.ct(v[e], v[e + (1u << p)], (1u << s) + (r >> (M - s)));
+        }
+    }
+#pragma unroll
+    for (u32 e = 0; e < 16; ++e) {
+        const u32 r = ((((o << U) | (e >> R2)) << R2) | (e & JM));
+        out[((u64)r << B) + c0 + cc] = ar.to_mid(v[e]);
+    }
+}
+
+template <u32 M, class Ar>
+CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out, u64* lds, u32 log_n) {
+    constexpr u32 R2 = M - 4, U = 4 - R2, TRL = TL - M, JM = (1u << R2) - 1;
+    const u32 A = log_n - M;
+    const u32 tid = threadIdx.x, o = tid & JM, rr = tid >> R2;
+    const u32 row = (blockIdx.x << TRL) + rr;
+    const u64 base = (u64)row << M;
+    typename Ar::V v[16];
+#pragma unroll
+    for (u32 k = 0; k < 16; ++k) v[k] = ar.from_mid(in[base + o + (k << R2)]);
+#pragma unroll
+    for (u32 s = 0; s < 4; ++s) {
+#pragma unroll
+        for (u32 k = 0; k < 16; ++k) {
+            const u32 d = 1u << (3 - s);
+            if (k & d) continue;
+            ar.ct(v[k], v[k + d], (1u << (A + s)) + (row << s) + (k >> (4 - s)));
+        }
+    }
+#pragma unroll
+    for (u32 k = 0; k < 16; ++k) lds[pad((rr << M) + o + (k << R2))] = ar.to_mid(v[k]);
+    __syncthreads();
+#pragma unroll
+    for (u32 e = 0; e < 16; ++e) {
+        const u32 c = ((((o << U) | (e >> R2)) << R2) | (e & JM));
+        v[e] = ar.from_mid(lds[pad((rr << M) + c)]);
+    }
+#pragma unroll
+    for (u32 s = 4; s < M; ++s) {
+        const u32 p = M - 1 - s;
+#pragma unroll
+        for (u32 e = 0; e < 16; ++e) {
+            if (e & (1u << p)) continue;
+            const u32 c = ((((o << U) | (e >> R2)) << R2) | (e & JM));
+            ar.ct(v[e], v[e + (1u << p)], (1u << (A + s)) + (row << s) + (c >> (M - s)));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (u32 e = 0; e < 16; ++e) {
+        const u32 c = ((((o << U) | (e >> R2)) << R2) | (e & JM));
+        lds[pad((rr << M) + c)] = ar.fwd_final(v[e]);
+    }
+    __syncthreads();
+    const u64 tbase = (u64)blockIdx.x << TL;
+#pragma unroll
+    for (u32 k = 0; k < 16; ++k) {
+        const u32 idx = tid + k * NT;
+        out[tbase + idx] = lds[pad(idx)];
+    }
+}
+
+template <u32 M, class Ar>
+CKH_DEV void inv_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out, u64* lds, u32 log_n) {
+    constexpr u32 R2 = M - 4, U = 4 - R2, TRL = TL - M, JM = (1u << R2) - 1;
+    const u32 tid = threadIdx.x, o = tid & JM, rr = tid >> R2;
+    const u32 row = (blockIdx.x << TRL) + rr;
+    const u64 tbase = (u64)blockIdx.x << TL;
+#pragma unroll
+    for (u32 k = 0; k < 16; ++k) {
+        const u32 idx = tid + k * NT;
+        lds[pad(idx)] = in[tbase + idx];
+    }
+    __syncthreads();
+    typename Ar::V v[16];
+#pragma unroll
+    for (u32 e = 0; e < 16; ++e) {
+        const u32 c = ((((o << U) | (e >> R2)) << R2) | (e & JM));
+        v[e] = ar.from_canon(lds[pad((rr << M) + c)]);
+    }
+#pragma unroll
+    for (u32 u = 0; u < R2; ++u) {
+#pragma unroll
+        for (u32 e = 0; e < 16; ++e) {
+            if (e & (1u << u)) continue;
+            const u32 c = ((((o << U) | (e >> R2)) << R2) | (e & JM));
+            ar.gs(v[e], v[e + (1u << u)], (1u << (log_n - 1 - u)) + (row << (M - 1 - u)) + (c >> (u + 1)));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (u32 e = 0; e < 16; ++e) {
+        const u32 c = ((((o << U) | (e >> R2)) << R2) | (e & JM));
+        lds[pad((rr << M) + c)] = ar.to_mid(v[e]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (u32 k = 0; k < 16; ++k) v[k] = ar.from_mid(lds[pad((rr << M) + o + (k << R2))]);
+#pragma unroll
+    for (u32 u = R2; u < M; ++u) {
+        const u32 d = 1u << (u - R2);
+#pragma unroll
+        for (u32 k = 0; k < 16; ++k) {
+            if (k & d) continue;
+            const u32 c = o + (k << R2);
+            ar.gs(v[k], v[k + d], (1u << (log_n - 1 - u)) + (row << (M - 1 - u)) + (c >> (u + 1)));
+        }
+    }
+    const u64 base = (u64)row << M;
+#pragma unroll
+    for (u32 k = 0; k < 16; ++k) out[base + o + (k << R2)] = ar.to_mid(v[k]);
+}
+
+template <u32 M, class Ar>
+CKH_DEV void inv_cols(const Ar& ar, u64* __restrict__ io, u64* lds, u32 log_n) {
+    constexpr u32 R2 = M - 4, U = 4 - R2, TCL = TL - M, TC = 1u << TCL, JM = (1u << R2) - 1;
+    const u32 B = log_n - M;
+    const u32 tid = threadIdx.x, cc = tid & (TC - 1), o = tid >> TCL;
+    const u32 c0 = blockIdx.x * TC;
+    typename Ar::V v[16];
+#pragma unroll
+    for (u32 e = 0; e < 16; ++e) {
+        const u32 r = ((((o << U) | (e >> R2)) << R2) | (e & JM));
+        v[e] = ar.from_mid(io[((u64)r << B) + c0 + cc]);
+    }
+#pragma unroll
+    for (u32 u = 0; u < R2; ++u) {
+#pragma unroll
+        for (u32 e = 0; e < 16; ++e) {
+            if (e & (1u << u)) continue;
+            const u32 r = ((((o << U) | (e >> R2)) << R2) | (e & JM));
+            ar.gs(v[e], v[e + (1u << u)], (1u << (log_n - 1 - B - u)) + (r >> (u + 1)));
+        }
+    }
+#pragma unroll
+    for (u32 e = 0; e < 16; ++e) {
+        const u32 r = ((((o << U) | (e >> R2)) << R2) | (e & JM));
+        lds[pad((r << TCL) + cc)] = ar.to_mid(v[e]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (u32 k = 0; k < 16; ++k) v[k] = ar.from_mid(lds[pad(((o + (k << R2)) << TCL) + cc)]);
+#pragma unroll
+    for (u32 u = R2; u < M; ++u) {
+        const u32 d = 1u << (u - R2);
+#pragma unroll
+        for (u32 k = 0; k < 16; ++k) {
+            if (k & d) continue;
+            if (u == M - 1) {
+                ar.gs_final(v[k], v[k + d]);
+            } else {
+                const u32 r = o + (k << R2);
+                ar.gs(v[k], v[k + d], (1u << (log_n - 1 - B - u)) + (r >> (u + 1)));
+            }
+        }
+    }
+#pragma unroll
+    for (u32 k = 0; k < 16; ++k) io[((u64)(o + (k << R2)) << B) + c0 + cc] = ar.canon_out(v[k]);
+}
+
+// ---------------------------------------------------------------- kernels
+
+#define CKH_AR_DISPATCH(CALL)                                                                   \
+    const LimbRef ref = refs[blockIdx.y];                                                       \
+    const PrimeConst* pc = pcs + ref.prime;                                                     \
+    const u64 toff = (u64)ref.prime << log_n;                                                   \
+    if (pc->q < FP_PRIME_BOUND) {                                                               \
+        const FpAr ar{twf + toff, pc->qf, pc->qinv, pc->n_inv_f, pc->w_n_inv_f};                \
+        CALL;                                                                                   \
+    } else {                                                                                    \
+        const IntAr ar{tw + toff, pc->q, pc->n_inv, pc->n_inv_s, pc->w_n_inv, pc->w_n_inv_s};   \
+        CALL;                                                                                   \
+    }
+
+template <u32 M>
+__global__ __launch_bounds__(NT) void k_fwd_cols(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
+                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n) {
+    __shared__ u64 lds[LDS_WORDS];
+    CKH_AR_DISPATCH((fwd_cols<M>(ar, ref.src, ref.dst, lds, log_n)))
+}
+
+template <u32 M>
+__global__ __launch_bounds__(NT) void k_fwd_rows(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
+                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n) {
+    __shared__ u64 lds[LDS_WORDS];
+    CKH_AR_DISPATCH((fwd_rows<M>(ar, ref.dst, ref.dst, lds, log_n)))
+}
+
+template <u32 M>
+__global__ __launch_bounds__(NT) void k_inv_rows(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
+                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n) {
+    __shared__ u64 lds[LDS_WORDS];
+    CKH_AR_DISPATCH((inv_rows<M>(ar, ref.src, ref.dst, lds, log_n)))
+}
+
+template <u32 M>
+__global__ __launch_bounds__(NT) void k_inv_cols(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
+                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n) {
+    __shared__ u64 lds[LDS_WORDS];
+    CKH_AR_DISPATCH((inv_cols<M>(ar, ref.dst, lds, log_n)))
+}
+
+}  // namespace fast
+
+bool ntt_fast_supported(u32 log_n) { return log_n >= 12 && log_n <= 16; }
+
+template <u32 M>
+static void launch_one(bool fwd, bool cols, const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s) {
+    dim3 grid(1u << (t.log_n - fast::TL), nrefs);
+    if (fwd && cols)
+        hipLaunchKernelGGL(fast::k_fwd_cols<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.log_n);
+    else if (fwd)
+        hipLaunchKernelGGL(fast::k_fwd_rows<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.log_n);
+    else if (cols)
+        hipLaunchKernelGGL(fast::k_inv_cols<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.log_n);
+    else
+        hipLaunchKernelGGL(fast::k_inv_rows<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.log_n);
+}
+
+static void launch_m(u32 m, bool fwd, bool cols, const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s) {
+    switch (m) {
+        case 6: launch_one<6>(fwd, cols, t, refs, nrefs, s); break;
+        case 7: launch_one<7>(fwd, cols, t, refs, nrefs, s); break;
+        default: launch_one<8>(fwd, cols, t, refs, nrefs, s); break;
+    }
+}
+
+// Split N = 2^(A+B): B = floor(log N / 2) row stages, A column stages (6..8 each).
+void ntt_fast_forward(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s) {
+    const u32 B = t.log_n / 2, A = t.log_n - B;
+    launch_m(A, true, true, t, refs, nrefs, s);
+    launch_m(B, true, false, t, refs, nrefs, s);
+}
+
+void ntt_fast_inverse(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s) {
+    const u32 B = t.log_n / 2, A = t.log_n - B;
+    launch_m(B, false, false, t, refs, nrefs, s);
+    launch_m(A, false, true, t, refs, nrefs, s);
+}
+
+}  // namespace ckh

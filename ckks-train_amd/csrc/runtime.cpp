@@ -154,6 +154,7 @@ void Context::build_tables() {
     const u32 np_all = nq + np;
     std::vector<PrimeConst> pc(np_all);
     std::vector<Tw> fwd((size_t)np_all * n), inv((size_t)np_all * n);
+    std::vector<FTw> fwd_f((size_t)np_all * n), inv_f((size_t)np_all * n);
     for (u32 k = 0; k < np_all; ++k) {
         const u64 q = primes[k];
         const u64 psi = hm::min_root(n, q), psi_inv = hm::inv(psi, q), n_inv = hm::inv(n % q, q);
@@ -162,6 +163,8 @@ void Context::build_tables() {
             const u32 j = hm::bitrev((u32)i, log_n);
             fwd[(size_t)k * n + j] = {p1, hm::shoup(p1, q)};
             inv[(size_t)k * n + j] = {p2, hm::shoup(p2, q)};
+            fwd_f[(size_t)k * n + j] = {(double)p1, (double)p1 / (double)q};
+            inv_f[(size_t)k * n + j] = {(double)p2, (double)p2 / (double)q};
             p1 = hm::mul(p1, psi, q);
             p2 = hm::mul(p2, psi_inv, q);
         }
@@ -175,12 +178,18 @@ void Context::build_tables() {
         c.one_s = hm::shoup(1, q);
         c.two64 = (u64)(((hm::u128)1 << 64) % q);
         c.two64_s = hm::shoup(c.two64, q);
+        c.qf = (double)q;
+        c.qinv = 1.0 / (double)q;
+        c.n_inv_f = {(double)n_inv, (double)n_inv / (double)q};
+        c.w_n_inv_f = {(double)c.w_n_inv, (double)c.w_n_inv / (double)q};
     }
     d_pc = dev_copy(pc);
     ntt.log_n = log_n;
     ntt.pc = d_pc;
     ntt.fwd = dev_copy(fwd);
     ntt.inv = dev_copy(inv);
+    ntt.fwd_f = dev_copy(fwd_f);
+    ntt.inv_f = dev_copy(inv_f);
 
     std::vector<ConvSrc> srcs;
     std::vector<ConvTgt> tgts;

@@ -11,6 +11,8 @@ SMALL = ("small_a2", 10, [60, 40, 40, 40, 40, 40], [60, 60], 3)     # alpha 2, t
 ALPHA1 = ("alpha1", 10, [60, 40, 40, 40], [60], 4)                  # one-prime digits, k = 1
 C2_SHAPE_SMALL_N = ("c2_shape_n1024", 10, [50] * 24, [60] * 6, 4)   # C2 prime shape at N = 2^10
 C4_SHAPE_SMALL_N = ("c4_shape_n1024", 10, [50] * 20, [60] * 4, 5)   # C4 prime shape at N = 2^10
+C2_SHAPE_N4096 = ("c2_shape_n4096", 12, [50] * 24, [60] * 6, 4)   # register-radix NTT path
+SMALL_N8192 = ("small_a3_n8192", 13, [50] * 6, [60] * 2, 2)        # alpha 3, odd log N split
 C2 = ("c2", 16, [50] * 24, [60] * 6, 4)                             # the headline key switch
 C4 = ("c4", 15, [50] * 20, [60] * 4, 5)
 

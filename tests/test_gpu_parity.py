@@ -9,8 +9,8 @@ import pytest
 
 import ckh
 import oracle as O
-from gpu_cases import (ALPHA1, C2, C2_SHAPE_SMALL_N, C4_SHAPE_SMALL_N, SMALL, host_uniform, levels_for, make,
-                       near_tie_digits, near_tie_p_part)
+from gpu_cases import (ALPHA1, C2, C2_SHAPE_N4096, C2_SHAPE_SMALL_N, C4_SHAPE_SMALL_N, SMALL, SMALL_N8192,
+                       host_uniform, levels_for, make, near_tie_digits, near_tie_p_part)
 
 pytestmark = pytest.mark.gpu
 
@@ -20,7 +20,8 @@ def say(*a):
     sys.stdout.flush()
 
 
-@pytest.fixture(scope="module", params=[SMALL, ALPHA1, C2_SHAPE_SMALL_N, C4_SHAPE_SMALL_N], ids=lambda c: c[0])
+@pytest.fixture(scope="module", params=[SMALL, ALPHA1, C2_SHAPE_SMALL_N, C4_SHAPE_SMALL_N, C2_SHAPE_N4096, SMALL_N8192],
+                ids=lambda c: c[0])
 def env(request):
     say("case", request.param[0])
     ctx, ref = make(request.param)
