@@ -87,6 +87,7 @@ _SIGS = {
     "ckh_intt": (C.c_int, [VP, VP, C.POINTER(VP)]),
     "ckh_automorphism": (C.c_int, [VP, VP, u64, C.POINTER(VP)]),
     "ckh_rescale_poly": (C.c_int, [VP, VP, C.POINTER(VP)]),
+    "ckh_time_ntt": (C.c_int, [VP, C.POINTER(VP), u32, C.c_int, u32, C.POINTER(C.c_double)]),
     "ckh_mod_down": (C.c_int, [VP, VP, C.POINTER(VP)]),
     "ckh_ksk_upload": (C.c_int, [VP, P64, u32, u64, C.POINTER(VP)]),
     "ckh_ksk_uniform": (C.c_int, [VP, u32, u64, u64, C.POINTER(VP)]),
@@ -262,6 +263,14 @@ class Context:
         kb, ka = (VP * n)(), (VP * n)()
         _check(lib().ckh_key_switch_batch(self.h, n, arr, key.h, kb, ka))
         return [(Poly(self, VP(kb[i])), Poly(self, VP(ka[i]))) for i in range(n)]
+
+    def time_ntt(self, polys, forward=True, reps=20):
+        """Mean ms per batched in-place NTT launch over every limb of `polys` (HIP events on
+        this context's stream). Overwrites the polynomials."""
+        arr = (VP * len(polys))(*[p.h for p in polys])
+        ms = C.c_double()
+        _check(lib().ckh_time_ntt(self.h, arr, len(polys), int(forward), reps, C.byref(ms)))
+        return ms.value
 
     def ct_upload(self, polys, level, scale):
         a = np.ascontiguousarray(np.asarray(polys, dtype=np.uint64))

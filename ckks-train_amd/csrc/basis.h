@@ -12,7 +12,9 @@ struct ConvSrc {
     u64 h;               // q_j >> 1: [x + (Q-1)/2]_Q has residue x + h mod q_j
     u64 qhinv, qhinv_s;  // (Q / q_j)^-1 mod q_j
     double inv;          // 1.0 / f64(q_j), round to nearest (host computed)
-    u64 pad;
+    double qf;           // q_j as f64 (f64 path)
+    double qhinv_f, qhinv_q;  // (Q / q_j)^-1 and its quotient by q_j, f64 path
+    u32 fp, pad;              // q_j < 2^50
 };
 
 // One target prime r of a conversion.
@@ -21,7 +23,10 @@ struct ConvTgt {
     u32 limb;   // output limb index within the job's output polynomial
     u64 qmod, qmod_s;  // Q mod r
     u64 hmod;          // (Q - 1)/2 mod r
-    u64 qhat[MAX_SRC], qhat_s[MAX_SRC];  // (Q / q_j) mod r
+    u64 qhat[MAX_SRC];                   // (Q / q_j) mod r
+    double qhat_f[MAX_SRC], qhat_q[MAX_SRC];  // f64 path: (Q / q_j) mod r and its quotient by r
+    double qmod_f, qmod_q, hmod_f, rf, rinv;
+    u32 fp, pad;  // r and every source below 2^50: f64 multiply-accumulate
 };
 
 // A conversion: sources [src_first, src_first + nsrc) of the ConvSrc table (their input limbs

@@ -23,6 +23,23 @@ CKH_DEV double u64_to_f64_rn(u64 v) {
     return hi + lo;
 }
 
+// f64 helpers for primes below 2^50 (exactness argument in ntt_fast.hip): u2d/d2u convert
+// exactly, fmulmod returns y w mod r as an integer in [-0.75 r, 0.75 r] for 0 <= y < 2^50.
+CKH_DEV double cv_u2d(u64 v) { return __builtin_fma((double)(u32)(v >> 32), 4294967296.0, (double)(u32)v); }
+
+CKH_DEV u64 cv_d2u(double d) {
+    const u32 hi = (u32)(d * (1.0 / 4294967296.0));
+    const double lo = __builtin_fma(-(double)hi, 4294967296.0, d);
+    return ((u64)hi << 32) | (u32)lo;
+}
+
+CKH_DEV double cv_fmulmod(double y, double w, double wq, double r) {
+    const double h = y * w;
+    const double l = __builtin_fma(y, w, -h);
+    const double k = __builtin_rint(y * wq);
+    return __builtin_fma(-k, r, h) + l;
+}
+
 template <u32 MAXS>
 __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __restrict__ jobs, u32 tg) {
     const u32 n = 1u << T.log_n;
@@ -34,16 +51,27 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
     const u32 t1 = min(t0 + tg, P.ntgt);
     const u64* inp = job.in;
     u64 y[MAXS];
+    double yd[MAXS];
     double f = 0.0;
 #pragma unroll
     for (u32 j = 0; j < MAXS; ++j) {
         y[j] = 0;
+        yd[j] = 0.0;
         if (j < P.nsrc) {
-            const ConvSrc s = T.src[P.src_first + j];
+            const ConvSrc* s = T.src + P.src_first + j;
             const u64 x = inp[(u64)(P.src_limb0 + j) * n + c];
-            y[j] = shoup(add_mod(x, s.h, s.q), s.qhinv, s.qhinv_s, s.q);
+            if (s->fp) {
+                double sv = cv_u2d(x) + (double)s->h;
+                sv = sv >= s->qf ? sv - s->qf : sv;
+                double v = cv_fmulmod(sv, s->qhinv_f, s->qhinv_q, s->qf);
+                v = v < 0.0 ? v + s->qf : v;
+                yd[j] = v;
+                y[j] = cv_d2u(v);
+            } else {
+                y[j] = shoup(add_mod(x, s->h, s->q), s->qhinv, s->qhinv_s, s->q);
+            }
             if (P.exact) {
-                const double prod = u64_to_f64_rn(y[j]) * s.inv;
+                const double prod = u64_to_f64_rn(y[j]) * s->inv;
                 f = f + prod;
             }
         }
@@ -52,13 +80,31 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
     u64* outp = job.out;
     for (u32 t = t0; t < t1; ++t) {
         const ConvTgt* g = T.tgt + P.tgt_first + t;
-        const u64 r = T.pc[g->prime].q;
-        u64 o = 0;
+        u64 o;
+        if (g->fp) {
+            const double r = g->rf, rinv = g->rinv;
+            double acc = 0.0;
 #pragma unroll
-        for (u32 j = 0; j < MAXS; ++j)
-            if (j < P.nsrc) o = add_mod(o, shoup(y[j], g->qhat[j], g->qhat_s[j], r), r);
-        if (P.exact) o = sub_mod(o, shoup(e, g->qmod, g->qmod_s, r), r);
-        o = sub_mod(o, g->hmod, r);
+            for (u32 j = 0; j < MAXS; ++j) {
+                if (j < P.nsrc) {
+                    acc = acc + cv_fmulmod(yd[j], g->qhat_f[j], g->qhat_q[j], r);
+                    if ((j & 3) == 3) acc = __builtin_fma(-__builtin_rint(acc * rinv), r, acc);
+                }
+            }
+            if (P.exact) acc = acc - cv_fmulmod((double)e, g->qmod_f, g->qmod_q, r);
+            acc = acc - g->hmod_f;
+            acc = __builtin_fma(-__builtin_rint(acc * rinv), r, acc);
+            o = cv_d2u(acc < 0.0 ? acc + r : acc);
+        } else {
+            const PrimeConst C = T.pc[g->prime];
+            Acc128 acc = {0, 0};
+#pragma unroll
+            for (u32 j = 0; j < MAXS; ++j)
+                if (j < P.nsrc) mac(acc, y[j], g->qhat[j]);
+            o = reduce128(acc.hi, acc.lo, C);
+            if (P.exact) o = sub_mod(o, shoup(e, g->qmod, g->qmod_s, C.q), C.q);
+            o = sub_mod(o, g->hmod, C.q);
+        }
         outp[(u64)g->limb * n + c] = o;
     }
 }

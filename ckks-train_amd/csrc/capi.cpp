@@ -236,6 +236,17 @@ ckh_status ckh_automorphism(ckh_context* ctx, const ckh_poly* in, uint64_t galoi
     });
 }
 
+ckh_status ckh_time_ntt(ckh_context* ctx, ckh_poly* const* polys, uint32_t n, int forward, uint32_t reps,
+                        double* ms_per_launch) {
+    return guarded([&] {
+        need(polys, "polys");
+        need(ms_per_launch, "ms_per_launch");
+        std::vector<const Poly*> v;
+        for (uint32_t i = 0; i < n; ++i) v.push_back(&poly_of(polys[i]));
+        *ms_per_launch = time_ntt(ctx_of(ctx), v, forward != 0, reps);
+    });
+}
+
 ckh_status ckh_rescale_poly(ckh_context* ctx, const ckh_poly* in, ckh_poly** out) {
     return guarded([&] {
         need(out, "out");

@@ -81,6 +81,32 @@ PolyPtr poly_automorphism(Context& c, const Poly& in, u64 g) {
     return out;
 }
 
+double time_ntt(Context& c, const std::vector<const Poly*>& polys, bool forward, u32 reps) {
+    std::vector<LimbRef> refs;
+    for (const Poly* p : polys)
+        for (u32 i = 0; i < p->limbs(); ++i) {
+            u32 prime = i <= p->level ? i : c.nq + (i - p->level - 1);
+            refs.push_back({p->data() + i * c.n, p->data() + i * c.n, prime, 0});
+        }
+    if (refs.empty() || reps == 0) fail(CKH_ERR_INVALID_PARAMS, "nothing to time");
+    hipStream_t s = c.stream();
+    BufPtr d = c.upload_blob(refs.data(), refs.size() * sizeof(LimbRef), s);
+    const LimbRef* dr = (const LimbRef*)d->ptr;
+    launch_ntt(c, dr, (u32)refs.size(), forward, s);  // warm up
+    hipEvent_t a, b;
+    hip_check(hipEventCreate(&a), "event");
+    hip_check(hipEventCreate(&b), "event");
+    hip_check(hipEventRecord(a, s), "event");
+    for (u32 r = 0; r < reps; ++r) launch_ntt(c, dr, (u32)refs.size(), forward, s);
+    hip_check(hipEventRecord(b, s), "event");
+    hip_check(hipEventSynchronize(b), "event");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, a, b), "event");
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return (double)ms / reps;
+}
+
 PolyPtr poly_rescale(Context& c, const Poly& in) {
     if (!in.ntt || in.p_limbs) fail(CKH_ERR_INVALID_PARAMS, "rescale needs an NTT-form Q polynomial");
     if (in.level == 0) fail(CKH_ERR_NO_LEVELS_LEFT, "cannot rescale at level 0");

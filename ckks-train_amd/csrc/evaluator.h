@@ -17,6 +17,7 @@ PolyPtr poly_uniform(Context& c, u32 level, u32 p_limbs, bool ntt, u64 seed, u64
 PolyPtr poly_transform(Context& c, const Poly& in, bool forward);
 PolyPtr poly_automorphism(Context& c, const Poly& in, u64 g);
 PolyPtr poly_rescale(Context& c, const Poly& in);
+double time_ntt(Context& c, const std::vector<const Poly*>& polys, bool forward, u32 reps);
 PolyPtr poly_mod_down(Context& c, const Poly& qp);
 PolyPtr poly_pointwise(Context& c, PwOp op, const Poly& a, const Poly* b);
 // keyswitch::decompose: one buffer holding all digits; returns one QpPoly handle per digit.

@@ -215,6 +215,10 @@ void Context::build_tables() {
             cs.qhinv = hm::inv(qh, q);
             cs.qhinv_s = hm::shoup(cs.qhinv, q);
             cs.inv = 1.0 / (double)q;
+            cs.fp = q < FP_PRIME_BOUND;
+            cs.qf = (double)q;
+            cs.qhinv_f = (double)cs.qhinv;
+            cs.qhinv_q = (double)cs.qhinv / (double)q;
             srcs.push_back(cs);
         }
         if (share_tgt_first) {
@@ -229,12 +233,21 @@ void Context::build_tables() {
                 t.qmod = hm::prod_mod(sq.data(), sq.size(), r);
                 t.qmod_s = hm::shoup(t.qmod, r);
                 t.hmod = hm::half_mod(sq.data(), sq.size(), r);
+                bool fp = r < FP_PRIME_BOUND;
+                for (u64 sqj : sq) fp = fp && sqj < FP_PRIME_BOUND;
+                t.fp = fp;
+                t.rf = (double)r;
+                t.rinv = 1.0 / (double)r;
+                t.qmod_f = (double)t.qmod;
+                t.qmod_q = (double)t.qmod / (double)r;
+                t.hmod_f = (double)t.hmod;
                 for (size_t j = 0; j < src.size(); ++j) {
                     u64 qh = 1 % r;
                     for (size_t i = 0; i < src.size(); ++i)
                         if (i != j) qh = hm::mul(qh, sq[i] % r, r);
                     t.qhat[j] = qh;
-                    t.qhat_s[j] = hm::shoup(qh, r);
+                    t.qhat_f[j] = (double)qh;
+                    t.qhat_q[j] = (double)qh / (double)r;
                 }
                 tgts.push_back(t);
             }
