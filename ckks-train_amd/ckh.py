@@ -116,6 +116,8 @@ _SIGS = {
     "ckh_add": (C.c_int, [VP, VP, VP, C.POINTER(VP)]),
     "ckh_sub": (C.c_int, [VP, VP, VP, C.POINTER(VP)]),
     "ckh_negate": (C.c_int, [VP, VP, C.POINTER(VP)]),
+    "ckh_add_plain": (C.c_int, [VP, VP, VP, C.c_double, C.POINTER(VP)]),
+    "ckh_mul_plain": (C.c_int, [VP, VP, VP, C.c_double, C.POINTER(VP)]),
     "ckh_mul_no_relin": (C.c_int, [VP, VP, VP, C.POINTER(VP)]),
     "ckh_relinearize": (C.c_int, [VP, VP, C.POINTER(VP)]),
     "ckh_mul": (C.c_int, [VP, VP, VP, C.POINTER(VP)]),
@@ -126,6 +128,7 @@ _SIGS = {
     "ckh_conjugate": (C.c_int, [VP, VP, C.POINTER(VP)]),
     "ckh_hoisted_rotate": (C.c_int, [VP, VP, C.POINTER(C.c_int32), u32, C.POINTER(VP)]),
     "ckh_relinearize_batch": (C.c_int, [VP, u32, C.POINTER(VP), C.POINTER(VP)]),
+    "ckh_matvec_plain": (C.c_int, [VP, VP, C.POINTER(VP), C.c_double, u32, u32, u32, C.POINTER(VP)]),
 }
 
 _lib = None
@@ -447,6 +450,12 @@ class Evaluator:
     def negate(self, a):
         return self._op(lib().ckh_negate, a.h)
 
+    def add_plain(self, a, pt, pt_scale):
+        return self._op(lib().ckh_add_plain, a.h, pt.h, pt_scale)
+
+    def mul_plain(self, a, pt, pt_scale):
+        return self._op(lib().ckh_mul_plain, a.h, pt.h, pt_scale)
+
     def mul_no_relin(self, a, b):
         return self._op(lib().ckh_mul_no_relin, a.h, b.h)
 
@@ -477,6 +486,11 @@ class Evaluator:
         outs = (VP * max(n, 1))()
         _check(lib().ckh_hoisted_rotate(self.h, ct.h, st, n, outs))
         return [Ct(self.ctx, VP(outs[i])) for i in range(n)]
+
+    def matvec_plain(self, x, entries, pt_scale, baby, giant, stride=1):
+        """BSGS product; entries[k1 * baby + k2] is a Poly or None (zero entry)."""
+        arr = (VP * len(entries))(*[e.h if e is not None else None for e in entries])
+        return self._op(lib().ckh_matvec_plain, x.h, arr, pt_scale, baby, giant, stride)
 
     def relinearize_batch(self, cts):
         n = len(cts)

@@ -488,6 +488,12 @@ ckh_status ckh_sub(ckh_evaluator* ev, const ckh_ct* a, const ckh_ct* b, ckh_ct**
     CT_OP(ev_of(ev).add(ct_of(a), ct_of(b), true));
 }
 ckh_status ckh_negate(ckh_evaluator* ev, const ckh_ct* a, ckh_ct** out) { CT_OP(ev_of(ev).negate(ct_of(a))); }
+ckh_status ckh_add_plain(ckh_evaluator* ev, const ckh_ct* a, const ckh_poly* pt, double pt_scale, ckh_ct** out) {
+    CT_OP(ev_of(ev).add_plain(ct_of(a), poly_of(pt), pt_scale));
+}
+ckh_status ckh_mul_plain(ckh_evaluator* ev, const ckh_ct* a, const ckh_poly* pt, double pt_scale, ckh_ct** out) {
+    CT_OP(ev_of(ev).mul_plain(ct_of(a), poly_of(pt), pt_scale));
+}
 ckh_status ckh_mul_no_relin(ckh_evaluator* ev, const ckh_ct* a, const ckh_ct* b, ckh_ct** out) {
     CT_OP(ev_of(ev).mul_no_relin(ct_of(a), ct_of(b)));
 }
@@ -518,6 +524,17 @@ ckh_status ckh_hoisted_rotate(ckh_evaluator* ev, const ckh_ct* ct, const int32_t
         if (n_steps) need(steps, "steps");
         auto r = ev_of(ev).hoisted_rotate(ct_of(ct), std::vector<int32_t>(steps, steps + n_steps));
         for (uint32_t i = 0; i < n_steps; ++i) outs[i] = wrap(r[i]);
+    });
+}
+
+ckh_status ckh_matvec_plain(ckh_evaluator* ev, const ckh_ct* x, const ckh_poly* const* entries, double pt_scale,
+                            uint32_t baby, uint32_t giant, uint32_t stride, ckh_ct** out) {
+    return guarded([&] {
+        need(out, "out");
+        need(entries, "entries");
+        std::vector<const Poly*> v;
+        for (uint64_t i = 0; i < (uint64_t)baby * giant; ++i) v.push_back(entries[i] ? &poly_of(entries[i]) : nullptr);
+        *out = wrap(ev_of(ev).matvec_plain(ct_of(x), v, pt_scale, baby, giant, stride));
     });
 }
 

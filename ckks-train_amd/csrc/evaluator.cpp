@@ -427,6 +427,34 @@ CtPtr Evaluator::negate(const Ct& a) {
     return make_ct(std::move(out), a.scale);
 }
 
+static void check_plain(const Context& c, const Poly& pt) {
+    if (pt.buf->ctx != &c || pt.p_limbs || !pt.ntt)
+        fail(CKH_ERR_INVALID_PARAMS, "operand is not over this evaluator's data basis");
+}
+
+// Evaluator::add_plain (evaluator.rs:287-296): c0 + p, scales within tolerance.
+CtPtr Evaluator::add_plain(const Ct& a, const Poly& pt, double pt_scale) {
+    check_basis(a);
+    check_plain(ctx, pt);
+    check_level(a.level(), pt.level);
+    check_scale(a.scale, pt_scale);
+    std::vector<PolyPtr> out = a.comps;
+    out[0] = poly_pointwise(ctx, PW_ADD, *a.comps[0], &pt);
+    return make_ct(std::move(out), a.scale);
+}
+
+// Evaluator::mul_plain (evaluator.rs:298-308): every component times p, scale product.
+CtPtr Evaluator::mul_plain(const Ct& a, const Poly& pt, double pt_scale) {
+    check_basis(a);
+    check_plain(ctx, pt);
+    check_level(a.level(), pt.level);
+    std::vector<PolyPtr> out;
+    for (auto& c : a.comps) out.push_back(poly_pointwise(ctx, PW_MUL, *c, &pt));
+    CtPtr r = make_ct(std::move(out), a.scale * pt_scale);
+    count(&ckh_op_counter::ct_pt_mults);
+    return r;
+}
+
 void Evaluator::check_mul_operands(const Ct& a, const Ct& b) const {
     check_basis(a);
     check_basis(b);
@@ -616,6 +644,104 @@ std::vector<CtPtr> Evaluator::hoisted_rotate(const Ct& ct, const std::vector<int
     hip_check(hipGetLastError(), "hoisted rotate");
     count(&ckh_op_counter::rotations, nontrivial);
     return res;
+}
+
+std::vector<CtPtr> Evaluator::rotate_batch(const std::vector<std::pair<const Ct*, int32_t>>& items) {
+    std::vector<CtPtr> res(items.size());
+    std::vector<std::tuple<size_t, u64, const Ksk*>> todo;
+    for (size_t i = 0; i < items.size(); ++i) {
+        const Ct& ct = *items[i].first;
+        check_basis(ct);
+        check_galois_input(ct);
+        const u64 e = rotation_elt(items[i].second);
+        if (e == 0)
+            res[i] = make_ct(ct.comps, ct.scale);
+        else
+            todo.push_back({i, e, &galois_key(e, ct.level())});
+    }
+    if (todo.empty()) return res;
+    u64 total = 0;
+    for (auto& [i, e, k] : todo) total += 4 * (u64)(items[i].first->level() + 1) * ctx.n;
+    BufPtr buf = ctx.alloc(total);  // per item: sigma(c1), sigma(c0), kb + sigma(c0), ka
+    std::vector<KsReq> reqs;
+    std::vector<Poly> dpolys;
+    dpolys.reserve(todo.size());
+    u64 off = 0;
+    hipStream_t s = ctx.stream();
+    for (auto& [i, e, k] : todo) {
+        const Ct& ct = *items[i].first;
+        const u32 level = ct.level();
+        const u64 Lw = (u64)(level + 1) * ctx.n;
+        u64* d = buf->ptr + off;
+        automorphism_ntt_launch(ct.comps[1]->data(), d, level + 1, e, ctx.log_n, s);
+        automorphism_ntt_launch(ct.comps[0]->data(), d + Lw, level + 1, e, ctx.log_n, s);
+        dpolys.push_back(Poly{buf, off, level, 0, true});
+        reqs.push_back({&dpolys.back(), k, d + Lw, nullptr, d + 2 * Lw, d + 3 * Lw});
+        res[i] = make_ct({view(buf, off + 2 * Lw, level, 0, true), view(buf, off + 3 * Lw, level, 0, true)}, ct.scale);
+        off += 4 * Lw;
+    }
+    run_ks(ctx, reqs);
+    count(&ckh_op_counter::rotations, todo.size());
+    return res;
+}
+
+// Evaluator::bsgs for one plaintext-diagonal term (crates/ckks-train-ckks/src/linalg.rs:800-888):
+// every check before work, hoisted baby rotations for the baby steps a non-zero entry uses,
+// per giant block the sum of mul_plain(rot(x, k2 s), entry) in k2 order, the nonzero giant
+// shifts rotated (in one batch), blocks summed in k1 order.
+CtPtr Evaluator::matvec_plain(const Ct& x, const std::vector<const Poly*>& entries, double pt_scale, u32 baby,
+                              u32 giant, u32 stride) {
+    check_basis(x);
+    check_galois_input(x);
+    if (baby == 0 || giant == 0 || entries.size() != (size_t)baby * giant)
+        fail(CKH_ERR_INVALID_PARAMS, "entries must hold baby * giant diagonals");
+    const u64 slots = ctx.n / 2;
+    if (stride == 0 || ((u64)baby * giant * stride) > slots || slots % ((u64)baby * giant * stride) != 0)
+        fail(CKH_ERR_INVALID_PARAMS, "d * stride must divide the slot count");
+    std::vector<bool> used(baby, false);
+    bool any = false;
+    for (u32 k1 = 0; k1 < giant; ++k1)
+        for (u32 k2 = 0; k2 < baby; ++k2) {
+            const Poly* w = entries[k1 * baby + k2];
+            if (!w) continue;
+            check_plain(ctx, *w);
+            check_level(x.level(), w->level);
+            used[k2] = any = true;
+        }
+    if (!any) fail(CKH_ERR_INVALID_PARAMS, "every diagonal entry is zero");
+    std::vector<int32_t> steps;
+    std::vector<u32> used_k2;
+    for (u32 k2 = 0; k2 < baby; ++k2)
+        if (used[k2]) {
+            steps.push_back((int32_t)(k2 * stride));
+            used_k2.push_back(k2);
+        }
+    for (int32_t st : steps)
+        if (rotation_elt(st)) galois_key(rotation_elt(st), x.level());
+    for (u32 k1 = 1; k1 < giant; ++k1)
+        if (rotation_elt((int32_t)(baby * k1 * stride))) galois_key(rotation_elt((int32_t)(baby * k1 * stride)), x.level());
+    std::vector<CtPtr> rot = hoisted_rotate(x, steps);
+    std::vector<CtPtr> xs(baby);
+    for (size_t i = 0; i < used_k2.size(); ++i) xs[used_k2[i]] = rot[i];
+    std::vector<CtPtr> blocks(giant);
+    for (u32 k1 = 0; k1 < giant; ++k1)
+        for (u32 k2 = 0; k2 < baby; ++k2) {
+            const Poly* w = entries[k1 * baby + k2];
+            if (!w) continue;
+            CtPtr t = mul_plain(*xs[k2], *w, pt_scale);
+            blocks[k1] = blocks[k1] ? add(*blocks[k1], *t, false) : t;
+        }
+    std::vector<std::pair<const Ct*, int32_t>> giant_rot;
+    std::vector<u32> idx;
+    for (u32 k1 = 0; k1 < giant; ++k1)
+        if (blocks[k1]) {
+            giant_rot.push_back({blocks[k1].get(), (int32_t)(baby * k1 * stride)});
+            idx.push_back(k1);
+        }
+    std::vector<CtPtr> rotated = rotate_batch(giant_rot);
+    CtPtr acc;
+    for (size_t i = 0; i < rotated.size(); ++i) acc = acc ? add(*acc, *rotated[i], false) : rotated[i];
+    return acc;
 }
 
 }  // namespace ckh

@@ -48,6 +48,8 @@ class Evaluator {
 
     CtPtr add(const Ct& a, const Ct& b, bool sub);
     CtPtr negate(const Ct& a);
+    CtPtr add_plain(const Ct& a, const Poly& pt, double pt_scale);
+    CtPtr mul_plain(const Ct& a, const Poly& pt, double pt_scale);
     CtPtr mul_no_relin(const Ct& a, const Ct& b);
     CtPtr relinearize(const Ct& ct);
     std::vector<CtPtr> relinearize_batch(const std::vector<const Ct*>& cts);
@@ -58,6 +60,10 @@ class Evaluator {
     CtPtr rotate(const Ct& ct, int32_t steps);
     CtPtr conjugate(const Ct& ct);
     std::vector<CtPtr> hoisted_rotate(const Ct& ct, const std::vector<int32_t>& steps);
+    // Rotations of several ciphertexts (their own steps) switched in one batch.
+    std::vector<CtPtr> rotate_batch(const std::vector<std::pair<const Ct*, int32_t>>& items);
+    CtPtr matvec_plain(const Ct& x, const std::vector<const Poly*>& entries, double pt_scale, u32 baby, u32 giant,
+                       u32 stride);
 
    private:
     std::mutex mu_;

@@ -184,6 +184,12 @@ int ckh_owns(ckh_evaluator *ev, const ckh_ct *ct);
 ckh_status ckh_add(ckh_evaluator *ev, const ckh_ct *a, const ckh_ct *b, ckh_ct **out);
 ckh_status ckh_sub(ckh_evaluator *ev, const ckh_ct *a, const ckh_ct *b, ckh_ct **out);
 ckh_status ckh_negate(ckh_evaluator *ev, const ckh_ct *a, ckh_ct **out);
+/* HeOps::add_plain / mul_plain (evaluator.rs:287-308): the plaintext is an NTT-form Q polynomial
+ * at the ciphertext's level with f64 scale pt_scale. */
+ckh_status ckh_add_plain(ckh_evaluator *ev, const ckh_ct *a, const ckh_poly *pt, double pt_scale,
+                         ckh_ct **out);
+ckh_status ckh_mul_plain(ckh_evaluator *ev, const ckh_ct *a, const ckh_poly *pt, double pt_scale,
+                         ckh_ct **out);
 /* HeOps::mul_no_relin (evaluator.rs:422-438), relinearize (441-458), mul (463-467). */
 ckh_status ckh_mul_no_relin(ckh_evaluator *ev, const ckh_ct *a, const ckh_ct *b, ckh_ct **out);
 ckh_status ckh_relinearize(ckh_evaluator *ev, const ckh_ct *ct, ckh_ct **out);
@@ -198,6 +204,14 @@ ckh_status ckh_rotate(ckh_evaluator *ev, const ckh_ct *ct, int32_t steps, ckh_ct
 ckh_status ckh_conjugate(ckh_evaluator *ev, const ckh_ct *ct, ckh_ct **out);
 ckh_status ckh_hoisted_rotate(ckh_evaluator *ev, const ckh_ct *ct, const int32_t *steps,
                               uint32_t n_steps, ckh_ct **outs);
+/* Evaluator::matvec_plain's BSGS product (crates/ckks-train-ckks/src/linalg.rs:608-621,800-888):
+ * entries[k1 * baby + k2] is the pre-rotated plaintext diagonal of giant block k1, baby step k2
+ * (NULL = a zero entry, skipped), all at x's level with scale pt_scale. The baby rotations
+ * k2 * stride share one decomposition; block k1 is rotated by baby * k1 * stride (the giant
+ * rotations of all blocks are switched in one batch). Returns the unrescaled sum. */
+ckh_status ckh_matvec_plain(ckh_evaluator *ev, const ckh_ct *x, const ckh_poly *const *entries,
+                            double pt_scale, uint32_t baby, uint32_t giant, uint32_t stride,
+                            ckh_ct **out);
 /* n relinearizations in batched launches (bit-identical to n ckh_relinearize calls). */
 ckh_status ckh_relinearize_batch(ckh_evaluator *ev, uint32_t n, const ckh_ct *const *in,
                                  ckh_ct **out);

@@ -314,6 +314,16 @@ class Params:
                               ptr(d[0]), ptr(d[1]), ptr(d[2]))
         return d
 
+    def poly_add(self, a, b, level):
+        out = arr(a).copy()
+        lib().or_poly_add(self.h, ptr(out), ptr(arr(b)), level + 1)
+        return out.reshape(level + 1, self.n)
+
+    def poly_mul(self, a, b, level):
+        out = arr(a).copy()
+        lib().or_poly_mul(self.h, ptr(out), ptr(arr(b)), level + 1)
+        return out.reshape(level + 1, self.n)
+
     def key_shape(self, key_max_level):
         """(parts, limbs per key polynomial) of a key serving levels 0..=key_max_level."""
         return len(self.digit_ranges(key_max_level)), key_max_level + 1 + self.k

@@ -69,3 +69,49 @@ def near_tie_p_part(ref):
             sv = m % r if c % 2 == 0 else (r - m % r) % r
             limbs[t, c] = (sv - (r >> 1)) % r
     return O.ntt_forward(n, ref.p, limbs.ravel()).reshape(len(ref.p), n)
+
+
+def oracle_matvec(ref, c0, c1, level, entries, baby, giant, stride, key_of_step, kml):
+    """The BSGS product composed from oracle operations, in the reference's order
+    (crates/ckks-train-ckks/src/linalg.rs:800-888): hoisted baby rotations of the used steps,
+    per giant block the sum over k2 of mul_plain(rot(x, k2 s), entry), the block rotated by
+    baby k1 s, blocks summed in k1 order. entries[k1 * baby + k2] is a host array or None;
+    key_of_step(step) returns the host key of that rotation step."""
+    slots = ref.n // 2
+    used = [k2 for k2 in range(baby) if any(entries[k1 * baby + k2] is not None for k1 in range(giant))]
+    nontriv = [k2 for k2 in used if (k2 * stride) % slots]
+    elts = [galois_elt_host(ref.n, k2 * stride) for k2 in nontriv]
+    hoisted = ref.hoisted_galois(c0, c1, level, elts, [key_of_step(k2 * stride) for k2 in nontriv], kml)
+    xs = {}
+    for k2 in used:
+        if k2 in nontriv:
+            i = nontriv.index(k2)
+            xs[k2] = (hoisted[i, 0], hoisted[i, 1])
+        else:
+            xs[k2] = (c0.reshape(level + 1, ref.n), c1.reshape(level + 1, ref.n))
+    acc = None
+    for k1 in range(giant):
+        blk = None
+        for k2 in range(baby):
+            w = entries[k1 * baby + k2]
+            if w is None:
+                continue
+            t = tuple(ref.poly_mul(x, w, level) for x in xs[k2])
+            blk = t if blk is None else tuple(ref.poly_add(a, b, level) for a, b in zip(blk, t))
+        if blk is None:
+            continue
+        shift = (baby * k1 * stride) % slots
+        if shift:
+            blk = ref.apply_galois(blk[0], blk[1], level, galois_elt_host(ref.n, baby * k1 * stride),
+                                   key_of_step(baby * k1 * stride), kml)
+        acc = blk if acc is None else tuple(ref.poly_add(a, b, level) for a, b in zip(acc, blk))
+    return acc
+
+
+def galois_elt_host(n, step):
+    return O.galois_elt(n, step)
+
+
+def c4_entries_mask(baby=32, giant=32):
+    """The C4 product: in giant block k1 the diagonals at baby steps k1 and k1 + 16 (mod 32)."""
+    return [[(k2 == k1 % baby) or (k2 == (k1 + baby // 2) % baby) for k2 in range(baby)] for k1 in range(giant)]

@@ -9,8 +9,9 @@ import pytest
 
 import ckh
 import oracle as O
-from gpu_cases import (ALPHA1, C2, C2_SHAPE_N4096, C2_SHAPE_SMALL_N, C4_SHAPE_SMALL_N, SMALL, SMALL_N8192,
-                       host_uniform, levels_for, make, near_tie_digits, near_tie_p_part)
+from gpu_cases import (ALPHA1, C2, C2_SHAPE_N4096, C2_SHAPE_SMALL_N, C4, C4_SHAPE_SMALL_N, SMALL, SMALL_N8192,
+                       c4_entries_mask, host_uniform, levels_for, make, near_tie_digits, near_tie_p_part,
+                       oracle_matvec)
 
 pytestmark = pytest.mark.gpu
 
@@ -319,3 +320,60 @@ def test_c2_full_size_key_switch():
     O.set_threads(16)
     wb, wa = ref.key_switch(host_uniform(ref, L, 4242).ravel(), L, ref.uniform_key(L, 424242), L)
     assert (kb.download() == wb).all() and (ka.download() == wa).all()
+
+
+# ---------------------------------------------------------------- BSGS (hoisted rotations)
+
+def _bsgs_case(case, baby, giant, mask, seed, level=None):
+    ctx, ref = make(case)
+    L = len(ref.q) - 1
+    level = L if level is None else level
+    steps = sorted({k2 for k2 in range(1, baby)} | {baby * k1 for k1 in range(1, giant)})
+    keys, hkeys = [], {}
+    for i, st in enumerate(steps):
+        g = ckh.galois_elt(ref.log_n, st)
+        keys.append(ctx.ksk_uniform(L, seed=seed + 10 * i, galois=g))
+        hkeys[st] = seed + 10 * i
+    ev = ckh.Evaluator(ctx, None, keys)
+    c0, c1 = host_uniform(ref, level, seed + 1), host_uniform(ref, level, seed + 2)
+    x = ctx.ct_upload(np.stack([c0, c1]), level, 2.0 ** 40)
+    entries_h, entries_d = [], []
+    for k1 in range(giant):
+        for k2 in range(baby):
+            if mask[k1][k2]:
+                w = host_uniform(ref, level, seed + 1000 + k1 * baby + k2)
+                entries_h.append(w)
+                entries_d.append(ctx.upload(w, level))
+            else:
+                entries_h.append(None)
+                entries_d.append(None)
+    got = ev.matvec_plain(x, entries_d, 2.0 ** 40, baby, giant)
+    key_cache = {}
+
+    def key_of_step(st):
+        if st not in key_cache:
+            key_cache[st] = ref.uniform_key(L, hkeys[st])
+        return key_cache[st]
+
+    want = oracle_matvec(ref, c0, c1, level, entries_h, baby, giant, 1, key_of_step, L)
+    g = got.download()
+    assert (g[0] == want[0]).all() and (g[1] == want[1]).all()
+    assert got.scale == 2.0 ** 80
+    nrot = sum(1 for k2 in range(1, baby) if any(mask[k1][k2] for k1 in range(giant)))
+    nrot += sum(1 for k1 in range(1, giant) if any(mask[k1]))
+    c = ev.counter()
+    assert c.rotations == nrot and c.ct_pt_mults == sum(map(sum, mask))
+
+
+def test_bsgs_matvec_small():
+    mask = [[True, False, True, True], [False, False, False, False], [True, True, False, False],
+            [False, True, True, True]]
+    _bsgs_case(C4_SHAPE_SMALL_N, 4, 4, mask, seed=900)
+
+
+def test_c4_hoisted_bsgs_full_size():
+    """N = 2^15, 20 x 50-bit q, 4 x 60-bit p, dnum 5: 32 hoisted baby steps, 32 giant steps,
+    64 diagonals (two per giant block), against the oracle composition."""
+    say("c4 full size")
+    O.set_threads(16)
+    _bsgs_case(C4, 32, 32, c4_entries_mask(), seed=4000)
