@@ -95,12 +95,103 @@ def cpu_baseline(q, p, levels, budget_s=12.0):
                       f"key_switch with {threads} OpenMP threads, {el:.1f} s"}
 
 
+def served_stream(ckh, n_ops=200, rate=1500.0, seed=SEED + 5):
+    """The served mixed stream: Poisson arrivals (rate ops/s) over N = 2^16 (24 + 6 primes,
+    dnum 4) and N = 2^15 (20 + 4 primes, dnum 5); 50% relinearize, 30% hoisted 16-rotation
+    batch, 20% mul + relinearize + rescale; limbs U{4..24} (at most 20 at 2^15); 64 rotation
+    keys and a relinearization key resident per parameter set; 200 GB admission budget.
+    The same arrival trace runs through the plain path (one op at a time, one stream) and
+    every served result is compared with the plain one by device digest."""
+    sets = [(16, [50] * 24, [60] * 6, 4), (15, [50] * 20, [60] * 4, 5)]
+    envs = []
+    for i, (log_n, qb, pb, dnum) in enumerate(sets):
+        q, p = ckh.params_primes(log_n, qb, pb)
+        ctx = ckh.Context(log_n, q, p, dnum, n_streams=4)
+        L = len(q) - 1
+        relin = ctx.ksk_uniform(L, seed=seed + 1000 * i)
+        rot = [ctx.ksk_uniform(L, seed=seed + 1000 * i + s, galois=ckh.galois_elt(log_n, s)) for s in range(1, 65)]
+        envs.append((ctx, L, ckh.Evaluator(ctx, relin, rot)))
+    rng = np.random.default_rng(seed)
+    ops, t = [], 0.0
+    for i in range(n_ops):
+        t += rng.exponential(1.0 / rate)
+        e = int(rng.integers(0, 2))
+        ctx, L, ev = envs[e]
+        level = min(int(rng.integers(3, 24)), L)
+        u = rng.random()
+
+        def ct(ncomp, base):
+            return ctx.ct_from_polys([ctx.uniform(level, seed=base + j) for j in range(ncomp)], 2.0 ** 40)
+
+        if u < 0.5:
+            ops.append((ckh.OP_RELINEARIZE, e, ct(3, 10 * i), None, [], t))
+        elif u < 0.8:
+            steps = sorted(int(s) for s in rng.choice(np.arange(1, 65), size=16, replace=False))
+            ops.append((ckh.OP_HOISTED_ROTATE, e, ct(2, 10 * i), None, steps, t))
+        else:
+            lv = max(level, 1)
+            ops.append((ckh.OP_MUL_RELIN_RESCALE, e,
+                        ctx.ct_from_polys([ctx.uniform(lv, seed=10 * i + j) for j in range(2)], 2.0 ** 40),
+                        ctx.ct_from_polys([ctx.uniform(lv, seed=10 * i + 5 + j) for j in range(2)], 2.0 ** 40), [], t))
+    for ctx, _, _ in envs:
+        ctx.synchronize()
+
+    def run_plain(op):
+        kind, e, a, b, steps, _ = op
+        ev = envs[e][2]
+        if kind == ckh.OP_RELINEARIZE:
+            return [ev.relinearize(a)]
+        if kind == ckh.OP_HOISTED_ROTATE:
+            return ev.hoisted_rotate(a, steps)
+        return [ev.rescale(ev.relinearize(ev.mul_no_relin(a, b)))]
+
+    for op in ops[:6]:  # warm both contexts' kernels and pools
+        run_plain(op)
+    for ctx, _, _ in envs:
+        ctx.synchronize()
+
+    # served
+    srv = ckh.Server(hbm_budget_bytes=200 * 10 ** 9, max_batch=64)
+    t0 = srv.now() + 0.02
+    tickets = [srv.submit(k, envs[e][2], a, b, steps, arrival_s=t0 + ta) for (k, e, a, b, steps, ta) in ops]
+    served = [srv.poll(tk) for tk in tickets]
+    st = srv.stats()
+    srv.close()
+
+    # plain, same arrivals, one op at a time
+    lat, plain_out = [], []
+    start = time.perf_counter() + 0.02
+    for op in ops:
+        arrival = start + op[5]
+        while time.perf_counter() < arrival:
+            pass
+        out = run_plain(op)
+        envs[op[1]][0].synchronize()
+        lat.append((time.perf_counter() - arrival) * 1e3)
+        plain_out.append(out)
+    makespan = time.perf_counter() - start
+    identical = all(s[0] == 0 and len(s[2]) == len(p) and all(a.digests() == b.digests() for a, b in zip(s[2], p))
+                    for s, p in zip(served, plain_out))
+    lat = np.array(lat)
+    mix = {k: sum(1 for o in ops if o[0] == k) for k in (0, 1, 2)}
+    return {
+        "ops": n_ops, "arrival_rate_per_s": rate,
+        "mix": {"relinearize": mix[0], "hoisted_16": mix[1], "mul_relin_rescale": mix[2]},
+        "served": {"ops_per_s": st.ops_per_s, "p50_ms": st.p50_ms, "p99_ms": st.p99_ms, "mean_ms": st.mean_ms,
+                   "batches": st.batches, "peak_admitted_GB": st.peak_bytes / 1e9, "failed": st.failed},
+        "plain": {"ops_per_s": n_ops / makespan, "p50_ms": float(np.percentile(lat, 50)),
+                  "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean())},
+        "identical_alone_and_in_mix": bool(identical),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the key-switch backend shards by ciphertext with no "
@@ -186,6 +277,11 @@ def main():
     achieved = ntt_bytes / (ntt_ms * 1e-3) / 1e9
     del tpolys
 
+    del polys, key, top
+    ctx.synchronize()
+    serve = None if args.no_serve else served_stream(ckh)
+    print(f"# served stream: {json.dumps(serve)}", file=sys.stderr, flush=True)
+
     cpu = None if args.no_cpu else cpu_baseline(q, p, levels)
 
     mean_bytes = float(np.mean([ks_bytes(lv) for lv in levels]))
@@ -222,6 +318,7 @@ def main():
             "ms_per_launch": ntt_ms, "bytes_per_launch": ntt_bytes,
         },
         "cpu_baseline": cpu,
+        "served_stream": serve,
     }
     print(json.dumps(line), flush=True)
 

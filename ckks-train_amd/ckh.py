@@ -65,6 +65,25 @@ class OpCounter(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", u64), ("max_batch", u32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", u32), ("ev", VP), ("a", VP), ("b", VP), ("steps", C.POINTER(C.c_int32)),
+                ("n_steps", u32), ("arrival_s", C.c_double), ("inject_failure", u32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(f, u64) for f in ("submitted", "completed", "failed", "rejected", "batches", "peak_bytes")] + \
+               [(f, C.c_double) for f in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "makespan_s", "ops_per_s")]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+OP_RELINEARIZE, OP_HOISTED_ROTATE, OP_MUL_RELIN_RESCALE = 0, 1, 2
+
 # name: (restype, argtypes)
 _SIGS = {
     "ckh_last_error": (C.c_char_p, []),
@@ -80,6 +99,7 @@ _SIGS = {
     "ckh_poly_upload": (C.c_int, [VP, P64, u32, u32, C.c_int, C.POINTER(VP)]),
     "ckh_poly_download": (C.c_int, [VP, VP, P64]),
     "ckh_poly_info": (C.c_int, [VP, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_int)]),
+    "ckh_poly_digest": (C.c_int, [VP, VP, P64]),
     "ckh_poly_retain": (VP, [VP]),
     "ckh_poly_release": (None, [VP]),
     "ckh_poly_uniform": (C.c_int, [VP, u32, u32, C.c_int, u64, u64, C.POINTER(VP)]),
@@ -129,6 +149,13 @@ _SIGS = {
     "ckh_hoisted_rotate": (C.c_int, [VP, VP, C.POINTER(C.c_int32), u32, C.POINTER(VP)]),
     "ckh_relinearize_batch": (C.c_int, [VP, u32, C.POINTER(VP), C.POINTER(VP)]),
     "ckh_matvec_plain": (C.c_int, [VP, VP, C.POINTER(VP), C.c_double, u32, u32, u32, C.POINTER(VP)]),
+    "ckh_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(VP)]),
+    "ckh_server_destroy": (None, [VP]),
+    "ckh_server_now": (C.c_double, [VP]),
+    "ckh_server_submit": (C.c_int, [VP, C.POINTER(Request), P64]),
+    "ckh_server_poll": (C.c_int, [VP, u64, C.c_int, C.POINTER(C.c_int), C.POINTER(VP), u32, C.POINTER(u32),
+                                  C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "ckh_server_stats_get": (C.c_int, [VP, C.POINTER(ServerStats)]),
 }
 
 _lib = None
@@ -316,6 +343,11 @@ class Poly:
     def retain(self):
         return Poly(self.ctx, VP(lib().ckh_poly_retain(self.h)))
 
+    def digest(self):
+        d = u64(0)
+        _check(lib().ckh_poly_digest(self.ctx.h, self.h, C.byref(d)))
+        return d.value
+
     def ntt(self):
         h = VP()
         _check(lib().ckh_ntt(self.ctx.h, self.h, C.byref(h)))
@@ -390,6 +422,9 @@ class Ct:
 
     def component(self, i):
         return Poly(self.ctx, VP(lib().ckh_ct_component(self.h, i)))
+
+    def digests(self):
+        return tuple(self.component(i).digest() for i in range(self.size()))
 
 
 class Evaluator:
@@ -498,3 +533,51 @@ class Evaluator:
         outs = (VP * max(n, 1))()
         _check(lib().ckh_relinearize_batch(self.h, n, arr, outs))
         return [Ct(self.ctx, VP(outs[i])) for i in range(n)]
+
+
+class Server:
+    """The serving loop (include/ckh.h, "serving")."""
+
+    def __init__(self, hbm_budget_bytes, max_batch=64):
+        cfg = ServerConfig(hbm_budget_bytes, max_batch)
+        h = VP()
+        _check(lib().ckh_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ckh_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def now(self):
+        return lib().ckh_server_now(self.h)
+
+    def submit(self, kind, ev, a, b=None, steps=(), arrival_s=-1.0, inject_failure=False):
+        st = (C.c_int32 * max(len(steps), 1))(*steps)
+        r = Request(kind, ev.h, a.h, b.h if b is not None else None, st, len(steps), arrival_s, int(inject_failure))
+        t = u64(0)
+        _check(lib().ckh_server_submit(self.h, C.byref(r), C.byref(t)))
+        self._keep[t.value] = (ev, a, b)
+        return t.value
+
+    def poll(self, ticket, wait=True, max_outs=64):
+        """None if not finished, else (status, message, outputs, latency_ms)."""
+        fin, n, st, lat = C.c_int(0), u32(0), C.c_int(0), C.c_double(0)
+        outs = (VP * max_outs)()
+        _check(lib().ckh_server_poll(self.h, ticket, int(wait), C.byref(fin), outs, max_outs, C.byref(n),
+                                     C.byref(st), C.byref(lat)))
+        if not fin.value:
+            return None
+        ev, a, b = self._keep.pop(ticket)
+        msg = lib().ckh_last_error().decode(errors="replace") if st.value else ""
+        cts = [Ct(a.ctx, VP(outs[i])) for i in range(min(n.value, max_outs))]
+        return st.value, msg, cts, lat.value
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().ckh_server_stats_get(self.h, C.byref(s)))
+        return s

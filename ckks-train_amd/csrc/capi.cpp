@@ -7,6 +7,7 @@
 
 #include "evaluator.h"
 #include "hostmath.h"
+#include "server.h"
 
 using namespace ckh;
 
@@ -28,6 +29,9 @@ struct ckh_ct {
 struct ckh_evaluator {
     std::shared_ptr<Context> keep;
     std::unique_ptr<Evaluator> e;
+};
+struct ckh_server {
+    std::unique_ptr<Server> s;
 };
 
 namespace {
@@ -184,6 +188,13 @@ ckh_status ckh_poly_upload(ckh_context* ctx, const uint64_t* limbs, uint32_t lev
 
 ckh_status ckh_poly_download(ckh_context* ctx, const ckh_poly* poly, uint64_t* out) {
     return guarded([&] { poly_download(ctx_of(ctx), poly_of(poly), out); });
+}
+
+ckh_status ckh_poly_digest(ckh_context* ctx, const ckh_poly* poly, uint64_t* digest) {
+    return guarded([&] {
+        need(digest, "digest");
+        *digest = poly_digest(ctx_of(ctx), poly_of(poly));
+    });
 }
 
 ckh_status ckh_poly_info(const ckh_poly* poly, uint32_t* level, uint32_t* p_limbs, int* ntt_form) {
@@ -546,6 +557,63 @@ ckh_status ckh_relinearize_batch(ckh_evaluator* ev, uint32_t n, const ckh_ct* co
         for (uint32_t i = 0; i < n; ++i) v.push_back(&ct_of(in[i]));
         auto r = ev_of(ev).relinearize_batch(v);
         for (uint32_t i = 0; i < n; ++i) out[i] = wrap(r[i]);
+    });
+}
+
+ckh_status ckh_server_create(const ckh_server_config* cfg, ckh_server** out) {
+    return guarded([&] {
+        need(cfg, "cfg");
+        need(out, "out");
+        auto* h = new ckh_server();
+        h->s.reset(new Server(*cfg));
+        *out = h;
+    });
+}
+
+void ckh_server_destroy(ckh_server* srv) { delete srv; }
+
+double ckh_server_now(ckh_server* srv) { return srv && srv->s ? srv->s->now() : 0.0; }
+
+ckh_status ckh_server_submit(ckh_server* srv, const ckh_request* req, uint64_t* ticket) {
+    return guarded([&] {
+        need(srv, "server");
+        need(req, "request");
+        need(ticket, "ticket");
+        auto op = std::make_shared<ServedOp>();
+        op->kind = req->kind;
+        op->ev = &ev_of(req->ev);
+        op->a = req->a ? req->a->c : nullptr;
+        op->b = req->b ? req->b->c : nullptr;
+        if (req->steps && req->n_steps) op->steps.assign(req->steps, req->steps + req->n_steps);
+        op->t_arrival = req->arrival_s;
+        op->inject_failure = req->inject_failure != 0;
+        *ticket = srv->s->submit(op);
+    });
+}
+
+ckh_status ckh_server_poll(ckh_server* srv, uint64_t ticket, int wait, int* finished, ckh_ct** outs, uint32_t max_outs,
+                           uint32_t* n_outs, ckh_status* op_status, double* latency_ms) {
+    return guarded([&] {
+        need(srv, "server");
+        need(finished, "finished");
+        OpPtr op;
+        *finished = srv->s->poll(ticket, wait != 0, op) ? 1 : 0;
+        if (!*finished) return;
+        if (op_status) *op_status = op->status;
+        if (op->status != CKH_OK) tl_err = op->err;
+        if (latency_ms) *latency_ms = (op->t_done - op->t_arrival) * 1e3;
+        const uint32_t n = (uint32_t)std::min<size_t>(op->outs.size(), max_outs);
+        if (n) need(outs, "outs");
+        for (uint32_t i = 0; i < n; ++i) outs[i] = wrap(op->outs[i]);
+        if (n_outs) *n_outs = (uint32_t)op->outs.size();
+    });
+}
+
+ckh_status ckh_server_stats_get(ckh_server* srv, ckh_server_stats* out) {
+    return guarded([&] {
+        need(srv, "server");
+        need(out, "out");
+        *out = srv->s->stats();
     });
 }
 

@@ -57,6 +57,17 @@ void poly_download(Context& c, const Poly& p, u64* host) {
     hip_check(hipStreamSynchronize(s), "download");
 }
 
+u64 poly_digest(Context& c, const Poly& p) {
+    hipStream_t s = c.stream();
+    BufPtr acc = c.alloc(1, s);
+    hip_check(hipMemsetAsync(acc->ptr, 0, 8, s), "memset");
+    digest_launch(p.data(), (u64)p.limbs() * c.n, acc->ptr, s);
+    u64 v = 0;
+    hip_check(hipMemcpyAsync(&v, acc->ptr, 8, hipMemcpyDeviceToHost, s), "digest");
+    hip_check(hipStreamSynchronize(s), "digest");
+    return v;
+}
+
 PolyPtr poly_uniform(Context& c, u32 level, u32 p_limbs, bool ntt, u64 seed, u64 stream0) {
     PolyPtr p = poly_alloc(c, level, p_limbs, ntt);
     fill_uniform_launch(p->data(), level + 1, p_limbs, c.nq, seed, stream0, c.d_pc, c.log_n, c.stream());

@@ -13,6 +13,7 @@ namespace ckh {
 PolyPtr poly_alloc(Context& c, u32 level, u32 p_limbs, bool ntt);
 PolyPtr poly_upload(Context& c, const u64* host, u32 level, u32 p_limbs, bool ntt);
 void poly_download(Context& c, const Poly& p, u64* host);
+u64 poly_digest(Context& c, const Poly& p);
 PolyPtr poly_uniform(Context& c, u32 level, u32 p_limbs, bool ntt, u64 seed, u64 stream0);
 PolyPtr poly_transform(Context& c, const Poly& in, bool forward);
 PolyPtr poly_automorphism(Context& c, const Poly& in, u64 g);

@@ -39,4 +39,8 @@ void automorphism_ntt_launch(const u64* in, u64* out, u32 limbs, u64 g, u32 log_
 void fill_uniform_launch(u64* out, u32 q_limbs, u32 p_limbs, u32 nq, u64 seed, u64 stream0,
                          const PrimeConst* pc, u32 log_n, hipStream_t stream);
 
+// *out += sum over words of mix(word, index): an order-independent 64-bit digest of `words` words
+// (for comparing large results without copying them to the host). *out must start at 0.
+void digest_launch(const u64* data, u64 words, u64* out, hipStream_t stream);
+
 }  // namespace ckh

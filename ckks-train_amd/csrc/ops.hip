@@ -136,4 +136,25 @@ void fill_uniform_launch(u64* out, u32 q_limbs, u32 p_limbs, u32 nq, u64 seed, u
                        nq, seed, stream0, pc, log_n);
 }
 
+__global__ __launch_bounds__(256) void k_digest(const u64* __restrict__ data, u64 words, u64* out) {
+    __shared__ u64 part[256];
+    u64 acc = 0;
+    for (u64 i = (u64)blockIdx.x * 256 + threadIdx.x; i < words; i += (u64)gridDim.x * 256)
+        acc += mix64(data[i] ^ mix64(i + 0x632be59bd9b4e019ULL));
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (u32 w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd((unsigned long long*)out, (unsigned long long)part[0]);
+}
+
+void digest_launch(const u64* data, u64 words, u64* out, hipStream_t stream) {
+    u64 blocks = (words + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(k_digest, dim3((u32)blocks), dim3(256), 0, stream, data, words, out);
+}
+
 }  // namespace ckh

@@ -1,0 +1,276 @@
+// Serving loop (see include/ckh.h, "serving").
+#include "server.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace ckh {
+
+Server::Server(const ckh_server_config& cfg) : cfg_(cfg), t0_(std::chrono::steady_clock::now()) {
+    if (cfg_.max_batch == 0) cfg_.max_batch = 64;
+    th_ = std::thread([this] { loop(); });
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    th_.join();
+}
+
+double Server::now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+}
+
+// Device bytes of a request while it runs: outputs plus the scratch of its key switches.
+u64 Server::footprint(const ServedOp& op) const {
+    const Context& c = op.ev->ctx;
+    const u64 L = op.a->level() + 1, k = c.np, d = c.ndigits(op.a->level());
+    const u64 ks = L + d * (L + k) + 2 * (L + k) + 2 * k + 2 * L;  // one key switch, limbs
+    u64 limbs = 0;
+    switch (op.kind) {
+        case CKH_OP_RELINEARIZE: limbs = ks + 2 * L; break;
+        case CKH_OP_HOISTED_ROTATE:
+            limbs = d * (L + k) + L + op.steps.size() * (2 * (L + k) + 2 * k + 2 * L + L + 2 * L);
+            break;
+        default: limbs = 3 * L + ks + 2 * L + 2 * L + 2 * (1 + L); break;
+    }
+    return limbs * c.n * 8;
+}
+
+u64 Server::submit(const OpPtr& op) {
+    if (!op->ev || !op->a) fail(CKH_ERR_INVALID_PARAMS, "request needs an evaluator and an input");
+    if (op->kind > CKH_OP_MUL_RELIN_RESCALE) fail(CKH_ERR_INVALID_PARAMS, "unknown op kind");
+    if (op->kind == CKH_OP_MUL_RELIN_RESCALE && !op->b) fail(CKH_ERR_INVALID_PARAMS, "mul needs two inputs");
+    if (op->t_arrival < 0) op->t_arrival = now();
+    Context& c = op->ev->ctx;
+    hip_check(hipSetDevice(c.device), "hipSetDevice");
+    hip_check(hipEventCreateWithFlags(&op->ready, hipEventDisableTiming), "event");
+    hip_check(hipEventRecord(op->ready, c.stream()), "event");
+    std::lock_guard<std::mutex> lk(mu_);
+    op->ticket = next_ticket_++;
+    table_[op->ticket] = op;
+    pending_.push_back(op);
+    ++submitted_;
+    if (first_arrival_ < 0 || op->t_arrival < first_arrival_) first_arrival_ = op->t_arrival;
+    cv_.notify_all();
+    return op->ticket;
+}
+
+void Server::fail_op(const OpPtr& op, ckh_status st, const std::string& msg) {
+    op->status = st;
+    op->err = msg;
+    op->outs.clear();
+}
+
+// Runs one scheduler turn's admitted requests: grouped by evaluator, relinearizations (also the
+// one inside mul+relin+rescale) batched, on the next stream of the evaluator's context.
+void Server::launch(std::vector<OpPtr>& ops) {
+    std::vector<Evaluator*> evs;
+    for (auto& op : ops)
+        if (std::find(evs.begin(), evs.end(), op->ev) == evs.end()) evs.push_back(op->ev);
+    for (Evaluator* ev : evs) {
+        Context& c = ev->ctx;
+        hip_check(hipSetDevice(c.device), "hipSetDevice");
+        const u32 si = rr_[&c]++ % c.n_streams();
+        c.set_stream(si);
+        hipStream_t s = c.stream();
+        std::vector<OpPtr> mine;
+        for (auto& op : ops)
+            if (op->ev == ev) {
+                mine.push_back(op);
+                (void)hipStreamWaitEvent(s, op->ready, 0);
+            }
+        // Injected failures and invalid requests fail alone; the rest run batched.
+        std::vector<OpPtr> relin, mrr;
+        for (auto& op : mine) {
+            if (op->inject_failure) {
+                fail_op(op, CKH_ERR_DEVICE, "injected device failure");
+                continue;
+            }
+            if (op->kind == CKH_OP_RELINEARIZE) relin.push_back(op);
+            else if (op->kind == CKH_OP_MUL_RELIN_RESCALE) mrr.push_back(op);
+            else {
+                try {
+                    op->outs = ev->hoisted_rotate(*op->a, op->steps);
+                } catch (const Error& e) {
+                    fail_op(op, e.status, e.msg);
+                }
+            }
+        }
+        // mul_no_relin first (a failing product fails its request alone).
+        std::vector<CtPtr> prods;
+        std::vector<OpPtr> mrr_ok;
+        for (auto& op : mrr) {
+            try {
+                prods.push_back(ev->mul_no_relin(*op->a, *op->b));
+                mrr_ok.push_back(op);
+            } catch (const Error& e) {
+                fail_op(op, e.status, e.msg);
+            }
+        }
+        std::vector<const Ct*> batch;
+        std::vector<OpPtr> owners;
+        for (auto& op : relin) {
+            batch.push_back(op->a.get());
+            owners.push_back(op);
+        }
+        for (size_t i = 0; i < mrr_ok.size(); ++i) {
+            batch.push_back(prods[i].get());
+            owners.push_back(mrr_ok[i]);
+        }
+        std::vector<CtPtr> outs;
+        try {
+            outs = ev->relinearize_batch(batch);
+        } catch (const Error&) {
+            // One bad member: fall back to one call each so only that request fails.
+            outs.assign(batch.size(), nullptr);
+            for (size_t i = 0; i < batch.size(); ++i) {
+                try {
+                    outs[i] = ev->relinearize(*batch[i]);
+                } catch (const Error& e2) {
+                    fail_op(owners[i], e2.status, e2.msg);
+                }
+            }
+        }
+        for (size_t i = 0; i < owners.size(); ++i) {
+            if (!outs[i]) continue;
+            if (owners[i]->kind == CKH_OP_RELINEARIZE) {
+                owners[i]->outs = {outs[i]};
+            } else {
+                try {
+                    owners[i]->outs = {ev->rescale(*outs[i])};
+                } catch (const Error& e) {
+                    fail_op(owners[i], e.status, e.msg);
+                }
+            }
+        }
+        Batch b{&c, nullptr, mine};
+        hip_check(hipEventCreateWithFlags(&b.done, hipEventDisableTiming), "event");
+        hip_check(hipEventRecord(b.done, s), "event");
+        std::lock_guard<std::mutex> lk(mu_);
+        inflight_.push_back(b);
+        ++batches_;
+    }
+}
+
+void Server::loop() {
+    for (;;) {
+        std::vector<OpPtr> admitted;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            // completions
+            for (size_t i = 0; i < inflight_.size();) {
+                Batch& b = inflight_[i];
+                (void)hipSetDevice(b.ctx->device);
+                if (hipEventQuery(b.done) == hipSuccess) {
+                    const double t = now();
+                    for (auto& op : b.ops) {
+                        op->t_done = t;
+                        op->done = true;
+                        in_use_ -= op->footprint;
+                        if (op->status == CKH_OK) {
+                            ++completed_;
+                            lat_ms_.push_back((t - op->t_arrival) * 1e3);
+                        } else {
+                            ++failed_;
+                        }
+                        last_done_ = std::max(last_done_, t);
+                    }
+                    (void)hipEventDestroy(b.done);
+                    inflight_.erase(inflight_.begin() + i);
+                    done_cv_.notify_all();
+                } else {
+                    ++i;
+                }
+            }
+            if (stop_ && pending_.empty() && inflight_.empty()) return;
+            // admission in arrival order within the budget
+            while (!pending_.empty() && admitted.size() < cfg_.max_batch) {
+                OpPtr op = pending_.front();
+                if (op->t_arrival > now()) break;  // not arrived yet (pre-scheduled arrivals)
+                u64 fp = 0;
+                try {
+                    fp = footprint(*op);
+                } catch (...) {
+                    fp = 0;
+                }
+                if (fp > cfg_.hbm_budget_bytes) {
+                    pending_.pop_front();
+                    fail_op(op, CKH_ERR_DEVICE, "request exceeds the HBM budget");
+                    op->done = true;
+                    op->t_done = now();
+                    ++rejected_;
+                    done_cv_.notify_all();
+                    continue;
+                }
+                if (in_use_ + fp > cfg_.hbm_budget_bytes) break;
+                pending_.pop_front();
+                op->footprint = fp;
+                in_use_ += fp;
+                peak_ = std::max(peak_, in_use_);
+                admitted.push_back(op);
+            }
+            if (admitted.empty()) {
+                cv_.wait_for(lk, std::chrono::microseconds(inflight_.empty() ? 200 : 20));
+                continue;
+            }
+        }
+        try {
+            launch(admitted);
+        } catch (const Error& e) {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (auto& op : admitted)
+                if (!op->done) {
+                    fail_op(op, e.status, e.msg);
+                    op->done = true;
+                    in_use_ -= op->footprint;
+                    ++failed_;
+                }
+            done_cv_.notify_all();
+        }
+    }
+}
+
+bool Server::poll(u64 ticket, bool wait, OpPtr& op) {
+    std::unique_lock<std::mutex> lk(mu_);
+    auto it = table_.find(ticket);
+    if (it == table_.end()) fail(CKH_ERR_INVALID_PARAMS, "unknown ticket");
+    OpPtr o = it->second;
+    if (wait) done_cv_.wait(lk, [&] { return o->done; });
+    if (!o->done) return false;
+    op = o;
+    table_.erase(it);
+    if (o->ready) (void)hipEventDestroy(o->ready);
+    o->ready = nullptr;
+    return true;
+}
+
+ckh_server_stats Server::stats() {
+    std::lock_guard<std::mutex> lk(mu_);
+    ckh_server_stats s{};
+    s.submitted = submitted_;
+    s.completed = completed_;
+    s.failed = failed_;
+    s.rejected = rejected_;
+    s.batches = batches_;
+    s.peak_bytes = peak_;
+    if (!lat_ms_.empty()) {
+        std::vector<double> v = lat_ms_;
+        std::sort(v.begin(), v.end());
+        auto pct = [&](double p) { return v[std::min(v.size() - 1, (size_t)std::ceil(p * v.size()) - 1)]; };
+        s.p50_ms = pct(0.50);
+        s.p99_ms = pct(0.99);
+        double sum = 0;
+        for (double x : v) sum += x;
+        s.mean_ms = sum / v.size();
+        s.max_ms = v.back();
+        s.makespan_s = last_done_ - first_arrival_;
+        s.ops_per_s = s.makespan_s > 0 ? completed_ / s.makespan_s : 0;
+    }
+    return s;
+}
+
+}  // namespace ckh

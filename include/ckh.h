@@ -96,6 +96,9 @@ ckh_status ckh_poly_info(const ckh_poly *poly, uint32_t *level, uint32_t *p_limb
                          int *ntt_form);
 ckh_poly *ckh_poly_retain(ckh_poly *poly);
 void ckh_poly_release(ckh_poly *poly);
+/* An order-independent 64-bit digest of every word of the polynomial (sum of a 64-bit mix of
+ * (word, index)), for comparing large results on the device. No counterpart in the reference. */
+ckh_status ckh_poly_digest(ckh_context *ctx, const ckh_poly *poly, uint64_t *digest);
 /* Seeded uniform residues on the device: limb i from generator stream stream0 + i. */
 ckh_status ckh_poly_uniform(ckh_context *ctx, uint32_t level, uint32_t p_limbs, int ntt_form,
                             uint64_t seed, uint64_t stream0, ckh_poly **out);
@@ -215,6 +218,57 @@ ckh_status ckh_matvec_plain(ckh_evaluator *ev, const ckh_ct *x, const ckh_poly *
 /* n relinearizations in batched launches (bit-identical to n ckh_relinearize calls). */
 ckh_status ckh_relinearize_batch(ckh_evaluator *ev, uint32_t n, const ckh_ct *const *in,
                                  ckh_ct **out);
+
+/* ------------------------------------------------------------------ serving */
+
+/* No counterpart in the reference (its trainers call HeOps synchronously). A scheduler thread
+ * keeps evaluators and their keys resident, admits requests in arrival order while the bytes
+ * of the admitted work stay within hbm_budget_bytes, and batches whatever is admitted at each
+ * turn: relinearizations of one evaluator as one batched key switch, mul+relin+rescale with its
+ * relinearization batched likewise, hoisted rotations per request. Batches run on the contexts'
+ * streams round-robin, so work admitted while other batches run overlaps with them. Results
+ * are bit-identical to the same operations called alone. */
+typedef enum ckh_op_kind {
+    CKH_OP_RELINEARIZE = 0,       /* a: 3 components -> 1 output */
+    CKH_OP_HOISTED_ROTATE = 1,    /* a: 2 components, steps -> n_steps outputs */
+    CKH_OP_MUL_RELIN_RESCALE = 2  /* a, b: 2 components -> rescale(relinearize(a * b)) */
+} ckh_op_kind;
+
+typedef struct ckh_server_config {
+    uint64_t hbm_budget_bytes; /* bound on the device bytes of admitted, unfinished requests */
+    uint32_t max_batch;        /* requests admitted per scheduler turn */
+} ckh_server_config;
+
+typedef struct ckh_request {
+    uint32_t kind;           /* ckh_op_kind */
+    ckh_evaluator *ev;       /* its context's streams run the work */
+    const ckh_ct *a, *b;
+    const int32_t *steps;    /* CKH_OP_HOISTED_ROTATE */
+    uint32_t n_steps;
+    double arrival_s;        /* arrival on the server clock (ckh_server_now); < 0: now */
+    uint32_t inject_failure; /* test hook: the request fails with CKH_ERR_DEVICE */
+} ckh_request;
+
+typedef struct ckh_server_stats {
+    uint64_t submitted, completed, failed, rejected, batches, peak_bytes;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* latency: completion - arrival */
+    double makespan_s, ops_per_s;           /* first arrival to last completion */
+} ckh_server_stats;
+
+ckh_status ckh_server_create(const ckh_server_config *cfg, ckh_server **out);
+/* Waits for the requests in flight, then stops the scheduler. */
+void ckh_server_destroy(ckh_server *srv);
+double ckh_server_now(ckh_server *srv);
+/* The request's inputs must have been produced on the calling thread's stream of their context
+ * (the scheduler waits for that point before using them). */
+ckh_status ckh_server_submit(ckh_server *srv, const ckh_request *req, uint64_t *ticket);
+/* *finished = 1 once the request is done (wait != 0 blocks until then); then op_status holds
+ * its result (its message in ckh_last_error), outs[0..n_outs) its ciphertexts (released by the
+ * caller) and latency_ms its completion minus arrival. A finished ticket is forgotten. */
+ckh_status ckh_server_poll(ckh_server *srv, uint64_t ticket, int wait, int *finished, ckh_ct **outs,
+                           uint32_t max_outs, uint32_t *n_outs, ckh_status *op_status,
+                           double *latency_ms);
+ckh_status ckh_server_stats_get(ckh_server *srv, ckh_server_stats *out);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,109 @@
+"""Serving loop: continuous batching over two parameter sets must return, for every request,
+exactly what the same operation returns alone; an injected failure fails only its request; a
+request above the HBM budget is rejected with a Device error."""
+import numpy as np
+import pytest
+
+import ckh
+from gpu_cases import make
+
+pytestmark = pytest.mark.gpu
+
+SETS = [("c2_shape_n4096", 12, [50] * 24, [60] * 6, 4), ("c4_shape_n1024", 10, [50] * 20, [60] * 4, 5)]
+
+
+def _setup(seed):
+    envs = []
+    for i, case in enumerate(SETS):
+        ctx, ref = make(case, n_streams=3)
+        L = len(ref.q) - 1
+        relin = ctx.ksk_uniform(L, seed=seed + 100 * i)
+        gk = [ctx.ksk_uniform(L, seed=seed + 100 * i + s, galois=ckh.galois_elt(ref.log_n, s)) for s in range(1, 17)]
+        envs.append((ctx, ref, ckh.Evaluator(ctx, relin, gk)))
+    return envs
+
+
+def _requests(envs, n, seed):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    t = 0.0
+    for i in range(n):
+        t += rng.exponential(1.0 / 4000.0)
+        e = int(rng.integers(0, len(envs)))
+        ctx, ref, ev = envs[e]
+        L = len(ref.q) - 1
+        level = int(rng.integers(3, L + 1))
+        u = rng.random()
+        if u < 0.5:
+            kind = ckh.OP_RELINEARIZE
+            a = ctx.ct_upload(np.stack([ctx.uniform(level, seed + 10 * i + j).download() for j in range(3)]), level,
+                              2.0 ** 40)
+            reqs.append((kind, e, a, None, [], t))
+        elif u < 0.8:
+            kind = ckh.OP_HOISTED_ROTATE
+            a = ctx.ct_upload(np.stack([ctx.uniform(level, seed + 10 * i + j).download() for j in range(2)]), level,
+                              2.0 ** 40)
+            steps = list(range(1, 17))
+            reqs.append((kind, e, a, None, steps, t))
+        else:
+            kind = ckh.OP_MUL_RELIN_RESCALE
+            level = max(level, 1)
+            a = ctx.ct_upload(np.stack([ctx.uniform(level, seed + 10 * i + j).download() for j in range(2)]), level,
+                              2.0 ** 40)
+            b = ctx.ct_upload(np.stack([ctx.uniform(level, seed + 10 * i + 5 + j).download() for j in range(2)]), level,
+                              2.0 ** 40)
+            reqs.append((kind, e, a, b, [], t))
+    return reqs
+
+
+def _alone(envs, req):
+    kind, e, a, b, steps, _ = req
+    ev = envs[e][2]
+    if kind == ckh.OP_RELINEARIZE:
+        return [ev.relinearize(a)]
+    if kind == ckh.OP_HOISTED_ROTATE:
+        return ev.hoisted_rotate(a, steps)
+    return [ev.rescale(ev.relinearize(ev.mul_no_relin(a, b)))]
+
+
+def test_served_mix_equals_alone_and_failure_isolated():
+    envs = _setup(7000)
+    reqs = _requests(envs, 40, 7)
+    srv = ckh.Server(hbm_budget_bytes=64 << 30, max_batch=16)
+    t0 = srv.now() + 0.05
+    bad = 11
+    tickets = [srv.submit(k, envs[e][2], a, b, steps, arrival_s=t0 + t, inject_failure=(i == bad))
+               for i, (k, e, a, b, steps, t) in enumerate(reqs)]
+    results = [srv.poll(t, wait=True) for t in tickets]
+    st = srv.stats()
+    assert st.submitted == 40 and st.completed == 39 and st.failed == 1 and st.rejected == 0
+    assert st.batches >= 2 and st.p99_ms >= st.p50_ms > 0
+    for i, ((status, msg, outs, lat), req) in enumerate(zip(results, reqs)):
+        if i == bad:
+            assert status == ckh.Device.status and "injected" in msg and outs == []
+            continue
+        assert status == 0, msg
+        assert lat > 0
+        alone = _alone(envs, req)
+        assert len(outs) == len(alone)
+        for o, w in zip(outs, alone):
+            assert o.info() == w.info()
+            assert o.digests() == w.digests(), (i, req[0])
+    # spot-check one result word for word as well
+    status, _, outs, _ = results[0]
+    alone = _alone(envs, reqs[0])
+    assert (outs[0].download() == alone[0].download()).all()
+    srv.close()
+
+
+def test_request_above_budget_is_rejected():
+    envs = _setup(8000)
+    ctx, ref, ev = envs[0]
+    L = len(ref.q) - 1
+    a = ctx.ct_upload(np.stack([ctx.uniform(L, 1 + j).download() for j in range(3)]), L, 2.0 ** 40)
+    srv = ckh.Server(hbm_budget_bytes=1 << 20)
+    t = srv.submit(ckh.OP_RELINEARIZE, ev, a)
+    status, msg, outs, _ = srv.poll(t)
+    assert status == ckh.Device.status and "budget" in msg and outs == []
+    assert srv.stats().rejected == 1
+    srv.close()
