@@ -24,10 +24,15 @@ struct ConvTgt {
     u64 qmod, qmod_s;  // Q mod r
     u64 hmod;          // (Q - 1)/2 mod r
     u64 qhat[MAX_SRC];                   // (Q / q_j) mod r
-    double qhat_f[MAX_SRC], qhat_q[MAX_SRC];  // f64 path: (Q / q_j) mod r and its quotient by r
+    double qhat_f[MAX_SRC], qhat_q[MAX_SRC];      // f64 path: (Q / q_j) mod r and its quotient by r
+    double qhat30_f[MAX_SRC], qhat30_q[MAX_SRC];  // split path: 2^30 (Q / q_j) mod r, quotient by r
     double qmod_f, qmod_q, hmod_f, rf, rinv;
-    u32 fp, pad;  // r and every source below 2^50: f64 multiply-accumulate
+    // TGT_INT: 64-bit integer sums; TGT_FP: r and every source below 2^50, f64 products of the
+    // sources; TGT_SPLIT: r below 2^50 with larger sources, each source split into 30-bit halves.
+    u32 mode, pad;
 };
+
+enum : u32 { TGT_INT = 0, TGT_FP = 1, TGT_SPLIT = 2 };
 
 // A conversion: sources [src_first, src_first + nsrc) of the ConvSrc table (their input limbs
 // are src_limb0 + j of the job's input), targets [tgt_first, tgt_first + ntgt) of the ConvTgt

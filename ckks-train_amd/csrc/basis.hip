@@ -51,12 +51,13 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
     const u32 t1 = min(t0 + tg, P.ntgt);
     const u64* inp = job.in;
     u64 y[MAXS];
-    double yd[MAXS];
+    double yd[MAXS], yh[MAXS];
     double f = 0.0;
 #pragma unroll
     for (u32 j = 0; j < MAXS; ++j) {
         y[j] = 0;
         yd[j] = 0.0;
+        yh[j] = 0.0;
         if (j < P.nsrc) {
             const ConvSrc* s = T.src + P.src_first + j;
             const u64 x = inp[(u64)(P.src_limb0 + j) * n + c];
@@ -69,6 +70,8 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
                 y[j] = cv_d2u(v);
             } else {
                 y[j] = shoup(add_mod(x, s->h, s->q), s->qhinv, s->qhinv_s, s->q);
+                yd[j] = (double)(u32)(y[j] & ((1u << 30) - 1));  // low 30 bits, exact
+                yh[j] = (double)(u32)(y[j] >> 30);                // high 30 bits, exact
             }
             if (P.exact) {
                 const double prod = u64_to_f64_rn(y[j]) * s->inv;
@@ -81,7 +84,23 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
     for (u32 t = t0; t < t1; ++t) {
         const ConvTgt* g = T.tgt + P.tgt_first + t;
         u64 o;
-        if (g->fp) {
+        if (g->mode == TGT_SPLIT) {
+            // y_j = yh 2^30 + yl: y_j c = yh (2^30 c) + yl c (mod r), four terms between reductions.
+            const double r = g->rf, rinv = g->rinv;
+            double acc = 0.0;
+#pragma unroll
+            for (u32 j = 0; j < MAXS; ++j) {
+                if (j < P.nsrc) {
+                    acc = acc + cv_fmulmod(yd[j], g->qhat_f[j], g->qhat_q[j], r);
+                    acc = acc + cv_fmulmod(yh[j], g->qhat30_f[j], g->qhat30_q[j], r);
+                    if (j & 1) acc = __builtin_fma(-__builtin_rint(acc * rinv), r, acc);
+                }
+            }
+            if (P.exact) acc = acc - cv_fmulmod((double)e, g->qmod_f, g->qmod_q, r);
+            acc = acc - g->hmod_f;
+            acc = __builtin_fma(-__builtin_rint(acc * rinv), r, acc);
+            o = cv_d2u(acc < 0.0 ? acc + r : acc);
+        } else if (g->mode == TGT_FP) {
             const double r = g->rf, rinv = g->rinv;
             double acc = 0.0;
 #pragma unroll

@@ -19,6 +19,11 @@ struct NttTables {
 bool ntt_fast_supported(u32 log_n);
 void ntt_fast_forward(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream);
 void ntt_fast_inverse(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream);
+// Forward NTT whose row pass ends with fin[r]: out = (a - NTT(src)) * inv (+ add) instead of
+// storing the transform (fin is a FinishJob array, one per ref).
+struct FinishJob;
+void ntt_fast_forward_finish(const NttTables& t, const LimbRef* refs, const FinishJob* fin, u32 nrefs,
+                             hipStream_t stream);
 
 // ref.dst <- NTT(ref.src) for each of the nrefs descriptors (device array). src may equal
 // dst (in place). Inputs reduced, outputs reduced.

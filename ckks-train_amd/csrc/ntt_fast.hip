@@ -19,6 +19,7 @@
 //    values as f64 bits; the last stage writes canonical residues.
 //  * q >= 2^50 (IntAr): Harvey butterflies with 64-bit Shoup products, values in [0, 4q)
 //    (forward) or [0, 2q) (inverse), as in ntt.hip.
+#include "basis.h"
 #include "common.h"
 #include "ntt.h"
 
@@ -177,7 +178,8 @@ CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
 }
 
 template <u32 M, class Ar>
-CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out, u64* lds, u32 log_n) {
+CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out, u64* lds, u32 log_n,
+                      const FinishJob* fin, u64 q) {
     constexpr u32 R2 = M - 4, U = 4 - R2, TRL = TL - M, JM = (1u << R2) - 1;
     const u32 A = log_n - M;
     const u32 tid = threadIdx.x, o = tid & JM, rr = tid >> R2;
@@ -221,6 +223,18 @@ CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
     }
     __syncthreads();
     const u64 tbase = (u64)blockIdx.x << TL;
+    if (fin) {
+        // finish of rescale / mod_down: (a - t) * inv (+ add)
+        const u64 inv = fin->inv, inv_s = fin->inv_s;
+#pragma unroll
+        for (u32 k = 0; k < 16; ++k) {
+            const u64 i = tbase + tid + k * NT;
+            u64 v = shoup(sub_mod(fin->a[i], lds[pad(tid + k * NT)], q), inv, inv_s, q);
+            if (fin->add) v = add_mod(v, fin->add[i], q);
+            fin->out[i] = v;
+        }
+        return;
+    }
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) {
         const u32 idx = tid + k * NT;
@@ -349,9 +363,11 @@ __global__ __launch_bounds__(NT) void k_fwd_cols(const LimbRef* __restrict__ ref
 
 template <u32 M>
 __global__ __launch_bounds__(NT) void k_fwd_rows(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
-                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n) {
+                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n,
+                                                 const FinishJob* __restrict__ fins) {
     __shared__ u64 lds[LDS_WORDS];
-    CKH_AR_DISPATCH((fwd_rows<M>(ar, ref.dst, ref.dst, lds, log_n)))
+    const FinishJob* fin = fins ? fins + blockIdx.y : nullptr;
+    CKH_AR_DISPATCH((fwd_rows<M>(ar, ref.dst, ref.dst, lds, log_n, fin, pc->q)))
 }
 
 template <u32 M>
@@ -373,23 +389,26 @@ __global__ __launch_bounds__(NT) void k_inv_cols(const LimbRef* __restrict__ ref
 bool ntt_fast_supported(u32 log_n) { return log_n >= 12 && log_n <= 16; }
 
 template <u32 M>
-static void launch_one(bool fwd, bool cols, const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s) {
+static void launch_one(bool fwd, bool cols, const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s,
+                       const FinishJob* fins = nullptr) {
     dim3 grid(1u << (t.log_n - fast::TL), nrefs);
     if (fwd && cols)
         hipLaunchKernelGGL(fast::k_fwd_cols<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.log_n);
     else if (fwd)
-        hipLaunchKernelGGL(fast::k_fwd_rows<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.log_n);
+        hipLaunchKernelGGL(fast::k_fwd_rows<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.log_n,
+                           fins);
     else if (cols)
         hipLaunchKernelGGL(fast::k_inv_cols<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.log_n);
     else
         hipLaunchKernelGGL(fast::k_inv_rows<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.log_n);
 }
 
-static void launch_m(u32 m, bool fwd, bool cols, const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s) {
+static void launch_m(u32 m, bool fwd, bool cols, const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s,
+                     const FinishJob* fins = nullptr) {
     switch (m) {
-        case 6: launch_one<6>(fwd, cols, t, refs, nrefs, s); break;
-        case 7: launch_one<7>(fwd, cols, t, refs, nrefs, s); break;
-        default: launch_one<8>(fwd, cols, t, refs, nrefs, s); break;
+        case 6: launch_one<6>(fwd, cols, t, refs, nrefs, s, fins); break;
+        case 7: launch_one<7>(fwd, cols, t, refs, nrefs, s, fins); break;
+        default: launch_one<8>(fwd, cols, t, refs, nrefs, s, fins); break;
     }
 }
 
@@ -398,6 +417,13 @@ void ntt_fast_forward(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStr
     const u32 B = t.log_n / 2, A = t.log_n - B;
     launch_m(A, true, true, t, refs, nrefs, s);
     launch_m(B, true, false, t, refs, nrefs, s);
+}
+
+void ntt_fast_forward_finish(const NttTables& t, const LimbRef* refs, const FinishJob* fin, u32 nrefs,
+                             hipStream_t s) {
+    const u32 B = t.log_n / 2, A = t.log_n - B;
+    launch_m(A, true, true, t, refs, nrefs, s);
+    launch_m(B, true, false, t, refs, nrefs, s, fin);
 }
 
 void ntt_fast_inverse(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s) {

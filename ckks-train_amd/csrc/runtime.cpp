@@ -233,9 +233,9 @@ void Context::build_tables() {
                 t.qmod = hm::prod_mod(sq.data(), sq.size(), r);
                 t.qmod_s = hm::shoup(t.qmod, r);
                 t.hmod = hm::half_mod(sq.data(), sq.size(), r);
-                bool fp = r < FP_PRIME_BOUND;
-                for (u64 sqj : sq) fp = fp && sqj < FP_PRIME_BOUND;
-                t.fp = fp;
+                bool srcs_fp = true;
+                for (u64 sqj : sq) srcs_fp = srcs_fp && sqj < FP_PRIME_BOUND;
+                t.mode = r >= FP_PRIME_BOUND ? TGT_INT : (srcs_fp ? TGT_FP : TGT_SPLIT);
                 t.rf = (double)r;
                 t.rinv = 1.0 / (double)r;
                 t.qmod_f = (double)t.qmod;
@@ -248,6 +248,9 @@ void Context::build_tables() {
                     t.qhat[j] = qh;
                     t.qhat_f[j] = (double)qh;
                     t.qhat_q[j] = (double)qh / (double)r;
+                    const u64 qh30 = hm::mul(qh, (1ull << 30) % r, r);
+                    t.qhat30_f[j] = (double)qh30;
+                    t.qhat30_q[j] = (double)qh30 / (double)r;
                 }
                 tgts.push_back(t);
             }
@@ -468,8 +471,19 @@ void Context::mod_down(const std::vector<MdTask>& tasks, hipStream_t s) {
     BufPtr d = upload_blob(b.bytes.data(), b.bytes.size(), s);
     launch_ntt(*this, at<LimbRef>(d, o_intt), (u32)intt.size(), false, s);
     launch_conv(conv, at<ConvJob>(d, o_jobs), (u32)jobs.size(), max_l, np, s);
-    launch_ntt(*this, at<LimbRef>(d, o_ntt), (u32)nttr.size(), true, s);
-    launch_finish(at<FinishJob>(d, o_fin), (u32)fin.size(), d_pc, log_n, s);
+    launch_ntt_finish(at<LimbRef>(d, o_ntt), at<FinishJob>(d, o_fin), (u32)nttr.size(), s);
+}
+
+// NTT of the converted limbs followed by the finish; fused into the row pass where the
+// register-radix NTT applies.
+void Context::launch_ntt_finish(const LimbRef* refs, const FinishJob* fin, u32 n_refs, hipStream_t s) {
+    if (ntt_fast_supported(log_n)) {
+        for (u32 i = 0; i < n_refs; i += MAX_ROWS)
+            ntt_fast_forward_finish(ntt, refs + i, fin + i, std::min(MAX_ROWS, n_refs - i), s);
+        return;
+    }
+    launch_ntt(*this, refs, n_refs, true, s);
+    launch_finish(fin, n_refs, d_pc, log_n, s);
 }
 
 // RnsPoly::rescale_by_last in NTT form (baseconv.rs:258-291) for a batch.
@@ -493,8 +507,7 @@ void Context::rescale(const std::vector<RescaleTask>& tasks, hipStream_t s) {
     BufPtr d = upload_blob(b.bytes.data(), b.bytes.size(), s);
     launch_ntt(*this, at<LimbRef>(d, o_intt), (u32)intt.size(), false, s);
     launch_conv(conv, at<ConvJob>(d, o_jobs), (u32)jobs.size(), max_t, 1, s);
-    launch_ntt(*this, at<LimbRef>(d, o_ntt), (u32)nttr.size(), true, s);
-    launch_finish(at<FinishJob>(d, o_fin), (u32)fin.size(), d_pc, log_n, s);
+    launch_ntt_finish(at<LimbRef>(d, o_ntt), at<FinishJob>(d, o_fin), (u32)nttr.size(), s);
 }
 
 }  // namespace ckh

@@ -154,6 +154,7 @@ class Context : public std::enable_shared_from_this<Context> {
     void rescale(const std::vector<RescaleTask>& tasks, hipStream_t s);
     void ntt_poly(const u64* in, u64* out, u32 limbs, u32 level, u32 p_limbs, bool forward, hipStream_t s);
 
+    void launch_ntt_finish(const LimbRef* refs, const FinishJob* fin, u32 n_refs, hipStream_t s);
     // Copies a host blob to a fresh device buffer on stream s.
     BufPtr upload_blob(const void* host, size_t bytes, hipStream_t s);
 
