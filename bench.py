@@ -95,7 +95,7 @@ def cpu_baseline(q, p, levels, budget_s=12.0):
                       f"key_switch with {threads} OpenMP threads, {el:.1f} s"}
 
 
-def served_stream(ckh, n_ops=200, rate=1500.0, seed=SEED + 5):
+def served_stream(ckh, n_ops=240, rate=4000.0, seed=SEED + 5):
     """The served mixed stream: Poisson arrivals (rate ops/s) over N = 2^16 (24 + 6 primes,
     dnum 4) and N = 2^15 (20 + 4 primes, dnum 5); 50% relinearize, 30% hoisted 16-rotation
     batch, 20% mul + relinearize + rescale; limbs U{4..24} (at most 20 at 2^15); 64 rotation
@@ -206,6 +206,8 @@ def main():
     stream = ctx.stream_handle()
     ev = HipEvents()
     levels = workload_levels()
+    tuning = ctx.autotune()
+    print(f"# autotune: {json.dumps(tuning)}", file=sys.stderr, flush=True)
     key = ctx.ksk_uniform(23, seed=SEED + 1)
     polys = [ctx.uniform(lv, seed=SEED + 100 + i) for i, lv in enumerate(levels)]
     ctx.synchronize()
@@ -276,6 +278,12 @@ def main():
     ntt_bytes = 16 * N * limbs
     achieved = ntt_bytes / (ntt_ms * 1e-3) / 1e9
     del tpolys
+    # the same launch over data primes only (240 x 50-bit limbs, the f64 path)
+    qpolys = [ctx.uniform(23, seed=SEED + 19 + i, ntt=False) for i in range(10)]
+    ntt_q_ms = ctx.time_ntt(qpolys, forward=True, reps=20)
+    ntt_q = {"limbs": 240, "ms_per_launch": ntt_q_ms, "GBps": 16 * N * 240 / (ntt_q_ms * 1e-3) / 1e9}
+    ntt_q["frac"] = ntt_q["GBps"] / HBM_PEAK_GBS
+    del qpolys
 
     del polys, key, top
     ctx.synchronize()
@@ -317,6 +325,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "ms_per_launch": ntt_ms, "bytes_per_launch": ntt_bytes,
         },
+        "ntt_forward_q_primes_only": ntt_q,
+        "tuning": tuning,
         "cpu_baseline": cpu,
         "served_stream": serve,
     }

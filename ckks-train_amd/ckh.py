@@ -95,6 +95,8 @@ _SIGS = {
     "ckh_set_stream": (C.c_int, [VP, u32]),
     "ckh_stream_handle": (VP, [VP]),
     "ckh_synchronize": (C.c_int, [VP]),
+    "ckh_autotune": (C.c_int, [VP, C.c_char_p, C.c_size_t]),
+    "ckh_set_tuning": (C.c_int, [VP, C.c_char_p, C.c_int64]),
     "ckh_memory_info": (C.c_int, [VP, P64, P64]),
     "ckh_poly_upload": (C.c_int, [VP, P64, u32, u32, C.c_int, C.POINTER(VP)]),
     "ckh_poly_download": (C.c_int, [VP, VP, P64]),
@@ -234,6 +236,15 @@ class Context:
 
     def synchronize(self):
         _check(lib().ckh_synchronize(self.h))
+
+    def autotune(self):
+        import json
+        buf = C.create_string_buffer(8192)
+        _check(lib().ckh_autotune(self.h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def set_tuning(self, key, value):
+        _check(lib().ckh_set_tuning(self.h, key.encode(), value))
 
     def set_stream(self, i):
         _check(lib().ckh_set_stream(self.h, i))

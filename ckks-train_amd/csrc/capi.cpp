@@ -161,6 +161,23 @@ ckh_status ckh_set_stream(ckh_context* ctx, uint32_t index) {
 
 void* ckh_stream_handle(ckh_context* ctx) { return ctx && ctx->c ? (void*)ctx->c->stream() : nullptr; }
 
+ckh_status ckh_autotune(ckh_context* ctx, char* report, size_t report_len) {
+    return guarded([&] {
+        std::string r = ctx_of(ctx).autotune();
+        if (report && report_len) {
+            std::strncpy(report, r.c_str(), report_len - 1);
+            report[report_len - 1] = 0;
+        }
+    });
+}
+
+ckh_status ckh_set_tuning(ckh_context* ctx, const char* key, int64_t value) {
+    return guarded([&] {
+        need(key, "key");
+        ctx_of(ctx).set_tuning(key, value);
+    });
+}
+
 ckh_status ckh_synchronize(ckh_context* ctx) {
     return guarded([&] {
         Context& c = ctx_of(ctx);

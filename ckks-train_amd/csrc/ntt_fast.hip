@@ -135,11 +135,12 @@ struct IntAr {
 // R2 low bits vary, plus U of the top bits).
 
 template <u32 M, class Ar>
-CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out, u64* lds, u32 log_n) {
+CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out, u64* lds, u32 log_n,
+                      u32 tile) {
     constexpr u32 R2 = M - 4, U = 4 - R2, TCL = TL - M, TC = 1u << TCL, JM = (1u << R2) - 1;
     const u32 B = log_n - M;
     const u32 tid = threadIdx.x, cc = tid & (TC - 1), o = tid >> TCL;
-    const u32 c0 = blockIdx.x * TC;
+    const u32 c0 = tile * TC;
     typename Ar::V v[16];
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) v[k] = ar.from_canon(in[((u64)(o + (k << R2)) << B) + c0 + cc]);
@@ -179,11 +180,11 @@ CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
 
 template <u32 M, class Ar>
 CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out, u64* lds, u32 log_n,
-                      const FinishJob* fin, u64 q) {
+                      const FinishJob* fin, u64 q, u32 tile) {
     constexpr u32 R2 = M - 4, U = 4 - R2, TRL = TL - M, JM = (1u << R2) - 1;
     const u32 A = log_n - M;
     const u32 tid = threadIdx.x, o = tid & JM, rr = tid >> R2;
-    const u32 row = (blockIdx.x << TRL) + rr;
+    const u32 row = (tile << TRL) + rr;
     const u64 base = (u64)row << M;
     typename Ar::V v[16];
 #pragma unroll
@@ -222,7 +223,7 @@ CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
         lds[pad((rr << M) + c)] = ar.fwd_final(v[e]);
     }
     __syncthreads();
-    const u64 tbase = (u64)blockIdx.x << TL;
+    const u64 tbase = (u64)tile << TL;
     if (fin) {
         // finish of rescale / mod_down: (a - t) * inv (+ add)
         const u64 inv = fin->inv, inv_s = fin->inv_s;
@@ -243,11 +244,12 @@ CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
 }
 
 template <u32 M, class Ar>
-CKH_DEV void inv_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out, u64* lds, u32 log_n) {
+CKH_DEV void inv_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out, u64* lds, u32 log_n,
+                      u32 tile) {
     constexpr u32 R2 = M - 4, U = 4 - R2, TRL = TL - M, JM = (1u << R2) - 1;
     const u32 tid = threadIdx.x, o = tid & JM, rr = tid >> R2;
-    const u32 row = (blockIdx.x << TRL) + rr;
-    const u64 tbase = (u64)blockIdx.x << TL;
+    const u32 row = (tile << TRL) + rr;
+    const u64 tbase = (u64)tile << TL;
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) {
         const u32 idx = tid + k * NT;
@@ -294,11 +296,11 @@ CKH_DEV void inv_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
 }
 
 template <u32 M, class Ar>
-CKH_DEV void inv_cols(const Ar& ar, u64* __restrict__ io, u64* lds, u32 log_n) {
+CKH_DEV void inv_cols(const Ar& ar, u64* __restrict__ io, u64* lds, u32 log_n, u32 tile) {
     constexpr u32 R2 = M - 4, U = 4 - R2, TCL = TL - M, TC = 1u << TCL, JM = (1u << R2) - 1;
     const u32 B = log_n - M;
     const u32 tid = threadIdx.x, cc = tid & (TC - 1), o = tid >> TCL;
-    const u32 c0 = blockIdx.x * TC;
+    const u32 c0 = tile * TC;
     typename Ar::V v[16];
 #pragma unroll
     for (u32 e = 0; e < 16; ++e) {
@@ -342,8 +344,25 @@ CKH_DEV void inv_cols(const Ar& ar, u64* __restrict__ io, u64* lds, u32 log_n) {
 
 // ---------------------------------------------------------------- kernels
 
+// Block -> (limb, tile): workgroups are dealt round-robin to the 8 XCDs by linear id, so the
+// tiles of one limb are given ids that land on one XCD (limbs in groups of 8); the column and
+// row passes of a limb then run on the same XCD and the intermediate can stay in its L2.
+CKH_DEV void xcd_map(u32 b, u32 tiles, u32 nrefs, u32& ref, u32& tile) {
+    const u32 full = (nrefs / 8) * 8 * tiles;
+    if (b < full) {
+        const u32 xcd = b & 7, slot = b >> 3;
+        ref = (slot / tiles) * 8 + xcd;
+        tile = slot % tiles;
+    } else {
+        ref = (nrefs / 8) * 8 + (b - full) / tiles;
+        tile = (b - full) % tiles;
+    }
+}
+
 #define CKH_AR_DISPATCH(CALL)                                                                   \
-    const LimbRef ref = refs[blockIdx.y];                                                       \
+    u32 ref_i, tile;                                                                            \
+    xcd_map(blockIdx.x, 1u << (log_n - TL), nrefs, ref_i, tile);                                \
+    const LimbRef ref = refs[ref_i];                                                            \
     const PrimeConst* pc = pcs + ref.prime;                                                     \
     const u64 toff = (u64)ref.prime << log_n;                                                   \
     if (pc->q < FP_PRIME_BOUND) {                                                               \
@@ -356,32 +375,34 @@ CKH_DEV void inv_cols(const Ar& ar, u64* __restrict__ io, u64* lds, u32 log_n) {
 
 template <u32 M>
 __global__ __launch_bounds__(NT) void k_fwd_cols(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
-                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n) {
+                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n,
+                                                 u32 nrefs) {
     __shared__ u64 lds[LDS_WORDS];
-    CKH_AR_DISPATCH((fwd_cols<M>(ar, ref.src, ref.dst, lds, log_n)))
+    CKH_AR_DISPATCH((fwd_cols<M>(ar, ref.src, ref.dst, lds, log_n, tile)))
 }
 
 template <u32 M>
 __global__ __launch_bounds__(NT) void k_fwd_rows(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
                                                  const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n,
-                                                 const FinishJob* __restrict__ fins) {
+                                                 u32 nrefs, const FinishJob* __restrict__ fins) {
     __shared__ u64 lds[LDS_WORDS];
-    const FinishJob* fin = fins ? fins + blockIdx.y : nullptr;
-    CKH_AR_DISPATCH((fwd_rows<M>(ar, ref.dst, ref.dst, lds, log_n, fin, pc->q)))
+    CKH_AR_DISPATCH((fwd_rows<M>(ar, ref.dst, ref.dst, lds, log_n, fins ? fins + ref_i : nullptr, pc->q, tile)))
 }
 
 template <u32 M>
 __global__ __launch_bounds__(NT) void k_inv_rows(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
-                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n) {
+                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n,
+                                                 u32 nrefs) {
     __shared__ u64 lds[LDS_WORDS];
-    CKH_AR_DISPATCH((inv_rows<M>(ar, ref.src, ref.dst, lds, log_n)))
+    CKH_AR_DISPATCH((inv_rows<M>(ar, ref.src, ref.dst, lds, log_n, tile)))
 }
 
 template <u32 M>
 __global__ __launch_bounds__(NT) void k_inv_cols(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
-                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n) {
+                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n,
+                                                 u32 nrefs) {
     __shared__ u64 lds[LDS_WORDS];
-    CKH_AR_DISPATCH((inv_cols<M>(ar, ref.dst, lds, log_n)))
+    CKH_AR_DISPATCH((inv_cols<M>(ar, ref.dst, lds, log_n, tile)))
 }
 
 }  // namespace fast
@@ -391,16 +412,17 @@ bool ntt_fast_supported(u32 log_n) { return log_n >= 12 && log_n <= 16; }
 template <u32 M>
 static void launch_one(bool fwd, bool cols, const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s,
                        const FinishJob* fins = nullptr) {
-    dim3 grid(1u << (t.log_n - fast::TL), nrefs);
+    dim3 grid((1u << (t.log_n - fast::TL)) * nrefs);
     if (fwd && cols)
-        hipLaunchKernelGGL(fast::k_fwd_cols<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.log_n);
+        hipLaunchKernelGGL(fast::k_fwd_cols<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.log_n, nrefs);
     else if (fwd)
-        hipLaunchKernelGGL(fast::k_fwd_rows<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.log_n,
+        hipLaunchKernelGGL(fast::k_fwd_rows<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.log_n, nrefs,
                            fins);
     else if (cols)
-        hipLaunchKernelGGL(fast::k_inv_cols<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.log_n);
+        hipLaunchKernelGGL(fast::k_inv_cols<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.log_n, nrefs);
     else
-        hipLaunchKernelGGL(fast::k_inv_rows<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.log_n);
+        hipLaunchKernelGGL(fast::k_inv_rows<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.log_n,
+                           nrefs);
 }
 
 static void launch_m(u32 m, bool fwd, bool cols, const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s,

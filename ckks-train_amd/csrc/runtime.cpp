@@ -320,26 +320,14 @@ void Context::build_tables() {
 // ------------------------------------------------------------------ descriptor blobs
 
 namespace {
-struct Blob {
-    std::vector<uint8_t> bytes;
-    template <class T>
-    size_t add(const std::vector<T>& v) {
-        size_t off = (bytes.size() + 15) & ~(size_t)15;
-        bytes.resize(off + v.size() * sizeof(T));
-        if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
-        return off;
-    }
-};
-template <class T>
-const T* at(const BufPtr& b, size_t off) {
-    return (const T*)((const uint8_t*)b->ptr + off);
-}
 constexpr u32 MAX_ROWS = 65535;
 }  // namespace
 
 void launch_ntt(const Context& c, const LimbRef* d_refs, u32 nrefs, bool forward, hipStream_t s) {
-    for (u32 i = 0; i < nrefs; i += MAX_ROWS) {
-        u32 cnt = std::min(MAX_ROWS, nrefs - i);
+    u32 chunk = forward ? c.ntt_chunk_fwd : c.ntt_chunk_inv;
+    chunk = (chunk == 0 || chunk > MAX_ROWS) ? MAX_ROWS : chunk;
+    for (u32 i = 0; i < nrefs; i += chunk) {
+        u32 cnt = std::min(chunk, nrefs - i);
         if (forward)
             ntt_forward(c.ntt, d_refs + i, cnt, s);
         else
@@ -364,6 +352,45 @@ static void launch_finish(const FinishJob* d, u32 njobs, const PrimeConst* pc, u
 }
 
 // ------------------------------------------------------------------ engine
+//
+// Each stage builds its descriptors into a Program (host bytes + steps); execute() copies the
+// bytes to the device in one transfer and launches the steps. A Program's steps depend only on
+// the shapes of its tasks, so a captured launch sequence can be replayed with new bytes.
+
+template <class T>
+size_t Program::add(const std::vector<T>& v) {
+    size_t off = (bytes.size() + 15) & ~(size_t)15;
+    bytes.resize(off + v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+}
+
+void Program::step(StepKind k, size_t off, u32 count, size_t off2, u32 a, u32 b) {
+    if (count) steps.push_back({k, count, off, off2, a, b});
+}
+
+void Context::run(const Program& p, const uint8_t* d, hipStream_t s) const {
+    for (const Program::Step& st : p.steps) {
+        const void* x = d + st.off;
+        switch (st.kind) {
+            case S_NTT_FWD: launch_ntt(*this, (const LimbRef*)x, st.count, true, s); break;
+            case S_NTT_INV: launch_ntt(*this, (const LimbRef*)x, st.count, false, s); break;
+            case S_CONV: launch_conv(conv, (const ConvJob*)x, st.count, st.a, st.b, s); break;
+            case S_COPY: launch_copy((const LimbRef*)x, st.count, log_n, s); break;
+            case S_IP: launch_ip((const IpJob*)x, st.count, d_pc, log_n, s); break;
+            case S_NTT_FWD_FIN:
+                launch_ntt_finish((const LimbRef*)x, (const FinishJob*)(d + st.off2), st.count, s);
+                break;
+            case S_FINISH: launch_finish((const FinishJob*)x, st.count, d_pc, log_n, s); break;
+        }
+    }
+}
+
+void Context::execute(const Program& p, hipStream_t s) {
+    if (p.steps.empty()) return;
+    BufPtr d = upload_blob(p.bytes.data(), p.bytes.size(), s);
+    run(p, (const uint8_t*)d->ptr, s);
+}
 
 void Context::ntt_poly(const u64* in, u64* out, u32 limbs, u32 level, u32 p_limbs, bool forward, hipStream_t s) {
     std::vector<LimbRef> refs;
@@ -372,16 +399,15 @@ void Context::ntt_poly(const u64* in, u64* out, u32 limbs, u32 level, u32 p_limb
         refs.push_back({in + i * n, out + i * n, prime, 0});
     }
     (void)p_limbs;
-    Blob b;
-    size_t o = b.add(refs);
-    BufPtr d = upload_blob(b.bytes.data(), b.bytes.size(), s);
-    launch_ntt(*this, at<LimbRef>(d, o), (u32)refs.size(), forward, s);
+    Program p;
+    p.step(forward ? S_NTT_FWD : S_NTT_INV, p.add(refs), (u32)refs.size());
+    execute(p, s);
 }
 
 // keyswitch::decompose (keyswitch.rs:82-124) for a batch: INTT of every input limb, the
 // centered exact conversion of every digit to its targets, the digit limbs copied, then the
 // NTT of every converted limb (lift_digit_exact, keyswitch.rs:140-210).
-void Context::decompose(const std::vector<DecTask>& tasks, hipStream_t s) {
+void Context::prog_decompose(Program& p, const std::vector<DecTask>& tasks) const {
     std::vector<LimbRef> intt, copies, nttr;
     std::vector<ConvJob> jobs;
     u32 max_tgt = 0;
@@ -400,19 +426,15 @@ void Context::decompose(const std::vector<DecTask>& tasks, hipStream_t s) {
         }
         max_tgt = std::max(max_tgt, modup_maxtgt_[t.level]);
     }
-    Blob b;
-    size_t o_intt = b.add(intt), o_jobs = b.add(jobs), o_copy = b.add(copies), o_ntt = b.add(nttr);
-    BufPtr d = upload_blob(b.bytes.data(), b.bytes.size(), s);
-    launch_ntt(*this, at<LimbRef>(d, o_intt), (u32)intt.size(), false, s);
-    launch_conv(conv, at<ConvJob>(d, o_jobs), (u32)jobs.size(), max_tgt, alpha, s);
-    launch_copy(at<LimbRef>(d, o_copy), (u32)copies.size(), log_n, s);
-    launch_ntt(*this, at<LimbRef>(d, o_ntt), (u32)nttr.size(), true, s);
+    p.step(S_NTT_INV, p.add(intt), (u32)intt.size());
+    p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_tgt, alpha);
+    p.step(S_COPY, p.add(copies), (u32)copies.size());
+    p.step(S_NTT_FWD, p.add(nttr), (u32)nttr.size());
 }
 
 // keyswitch::apply_key (keyswitch.rs:230-309) for a batch: the inner product over Q_l * P, then
-// RnsPoly::mod_down of both sums (baseconv.rs:459-511): INTT of the P limbs, the centered
-// exact conversion P -> Q_l, NTT, and (x - t) * P^-1 (+ the optional addend).
-void Context::apply(const std::vector<ApplyTask>& tasks, hipStream_t s) {
+// mod_down of both sums.
+void Context::prog_apply(Program& p, const std::vector<ApplyTask>& tasks) const {
     std::vector<IpJob> ip;
     u32 max_l = 0;
     for (const ApplyTask& t : tasks) max_l = std::max(max_l, t.level + 1);
@@ -434,10 +456,7 @@ void Context::apply(const std::vector<ApplyTask>& tasks, hipStream_t s) {
             ip.push_back({t.dig + i * n, digit_words(t.level), t.key->buf->ptr + i * n, t.key->poly_words(),
                           t.acc + i * n, t.acc + aw + i * n, t.galois, ndigits(t.level), i});
         }
-    Blob b;
-    size_t o_ip = b.add(ip);
-    BufPtr d = upload_blob(b.bytes.data(), b.bytes.size(), s);
-    launch_ip(at<IpJob>(d, o_ip), (u32)ip.size(), d_pc, log_n, s);
+    p.step(S_IP, p.add(ip), (u32)ip.size());
     std::vector<MdTask> md;
     for (const ApplyTask& t : tasks) {
         const u32 L = t.level + 1;
@@ -445,12 +464,12 @@ void Context::apply(const std::vector<ApplyTask>& tasks, hipStream_t s) {
         md.push_back({t.acc, t.level, t.out_b, t.add_b, t.y, t.z});
         md.push_back({t.acc + aw, t.level, t.out_a, t.add_a, t.y + (u64)np * n, t.z + (u64)L * n});
     }
-    mod_down(md, s);
+    prog_mod_down(p, md);
 }
 
 // RnsPoly::mod_down (baseconv.rs:459-511) for a batch: INTT of the P limbs, the centered exact
 // conversion P -> Q_l, NTT, and (x - t) * P^-1 (+ the optional addend).
-void Context::mod_down(const std::vector<MdTask>& tasks, hipStream_t s) {
+void Context::prog_mod_down(Program& p, const std::vector<MdTask>& tasks) const {
     std::vector<LimbRef> intt, nttr;
     std::vector<ConvJob> jobs;
     std::vector<FinishJob> fin;
@@ -466,28 +485,14 @@ void Context::mod_down(const std::vector<MdTask>& tasks, hipStream_t s) {
             fin.push_back({t.acc + i * n, zi, t.out + i * n, t.add ? t.add + i * n : nullptr, pinv_[i], pinv_s_[i], i, 0});
         }
     }
-    Blob b;
-    size_t o_intt = b.add(intt), o_jobs = b.add(jobs), o_ntt = b.add(nttr), o_fin = b.add(fin);
-    BufPtr d = upload_blob(b.bytes.data(), b.bytes.size(), s);
-    launch_ntt(*this, at<LimbRef>(d, o_intt), (u32)intt.size(), false, s);
-    launch_conv(conv, at<ConvJob>(d, o_jobs), (u32)jobs.size(), max_l, np, s);
-    launch_ntt_finish(at<LimbRef>(d, o_ntt), at<FinishJob>(d, o_fin), (u32)nttr.size(), s);
-}
-
-// NTT of the converted limbs followed by the finish; fused into the row pass where the
-// register-radix NTT applies.
-void Context::launch_ntt_finish(const LimbRef* refs, const FinishJob* fin, u32 n_refs, hipStream_t s) {
-    if (ntt_fast_supported(log_n)) {
-        for (u32 i = 0; i < n_refs; i += MAX_ROWS)
-            ntt_fast_forward_finish(ntt, refs + i, fin + i, std::min(MAX_ROWS, n_refs - i), s);
-        return;
-    }
-    launch_ntt(*this, refs, n_refs, true, s);
-    launch_finish(fin, n_refs, d_pc, log_n, s);
+    p.step(S_NTT_INV, p.add(intt), (u32)intt.size());
+    p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_l, np);
+    const size_t o_ntt = p.add(nttr), o_fin = p.add(fin);
+    p.step(S_NTT_FWD_FIN, o_ntt, (u32)nttr.size(), o_fin);
 }
 
 // RnsPoly::rescale_by_last in NTT form (baseconv.rs:258-291) for a batch.
-void Context::rescale(const std::vector<RescaleTask>& tasks, hipStream_t s) {
+void Context::prog_rescale(Program& p, const std::vector<RescaleTask>& tasks) const {
     std::vector<LimbRef> intt, nttr;
     std::vector<ConvJob> jobs;
     std::vector<FinishJob> fin;
@@ -502,12 +507,47 @@ void Context::rescale(const std::vector<RescaleTask>& tasks, hipStream_t s) {
         }
         max_t = std::max(max_t, l);
     }
-    Blob b;
-    size_t o_intt = b.add(intt), o_jobs = b.add(jobs), o_ntt = b.add(nttr), o_fin = b.add(fin);
-    BufPtr d = upload_blob(b.bytes.data(), b.bytes.size(), s);
-    launch_ntt(*this, at<LimbRef>(d, o_intt), (u32)intt.size(), false, s);
-    launch_conv(conv, at<ConvJob>(d, o_jobs), (u32)jobs.size(), max_t, 1, s);
-    launch_ntt_finish(at<LimbRef>(d, o_ntt), at<FinishJob>(d, o_fin), (u32)nttr.size(), s);
+    p.step(S_NTT_INV, p.add(intt), (u32)intt.size());
+    p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_t, 1);
+    const size_t o_ntt = p.add(nttr), o_fin = p.add(fin);
+    p.step(S_NTT_FWD_FIN, o_ntt, (u32)nttr.size(), o_fin);
+}
+
+void Context::decompose(const std::vector<DecTask>& tasks, hipStream_t s) {
+    Program p;
+    prog_decompose(p, tasks);
+    execute(p, s);
+}
+
+void Context::apply(const std::vector<ApplyTask>& tasks, hipStream_t s) {
+    Program p;
+    prog_apply(p, tasks);
+    execute(p, s);
+}
+
+void Context::mod_down(const std::vector<MdTask>& tasks, hipStream_t s) {
+    Program p;
+    prog_mod_down(p, tasks);
+    execute(p, s);
+}
+
+void Context::rescale(const std::vector<RescaleTask>& tasks, hipStream_t s) {
+    Program p;
+    prog_rescale(p, tasks);
+    execute(p, s);
+}
+
+// NTT of the converted limbs followed by the finish; fused into the row pass where the
+// register-radix NTT applies.
+void Context::launch_ntt_finish(const LimbRef* refs, const FinishJob* fin, u32 n_refs, hipStream_t s) const {
+    if (ntt_fast_supported(log_n)) {
+        const u32 chunk = (ntt_chunk_fwd == 0 || ntt_chunk_fwd > MAX_ROWS) ? MAX_ROWS : ntt_chunk_fwd;
+        for (u32 i = 0; i < n_refs; i += chunk)
+            ntt_fast_forward_finish(ntt, refs + i, fin + i, std::min(chunk, n_refs - i), s);
+        return;
+    }
+    launch_ntt(*this, refs, n_refs, true, s);
+    launch_finish(fin, n_refs, d_pc, log_n, s);
 }
 
 }  // namespace ckh

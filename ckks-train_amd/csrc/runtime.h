@@ -116,6 +116,22 @@ struct RescaleTask {
     u64* z;
 };
 
+// Descriptor bytes plus the launch steps that read them (see runtime.cpp, "engine").
+enum StepKind : uint8_t { S_NTT_FWD, S_NTT_INV, S_CONV, S_COPY, S_IP, S_NTT_FWD_FIN, S_FINISH };
+struct Program {
+    struct Step {
+        StepKind kind;
+        u32 count;
+        size_t off, off2;
+        u32 a, b;
+    };
+    std::vector<uint8_t> bytes;
+    std::vector<Step> steps;
+    template <class T>
+    size_t add(const std::vector<T>& v);
+    void step(StepKind k, size_t off, u32 count, size_t off2 = 0, u32 a = 0, u32 b = 0);
+};
+
 class Context : public std::enable_shared_from_this<Context> {
    public:
     Context(int ordinal, const ckh_params& p, u32 n_streams);
@@ -147,14 +163,26 @@ class Context : public std::enable_shared_from_this<Context> {
     u32 ndigits(u32 level) const;
     u64 digit_words(u32 level) const { return (u64)(level + 1 + np) * n; }
 
-    // The engine.
+    // The engine: build programs, run them (descriptors at d), or both (execute).
+    void prog_decompose(Program& p, const std::vector<DecTask>& tasks) const;
+    void prog_apply(Program& p, const std::vector<ApplyTask>& tasks) const;
+    void prog_mod_down(Program& p, const std::vector<MdTask>& tasks) const;
+    void prog_rescale(Program& p, const std::vector<RescaleTask>& tasks) const;
+    void run(const Program& p, const uint8_t* d, hipStream_t s) const;
+    void execute(const Program& p, hipStream_t s);
     void decompose(const std::vector<DecTask>& tasks, hipStream_t s);
     void apply(const std::vector<ApplyTask>& tasks, hipStream_t s);
     void mod_down(const std::vector<MdTask>& tasks, hipStream_t s);
     void rescale(const std::vector<RescaleTask>& tasks, hipStream_t s);
     void ntt_poly(const u64* in, u64* out, u32 limbs, u32 level, u32 p_limbs, bool forward, hipStream_t s);
 
-    void launch_ntt_finish(const LimbRef* refs, const FinishJob* fin, u32 n_refs, hipStream_t s);
+    void launch_ntt_finish(const LimbRef* refs, const FinishJob* fin, u32 n_refs, hipStream_t s) const;
+
+    // Tuning (autotune.cpp): limbs per column-pass/row-pass launch pair (0 = all at once). A
+    // chunk small enough keeps the pass intermediate in the on-die caches between the passes.
+    u32 ntt_chunk_fwd = 0, ntt_chunk_inv = 0;
+    std::string autotune();
+    void set_tuning(const std::string& key, int64_t value);
     // Copies a host blob to a fresh device buffer on stream s.
     BufPtr upload_blob(const void* host, size_t bytes, hipStream_t s);
 

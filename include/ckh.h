@@ -80,6 +80,12 @@ void ckh_device_close(ckh_context *ctx);
 ckh_status ckh_set_stream(ckh_context *ctx, uint32_t index);
 /* The calling thread's hipStream_t (for event timing by callers). */
 void *ckh_stream_handle(ckh_context *ctx);
+/* The project's autotuner (no counterpart in the reference): times candidate launch schedules
+ * on this device with synthetic limbs of the context's primes and keeps the fastest; writes a
+ * JSON report (candidates and choices) to report (may be NULL). */
+ckh_status ckh_autotune(ckh_context *ctx, char *report, size_t report_len);
+/* Forces one tuning value ("ntt_chunk_fwd", "ntt_chunk_inv": limbs per launch, 0 = all). */
+ckh_status ckh_set_tuning(ckh_context *ctx, const char *key, int64_t value);
 /* Waits for all work of the context. */
 ckh_status ckh_synchronize(ckh_context *ctx);
 /* Device memory: bytes in use by live handles, and free device bytes (evaluator.rs:496-519). */

@@ -377,3 +377,25 @@ def test_c4_hoisted_bsgs_full_size():
     say("c4 full size")
     O.set_threads(16)
     _bsgs_case(C4, 32, 32, c4_entries_mask(), seed=4000)
+
+
+# ---------------------------------------------------------------- tuning never changes results
+
+def test_every_ntt_chunk_gives_the_same_switch():
+    ctx, ref = make(C2_SHAPE_N4096)
+    L = len(ref.q) - 1
+    key = ctx.ksk_uniform(L, seed=31)
+    polys = [ctx.uniform(l, seed=40 + l) for l in (L, L // 2, 3, 0)]
+    want = None
+    for chunk in (0, 1, 3, 8, 64):
+        ctx.set_tuning("ntt_chunk_fwd", chunk)
+        ctx.set_tuning("ntt_chunk_inv", chunk)
+        got = [(kb.digest(), ka.digest()) for kb, ka in ctx.key_switch_batch(polys, key)]
+        want = want or got
+        assert got == want, chunk
+    rep = ctx.autotune()
+    assert rep["ntt_chunk_fwd"] in (0, 8, 16, 32, 64, 128, 256)
+    got = [(kb.digest(), ka.digest()) for kb, ka in ctx.key_switch_batch(polys, key)]
+    assert got == want
+    with pytest.raises(ckh.InvalidParams):
+        ctx.set_tuning("no_such_knob", 1)
