@@ -185,6 +185,30 @@ def served_stream(ckh, n_ops=240, rate=4000.0, seed=SEED + 5):
     }
 
 
+def roofline_only(ctx, ev, reps=3):
+    """The roofline launch alone (forward NTT over 8 polynomials x 30 limbs), then a
+    device-to-device copy of the same byte count: the copy calibrates the HBM byte counters
+    (its algorithmic traffic is exactly bytes read + bytes written)."""
+    tpolys = [ctx.uniform(23, seed=SEED + 9 + i, p_limbs=6, ntt=False) for i in range(8)]
+    ctx.synchronize()
+    ms = ctx.time_ntt(tpolys, forward=True, reps=reps)
+    nbytes = 8 * N * 8 * 30
+    h = ev.h
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    h.hipFree.argtypes = [C.c_void_p]
+    h.hipDeviceSynchronize.argtypes = []
+    a, b = C.c_void_p(), C.c_void_p()
+    assert h.hipMalloc(C.byref(a), nbytes) == 0 and h.hipMalloc(C.byref(b), nbytes) == 0
+    for _ in range(reps):
+        assert h.hipMemcpy(b, a, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+    assert h.hipDeviceSynchronize() == 0
+    h.hipFree(a)
+    h.hipFree(b)
+    print(json.dumps({"ntt_forward_limbs": 240, "ntt_ms_per_launch": ms, "ntt_bytes_alg": 16 * N * 240,
+                      "copy_bytes": nbytes, "copy_reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +216,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="run only the dominant-kernel launch and a device copy of the same size (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the key-switch backend shards by ciphertext with no "
@@ -205,6 +231,9 @@ def main():
     ctx = ckh.Context(LOG_N, q, p, DNUM)
     stream = ctx.stream_handle()
     ev = HipEvents()
+    if args.roofline_only:
+        roofline_only(ctx, ev)
+        return
     levels = workload_levels()
     tuning = ctx.autotune()
     print(f"# autotune: {json.dumps(tuning)}", file=sys.stderr, flush=True)
@@ -327,6 +356,7 @@ def main():
         },
         "ntt_forward_q_primes_only": ntt_q,
         "tuning": tuning,
+        "graphs": dict(zip(("captured", "replays"), ctx.graph_stats())),
         "cpu_baseline": cpu,
         "served_stream": serve,
     }

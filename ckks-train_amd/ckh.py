@@ -98,6 +98,7 @@ _SIGS = {
     "ckh_autotune": (C.c_int, [VP, C.c_char_p, C.c_size_t]),
     "ckh_set_tuning": (C.c_int, [VP, C.c_char_p, C.c_int64]),
     "ckh_memory_info": (C.c_int, [VP, P64, P64]),
+    "ckh_graph_stats": (C.c_int, [VP, P64, P64]),
     "ckh_poly_upload": (C.c_int, [VP, P64, u32, u32, C.c_int, C.POINTER(VP)]),
     "ckh_poly_download": (C.c_int, [VP, VP, P64]),
     "ckh_poly_info": (C.c_int, [VP, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_int)]),
@@ -245,6 +246,12 @@ class Context:
 
     def set_tuning(self, key, value):
         _check(lib().ckh_set_tuning(self.h, key.encode(), value))
+
+    def graph_stats(self):
+        """(captured graphs, graph replays) of this context."""
+        a, b = u64(0), u64(0)
+        _check(lib().ckh_graph_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def set_stream(self, i):
         _check(lib().ckh_set_stream(self.h, i))

@@ -1,5 +1,6 @@
 // The project's autotuner: picks launch schedules per context by timing candidates on the
 // device (HIP events on the context's stream) with synthetic limbs of the context's primes.
+#include <algorithm>
 #include <sstream>
 
 #include "evaluator.h"
@@ -12,8 +13,13 @@ void Context::set_tuning(const std::string& key, int64_t value) {
         ntt_chunk_fwd = (u32)value;
     else if (key == "ntt_chunk_inv")
         ntt_chunk_inv = (u32)value;
+    else if (key == "graphs")
+        use_graphs = value != 0;
+    else if (key == "graph_cache")
+        graph_cache_max = (u32)std::max<int64_t>(value, 1);
     else
         fail(CKH_ERR_INVALID_PARAMS, "unknown tuning key " + key);
+    clear_plans();
 }
 
 // NTT chunk size: 16 polynomials of every Q and P prime (the limb mix of a key switch),
@@ -43,6 +49,7 @@ std::string Context::autotune() {
     }
     ntt_chunk_fwd = arg_f;
     ntt_chunk_inv = arg_i;
+    clear_plans();
     os << "}, \"ntt_chunk_fwd\": " << arg_f << ", \"ntt_chunk_inv\": " << arg_i << "}";
     return os.str();
 }

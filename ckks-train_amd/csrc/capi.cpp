@@ -178,6 +178,14 @@ ckh_status ckh_set_tuning(ckh_context* ctx, const char* key, int64_t value) {
     });
 }
 
+ckh_status ckh_graph_stats(ckh_context* ctx, uint64_t* captures, uint64_t* launches) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        if (captures) *captures = c.graph_captures;
+        if (launches) *launches = c.graph_launches;
+    });
+}
+
 ckh_status ckh_synchronize(ckh_context* ctx) {
     return guarded([&] {
         Context& c = ctx_of(ctx);

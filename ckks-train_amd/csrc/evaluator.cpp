@@ -230,8 +230,10 @@ void run_ks(Context& c, const std::vector<KsReq>& reqs) {
         dec.push_back({r.d->data(), r.d->level, dig, x});
         app.push_back({dig, r.d->level, 0, r.key, r.out_b, r.out_a, r.add_b, r.add_a, acc, y, z});
     }
-    c.decompose(dec, s);
-    c.apply(app, s);
+    Program prog;  // one program (one graph when captured) per batched switch
+    c.prog_decompose(prog, dec);
+    c.prog_apply(prog, app);
+    c.execute(prog, s);
     hip_check(hipGetLastError(), "key switch");
 }
 }  // namespace

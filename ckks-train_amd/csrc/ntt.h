@@ -13,7 +13,14 @@ struct NttTables {
     const Tw* inv;     // [prime][N]
     const FTw* fwd_f;  // [prime][N], f64 path (primes below 2^50)
     const FTw* inv_f;
+    // Row factors of the register-radix row pass (N = 2^(A+M), M = log N / 2 row stages):
+    // rf[prime][row][s] = fwd_f[prime][2^(A+s) + row 2^s] for s < M (8 slots per row); ri the
+    // same over inv_f. Every row-pass twiddle is a row factor times one of the first 2^M entries.
+    const FTw* rf;
+    const FTw* ri;
 };
+
+constexpr u32 ROW_FACTOR_SLOTS = 8;
 
 // Register-radix passes (ntt_fast.hip) for 2^12 <= N <= 2^16.
 bool ntt_fast_supported(u32 log_n);

@@ -62,22 +62,54 @@ CKH_DEV double fcanon(double x, double q, double qinv) {
 
 struct FpAr {
     typedef double V;
+    typedef FTw W;
     const FTw* tab;
     double q, qinv;
     FTw ninv, wninv;
+    const FTw* rt;  // this prime's row factors, [row][ROW_FACTOR_SLOTS]
     CKH_DEV V from_canon(u64 x) const { return u2d(x); }
     CKH_DEV V from_mid(u64 x) const { return __longlong_as_double((long long)x); }
     CKH_DEV u64 to_mid(V v) const { return (u64)__double_as_longlong(v); }
-    CKH_DEV void ct(V& x, V& y, u32 idx) const {
-        const FTw w = tab[idx];
+    CKH_DEV W tw(u32 idx) const { return tab[idx]; }
+    // Row-pass twiddles tab[2^(A+s) + row 2^s + j0 + m], m < 2^D (j0 + 2^D <= 2^s): the table
+    // holds psi^bitrev(i) and bit reversal maps disjoint bits to disjoint bits, so the entry is
+    // the row factor rt[row][s] = tab[2^(A+s) + row 2^s] (one 128-byte line per row) times
+    // tab[j0 + m] (among the first 2^M entries, cache resident). m = 0 with j0 = 0 is the
+    // factor itself. Products are signed, of magnitude below 0.75 q, which every butterfly
+    // accepts as a twiddle (w / q rounds twice: |k error| <= 3 |y| / (16 q) + 1/2 stays below 1
+    // for |y| <= 2^51).
+    template <u32 D>
+    CKH_DEV void row_group(u32 /*full_base*/, u32 row, u32 s, u32 j0, W* g) const {
+        const FTw r = rt[row * ROW_FACTOR_SLOTS + s];
+#pragma unroll
+        for (u32 m = 0; m < (1u << D); ++m) {
+            if (m == 0 && j0 == 0) {
+                g[0] = r;
+                continue;
+            }
+            const FTw c = tab[j0 + m];
+            const double w = fmulmod(r.w, c.w, c.wq, q);
+            g[m] = {w, w * qinv};
+        }
+    }
+    CKH_DEV void ct(V& x, V& y, const W& w) const {
         const double xr = freduce(x, q, qinv);
         const double t = fmulmod(y, w.w, w.wq, q);
         x = xr + t;
         y = xr - t;
     }
+    // Without the reduction of x. Stages alternate (lazy on even stages of each pass): with
+    // |t| <= (1/2 + |y| / (4 q)) q the magnitudes stay below 1.7 q after a reduced stage,
+    // 2.6 q after a lazy one, and 3.7 q where two lazy stages meet at the pass boundary; every
+    // value and every fmulmod intermediate stays an exact integer below 2^53.
+    CKH_DEV void ct_lazy(V& x, V& y, const W& w) const {
+        const double t = fmulmod(y, w.w, w.wq, q);
+        const double x0 = x;
+        x = x0 + t;
+        y = x0 - t;
+    }
     CKH_DEV u64 fwd_final(V v) const { return d2u(fcanon(v, q, qinv)); }
-    CKH_DEV void gs(V& x, V& y, u32 idx) const {
-        const FTw w = tab[idx];
+    CKH_DEV void gs(V& x, V& y, const W& w) const {
         const double s = x + y, d = x - y;
         x = freduce(s, q, qinv);
         y = fmulmod(d, w.w, w.wq, q);
@@ -94,26 +126,32 @@ struct FpAr {
 
 struct IntAr {
     typedef u64 V;
+    typedef Tw W;
     const Tw* tab;
     u64 q;
     u64 n_inv, n_inv_s, w_n_inv, w_n_inv_s;
     CKH_DEV V from_canon(u64 x) const { return x; }
     CKH_DEV V from_mid(u64 x) const { return x; }
     CKH_DEV u64 to_mid(V v) const { return v; }
-    CKH_DEV void ct(V& x, V& y, u32 idx) const {
-        const Tw w = tab[idx];
+    CKH_DEV W tw(u32 idx) const { return tab[idx]; }
+    template <u32 D>
+    CKH_DEV void row_group(u32 full_base, u32, u32, u32, W* g) const {
+#pragma unroll
+        for (u32 m = 0; m < (1u << D); ++m) g[m] = tab[full_base + m];
+    }
+    CKH_DEV void ct(V& x, V& y, const W& w) const {
         const u64 two_q = 2 * q;
         const u64 xr = x >= two_q ? x - two_q : x;
         const u64 t = shoup_lazy(y, w.w, w.ws, q);
         x = xr + t;
         y = xr + two_q - t;
     }
+    CKH_DEV void ct_lazy(V& x, V& y, const W& w) const { ct(x, y, w); }
     CKH_DEV u64 fwd_final(V v) const {
         v = v >= 2 * q ? v - 2 * q : v;
         return v >= q ? v - q : v;
     }
-    CKH_DEV void gs(V& x, V& y, u32 idx) const {
-        const Tw w = tab[idx];
+    CKH_DEV void gs(V& x, V& y, const W& w) const {
         const u64 two_q = 2 * q;
         const u64 s = x + y, d = x + two_q - y;
         x = s >= two_q ? s - two_q : s;
@@ -126,6 +164,96 @@ struct IntAr {
     }
     CKH_DEV u64 canon_out(V v) const { return v; }
 };
+
+// ---------------------------------------------------------------- row-pass twiddle groups
+//
+// In the row passes every thread needs, per stage, 2^D twiddles at consecutive table indices
+// starting at a multiple of 2^D; Ar::group builds them (for the f64 path from one load).
+
+// Forward butterfly of stage s of a pass (s is a constant after unrolling).
+template <class Ar>
+CKH_DEV void ct_stage(const Ar& ar, u32 s, typename Ar::V& x, typename Ar::V& y, const typename Ar::W& w) {
+    if (s & 1)
+        ar.ct(x, y, w);
+    else
+        ar.ct_lazy(x, y, w);
+}
+
+// Stage twiddles of the row passes: tab[full_base + m], m < 2^D, where full_base =
+// 2^(A+s) + row 2^s + j0. D is a compile-time constant once the callers' loops are unrolled.
+template <class Ar>
+CKH_DEV void row_tw(const Ar& ar, u32 A, u32 row, u32 s, u32 j0, u32 D, typename Ar::W* g) {
+    const u32 full = (1u << (A + s)) + (row << s) + j0;
+    switch (D) {
+        case 0: ar.template row_group<0>(full, row, s, j0, g); break;
+        case 1: ar.template row_group<1>(full, row, s, j0, g); break;
+        case 2: ar.template row_group<2>(full, row, s, j0, g); break;
+        default: ar.template row_group<3>(full, row, s, j0, g); break;
+    }
+}
+
+template <class Ar>
+CKH_DEV void rows_round1_fwd(const Ar& ar, typename Ar::V (&v)[16], u32 A, u32 row) {
+#pragma unroll
+    for (u32 s = 0; s < 4; ++s) {
+        typename Ar::W g[8];
+        row_tw(ar, A, row, s, 0, s, g);
+#pragma unroll
+        for (u32 k = 0; k < 16; ++k) {
+            const u32 d = 1u << (3 - s);
+            if (k & d) continue;
+            ct_stage(ar, s, v[k], v[k + d], g[k >> (4 - s)]);
+        }
+    }
+}
+
+template <u32 M, class Ar>
+CKH_DEV void rows_round2_fwd(const Ar& ar, typename Ar::V (&v)[16], u32 A, u32 row, u32 o) {
+#pragma unroll
+    for (u32 s = 4; s < M; ++s) {
+        const u32 D = s + 4 - M, p = M - 1 - s;
+        typename Ar::W g[8];
+        row_tw(ar, A, row, s, o << D, D, g);
+#pragma unroll
+        for (u32 e = 0; e < 16; ++e) {
+            if (e & (1u << p)) continue;
+            ct_stage(ar, s, v[e], v[e + (1u << p)], g[e >> (p + 1)]);
+        }
+    }
+}
+
+// Inverse stage u of the row pass is row stage s = M - 1 - u (table index 2^(A+s) + row 2^s + j).
+template <u32 M, class Ar>
+CKH_DEV void rows_round1_inv(const Ar& ar, typename Ar::V (&v)[16], u32 A, u32 row, u32 o) {
+    constexpr u32 R2 = M - 4;
+#pragma unroll
+    for (u32 u = 0; u < R2; ++u) {
+        const u32 D = 3 - u;
+        typename Ar::W g[8];
+        row_tw(ar, A, row, M - 1 - u, o << D, D, g);
+#pragma unroll
+        for (u32 e = 0; e < 16; ++e) {
+            if (e & (1u << u)) continue;
+            ar.gs(v[e], v[e + (1u << u)], g[e >> (u + 1)]);
+        }
+    }
+}
+
+template <u32 M, class Ar>
+CKH_DEV void rows_round2_inv(const Ar& ar, typename Ar::V (&v)[16], u32 A, u32 row) {
+    constexpr u32 R2 = M - 4;
+#pragma unroll
+    for (u32 u = R2; u < M; ++u) {
+        const u32 D = M - 1 - u, d = 1u << (u - R2);
+        typename Ar::W g[8];
+        row_tw(ar, A, row, M - 1 - u, 0, D, g);
+#pragma unroll
+        for (u32 k = 0; k < 16; ++k) {
+            if (k & d) continue;
+            ar.gs(v[k], v[k + d], g[k >> (u + 1 - R2)]);
+        }
+    }
+}
 
 // ---------------------------------------------------------------- passes
 //
@@ -150,7 +278,7 @@ CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
         for (u32 k = 0; k < 16; ++k) {
             const u32 d = 1u << (3 - s);
             if (k & d) continue;
-            ar.ct(v[k], v[k + d], (1u << s) + (k >> (4 - s)));
+            ct_stage(ar, s, v[k], v[k + d], ar.tw((1u << s) + (k >> (4 - s))));
         }
     }
 #pragma unroll
@@ -168,7 +296,7 @@ CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
         for (u32 e = 0; e < 16; ++e) {
             if (e & (1u << p)) continue;
             const u32 r = ((((o << U) | (e >> R2)) << R2) | (e & JM));
-            ar.ct(v[e], v[e + (1u << p)], (1u << s) + (r >> (M - s)));
+            ct_stage(ar, s, v[e], v[e + (1u << p)], ar.tw((1u << s) + (r >> (M - s))));
         }
     }
 #pragma unroll
@@ -189,15 +317,8 @@ CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
     typename Ar::V v[16];
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) v[k] = ar.from_mid(in[base + o + (k << R2)]);
-#pragma unroll
-    for (u32 s = 0; s < 4; ++s) {
-#pragma unroll
-        for (u32 k = 0; k < 16; ++k) {
-            const u32 d = 1u << (3 - s);
-            if (k & d) continue;
-            ar.ct(v[k], v[k + d], (1u << (A + s)) + (row << s) + (k >> (4 - s)));
-        }
-    }
+    // Round 1: stage A + s uses tab[2^(A+s) + row 2^s + m], m = k >> (4 - s) < 2^s.
+    rows_round1_fwd<Ar>(ar, v, A, row);
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) lds[pad((rr << M) + o + (k << R2))] = ar.to_mid(v[k]);
     __syncthreads();
@@ -206,16 +327,9 @@ CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
         const u32 c = ((((o << U) | (e >> R2)) << R2) | (e & JM));
         v[e] = ar.from_mid(lds[pad((rr << M) + c)]);
     }
-#pragma unroll
-    for (u32 s = 4; s < M; ++s) {
-        const u32 p = M - 1 - s;
-#pragma unroll
-        for (u32 e = 0; e < 16; ++e) {
-            if (e & (1u << p)) continue;
-            const u32 c = ((((o << U) | (e >> R2)) << R2) | (e & JM));
-            ar.ct(v[e], v[e + (1u << p)], (1u << (A + s)) + (row << s) + (c >> (M - s)));
-        }
-    }
+    // Round 2: the thread's column c = 16 o + e, so stage A + s uses tab[2^(A+s) + row 2^s +
+    // o 2^D + m] with D = s + 4 - M and m = e >> (M - s) < 2^D.
+    rows_round2_fwd<M, Ar>(ar, v, A, row, o);
     __syncthreads();
 #pragma unroll
     for (u32 e = 0; e < 16; ++e) {
@@ -262,15 +376,9 @@ CKH_DEV void inv_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
         const u32 c = ((((o << U) | (e >> R2)) << R2) | (e & JM));
         v[e] = ar.from_canon(lds[pad((rr << M) + c)]);
     }
-#pragma unroll
-    for (u32 u = 0; u < R2; ++u) {
-#pragma unroll
-        for (u32 e = 0; e < 16; ++e) {
-            if (e & (1u << u)) continue;
-            const u32 c = ((((o << U) | (e >> R2)) << R2) | (e & JM));
-            ar.gs(v[e], v[e + (1u << u)], (1u << (log_n - 1 - u)) + (row << (M - 1 - u)) + (c >> (u + 1)));
-        }
-    }
+    // Round 1: c = 16 o + e; stage u uses tab[2^(logN-1-u) + row 2^(M-1-u) + o 2^(3-u) + m],
+    // m = e >> (u + 1).
+    rows_round1_inv<M, Ar>(ar, v, log_n - M, row, o);
     __syncthreads();
 #pragma unroll
     for (u32 e = 0; e < 16; ++e) {
@@ -280,16 +388,9 @@ CKH_DEV void inv_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
     __syncthreads();
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) v[k] = ar.from_mid(lds[pad((rr << M) + o + (k << R2))]);
-#pragma unroll
-    for (u32 u = R2; u < M; ++u) {
-        const u32 d = 1u << (u - R2);
-#pragma unroll
-        for (u32 k = 0; k < 16; ++k) {
-            if (k & d) continue;
-            const u32 c = o + (k << R2);
-            ar.gs(v[k], v[k + d], (1u << (log_n - 1 - u)) + (row << (M - 1 - u)) + (c >> (u + 1)));
-        }
-    }
+    // Round 2: c = o + (k << R2) with o < 2^R2, so c >> (u + 1) = k >> (u + 1 - R2): stage u
+    // uses tab[2^(logN-1-u) + row 2^(M-1-u) + m], m < 2^(M-1-u).
+    rows_round2_inv<M, Ar>(ar, v, log_n - M, row);
     const u64 base = (u64)row << M;
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) out[base + o + (k << R2)] = ar.to_mid(v[k]);
@@ -313,7 +414,7 @@ CKH_DEV void inv_cols(const Ar& ar, u64* __restrict__ io, u64* lds, u32 log_n, u
         for (u32 e = 0; e < 16; ++e) {
             if (e & (1u << u)) continue;
             const u32 r = ((((o << U) | (e >> R2)) << R2) | (e & JM));
-            ar.gs(v[e], v[e + (1u << u)], (1u << (log_n - 1 - B - u)) + (r >> (u + 1)));
+            ar.gs(v[e], v[e + (1u << u)], ar.tw((1u << (log_n - 1 - B - u)) + (r >> (u + 1))));
         }
     }
 #pragma unroll
@@ -334,7 +435,7 @@ CKH_DEV void inv_cols(const Ar& ar, u64* __restrict__ io, u64* lds, u32 log_n, u
                 ar.gs_final(v[k], v[k + d]);
             } else {
                 const u32 r = o + (k << R2);
-                ar.gs(v[k], v[k + d], (1u << (log_n - 1 - B - u)) + (r >> (u + 1)));
+                ar.gs(v[k], v[k + d], ar.tw((1u << (log_n - 1 - B - u)) + (r >> (u + 1))));
             }
         }
     }
@@ -366,7 +467,7 @@ CKH_DEV void xcd_map(u32 b, u32 tiles, u32 nrefs, u32& ref, u32& tile) {
     const PrimeConst* pc = pcs + ref.prime;                                                     \
     const u64 toff = (u64)ref.prime << log_n;                                                   \
     if (pc->q < FP_PRIME_BOUND) {                                                               \
-        const FpAr ar{twf + toff, pc->qf, pc->qinv, pc->n_inv_f, pc->w_n_inv_f};                \
+        const FpAr ar{twf + toff, pc->qf, pc->qinv, pc->n_inv_f, pc->w_n_inv_f, rt + ((u64)ref.prime << (log_n - log_n / 2)) * ROW_FACTOR_SLOTS};                \
         CALL;                                                                                   \
     } else {                                                                                    \
         const IntAr ar{tw + toff, pc->q, pc->n_inv, pc->n_inv_s, pc->w_n_inv, pc->w_n_inv_s};   \
@@ -375,7 +476,8 @@ CKH_DEV void xcd_map(u32 b, u32 tiles, u32 nrefs, u32& ref, u32& tile) {
 
 template <u32 M>
 __global__ __launch_bounds__(NT) void k_fwd_cols(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
-                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n,
+                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf,
+                                                 const FTw* __restrict__ rt, u32 log_n,
                                                  u32 nrefs) {
     __shared__ u64 lds[LDS_WORDS];
     CKH_AR_DISPATCH((fwd_cols<M>(ar, ref.src, ref.dst, lds, log_n, tile)))
@@ -383,7 +485,8 @@ __global__ __launch_bounds__(NT) void k_fwd_cols(const LimbRef* __restrict__ ref
 
 template <u32 M>
 __global__ __launch_bounds__(NT) void k_fwd_rows(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
-                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n,
+                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf,
+                                                 const FTw* __restrict__ rt, u32 log_n,
                                                  u32 nrefs, const FinishJob* __restrict__ fins) {
     __shared__ u64 lds[LDS_WORDS];
     CKH_AR_DISPATCH((fwd_rows<M>(ar, ref.dst, ref.dst, lds, log_n, fins ? fins + ref_i : nullptr, pc->q, tile)))
@@ -391,7 +494,8 @@ __global__ __launch_bounds__(NT) void k_fwd_rows(const LimbRef* __restrict__ ref
 
 template <u32 M>
 __global__ __launch_bounds__(NT) void k_inv_rows(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
-                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n,
+                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf,
+                                                 const FTw* __restrict__ rt, u32 log_n,
                                                  u32 nrefs) {
     __shared__ u64 lds[LDS_WORDS];
     CKH_AR_DISPATCH((inv_rows<M>(ar, ref.src, ref.dst, lds, log_n, tile)))
@@ -399,7 +503,8 @@ __global__ __launch_bounds__(NT) void k_inv_rows(const LimbRef* __restrict__ ref
 
 template <u32 M>
 __global__ __launch_bounds__(NT) void k_inv_cols(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
-                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf, u32 log_n,
+                                                 const Tw* __restrict__ tw, const FTw* __restrict__ twf,
+                                                 const FTw* __restrict__ rt, u32 log_n,
                                                  u32 nrefs) {
     __shared__ u64 lds[LDS_WORDS];
     CKH_AR_DISPATCH((inv_cols<M>(ar, ref.dst, lds, log_n, tile)))
@@ -414,14 +519,14 @@ static void launch_one(bool fwd, bool cols, const NttTables& t, const LimbRef* r
                        const FinishJob* fins = nullptr) {
     dim3 grid((1u << (t.log_n - fast::TL)) * nrefs);
     if (fwd && cols)
-        hipLaunchKernelGGL(fast::k_fwd_cols<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.log_n, nrefs);
+        hipLaunchKernelGGL(fast::k_fwd_cols<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.rf, t.log_n, nrefs);
     else if (fwd)
-        hipLaunchKernelGGL(fast::k_fwd_rows<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.log_n, nrefs,
+        hipLaunchKernelGGL(fast::k_fwd_rows<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.rf, t.log_n, nrefs,
                            fins);
     else if (cols)
-        hipLaunchKernelGGL(fast::k_inv_cols<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.log_n, nrefs);
+        hipLaunchKernelGGL(fast::k_inv_cols<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.ri, t.log_n, nrefs);
     else
-        hipLaunchKernelGGL(fast::k_inv_rows<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.log_n,
+        hipLaunchKernelGGL(fast::k_inv_rows<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.ri, t.log_n,
                            nrefs);
 }
 

@@ -78,6 +78,11 @@ Context::Context(int ordinal, const ckh_params& p, u32 n_streams) {
 Context::~Context() {
     (void)hipSetDevice(device);
     for (auto s : streams_) (void)hipStreamSynchronize(s);
+    for (auto& pl : plans_) {
+        if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+        if (pl->desc) (void)hipFree(pl->desc);
+    }
+    plans_.clear();
     for (void* a : perm_allocs_) (void)hipFree(a);
     for (auto s : streams_) (void)hipStreamDestroy(s);
 }
@@ -190,6 +195,20 @@ void Context::build_tables() {
     ntt.inv = dev_copy(inv);
     ntt.fwd_f = dev_copy(fwd_f);
     ntt.inv_f = dev_copy(inv_f);
+    if (ntt_fast_supported(log_n)) {
+        const u32 M = log_n / 2, A = log_n - M, rows = 1u << A;
+        std::vector<FTw> rf((size_t)np_all * rows * ROW_FACTOR_SLOTS), ri(rf.size());
+        for (u32 k = 0; k < np_all; ++k)
+            for (u32 row = 0; row < rows; ++row)
+                for (u32 s = 0; s < M; ++s) {
+                    const size_t at = ((size_t)k * rows + row) * ROW_FACTOR_SLOTS + s;
+                    const size_t src = (size_t)k * n + (1u << (A + s)) + ((size_t)row << s);
+                    rf[at] = fwd_f[src];
+                    ri[at] = inv_f[src];
+                }
+        ntt.rf = dev_copy(rf);
+        ntt.ri = dev_copy(ri);
+    }
 
     std::vector<ConvSrc> srcs;
     std::vector<ConvTgt> tgts;
@@ -388,8 +407,75 @@ void Context::run(const Program& p, const uint8_t* d, hipStream_t s) const {
 
 void Context::execute(const Program& p, hipStream_t s) {
     if (p.steps.empty()) return;
+    if (use_graphs && execute_graph(p, s)) return;
     BufPtr d = upload_blob(p.bytes.data(), p.bytes.size(), s);
     run(p, (const uint8_t*)d->ptr, s);
+}
+
+void Context::drop_plan(size_t i) {
+    Plan& pl = *plans_[i];
+    if (pl.exec || pl.desc) (void)hipStreamSynchronize(pl.s);  // the last replay may still run
+    if (pl.exec) (void)hipGraphExecDestroy(pl.exec);
+    if (pl.desc) (void)hipFree(pl.desc);
+    plans_.erase(plans_.begin() + i);
+}
+
+void Context::clear_plans() {
+    std::lock_guard<std::mutex> lk(plan_mu_);
+    while (!plans_.empty()) drop_plan(plans_.size() - 1);
+}
+
+// Returns false when the program ran on no graph (first sighting of its shape: the caller runs
+// it directly). The graph holds only kernel launches; descriptors travel in the plan's buffer.
+bool Context::execute_graph(const Program& p, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(plan_mu_);
+    Plan* pl = nullptr;
+    for (auto& q : plans_)
+        if (q->s == s && q->bytes == p.bytes.size() && q->steps == p.steps) {
+            pl = q.get();
+            break;
+        }
+    if (!pl) {
+        if (plans_.size() >= std::max<u32>(graph_cache_max, 1)) {
+            size_t lru = 0;
+            for (size_t i = 1; i < plans_.size(); ++i)
+                if (plans_[i]->last_use < plans_[lru]->last_use) lru = i;
+            drop_plan(lru);
+        }
+        auto np_ = std::make_unique<Plan>();
+        np_->s = s;
+        np_->steps = p.steps;
+        np_->bytes = p.bytes.size();
+        np_->last_use = ++plan_clock_;
+        plans_.push_back(std::move(np_));
+        return false;
+    }
+    pl->last_use = ++plan_clock_;
+    if (!pl->desc) {
+        hipError_t e = hipMalloc(&pl->desc, std::max<size_t>(p.bytes.size(), 16));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            pl->desc = nullptr;
+            return false;
+        }
+    }
+    hip_check(hipMemcpyAsync(pl->desc, p.bytes.data(), p.bytes.size(), hipMemcpyHostToDevice, s),
+              "descriptor upload");
+    if (!pl->exec) {
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+        run(p, (const uint8_t*)pl->desc, s);
+        const hipError_t e_end = hipStreamEndCapture(s, &g);
+        hip_check(e_end, "hipStreamEndCapture");
+        const hipError_t e_inst = hipGraphInstantiate(&pl->exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e_inst != hipSuccess) pl->exec = nullptr;
+        hip_check(e_inst, "hipGraphInstantiate");
+        ++graph_captures;
+    }
+    hip_check(hipGraphLaunch(pl->exec, s), "hipGraphLaunch");
+    ++graph_launches;
+    return true;
 }
 
 void Context::ntt_poly(const u64* in, u64* out, u32 limbs, u32 level, u32 p_limbs, bool forward, hipStream_t s) {

@@ -124,6 +124,9 @@ struct Program {
         u32 count;
         size_t off, off2;
         u32 a, b;
+        bool operator==(const Step& o) const {
+            return kind == o.kind && count == o.count && off == o.off && off2 == o.off2 && a == o.a && b == o.b;
+        }
     };
     std::vector<uint8_t> bytes;
     std::vector<Step> steps;
@@ -186,7 +189,32 @@ class Context : public std::enable_shared_from_this<Context> {
     // Copies a host blob to a fresh device buffer on stream s.
     BufPtr upload_blob(const void* host, size_t bytes, hipStream_t s);
 
+    // Captured programs (hipGraph): a program whose launch steps (kinds, grid sizes, descriptor
+    // offsets) were seen before on the same stream is captured once and then replayed, with its
+    // descriptor bytes copied into the plan's own device buffer before each replay. Tuning key
+    // "graphs" (1 = on) and "graph_cache" (plans kept per context, least recently used evicted).
+    u32 use_graphs = 1, graph_cache_max = 64;
+    std::atomic<u64> graph_captures{0}, graph_launches{0};
+
    private:
+    struct Plan {
+        hipStream_t s = nullptr;
+        std::vector<Program::Step> steps;
+        size_t bytes = 0;
+        void* desc = nullptr;  // plain device allocation (a pool buffer would keep the context alive)
+        hipGraphExec_t exec = nullptr;
+        u64 last_use = 0;
+    };
+    std::mutex plan_mu_;
+    std::vector<std::unique_ptr<Plan>> plans_;
+    u64 plan_clock_ = 0;
+    bool execute_graph(const Program& p, hipStream_t s);
+    void drop_plan(size_t i);
+
+   public:
+    // Drops every captured plan (a tuning change alters the launch sequences).
+    void clear_plans();
+
     std::vector<hipStream_t> streams_;
     std::vector<void*> perm_allocs_;
     std::vector<std::vector<u32>> modup_plan_;  // [level][digit]

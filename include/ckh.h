@@ -84,8 +84,12 @@ void *ckh_stream_handle(ckh_context *ctx);
  * on this device with synthetic limbs of the context's primes and keeps the fastest; writes a
  * JSON report (candidates and choices) to report (may be NULL). */
 ckh_status ckh_autotune(ckh_context *ctx, char *report, size_t report_len);
-/* Forces one tuning value ("ntt_chunk_fwd", "ntt_chunk_inv": limbs per launch, 0 = all). */
+/* Forces one tuning value: "ntt_chunk_fwd", "ntt_chunk_inv" (limbs per launch, 0 = all);
+ * "graphs" (1 = replay repeated launch sequences from captured hipGraphs, the default; 0 = launch
+ * directly); "graph_cache" (captured sequences kept per context). */
 ckh_status ckh_set_tuning(ckh_context *ctx, const char *key, int64_t value);
+/* Captured hipGraphs so far and their replays (no counterpart in the reference). */
+ckh_status ckh_graph_stats(ckh_context *ctx, uint64_t *captures, uint64_t *launches);
 /* Waits for all work of the context. */
 ckh_status ckh_synchronize(ckh_context *ctx);
 /* Device memory: bytes in use by live handles, and free device bytes (evaluator.rs:496-519). */

@@ -394,8 +394,41 @@ def test_every_ntt_chunk_gives_the_same_switch():
         want = want or got
         assert got == want, chunk
     rep = ctx.autotune()
-    assert rep["ntt_chunk_fwd"] in (0, 8, 16, 32, 64, 128, 256)
+    assert rep["ntt_chunk_fwd"] in (0, 16, 24, 32, 48, 64, 96, 128)
     got = [(kb.digest(), ka.digest()) for kb, ka in ctx.key_switch_batch(polys, key)]
     assert got == want
     with pytest.raises(ckh.InvalidParams):
         ctx.set_tuning("no_such_knob", 1)
+
+
+def test_graph_replays_equal_direct_launches():
+    """Repeated batched switches of one level composition are captured once and replayed with new
+    descriptors; every replay (new inputs, eager work and synchronizations in between) must equal
+    the directly launched result."""
+    ctx, ref = make(C2_SHAPE_N4096)
+    L = len(ref.q) - 1
+    key = ctx.ksk_uniform(L, seed=51)
+    lv = [L, 3, L // 2, 0, 7]
+    ctx.set_tuning("graphs", 0)
+    rounds = []
+    for r in range(4):
+        polys = [ctx.uniform(l, seed=500 + 10 * r + i) for i, l in enumerate(lv)]
+        rounds.append((polys, [(kb.digest(), ka.digest()) for kb, ka in ctx.key_switch_batch(polys, key)]))
+    c0, l0 = ctx.graph_stats()
+    assert l0 == 0
+    ctx.set_tuning("graphs", 1)
+    for r, (polys, want) in enumerate(rounds):
+        got = ctx.key_switch_batch(polys, key)
+        # eager work on the same stream between replays
+        extra = ctx.key_switch(polys[1], key)
+        ctx.synchronize()
+        assert [(kb.digest(), ka.digest()) for kb, ka in got] == want, r
+        assert extra[0].digest() == want[1][0]
+    caps, launches = ctx.graph_stats()
+    assert caps >= 1 and launches >= 3
+    # a replay word for word against the oracle
+    polys, _ = rounds[3]
+    kb, ka = ctx.key_switch_batch(polys, key)[2]
+    d = polys[2].download()
+    wb, wa = ref.key_switch(d.ravel(), lv[2], ref.uniform_key(L, 51), L)
+    assert (kb.download() == wb).all() and (ka.download() == wa).all()
