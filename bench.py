@@ -72,27 +72,36 @@ class HipEvents:
         return ms.value
 
 
-def cpu_baseline(q, p, levels, budget_s=12.0):
+def cpu_baseline(q, p, levels, budget_s=12.0, budget_1t_s=8.0):
     """The CPU oracle (C restatement of the ckks-train CPU path, OpenMP over digits and limbs as
     the reference's rayon feature) on the first ciphertexts of the same workload until the time
-    budget is spent (at least two switches)."""
+    budget is spent (at least two switches), at the host's thread count (at most 16, the box's
+    CPU share) and again at one thread."""
     import oracle as O
 
     threads = min(16, os.cpu_count() or 1)
-    O.set_threads(threads)
     ref = O.Params(LOG_N, q=q, p=p, dnum=DNUM)
     key = ref.uniform_key(23, SEED + 1)
-    done, t0 = 0, time.perf_counter()
-    for i, lv in enumerate(levels):
-        d = O.uniform_poly(q[: lv + 1], N, SEED + 100 + i)
-        ref.key_switch(d.ravel(), lv, key, 23)
-        done += 1
-        if done >= 2 and time.perf_counter() - t0 > budget_s:
-            break
-    el = time.perf_counter() - t0
+
+    def run(nthreads, budget):
+        O.set_threads(nthreads)
+        done, t0 = 0, time.perf_counter()
+        for i, lv in enumerate(levels):
+            d = O.uniform_poly(q[: lv + 1], N, SEED + 100 + i)
+            ref.key_switch(d.ravel(), lv, key, 23)
+            done += 1
+            if done >= 2 and time.perf_counter() - t0 > budget:
+                break
+        return done, time.perf_counter() - t0
+
+    done, el = run(threads, budget_s)
+    done1, el1 = run(1, budget_1t_s)
+    O.set_threads(threads)
     return {"value": done / el, "unit": "key_switches/s", "cores": threads, "kind": "port",
             "sample": f"first {done} ciphertexts of the workload (levels {levels[:done]}), oracle "
-                      f"key_switch with {threads} OpenMP threads, {el:.1f} s"}
+                      f"key_switch with {threads} OpenMP threads, {el:.1f} s",
+            "value_1_thread": done1 / el1,
+            "sample_1_thread": f"first {done1} ciphertexts, 1 thread, {el1:.1f} s"}
 
 
 def served_stream(ckh, n_ops=240, rate=4000.0, seed=SEED + 5):
@@ -216,6 +225,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched step (warmup + steps, no autotune) and print its time (for counter runs)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the dominant-kernel launch and a device copy of the same size (for counter runs)")
     args = ap.parse_args()
@@ -235,7 +246,10 @@ def main():
         roofline_only(ctx, ev)
         return
     levels = workload_levels()
-    tuning = ctx.autotune()
+    tuning = {"ntt_chunk_fwd": 0, "ntt_chunk_inv": 128} if args.batched_only else ctx.autotune()
+    if args.batched_only:
+        for k, v in tuning.items():
+            ctx.set_tuning(k, v)
     print(f"# autotune: {json.dumps(tuning)}", file=sys.stderr, flush=True)
     key = ctx.ksk_uniform(23, seed=SEED + 1)
     polys = [ctx.uniform(lv, seed=SEED + 100 + i) for i, lv in enumerate(levels)]
@@ -260,6 +274,9 @@ def main():
     ms_step = batched_ms / args.steps
     batched = N_CT / (ms_step * 1e-3)
     print(f"# batched: {ms_step:.2f} ms/step ({wall / args.steps * 1e3:.2f} ms wall)", file=sys.stderr, flush=True)
+    if args.batched_only:
+        print(json.dumps({"batched_ms_per_step": ms_step, "key_switches_per_s": batched, "steps": args.steps}))
+        return
 
     # ---- plain: one ciphertext per launch sequence, same switches
     for i in range(8):

@@ -15,6 +15,10 @@ void Context::set_tuning(const std::string& key, int64_t value) {
         ntt_chunk_inv = (u32)value;
     else if (key == "graphs")
         use_graphs = value != 0;
+    else if (key == "conv_variant")
+        conv_variant = (u32)std::min<int64_t>(value, 3);
+    else if (key == "fuse_ip")
+        fuse_ip = value != 0;
     else if (key == "graph_cache")
         graph_cache_max = (u32)std::max<int64_t>(value, 1);
     else
@@ -23,7 +27,8 @@ void Context::set_tuning(const std::string& key, int64_t value) {
 }
 
 // NTT chunk size: 16 polynomials of every Q and P prime (the limb mix of a key switch),
-// transformed in place with each candidate; the fastest per direction is kept.
+// transformed in place with each candidate; the fastest per direction is kept. Then the
+// conversion kernel variant and the row-pass fusion, timed on a batched key switch.
 std::string Context::autotune() {
     const u32 polys = 16, lp = nq + np;
     std::vector<PolyPtr> ps;
@@ -49,8 +54,56 @@ std::string Context::autotune() {
     }
     ntt_chunk_fwd = arg_f;
     ntt_chunk_inv = arg_i;
+    ps.clear();
+    raw.clear();
+
+    // Conversion schedule and row-pass fusion: a batched key switch of 12 polynomials at
+    // levels spread over the chain, under a synthetic key, timed per candidate.
+    const u32 top = nq - 1;
+    KskPtr key = ksk_uniform(*this, top, 0, 0x5eed5eed);
+    std::vector<PolyPtr> ds;
+    std::vector<const Poly*> draw;
+    for (u32 i = 0; i < 12; ++i) ds.push_back(poly_uniform(*this, top - (i * top) / 12, 0, true, 0x7eed + i, 0));
+    for (auto& p : ds) draw.push_back(p.get());
+    const u32 keep_graphs = use_graphs;
+    use_graphs = 0;
+    hipStream_t s = stream();
+    auto time_ks = [&]() {
+        clear_plans();
+        (void)ks_switch_batch(*this, draw, *key);  // warm
+        hipEvent_t a, b;
+        hip_check(hipEventCreate(&a), "event");
+        hip_check(hipEventCreate(&b), "event");
+        hip_check(hipEventRecord(a, s), "event");
+        for (u32 r = 0; r < 3; ++r) (void)ks_switch_batch(*this, draw, *key);
+        hip_check(hipEventRecord(b, s), "event");
+        hip_check(hipEventSynchronize(b), "event");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, a, b), "event");
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return (double)ms / 3;
+    };
+    double best = 1e30;
+    u32 best_var = conv_variant, best_fuse = fuse_ip;
+    os << "}, \"ks_candidates_ms\": {";
+    first = true;
+    for (u32 var : {0u, 1u, 2u, 3u})
+        for (u32 fuse : {0u, 1u}) {
+            if (fuse && !fuse_ip_possible()) continue;
+            conv_variant = var;
+            fuse_ip = fuse;
+            const double t = time_ks();
+            os << (first ? "" : ", ") << "\"conv" << var << (fuse ? "_fused" : "") << "\": " << t;
+            first = false;
+            if (t < best) best = t, best_var = var, best_fuse = fuse;
+        }
+    conv_variant = best_var;
+    fuse_ip = best_fuse;
+    use_graphs = keep_graphs;
     clear_plans();
-    os << "}, \"ntt_chunk_fwd\": " << arg_f << ", \"ntt_chunk_inv\": " << arg_i << "}";
+    os << "}, \"ntt_chunk_fwd\": " << arg_f << ", \"ntt_chunk_inv\": " << arg_i << ", \"conv_variant\": " << conv_variant
+       << ", \"fuse_ip\": " << fuse_ip << "}";
     return os.str();
 }
 

@@ -26,13 +26,16 @@ struct ConvTgt {
     u64 qhat[MAX_SRC];                   // (Q / q_j) mod r
     double qhat_f[MAX_SRC], qhat_q[MAX_SRC];      // f64 path: (Q / q_j) mod r and its quotient by r
     double qhat30_f[MAX_SRC], qhat30_q[MAX_SRC];  // split path: 2^30 (Q / q_j) mod r, quotient by r
+    double wide[3][MAX_SRC];  // wide path: (Q / q_j) mod r in 20-bit pieces (low, middle, high)
     double qmod_f, qmod_q, hmod_f, rf, rinv;
     // TGT_INT: 64-bit integer sums; TGT_FP: r and every source below 2^50, f64 products of the
-    // sources; TGT_SPLIT: r below 2^50 with larger sources, each source split into 30-bit halves.
+    // sources; TGT_SPLIT: r below 2^50 with larger sources, each source split into 30-bit halves;
+    // TGT_WIDE: r at least 2^50 with every source below 2^50: exact f64 partial sums of 25-bit
+    // source halves times 20-bit constant pieces, combined into 128 bits and reduced once.
     u32 mode, pad;
 };
 
-enum : u32 { TGT_INT = 0, TGT_FP = 1, TGT_SPLIT = 2 };
+enum : u32 { TGT_INT = 0, TGT_FP = 1, TGT_SPLIT = 2, TGT_WIDE = 3 };
 
 // A conversion: sources [src_first, src_first + nsrc) of the ConvSrc table (their input limbs
 // are src_limb0 + j of the job's input), targets [tgt_first, tgt_first + ntgt) of the ConvTgt
@@ -70,7 +73,10 @@ struct ConvTables {
 // For each job: coefficient-form input limbs (reduced) -> converted output limbs (reduced,
 // coefficient form): out_t = [x_c]_r with x_c the centered value of the source residues
 // (BaseConverter::convert_exact of [x + h]_Q minus h mod r).
-void conv_launch(const ConvTables& t, const ConvJob* jobs, u32 njobs, u32 max_tgt, u32 max_src,
+// variant (a tuning choice): 0 = per-target constants through the scalar cache, 128-bit integer
+// sums for 60-bit targets; 1 = the same with f64 partial sums for 60-bit targets of 50-bit
+// sources; 2 / 3 = constants staged in LDS, 1 / 2 coefficients per thread.
+void conv_launch(const ConvTables& t, const ConvJob* jobs, u32 njobs, u32 max_tgt, u32 max_src, u32 variant,
                  hipStream_t stream);
 
 void finish_launch(const FinishJob* jobs, u32 njobs, const PrimeConst* pc, u32 log_n,

@@ -76,6 +76,39 @@ CKH_DEV u64 reduce128(u64 hi, u64 lo, const PrimeConst& c) {
 // a * b mod q for reduced a, b.
 CKH_DEV u64 mul_mod(u64 a, u64 b, const PrimeConst& c) { return reduce128(mulhi(a, b), a * b, c); }
 
+// ---- f64 arithmetic for primes below 2^50 (FP_PRIME_BOUND): residues as integer-valued f64.
+
+// u64 < 2^53 -> f64, exact.
+CKH_DEV double u2d(u64 v) { return __builtin_fma((double)(u32)(v >> 32), 4294967296.0, (double)(u32)v); }
+
+// Non-negative integer-valued f64 below 2^53 -> u64, exact.
+CKH_DEV u64 d2u(double d) {
+    const u32 hi = (u32)(d * (1.0 / 4294967296.0));
+    const double lo = __builtin_fma(-(double)hi, 4294967296.0, d);
+    return ((u64)hi << 32) | (u32)lo;
+}
+
+// y w mod q as a signed integer-valued f64, for integer-valued y, w with |w| < q < 2^50 and
+// wq = w / q rounded once or twice: h = RN(y w) and l = fma(y, w, -h) split the product
+// exactly; k = rint(y wq) misses y w / q by at most 1/2 + 3 |y| / (16 q) (the rounding errors
+// of wq and of y wq), so t = y w - k q has |t| <= (1/2 + 3 |y| / (16 q)) q, e.g. 0.69 q for
+// |y| <= q; while that stays far below 2^53, fma(-k, q, h) + l computes t exactly.
+CKH_DEV double fmulmod(double y, double w, double wq, double q) {
+    const double h = y * w;
+    const double l = __builtin_fma(y, w, -h);
+    const double k = __builtin_rint(y * wq);
+    return __builtin_fma(-k, q, h) + l;
+}
+
+// x - rint(x / q) q: |result| <= q / 2 (plus rounding slack) for |x| < 2^53.
+CKH_DEV double freduce(double x, double q, double qinv) { return __builtin_fma(-__builtin_rint(x * qinv), q, x); }
+
+// The canonical residue in [0, q) as f64.
+CKH_DEV double fcanon(double x, double q, double qinv) {
+    const double r = freduce(x, q, qinv);
+    return r < 0.0 ? r + q : r;
+}
+
 // 128-bit accumulator.
 struct Acc128 {
     u64 lo, hi;

@@ -216,6 +216,7 @@ void run_ks(Context& c, const std::vector<KsReq>& reqs) {
     u64 total = 0;
     for (auto& r : reqs) total += ks_scratch_words(c, r.d->level, true);
     BufPtr scratch = c.alloc(total, s);
+    const bool fuse = c.fuse_ip && c.fuse_ip_possible();
     std::vector<DecTask> dec;
     std::vector<ApplyTask> app;
     u64* p = scratch->ptr;
@@ -227,8 +228,8 @@ void run_ks(Context& c, const std::vector<KsReq>& reqs) {
         u64* y = acc + 2 * (L + k) * c.n;
         u64* z = y + 2 * k * c.n;
         p = z + 2 * L * c.n;
-        dec.push_back({r.d->data(), r.d->level, dig, x});
-        app.push_back({dig, r.d->level, 0, r.key, r.out_b, r.out_a, r.add_b, r.add_a, acc, y, z});
+        dec.push_back({r.d->data(), r.d->level, dig, x, false, fuse});
+        app.push_back({dig, r.d->level, 0, r.key, r.out_b, r.out_a, r.add_b, r.add_a, acc, y, z, r.d->data(), fuse});
     }
     Program prog;  // one program (one graph when captured) per batched switch
     c.prog_decompose(prog, dec);

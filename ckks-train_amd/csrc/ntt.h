@@ -32,6 +32,27 @@ struct FinishJob;
 void ntt_fast_forward_finish(const NttTables& t, const LimbRef* refs, const FinishJob* fin, u32 nrefs,
                              hipStream_t stream);
 
+// The two passes of ntt_fast_forward separately (column pass: src -> dst; row pass: in place on
+// dst; between them dst holds f64 intermediates, not residues).
+void ntt_fast_forward_cols(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream);
+void ntt_fast_forward_rows(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream);
+
+// One limb of a key-switch inner product fused with the forward row pass (primes below 2^50):
+// digit j's limb is the column-pass output at mid + j * mid_stride, except digit own_j whose
+// limb is the NTT-form own (not transformed); key part j's b limb at key + 2 j key_poly, its a
+// limb key_poly further; acc_b / acc_a receive the sums.
+struct RowIpJob {
+    const u64* mid;
+    u64 mid_stride;
+    const u64* own;
+    const u64* key;
+    u64 key_poly;
+    u64* acc_b;
+    u64* acc_a;
+    u32 ndig, prime, own_j, pad;
+};
+void rows_ip_launch(const NttTables& t, const RowIpJob* jobs, u32 njobs, hipStream_t stream);
+
 // ref.dst <- NTT(ref.src) for each of the nrefs descriptors (device array). src may equal
 // dst (in place). Inputs reduced, outputs reduced.
 void ntt_forward(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream);

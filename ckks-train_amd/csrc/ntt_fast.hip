@@ -36,30 +36,6 @@ CKH_DEV u32 pad(u32 a) { return a + (a >> 4); }
 
 // ---------------------------------------------------------------- f64 arithmetic (q < 2^50)
 
-// u64 < 2^53 -> f64, exact.
-CKH_DEV double u2d(u64 v) { return __builtin_fma((double)(u32)(v >> 32), 4294967296.0, (double)(u32)v); }
-
-// Non-negative integer-valued f64 below 2^52 -> u64, exact.
-CKH_DEV u64 d2u(double d) {
-    const u32 hi = (u32)(d * (1.0 / 4294967296.0));
-    const double lo = __builtin_fma(-(double)hi, 4294967296.0, d);
-    return ((u64)hi << 32) | (u32)lo;
-}
-
-CKH_DEV double fmulmod(double y, double w, double wq, double q) {
-    const double h = y * w;
-    const double l = __builtin_fma(y, w, -h);
-    const double k = __builtin_rint(y * wq);
-    return __builtin_fma(-k, q, h) + l;
-}
-
-CKH_DEV double freduce(double x, double q, double qinv) { return __builtin_fma(-__builtin_rint(x * qinv), q, x); }
-
-CKH_DEV double fcanon(double x, double q, double qinv) {
-    const double r = freduce(x, q, qinv);
-    return r < 0.0 ? r + q : r;
-}
-
 struct FpAr {
     typedef double V;
     typedef FTw W;
@@ -109,6 +85,7 @@ struct FpAr {
         y = x0 - t;
     }
     CKH_DEV u64 fwd_final(V v) const { return d2u(fcanon(v, q, qinv)); }
+    CKH_DEV u64 fwd_final_f(V v) const { return (u64)__double_as_longlong(fcanon(v, q, qinv)); }
     CKH_DEV void gs(V& x, V& y, const W& w) const {
         const double s = x + y, d = x - y;
         x = freduce(s, q, qinv);
@@ -147,6 +124,7 @@ struct IntAr {
         y = xr + two_q - t;
     }
     CKH_DEV void ct_lazy(V& x, V& y, const W& w) const { ct(x, y, w); }
+    CKH_DEV u64 fwd_final_f(V v) const { return fwd_final(v); }  // not used (f64 primes only)
     CKH_DEV u64 fwd_final(V v) const {
         v = v >= 2 * q ? v - 2 * q : v;
         return v >= q ? v - q : v;
@@ -306,9 +284,10 @@ CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
     }
 }
 
-template <u32 M, class Ar>
-CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out, u64* lds, u32 log_n,
-                      const FinishJob* fin, u64 q, u32 tile) {
+// The row pass of one tile, its canonical outputs left in LDS in natural order (word i of the
+// tile at pad(i)): as u64 residues, or (F64) as the f64 bits of the residue.
+template <u32 M, bool F64, class Ar>
+CKH_DEV void fwd_rows_to_lds(const Ar& ar, const u64* __restrict__ in, u64* lds, u32 log_n, u32 tile) {
     constexpr u32 R2 = M - 4, U = 4 - R2, TRL = TL - M, JM = (1u << R2) - 1;
     const u32 A = log_n - M;
     const u32 tid = threadIdx.x, o = tid & JM, rr = tid >> R2;
@@ -334,9 +313,16 @@ CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
 #pragma unroll
     for (u32 e = 0; e < 16; ++e) {
         const u32 c = ((((o << U) | (e >> R2)) << R2) | (e & JM));
-        lds[pad((rr << M) + c)] = ar.fwd_final(v[e]);
+        lds[pad((rr << M) + c)] = F64 ? ar.fwd_final_f(v[e]) : ar.fwd_final(v[e]);
     }
     __syncthreads();
+}
+
+template <u32 M, class Ar>
+CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out, u64* lds, u32 log_n,
+                      const FinishJob* fin, u64 q, u32 tile) {
+    const u32 tid = threadIdx.x;
+    fwd_rows_to_lds<M, false>(ar, in, lds, log_n, tile);
     const u64 tbase = (u64)tile << TL;
     if (fin) {
         // finish of rescale / mod_down: (a - t) * inv (+ add)
@@ -354,6 +340,52 @@ CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
     for (u32 k = 0; k < 16; ++k) {
         const u32 idx = tid + k * NT;
         out[tbase + idx] = lds[pad(idx)];
+    }
+}
+
+// Row pass fused with the key inner product (f64 primes): for one limb and tile, every digit
+// not owning the limb is row-transformed from its column-pass output, the owning digit's limb
+// is read as it is (NTT form), and acc_b / acc_a = sum_j digit_j * key part j (b / a) are
+// written; the transformed digits never reach memory (keyswitch.rs:254-303).
+template <u32 M>
+CKH_DEV void rows_ip(const FpAr& ar, const RowIpJob& J, u64* lds, u32 log_n, u32 tile) {
+    const u32 tid = threadIdx.x;
+    const u64 tbase = (u64)tile << TL;
+    const double q = ar.q, qinv = ar.qinv;
+    double sb[16], sa[16];
+#pragma unroll
+    for (u32 k = 0; k < 16; ++k) sb[k] = sa[k] = 0.0;
+    for (u32 j = 0; j < J.ndig; ++j) {
+        double x[16];
+        if (j == J.own_j) {
+#pragma unroll
+            for (u32 k = 0; k < 16; ++k) x[k] = u2d(J.own[tbase + tid + k * NT]);
+        } else {
+            fwd_rows_to_lds<M, true>(ar, J.mid + j * J.mid_stride, lds, log_n, tile);
+#pragma unroll
+            for (u32 k = 0; k < 16; ++k) x[k] = __longlong_as_double((long long)lds[pad(tid + k * NT)]);
+            __syncthreads();  // the next digit's pass reuses the LDS
+        }
+        const u64* kb = J.key + (u64)(2 * j) * J.key_poly + tbase;
+        const u64* ka = kb + J.key_poly;
+#pragma unroll
+        for (u32 k = 0; k < 16; ++k) {
+            const double b = u2d(kb[tid + k * NT]), a = u2d(ka[tid + k * NT]);
+            sb[k] += fmulmod(x[k], b, b * qinv, q);
+            sa[k] += fmulmod(x[k], a, a * qinv, q);
+        }
+        if ((j & 3) == 3) {
+#pragma unroll
+            for (u32 k = 0; k < 16; ++k) {
+                sb[k] = freduce(sb[k], q, qinv);
+                sa[k] = freduce(sa[k], q, qinv);
+            }
+        }
+    }
+#pragma unroll
+    for (u32 k = 0; k < 16; ++k) {
+        J.acc_b[tbase + tid + k * NT] = d2u(fcanon(sb[k], q, qinv));
+        J.acc_a[tbase + tid + k * NT] = d2u(fcanon(sa[k], q, qinv));
     }
 }
 
@@ -510,6 +542,20 @@ __global__ __launch_bounds__(NT) void k_inv_cols(const LimbRef* __restrict__ ref
     CKH_AR_DISPATCH((inv_cols<M>(ar, ref.dst, lds, log_n, tile)))
 }
 
+template <u32 M>
+__global__ __launch_bounds__(NT) void k_rows_ip(const RowIpJob* __restrict__ jobs, const PrimeConst* __restrict__ pcs,
+                                                const FTw* __restrict__ twf, const FTw* __restrict__ rt, u32 log_n,
+                                                u32 njobs) {
+    __shared__ u64 lds[LDS_WORDS];
+    u32 ji, tile;
+    xcd_map(blockIdx.x, 1u << (log_n - TL), njobs, ji, tile);
+    const RowIpJob J = jobs[ji];
+    const PrimeConst* pc = pcs + J.prime;
+    const FpAr ar{twf + ((u64)J.prime << log_n), pc->qf, pc->qinv, pc->n_inv_f, pc->w_n_inv_f,
+                  rt + ((u64)J.prime << (log_n - log_n / 2)) * ROW_FACTOR_SLOTS};
+    rows_ip<M>(ar, J, lds, log_n, tile);
+}
+
 }  // namespace fast
 
 bool ntt_fast_supported(u32 log_n) { return log_n >= 12 && log_n <= 16; }
@@ -551,6 +597,24 @@ void ntt_fast_forward_finish(const NttTables& t, const LimbRef* refs, const Fini
     const u32 B = t.log_n / 2, A = t.log_n - B;
     launch_m(A, true, true, t, refs, nrefs, s);
     launch_m(B, true, false, t, refs, nrefs, s, fin);
+}
+
+void ntt_fast_forward_cols(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s) {
+    launch_m(t.log_n - t.log_n / 2, true, true, t, refs, nrefs, s);
+}
+
+void ntt_fast_forward_rows(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s) {
+    launch_m(t.log_n / 2, true, false, t, refs, nrefs, s);
+}
+
+void rows_ip_launch(const NttTables& t, const RowIpJob* jobs, u32 njobs, hipStream_t s) {
+    if (njobs == 0) return;
+    dim3 grid((1u << (t.log_n - fast::TL)) * njobs);
+    switch (t.log_n / 2) {
+        case 6: hipLaunchKernelGGL(fast::k_rows_ip<6>, grid, dim3(fast::NT), 0, s, jobs, t.pc, t.fwd_f, t.rf, t.log_n, njobs); break;
+        case 7: hipLaunchKernelGGL(fast::k_rows_ip<7>, grid, dim3(fast::NT), 0, s, jobs, t.pc, t.fwd_f, t.rf, t.log_n, njobs); break;
+        default: hipLaunchKernelGGL(fast::k_rows_ip<8>, grid, dim3(fast::NT), 0, s, jobs, t.pc, t.fwd_f, t.rf, t.log_n, njobs); break;
+    }
 }
 
 void ntt_fast_inverse(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s) {

@@ -7,7 +7,8 @@ namespace ckh {
 // One limb of a key-switch inner product: acc_b = sum_j dig_j * key_b_j, acc_a = sum_j dig_j *
 // key_a_j over ndig digits (digit j's limb at dig + j * dig_stride). The key limb of part j is
 // at key + (2 j) * key_poly (b) and key + (2 j + 1) * key_poly (a). galois != 0 reads the digit
-// through the NTT-form automorphism permutation (hoisted rotations).
+// through the NTT-form automorphism permutation (hoisted rotations). When own is set, digit
+// own_j's limb (the digit's own, unconverted limb) is read from own instead.
 struct IpJob {
     const u64* dig;
     u64 dig_stride;
@@ -16,7 +17,8 @@ struct IpJob {
     u64* acc_b;
     u64* acc_a;
     u64 galois;
-    u32 ndig, prime;
+    const u64* own;
+    u32 ndig, prime, own_j, pad;
 };
 
 void inner_product_launch(const IpJob* jobs, u32 njobs, const PrimeConst* pc, u32 log_n,

@@ -13,8 +13,10 @@ CKH_DEV u32 galois_index(u32 i, u64 g, u32 log_n) {
     return __brev(raw) >> (32 - log_n);
 }
 
-// acc_b = sum_j d_j * b_j and acc_a = sum_j d_j * a_j per limb of Q_l * P, exact 128-bit sums
-// reduced once (crates/ckks-train-ckks/src/keyswitch.rs:254-303). Digit limbs may be read
+// acc_b = sum_j d_j * b_j and acc_a = sum_j d_j * a_j per limb of Q_l * P
+// (crates/ckks-train-ckks/src/keyswitch.rs:254-303; residues are unique, so any exact summation
+// order gives the reference's words). Primes below 2^50 sum f64 modular products (reduced every
+// four digits); larger primes sum exact 128-bit products and reduce once. Digit limbs may be read
 // through the Galois permutation (hoisted rotations, evaluator.rs:596-604).
 __global__ __launch_bounds__(256) void k_inner_product(const IpJob* __restrict__ jobs,
                                                        const PrimeConst* __restrict__ pc, u32 log_n) {
@@ -24,9 +26,29 @@ __global__ __launch_bounds__(256) void k_inner_product(const IpJob* __restrict__
     const IpJob J = jobs[blockIdx.y];
     const PrimeConst C = pc[J.prime];
     const u32 cs = J.galois ? galois_index(c, J.galois, log_n) : c;
+    if (C.q < FP_PRIME_BOUND) {
+        const double q = C.qf, qinv = C.qinv;
+        double sb = 0.0, sa = 0.0;
+        for (u32 j = 0; j < J.ndig; ++j) {
+            const u64* src = (J.own && j == J.own_j) ? J.own : J.dig + j * J.dig_stride;
+            const double d = u2d(src[cs]);
+            const double b = u2d(J.key[(2 * j) * J.key_poly + c]);
+            const double a = u2d(J.key[(2 * j + 1) * J.key_poly + c]);
+            sb += fmulmod(d, b, b * qinv, q);
+            sa += fmulmod(d, a, a * qinv, q);
+            if ((j & 3) == 3) {
+                sb = freduce(sb, q, qinv);
+                sa = freduce(sa, q, qinv);
+            }
+        }
+        J.acc_b[c] = d2u(fcanon(sb, q, qinv));
+        J.acc_a[c] = d2u(fcanon(sa, q, qinv));
+        return;
+    }
     Acc128 sb = {0, 0}, sa = {0, 0};
     for (u32 j = 0; j < J.ndig; ++j) {
-        const u64 d = J.dig[j * J.dig_stride + cs];
+        const u64* src = (J.own && j == J.own_j) ? J.own : J.dig + j * J.dig_stride;
+        const u64 d = src[cs];
         const u64 b = J.key[(2 * j) * J.key_poly + c];
         const u64 k = J.key[(2 * j + 1) * J.key_poly + c];
         mac(sb, d, b);

@@ -254,7 +254,7 @@ void Context::build_tables() {
                 t.hmod = hm::half_mod(sq.data(), sq.size(), r);
                 bool srcs_fp = true;
                 for (u64 sqj : sq) srcs_fp = srcs_fp && sqj < FP_PRIME_BOUND;
-                t.mode = r >= FP_PRIME_BOUND ? TGT_INT : (srcs_fp ? TGT_FP : TGT_SPLIT);
+                t.mode = r >= FP_PRIME_BOUND ? (srcs_fp ? TGT_WIDE : TGT_INT) : (srcs_fp ? TGT_FP : TGT_SPLIT);
                 t.rf = (double)r;
                 t.rinv = 1.0 / (double)r;
                 t.qmod_f = (double)t.qmod;
@@ -270,6 +270,7 @@ void Context::build_tables() {
                     const u64 qh30 = hm::mul(qh, (1ull << 30) % r, r);
                     t.qhat30_f[j] = (double)qh30;
                     t.qhat30_q[j] = (double)qh30 / (double)r;
+                    for (u32 b = 0; b < 3; ++b) t.wide[b][j] = (double)((qh >> (20 * b)) & ((1u << 20) - 1));
                 }
                 tgts.push_back(t);
             }
@@ -358,8 +359,10 @@ static void launch_copy(const LimbRef* d, u32 nrefs, u32 log_n, hipStream_t s) {
     for (u32 i = 0; i < nrefs; i += MAX_ROWS) copy_launch(d + i, std::min(MAX_ROWS, nrefs - i), log_n, s);
 }
 
-static void launch_conv(const ConvTables& t, const ConvJob* d, u32 njobs, u32 max_tgt, u32 max_src, hipStream_t s) {
-    for (u32 i = 0; i < njobs; i += MAX_ROWS) conv_launch(t, d + i, std::min(MAX_ROWS, njobs - i), max_tgt, max_src, s);
+static void launch_conv(const ConvTables& t, const ConvJob* d, u32 njobs, u32 max_tgt, u32 max_src, u32 variant,
+                        hipStream_t s) {
+    for (u32 i = 0; i < njobs; i += MAX_ROWS)
+        conv_launch(t, d + i, std::min(MAX_ROWS, njobs - i), max_tgt, max_src, variant, s);
 }
 
 static void launch_ip(const IpJob* d, u32 njobs, const PrimeConst* pc, u32 log_n, hipStream_t s) {
@@ -394,13 +397,16 @@ void Context::run(const Program& p, const uint8_t* d, hipStream_t s) const {
         switch (st.kind) {
             case S_NTT_FWD: launch_ntt(*this, (const LimbRef*)x, st.count, true, s); break;
             case S_NTT_INV: launch_ntt(*this, (const LimbRef*)x, st.count, false, s); break;
-            case S_CONV: launch_conv(conv, (const ConvJob*)x, st.count, st.a, st.b, s); break;
+            case S_CONV: launch_conv(conv, (const ConvJob*)x, st.count, st.a, st.b, conv_variant, s); break;
             case S_COPY: launch_copy((const LimbRef*)x, st.count, log_n, s); break;
             case S_IP: launch_ip((const IpJob*)x, st.count, d_pc, log_n, s); break;
             case S_NTT_FWD_FIN:
                 launch_ntt_finish((const LimbRef*)x, (const FinishJob*)(d + st.off2), st.count, s);
                 break;
             case S_FINISH: launch_finish((const FinishJob*)x, st.count, d_pc, log_n, s); break;
+            case S_NTT_COLS: ntt_fast_forward_cols(ntt, (const LimbRef*)x, st.count, s); break;
+            case S_NTT_ROWS: ntt_fast_forward_rows(ntt, (const LimbRef*)x, st.count, s); break;
+            case S_ROWS_IP: rows_ip_launch(ntt, (const RowIpJob*)x, st.count, s); break;
         }
     }
 }
@@ -493,8 +499,15 @@ void Context::ntt_poly(const u64* in, u64* out, u32 limbs, u32 level, u32 p_limb
 // keyswitch::decompose (keyswitch.rs:82-124) for a batch: INTT of every input limb, the
 // centered exact conversion of every digit to its targets, the digit limbs copied, then the
 // NTT of every converted limb (lift_digit_exact, keyswitch.rs:140-210).
+bool Context::fuse_ip_possible() const {
+    if (!ntt_fast_supported(log_n)) return false;
+    for (u32 i = 0; i < nq; ++i)
+        if (primes[i] >= FP_PRIME_BOUND) return false;
+    return true;
+}
+
 void Context::prog_decompose(Program& p, const std::vector<DecTask>& tasks) const {
-    std::vector<LimbRef> intt, copies, nttr;
+    std::vector<LimbRef> intt, copies, nttr, cols, rows;
     std::vector<ConvJob> jobs;
     u32 max_tgt = 0;
     for (const DecTask& t : tasks) {
@@ -505,10 +518,19 @@ void Context::prog_decompose(Program& p, const std::vector<DecTask>& tasks) cons
         for (u32 j = 0; j < dr.size(); ++j) {
             u64* dj = t.dig + j * ds;
             jobs.push_back({t.x, dj, modup_plan_[t.level][j], 0});
-            for (u32 i = dr[j].first; i < dr[j].second; ++i) copies.push_back({t.d + i * n, dj + i * n, i, 0});
+            if (t.copy_own)
+                for (u32 i = dr[j].first; i < dr[j].second; ++i) copies.push_back({t.d + i * n, dj + i * n, i, 0});
             for (u32 i = 0; i < L; ++i)
-                if (i < dr[j].first || i >= dr[j].second) nttr.push_back({dj + i * n, dj + i * n, i, 0});
-            for (u32 k = 0; k < np; ++k) nttr.push_back({dj + (L + k) * n, dj + (L + k) * n, nq + k, 0});
+                if (i < dr[j].first || i >= dr[j].second) (t.fuse ? cols : nttr).push_back({dj + i * n, dj + i * n, i, 0});
+            for (u32 k = 0; k < np; ++k) {
+                const LimbRef r{dj + (L + k) * n, dj + (L + k) * n, nq + k, 0};
+                if (t.fuse) {
+                    cols.push_back(r);
+                    rows.push_back(r);
+                } else {
+                    nttr.push_back(r);
+                }
+            }
         }
         max_tgt = std::max(max_tgt, modup_maxtgt_[t.level]);
     }
@@ -516,6 +538,8 @@ void Context::prog_decompose(Program& p, const std::vector<DecTask>& tasks) cons
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_tgt, alpha);
     p.step(S_COPY, p.add(copies), (u32)copies.size());
     p.step(S_NTT_FWD, p.add(nttr), (u32)nttr.size());
+    p.step(S_NTT_COLS, p.add(cols), (u32)cols.size());
+    p.step(S_NTT_ROWS, p.add(rows), (u32)rows.size());
 }
 
 // keyswitch::apply_key (keyswitch.rs:230-309) for a batch: the inner product over Q_l * P, then
@@ -531,18 +555,27 @@ void Context::prog_apply(Program& p, const std::vector<ApplyTask>& tasks) const 
             const u32 L = t.level + 1, li = L + k;
             const u64 aw = (u64)(L + np) * n;
             ip.push_back({t.dig + li * n, digit_words(t.level), t.key->buf->ptr + (t.key->max_level + 1 + k) * n,
-                          t.key->poly_words(), t.acc + li * n, t.acc + aw + li * n, t.galois, ndigits(t.level),
-                          nq + k});
+                          t.key->poly_words(), t.acc + li * n, t.acc + aw + li * n, t.galois, nullptr,
+                          ndigits(t.level), nq + k, 0, 0});
         }
+    std::vector<RowIpJob> rip;
     for (u32 i = 0; i < max_l; ++i)
         for (const ApplyTask& t : tasks) {
             const u32 L = t.level + 1;
             if (i >= L) continue;
             const u64 aw = (u64)(L + np) * n;
+            if (t.fused) {
+                rip.push_back({t.dig + i * n, digit_words(t.level), t.own + i * n, t.key->buf->ptr + i * n,
+                               t.key->poly_words(), t.acc + i * n, t.acc + aw + i * n, ndigits(t.level), i,
+                               i / alpha, 0});
+                continue;
+            }
             ip.push_back({t.dig + i * n, digit_words(t.level), t.key->buf->ptr + i * n, t.key->poly_words(),
-                          t.acc + i * n, t.acc + aw + i * n, t.galois, ndigits(t.level), i});
+                          t.acc + i * n, t.acc + aw + i * n, t.galois, t.own ? t.own + i * n : nullptr,
+                          ndigits(t.level), i, i / alpha, 0});
         }
     p.step(S_IP, p.add(ip), (u32)ip.size());
+    p.step(S_ROWS_IP, p.add(rip), (u32)rip.size());
     std::vector<MdTask> md;
     for (const ApplyTask& t : tasks) {
         const u32 L = t.level + 1;

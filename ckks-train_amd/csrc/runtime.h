@@ -77,6 +77,9 @@ struct DecTask {
     u32 level;
     u64* dig;
     u64* x;
+    bool copy_own = true;  // false: the digits' own limbs stay in d (an ApplyTask reads them there)
+    bool fuse = false;     // Q-prime targets get only the column pass (an ApplyTask with fused set
+                           // finishes them inside its inner product)
 };
 
 // One inner product + mod_down: digits (from a DecTask) times key (galois != 0: digits read
@@ -94,6 +97,8 @@ struct ApplyTask {
     u64* acc;
     u64* y;
     u64* z;
+    const u64* own = nullptr;  // set: the digits' own limbs are read from this d (not copied)
+    bool fused = false;        // the digits come from a DecTask with fuse set (needs own)
 };
 
 // One mod_down of a QpPoly sum (level l, l + 1 + k limbs, NTT form) -> out (l + 1 limbs) plus an
@@ -117,7 +122,9 @@ struct RescaleTask {
 };
 
 // Descriptor bytes plus the launch steps that read them (see runtime.cpp, "engine").
-enum StepKind : uint8_t { S_NTT_FWD, S_NTT_INV, S_CONV, S_COPY, S_IP, S_NTT_FWD_FIN, S_FINISH };
+enum StepKind : uint8_t {
+    S_NTT_FWD, S_NTT_INV, S_CONV, S_COPY, S_IP, S_NTT_FWD_FIN, S_FINISH, S_NTT_COLS, S_NTT_ROWS, S_ROWS_IP
+};
 struct Program {
     struct Step {
         StepKind kind;
@@ -184,6 +191,7 @@ class Context : public std::enable_shared_from_this<Context> {
     // Tuning (autotune.cpp): limbs per column-pass/row-pass launch pair (0 = all at once). A
     // chunk small enough keeps the pass intermediate in the on-die caches between the passes.
     u32 ntt_chunk_fwd = 0, ntt_chunk_inv = 0;
+    u32 conv_variant = 0;  // base-conversion kernel variant (basis.h, conv_launch)
     std::string autotune();
     void set_tuning(const std::string& key, int64_t value);
     // Copies a host blob to a fresh device buffer on stream s.
@@ -194,6 +202,10 @@ class Context : public std::enable_shared_from_this<Context> {
     // descriptor bytes copied into the plan's own device buffer before each replay. Tuning key
     // "graphs" (1 = on) and "graph_cache" (plans kept per context, least recently used evicted).
     u32 use_graphs = 1, graph_cache_max = 64;
+    // Row pass fused into the inner product of batched key switches (tuning key "fuse_ip");
+    // possible when the register-radix NTT applies and every Q prime is below 2^50.
+    u32 fuse_ip = 1;
+    bool fuse_ip_possible() const;
     std::atomic<u64> graph_captures{0}, graph_launches{0};
 
    private:

@@ -432,3 +432,26 @@ def test_graph_replays_equal_direct_launches():
     d = polys[2].download()
     wb, wa = ref.key_switch(d.ravel(), lv[2], ref.uniform_key(L, 51), L)
     assert (kb.download() == wb).all() and (ka.download() == wa).all()
+
+
+@pytest.mark.parametrize("case", [C2_SHAPE_N4096, SMALL_N8192, C4])
+def test_schedules_give_the_same_switch(case):
+    """Every conversion kernel variant with and without the row pass fused into the inner product
+    gives the same batched switch, equal to the oracle."""
+    ctx, ref = make(case)
+    L = len(ref.q) - 1
+    key = ctx.ksk_uniform(L, seed=61)
+    lv = sorted({L, L // 2, 1, 0, max(L - 3, 0)})
+    polys = [ctx.uniform(l, seed=600 + i) for i, l in enumerate(lv)]
+    want = None
+    for var in (0, 1, 2, 3):
+        for fuse in (0, 1):
+            ctx.set_tuning("conv_variant", var)
+            ctx.set_tuning("fuse_ip", fuse)
+            got = [(kb.download(), ka.download()) for kb, ka in ctx.key_switch_batch(polys, key)]
+            if want is None:
+                want = got
+            for (wb, wa), (gb, ga) in zip(want, got):
+                assert (wb == gb).all() and (wa == ga).all(), (var, fuse)
+    ob, oa = ref.key_switch(polys[-1].download().ravel(), lv[-1], ref.uniform_key(L, 61), L)
+    assert (want[-1][0] == ob).all() and (want[-1][1] == oa).all()
