@@ -96,9 +96,11 @@ _SIGS = {
     "ckh_stream_handle": (VP, [VP]),
     "ckh_synchronize": (C.c_int, [VP]),
     "ckh_autotune": (C.c_int, [VP, C.c_char_p, C.c_size_t]),
+    "ckh_autotune_ks": (C.c_int, [VP, C.c_uint32, C.POINTER(VP), VP, C.c_char_p, C.c_size_t]),
     "ckh_set_tuning": (C.c_int, [VP, C.c_char_p, C.c_int64]),
     "ckh_memory_info": (C.c_int, [VP, P64, P64]),
     "ckh_graph_stats": (C.c_int, [VP, P64, P64]),
+    "ckh_get_tuning": (C.c_int, [VP, C.c_char_p, C.POINTER(C.c_int64)]),
     "ckh_poly_upload": (C.c_int, [VP, P64, u32, u32, C.c_int, C.POINTER(VP)]),
     "ckh_poly_download": (C.c_int, [VP, VP, P64]),
     "ckh_poly_info": (C.c_int, [VP, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_int)]),
@@ -244,8 +246,21 @@ class Context:
         _check(lib().ckh_autotune(self.h, buf, len(buf)))
         return json.loads(buf.value.decode())
 
+    def autotune_ks(self, polys, key):
+        """Picks the key-switch schedule on this batch (polys under key); returns the timings."""
+        import json
+        buf = C.create_string_buffer(4096)
+        arr = (VP * len(polys))(*[p.h for p in polys])
+        _check(lib().ckh_autotune_ks(self.h, len(polys), arr, key.h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
     def set_tuning(self, key, value):
         _check(lib().ckh_set_tuning(self.h, key.encode(), value))
+
+    def tuning(self, key):
+        v = C.c_int64(0)
+        _check(lib().ckh_get_tuning(self.h, key.encode(), C.byref(v)))
+        return v.value
 
     def graph_stats(self):
         """(captured graphs, graph replays) of this context."""

@@ -16,7 +16,7 @@ void Context::set_tuning(const std::string& key, int64_t value) {
     else if (key == "graphs")
         use_graphs = value != 0;
     else if (key == "conv_variant")
-        conv_variant = (u32)std::min<int64_t>(value, 3);
+        conv_variant = (u32)std::min<int64_t>(value, 1);
     else if (key == "fuse_ip")
         fuse_ip = value != 0;
     else if (key == "graph_cache")
@@ -24,6 +24,16 @@ void Context::set_tuning(const std::string& key, int64_t value) {
     else
         fail(CKH_ERR_INVALID_PARAMS, "unknown tuning key " + key);
     clear_plans();
+}
+
+int64_t Context::get_tuning(const std::string& key) const {
+    if (key == "ntt_chunk_fwd") return ntt_chunk_fwd;
+    if (key == "ntt_chunk_inv") return ntt_chunk_inv;
+    if (key == "graphs") return use_graphs;
+    if (key == "graph_cache") return graph_cache_max;
+    if (key == "conv_variant") return conv_variant;
+    if (key == "fuse_ip") return fuse_ip;
+    fail(CKH_ERR_INVALID_PARAMS, "unknown tuning key " + key);
 }
 
 // NTT chunk size: 16 polynomials of every Q and P prime (the limb mix of a key switch),
@@ -57,25 +67,36 @@ std::string Context::autotune() {
     ps.clear();
     raw.clear();
 
-    // Conversion schedule and row-pass fusion: a batched key switch of 12 polynomials at
-    // levels spread over the chain, under a synthetic key, timed per candidate.
+    // Conversion kernel and row-pass fusion on a batched key switch of 12 polynomials at levels
+    // spread over the chain under a synthetic key.
     const u32 top = nq - 1;
     KskPtr key = ksk_uniform(*this, top, 0, 0x5eed5eed);
     std::vector<PolyPtr> ds;
     std::vector<const Poly*> draw;
     for (u32 i = 0; i < 12; ++i) ds.push_back(poly_uniform(*this, top - (i * top) / 12, 0, true, 0x7eed + i, 0));
     for (auto& p : ds) draw.push_back(p.get());
+    os << "}, \"ks_candidates_ms\": " << autotune_ks(draw, *key);
+    os << ", \"ntt_chunk_fwd\": " << arg_f << ", \"ntt_chunk_inv\": " << arg_i << ", \"conv_variant\": " << conv_variant
+       << ", \"fuse_ip\": " << fuse_ip << "}";
+    return os.str();
+}
+
+// Times every (conversion variant, row-pass fusion) pair on the batched key switch of `ds`
+// under `key` (direct launches, one warm-up and three timed runs each), keeps the fastest, and
+// returns the timings as a JSON object.
+std::string Context::autotune_ks(const std::vector<const Poly*>& ds, const Ksk& key) {
+    if (ds.empty()) fail(CKH_ERR_INVALID_PARAMS, "nothing to tune on");
     const u32 keep_graphs = use_graphs;
     use_graphs = 0;
     hipStream_t s = stream();
     auto time_ks = [&]() {
         clear_plans();
-        (void)ks_switch_batch(*this, draw, *key);  // warm
+        (void)ks_switch_batch(*this, ds, key);  // warm
         hipEvent_t a, b;
         hip_check(hipEventCreate(&a), "event");
         hip_check(hipEventCreate(&b), "event");
         hip_check(hipEventRecord(a, s), "event");
-        for (u32 r = 0; r < 3; ++r) (void)ks_switch_batch(*this, draw, *key);
+        for (u32 r = 0; r < 3; ++r) (void)ks_switch_batch(*this, ds, key);
         hip_check(hipEventRecord(b, s), "event");
         hip_check(hipEventSynchronize(b), "event");
         float ms = 0;
@@ -84,11 +105,12 @@ std::string Context::autotune() {
         (void)hipEventDestroy(b);
         return (double)ms / 3;
     };
+    std::ostringstream os;
+    os << "{";
     double best = 1e30;
     u32 best_var = conv_variant, best_fuse = fuse_ip;
-    os << "}, \"ks_candidates_ms\": {";
-    first = true;
-    for (u32 var : {0u, 1u, 2u, 3u})
+    bool first = true;
+    for (u32 var : {0u, 1u})
         for (u32 fuse : {0u, 1u}) {
             if (fuse && !fuse_ip_possible()) continue;
             conv_variant = var;
@@ -102,8 +124,7 @@ std::string Context::autotune() {
     fuse_ip = best_fuse;
     use_graphs = keep_graphs;
     clear_plans();
-    os << "}, \"ntt_chunk_fwd\": " << arg_f << ", \"ntt_chunk_inv\": " << arg_i << ", \"conv_variant\": " << conv_variant
-       << ", \"fuse_ip\": " << fuse_ip << "}";
+    os << "}";
     return os.str();
 }
 

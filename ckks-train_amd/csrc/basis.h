@@ -73,9 +73,8 @@ struct ConvTables {
 // For each job: coefficient-form input limbs (reduced) -> converted output limbs (reduced,
 // coefficient form): out_t = [x_c]_r with x_c the centered value of the source residues
 // (BaseConverter::convert_exact of [x + h]_Q minus h mod r).
-// variant (a tuning choice): 0 = per-target constants through the scalar cache, 128-bit integer
-// sums for 60-bit targets; 1 = the same with f64 partial sums for 60-bit targets of 50-bit
-// sources; 2 / 3 = constants staged in LDS, 1 / 2 coefficients per thread.
+// variant (a tuning choice): 60-bit targets of 50-bit sources use 128-bit integer sums (0) or
+// f64 partial sums (1).
 void conv_launch(const ConvTables& t, const ConvJob* jobs, u32 njobs, u32 max_tgt, u32 max_src, u32 variant,
                  hipStream_t stream);
 

@@ -40,162 +40,14 @@ CKH_DEV void add_shl(u64& lo, u64& hi, u64 v, u32 k) {
     hi += vh + (lo < vl ? 1 : 0);
 }
 
-// Per-coefficient source values of one conversion: f64 source j: a = y_j, b / c = its high /
-// low 25 bits; integer source j: a / b = the low / high 30 bits of y_j, c = y_j's bits.
-template <u32 MAXS>
-struct ConvVals {
-    double a[MAXS], b[MAXS], c[MAXS];
-    u64 e;
-};
-
-template <u32 MAXS>
-CKH_DEV void conv_sources(const ConvTables& T, const ConvPlan& P, const u64* __restrict__ inp, u32 c, u32 n,
-                          ConvVals<MAXS>& V) {
-    double f = 0.0;
-#pragma unroll
-    for (u32 j = 0; j < MAXS; ++j) {
-        V.a[j] = V.b[j] = V.c[j] = 0.0;
-        if (j < P.nsrc) {
-            const ConvSrc* s = T.src + P.src_first + j;
-            const u64 x = inp[(u64)(P.src_limb0 + j) * n + c];
-            u64 y;
-            if (s->fp) {
-                double sv = u2d(x) + (double)s->h;
-                sv = sv >= s->qf ? sv - s->qf : sv;
-                double v = fmulmod(sv, s->qhinv_f, s->qhinv_q, s->qf);
-                v = v < 0.0 ? v + s->qf : v;
-                y = d2u(v);
-                V.a[j] = v;
-                V.b[j] = __builtin_floor(v * 0x1p-25);
-                V.c[j] = __builtin_fma(-V.b[j], 0x1p25, v);
-            } else {
-                y = shoup(add_mod(x, s->h, s->q), s->qhinv, s->qhinv_s, s->q);
-                V.a[j] = (double)(u32)(y & ((1u << 30) - 1));  // low 30 bits, exact
-                V.b[j] = (double)(u32)(y >> 30);                // high 30 bits, exact
-                V.c[j] = __longlong_as_double((long long)y);
-            }
-            if (P.exact) {
-                const double prod = u64_to_f64_rn(y) * s->inv;
-                f = f + prod;
-            }
-        }
-    }
-    V.e = P.exact ? (u64)f : 0;  // f >= 0: truncation is floor
-}
-
-// One output word of target g from the source values (fpmask: bit j set for an f64 source).
-template <u32 MAXS>
-CKH_DEV u64 conv_target(const ConvTables& T, const ConvPlan& P, const ConvTgt& g, u32 fpmask, const ConvVals<MAXS>& V) {
-    if (g.mode == TGT_SPLIT || g.mode == TGT_FP) {
-        // f64 products y_j c_j mod r (|each| < r); an integer source splits into 30-bit halves,
-        // y_j c_j = yl c_j + yh (2^30 c_j). Sums stay below 2^53 for up to 8 terms between
-        // reductions.
-        const double r = g.rf, rinv = g.rinv;
-        double acc = 0.0;
-        u32 terms = 0;
-#pragma unroll
-        for (u32 j = 0; j < MAXS; ++j) {
-            if (j < P.nsrc) {
-                acc = acc + fmulmod(V.a[j], g.qhat_f[j], g.qhat_q[j], r);
-                ++terms;
-                if (!((fpmask >> j) & 1)) {
-                    acc = acc + fmulmod(V.b[j], g.qhat30_f[j], g.qhat30_q[j], r);
-                    ++terms;
-                }
-                if (terms >= 7) {
-                    acc = freduce(acc, r, rinv);
-                    terms = 0;
-                }
-            }
-        }
-        if (P.exact) acc = acc - fmulmod((double)V.e, g.qmod_f, g.qmod_q, r);
-        acc = acc - g.hmod_f;
-        return d2u(fcanon(acc, r, rinv));
-    }
-    const PrimeConst C = T.pc[g.prime];
-    u64 o;
-    if (g.mode == TGT_WIDE) {
-        // sum_j y_j c_j = sum_{a,b} S_ab 2^(25 a + 20 b), S_ab = sum_j (25-bit half a of y_j)
-        // (20-bit piece b of c_j): every product is below 2^45 and every S_ab below 2^49, so
-        // the f64 sums are exact.
-        double s00 = 0.0, s01 = 0.0, s02 = 0.0, s10 = 0.0, s11 = 0.0, s12 = 0.0;
-#pragma unroll
-        for (u32 j = 0; j < MAXS; ++j) {
-            if (j < P.nsrc) {
-                const double c0 = g.wide[0][j], c1 = g.wide[1][j], c2 = g.wide[2][j];
-                s00 = __builtin_fma(V.c[j], c0, s00);
-                s01 = __builtin_fma(V.c[j], c1, s01);
-                s02 = __builtin_fma(V.c[j], c2, s02);
-                s10 = __builtin_fma(V.b[j], c0, s10);
-                s11 = __builtin_fma(V.b[j], c1, s11);
-                s12 = __builtin_fma(V.b[j], c2, s12);
-            }
-        }
-        u64 lo = 0, hi = 0;
-        add_shl(lo, hi, d2u(s00), 0);
-        add_shl(lo, hi, d2u(s01), 20);
-        add_shl(lo, hi, d2u(s10), 25);
-        add_shl(lo, hi, d2u(s02), 40);
-        add_shl(lo, hi, d2u(s11), 45);
-        add_shl(lo, hi, d2u(s12), 65);
-        o = reduce128(hi, lo, C);
-    } else {
-        Acc128 acc = {0, 0};
-#pragma unroll
-        for (u32 j = 0; j < MAXS; ++j)
-            if (j < P.nsrc) {
-                const u64 y = ((fpmask >> j) & 1) ? d2u(V.a[j]) : (u64)__double_as_longlong(V.c[j]);
-                mac(acc, y, g.qhat[j]);
-            }
-        o = reduce128(acc.hi, acc.lo, C);
-    }
-    if (P.exact) o = sub_mod(o, shoup(V.e, g.qmod, g.qmod_s, C.q), C.q);
-    return sub_mod(o, g.hmod, C.q);
-}
-
 constexpr u32 CONV_TG = 8;  // targets per block (z slice of the grid)
 
-// One block: CPT x 256 coefficients of one job, targets [t0, t0 + CONV_TG) of its plan, whose
-// constants are staged in LDS once per block.
-template <u32 MAXS, u32 CPT>
-__global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __restrict__ jobs) {
-    __shared__ ConvTgt tl[CONV_TG];
-    const u32 n = 1u << T.log_n;
-    const ConvJob job = jobs[blockIdx.y];
-    const ConvPlan P = T.plans[job.plan];
-    const u32 t0 = blockIdx.z * CONV_TG;
-    if (t0 >= P.ntgt) return;  // uniform over the block
-    const u32 nt = min(CONV_TG, P.ntgt - t0);
-    {
-        const u64* src = (const u64*)(T.tgt + P.tgt_first + t0);
-        u64* dst = (u64*)tl;
-        const u32 words = nt * (u32)(sizeof(ConvTgt) / 8);
-        for (u32 w = threadIdx.x; w < words; w += 256) dst[w] = src[w];
-    }
-    __syncthreads();
-    u32 fpmask = 0;
-    for (u32 j = 0; j < P.nsrc; ++j) fpmask |= (T.src[P.src_first + j].fp ? 1u : 0u) << j;
-    ConvVals<MAXS> V[CPT];
-    u32 cs[CPT];
-#pragma unroll
-    for (u32 i = 0; i < CPT; ++i) {
-        cs[i] = (blockIdx.x * CPT + i) * 256 + threadIdx.x;
-        if (cs[i] < n) conv_sources<MAXS>(T, P, job.in, cs[i], n, V[i]);
-    }
-    u64* outp = job.out;
-    for (u32 t = 0; t < nt; ++t) {
-        const ConvTgt& g = tl[t];
-#pragma unroll
-        for (u32 i = 0; i < CPT; ++i)
-            if (cs[i] < n) outp[(u64)g.limb * n + cs[i]] = conv_target<MAXS>(T, P, g, fpmask, V[i]);
-    }
-}
-
-// Variant with per-target constants read through the scalar cache, one coefficient per thread
-// (WIDE: 60-bit targets of f64 sources use the partial sums of conv_target; otherwise the
-// 128-bit integer sums).
+// One block: 256 coefficients of one job, targets [t0, t0 + CONV_TG) of its plan; per-target
+// constants are read through the scalar cache. 60-bit targets of f64 sources use (WIDE) six
+// exact f64 partial sums of 25-bit source halves times 20-bit constant pieces combined into 128
+// bits, or 128-bit integer sums.
 template <u32 MAXS, bool WIDE>
-__global__ __launch_bounds__(256) void k_conv_sc(ConvTables T, const ConvJob* __restrict__ jobs) {
+__global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __restrict__ jobs) {
     const u32 n = 1u << T.log_n;
     const u32 c = blockIdx.x * 256 + threadIdx.x;
     const ConvJob job = jobs[blockIdx.y];
@@ -245,11 +97,9 @@ __global__ __launch_bounds__(256) void k_conv_sc(ConvTables T, const ConvJob* __
             double acc = 0.0;
 #pragma unroll
             for (u32 j = 0; j < MAXS; ++j) {
-                if (j < P.nsrc) {
-                    acc = acc + fmulmod(yd[j], g->qhat_f[j], g->qhat_q[j], r);
-                    acc = acc + fmulmod(yh[j], g->qhat30_f[j], g->qhat30_q[j], r);
-                    if (j & 1) acc = freduce(acc, r, rinv);
-                }
+                acc = acc + fmulmod(yd[j], g->qhat_f[j], g->qhat_q[j], r);
+                acc = acc + fmulmod(yh[j], g->qhat30_f[j], g->qhat30_q[j], r);
+                if (j & 1) acc = freduce(acc, r, rinv);
             }
             if (P.exact) acc = acc - fmulmod((double)e, g->qmod_f, g->qmod_q, r);
             acc = acc - g->hmod_f;
@@ -259,10 +109,8 @@ __global__ __launch_bounds__(256) void k_conv_sc(ConvTables T, const ConvJob* __
             double acc = 0.0;
 #pragma unroll
             for (u32 j = 0; j < MAXS; ++j) {
-                if (j < P.nsrc) {
-                    acc = acc + fmulmod(yd[j], g->qhat_f[j], g->qhat_q[j], r);
-                    if ((j & 3) == 3) acc = freduce(acc, r, rinv);
-                }
+                acc = acc + fmulmod(yd[j], g->qhat_f[j], g->qhat_q[j], r);
+                if ((j & 3) == 3) acc = freduce(acc, r, rinv);
             }
             if (P.exact) acc = acc - fmulmod((double)e, g->qmod_f, g->qmod_q, r);
             acc = acc - g->hmod_f;
@@ -273,17 +121,15 @@ __global__ __launch_bounds__(256) void k_conv_sc(ConvTables T, const ConvJob* __
                 double s00 = 0.0, s01 = 0.0, s02 = 0.0, s10 = 0.0, s11 = 0.0, s12 = 0.0;
 #pragma unroll
                 for (u32 j = 0; j < MAXS; ++j) {
-                    if (j < P.nsrc) {
-                        const double hj = __builtin_floor(yd[j] * 0x1p-25);
-                        const double lj = __builtin_fma(-hj, 0x1p25, yd[j]);
-                        const double c0 = g->wide[0][j], c1 = g->wide[1][j], c2 = g->wide[2][j];
-                        s00 = __builtin_fma(lj, c0, s00);
-                        s01 = __builtin_fma(lj, c1, s01);
-                        s02 = __builtin_fma(lj, c2, s02);
-                        s10 = __builtin_fma(hj, c0, s10);
-                        s11 = __builtin_fma(hj, c1, s11);
-                        s12 = __builtin_fma(hj, c2, s12);
-                    }
+                    const double hj = __builtin_floor(yd[j] * 0x1p-25);
+                    const double lj = __builtin_fma(-hj, 0x1p25, yd[j]);
+                    const double c0 = g->wide[0][j], c1 = g->wide[1][j], c2 = g->wide[2][j];
+                    s00 = __builtin_fma(lj, c0, s00);
+                    s01 = __builtin_fma(lj, c1, s01);
+                    s02 = __builtin_fma(lj, c2, s02);
+                    s10 = __builtin_fma(hj, c0, s10);
+                    s11 = __builtin_fma(hj, c1, s11);
+                    s12 = __builtin_fma(hj, c2, s12);
                 }
                 u64 lo = 0, hi = 0;
                 add_shl(lo, hi, d2u(s00), 0);
@@ -296,8 +142,7 @@ __global__ __launch_bounds__(256) void k_conv_sc(ConvTables T, const ConvJob* __
             } else {
                 Acc128 acc = {0, 0};
 #pragma unroll
-                for (u32 j = 0; j < MAXS; ++j)
-                    if (j < P.nsrc) mac(acc, y[j], g->qhat[j]);
+                for (u32 j = 0; j < MAXS; ++j) mac(acc, y[j], g->qhat[j]);
                 o = reduce128(acc.hi, acc.lo, C);
             }
             if (P.exact) o = sub_mod(o, shoup(e, g->qmod, g->qmod_s, C.q), C.q);
@@ -312,27 +157,28 @@ static void conv_launch_s(const ConvTables& t, const ConvJob* jobs, u32 njobs, u
                           hipStream_t stream) {
     const u32 n = 1u << t.log_n;
     const u32 zt = (max_tgt + CONV_TG - 1) / CONV_TG;
-    if (variant == 0)
-        hipLaunchKernelGGL((k_conv_sc<MAXS, false>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs);
-    else if (variant == 1)
-        hipLaunchKernelGGL((k_conv_sc<MAXS, true>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs);
-    else if (variant >= 3 && n >= 512)
-        hipLaunchKernelGGL((k_conv<MAXS, 2>), dim3(n / 512, njobs, zt), dim3(256), 0, stream, t, jobs);
+    if (variant == 1)
+        hipLaunchKernelGGL((k_conv<MAXS, true>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs);
     else
-        hipLaunchKernelGGL((k_conv<MAXS, 1>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs);
+        hipLaunchKernelGGL((k_conv<MAXS, false>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs);
+
 }
 
 void conv_launch(const ConvTables& t, const ConvJob* jobs, u32 njobs, u32 max_tgt, u32 max_src, u32 variant,
                  hipStream_t stream) {
     if (njobs == 0 || max_tgt == 0) return;
-    if (max_src <= 1)
-        conv_launch_s<1>(t, jobs, njobs, max_tgt, variant, stream);
-    else if (max_src <= 4)
-        conv_launch_s<4>(t, jobs, njobs, max_tgt, variant, stream);
-    else if (max_src <= 8)
-        conv_launch_s<8>(t, jobs, njobs, max_tgt, variant, stream);
-    else
-        conv_launch_s<16>(t, jobs, njobs, max_tgt, variant, stream);
+    switch (max_src) {
+        case 0:
+        case 1: conv_launch_s<1>(t, jobs, njobs, max_tgt, variant, stream); break;
+        case 2: conv_launch_s<2>(t, jobs, njobs, max_tgt, variant, stream); break;
+        case 3: conv_launch_s<3>(t, jobs, njobs, max_tgt, variant, stream); break;
+        case 4: conv_launch_s<4>(t, jobs, njobs, max_tgt, variant, stream); break;
+        case 5: conv_launch_s<5>(t, jobs, njobs, max_tgt, variant, stream); break;
+        case 6: conv_launch_s<6>(t, jobs, njobs, max_tgt, variant, stream); break;
+        case 7: conv_launch_s<7>(t, jobs, njobs, max_tgt, variant, stream); break;
+        case 8: conv_launch_s<8>(t, jobs, njobs, max_tgt, variant, stream); break;
+        default: conv_launch_s<16>(t, jobs, njobs, max_tgt, variant, stream); break;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_finish(const FinishJob* __restrict__ jobs,

@@ -171,10 +171,32 @@ ckh_status ckh_autotune(ckh_context* ctx, char* report, size_t report_len) {
     });
 }
 
+ckh_status ckh_autotune_ks(ckh_context* ctx, uint32_t n, const ckh_poly* const* d, const ckh_ksk* key, char* report,
+                           size_t report_len) {
+    return guarded([&] {
+        need(d, "d");
+        std::vector<const Poly*> ds;
+        for (uint32_t i = 0; i < n; ++i) ds.push_back(&poly_of(d[i]));
+        std::string r = ctx_of(ctx).autotune_ks(ds, key_of(key));
+        if (report && report_len) {
+            std::strncpy(report, r.c_str(), report_len - 1);
+            report[report_len - 1] = 0;
+        }
+    });
+}
+
 ckh_status ckh_set_tuning(ckh_context* ctx, const char* key, int64_t value) {
     return guarded([&] {
         need(key, "key");
         ctx_of(ctx).set_tuning(key, value);
+    });
+}
+
+ckh_status ckh_get_tuning(ckh_context* ctx, const char* key, int64_t* value) {
+    return guarded([&] {
+        need(key, "key");
+        need(value, "value");
+        *value = ctx_of(ctx).get_tuning(key);
     });
 }
 

@@ -193,7 +193,10 @@ class Context : public std::enable_shared_from_this<Context> {
     u32 ntt_chunk_fwd = 0, ntt_chunk_inv = 0;
     u32 conv_variant = 0;  // base-conversion kernel variant (basis.h, conv_launch)
     std::string autotune();
+    // The key-switch candidates of autotune() timed on a caller's batch (autotune.cpp).
+    std::string autotune_ks(const std::vector<const Poly*>& ds, const Ksk& key);
     void set_tuning(const std::string& key, int64_t value);
+    int64_t get_tuning(const std::string& key) const;
     // Copies a host blob to a fresh device buffer on stream s.
     BufPtr upload_blob(const void* host, size_t bytes, hipStream_t s);
 

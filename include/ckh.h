@@ -84,10 +84,18 @@ void *ckh_stream_handle(ckh_context *ctx);
  * on this device with synthetic limbs of the context's primes and keeps the fastest; writes a
  * JSON report (candidates and choices) to report (may be NULL). */
 ckh_status ckh_autotune(ckh_context *ctx, char *report, size_t report_len);
+/* The autotuner's key-switch candidates (conversion kernel variant, row-pass fusion) timed on
+ * the caller's own batch: n NTT-form polynomials switched under key (no counterpart in the
+ * reference). Keeps the fastest; writes the timings as JSON to report (may be NULL). */
+ckh_status ckh_autotune_ks(ckh_context *ctx, uint32_t n, const ckh_poly *const *d, const ckh_ksk *key,
+                           char *report, size_t report_len);
 /* Forces one tuning value: "ntt_chunk_fwd", "ntt_chunk_inv" (limbs per launch, 0 = all);
  * "graphs" (1 = replay repeated launch sequences from captured hipGraphs, the default; 0 = launch
- * directly); "graph_cache" (captured sequences kept per context). */
+ * directly); "graph_cache" (captured sequences kept per context); "conv_variant" (0 or 1, the
+ * base-conversion kernel); "fuse_ip" (1 = row pass fused into the inner product). */
 ckh_status ckh_set_tuning(ckh_context *ctx, const char *key, int64_t value);
+/* Reads one tuning value (keys as ckh_set_tuning, plus "conv_variant" and "fuse_ip"). */
+ckh_status ckh_get_tuning(ckh_context *ctx, const char *key, int64_t *value);
 /* Captured hipGraphs so far and their replays (no counterpart in the reference). */
 ckh_status ckh_graph_stats(ckh_context *ctx, uint64_t *captures, uint64_t *launches);
 /* Waits for all work of the context. */

@@ -444,7 +444,7 @@ def test_schedules_give_the_same_switch(case):
     lv = sorted({L, L // 2, 1, 0, max(L - 3, 0)})
     polys = [ctx.uniform(l, seed=600 + i) for i, l in enumerate(lv)]
     want = None
-    for var in (0, 1, 2, 3):
+    for var in (0, 1):
         for fuse in (0, 1):
             ctx.set_tuning("conv_variant", var)
             ctx.set_tuning("fuse_ip", fuse)
