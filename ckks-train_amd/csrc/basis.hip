@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
     const u32 t0 = blockIdx.z * CONV_TG;
     if (t0 >= P.ntgt || c >= n) return;
     const u32 t1 = min(t0 + CONV_TG, P.ntgt);
-    const u64* inp = job.in;
+    const auto inp = glb(job.in);
     u64 y[MAXS];
     // f64 source: yd = y, yh = 0; integer source: yd / yh = its low / high 30 bits
     double yd[MAXS], yh[MAXS];
@@ -87,9 +87,9 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
         }
     }
     const u64 e = P.exact ? (u64)f : 0;  // f >= 0: truncation is floor
-    u64* outp = job.out;
+    const auto outp = glb(job.out);
     for (u32 t = t0; t < t1; ++t) {
-        const ConvTgt* g = T.tgt + P.tgt_first + t;
+        const auto* g = cst(T.tgt + P.tgt_first + t);
         u64 o;
         if (g->mode == TGT_SPLIT) {
             // y_j = yh 2^30 + yl: y_j c = yh (2^30 c) + yl c (mod r), four terms between reductions.
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
             acc = acc - g->hmod_f;
             o = d2u(fcanon(acc, r, rinv));
         } else {
-            const PrimeConst C = T.pc[g->prime];
+            const PrimeConst C = pc_int(T.pc + g->prime);
             if (WIDE && g->mode == TGT_WIDE) {
                 double s00 = 0.0, s01 = 0.0, s02 = 0.0, s10 = 0.0, s11 = 0.0, s12 = 0.0;
 #pragma unroll
@@ -188,9 +188,9 @@ __global__ __launch_bounds__(256) void k_finish(const FinishJob* __restrict__ jo
     if (c >= n) return;
     const FinishJob job = jobs[blockIdx.y];
     const u64 q = pc[job.prime].q;
-    u64 v = shoup(sub_mod(job.a[c], job.t[c], q), job.inv, job.inv_s, q);
-    if (job.add) v = add_mod(v, job.add[c], q);
-    job.out[c] = v;
+    u64 v = shoup(sub_mod(glb(job.a)[c], glb(job.t)[c], q), job.inv, job.inv_s, q);
+    if (job.add) v = add_mod(v, glb(job.add)[c], q);
+    glb(job.out)[c] = v;
 }
 
 void finish_launch(const FinishJob* jobs, u32 njobs, const PrimeConst* pc, u32 log_n, hipStream_t stream) {

@@ -109,6 +109,33 @@ CKH_DEV double fcanon(double x, double q, double qinv) {
     return r < 0.0 ? r + q : r;
 }
 
+// A pointer to memory that no thread writes during the kernel (tables, plans), as a constant
+// address-space pointer: loads through it may use the scalar cache even after the kernel's own
+// stores, which the compiler otherwise cannot prove do not alias.
+template <class T>
+CKH_DEV const __attribute__((address_space(4))) T* cst(const T* p) {
+    return (const __attribute__((address_space(4))) T*)p;
+}
+
+// The integer-reduction constants of a prime, read through the scalar cache.
+CKH_DEV PrimeConst pc_int(const PrimeConst* p) {
+    const auto* c = cst(p);
+    PrimeConst r{};
+    r.q = c->q;
+    r.one_s = c->one_s;
+    r.two64 = c->two64;
+    r.two64_s = c->two64_s;
+    return r;
+}
+
+// A device pointer read from a descriptor, as a global address-space pointer: accesses through
+// it compile to global (not flat) memory instructions, which do not share the scalar loads'
+// wait counter.
+template <class T>
+CKH_DEV __attribute__((address_space(1))) T* glb(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+
 // 128-bit accumulator.
 struct Acc128 {
     u64 lo, hi;

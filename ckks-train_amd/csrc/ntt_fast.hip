@@ -241,8 +241,10 @@ CKH_DEV void rows_round2_inv(const Ar& ar, typename Ar::V (&v)[16], u32 A, u32 r
 // R2 low bits vary, plus U of the top bits).
 
 template <u32 M, class Ar>
-CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out, u64* lds, u32 log_n,
+CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in_p, u64* __restrict__ out_p, u64* lds, u32 log_n,
                       u32 tile) {
+    const auto in = glb(in_p);
+    const auto out = glb(out_p);
     constexpr u32 R2 = M - 4, U = 4 - R2, TCL = TL - M, TC = 1u << TCL, JM = (1u << R2) - 1;
     const u32 B = log_n - M;
     const u32 tid = threadIdx.x, cc = tid & (TC - 1), o = tid >> TCL;
@@ -287,7 +289,8 @@ CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
 // The row pass of one tile, its canonical outputs left in LDS in natural order (word i of the
 // tile at pad(i)): as u64 residues, or (F64) as the f64 bits of the residue.
 template <u32 M, bool F64, class Ar>
-CKH_DEV void fwd_rows_to_lds(const Ar& ar, const u64* __restrict__ in, u64* lds, u32 log_n, u32 tile) {
+CKH_DEV void fwd_rows_to_lds(const Ar& ar, const u64* __restrict__ in_p, u64* lds, u32 log_n, u32 tile) {
+    const auto in = glb(in_p);
     constexpr u32 R2 = M - 4, U = 4 - R2, TRL = TL - M, JM = (1u << R2) - 1;
     const u32 A = log_n - M;
     const u32 tid = threadIdx.x, o = tid & JM, rr = tid >> R2;
@@ -319,20 +322,23 @@ CKH_DEV void fwd_rows_to_lds(const Ar& ar, const u64* __restrict__ in, u64* lds,
 }
 
 template <u32 M, class Ar>
-CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out, u64* lds, u32 log_n,
+CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out_p, u64* lds, u32 log_n,
                       const FinishJob* fin, u64 q, u32 tile) {
     const u32 tid = threadIdx.x;
+    const auto out = glb(out_p);
     fwd_rows_to_lds<M, false>(ar, in, lds, log_n, tile);
     const u64 tbase = (u64)tile << TL;
     if (fin) {
         // finish of rescale / mod_down: (a - t) * inv (+ add)
         const u64 inv = fin->inv, inv_s = fin->inv_s;
+        const auto fa = glb(fin->a), fadd = glb(fin->add);
+        const auto fout = glb(fin->out);
 #pragma unroll
         for (u32 k = 0; k < 16; ++k) {
             const u64 i = tbase + tid + k * NT;
-            u64 v = shoup(sub_mod(fin->a[i], lds[pad(tid + k * NT)], q), inv, inv_s, q);
-            if (fin->add) v = add_mod(v, fin->add[i], q);
-            fin->out[i] = v;
+            u64 v = shoup(sub_mod(fa[i], lds[pad(tid + k * NT)], q), inv, inv_s, q);
+            if (fin->add) v = add_mod(v, fadd[i], q);
+            fout[i] = v;
         }
         return;
     }
@@ -359,15 +365,15 @@ CKH_DEV void rows_ip(const FpAr& ar, const RowIpJob& J, u64* lds, u32 log_n, u32
         double x[16];
         if (j == J.own_j) {
 #pragma unroll
-            for (u32 k = 0; k < 16; ++k) x[k] = u2d(J.own[tbase + tid + k * NT]);
+            for (u32 k = 0; k < 16; ++k) x[k] = u2d(glb(J.own)[tbase + tid + k * NT]);
         } else {
             fwd_rows_to_lds<M, true>(ar, J.mid + j * J.mid_stride, lds, log_n, tile);
 #pragma unroll
             for (u32 k = 0; k < 16; ++k) x[k] = __longlong_as_double((long long)lds[pad(tid + k * NT)]);
             __syncthreads();  // the next digit's pass reuses the LDS
         }
-        const u64* kb = J.key + (u64)(2 * j) * J.key_poly + tbase;
-        const u64* ka = kb + J.key_poly;
+        const auto kb = glb(J.key + (u64)(2 * j) * J.key_poly + tbase);
+        const auto ka = glb(J.key + (u64)(2 * j + 1) * J.key_poly + tbase);
 #pragma unroll
         for (u32 k = 0; k < 16; ++k) {
             const double b = u2d(kb[tid + k * NT]), a = u2d(ka[tid + k * NT]);
@@ -384,14 +390,16 @@ CKH_DEV void rows_ip(const FpAr& ar, const RowIpJob& J, u64* lds, u32 log_n, u32
     }
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) {
-        J.acc_b[tbase + tid + k * NT] = d2u(fcanon(sb[k], q, qinv));
-        J.acc_a[tbase + tid + k * NT] = d2u(fcanon(sa[k], q, qinv));
+        glb(J.acc_b)[tbase + tid + k * NT] = d2u(fcanon(sb[k], q, qinv));
+        glb(J.acc_a)[tbase + tid + k * NT] = d2u(fcanon(sa[k], q, qinv));
     }
 }
 
 template <u32 M, class Ar>
-CKH_DEV void inv_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out, u64* lds, u32 log_n,
+CKH_DEV void inv_rows(const Ar& ar, const u64* __restrict__ in_p, u64* __restrict__ out_p, u64* lds, u32 log_n,
                       u32 tile) {
+    const auto in = glb(in_p);
+    const auto out = glb(out_p);
     constexpr u32 R2 = M - 4, U = 4 - R2, TRL = TL - M, JM = (1u << R2) - 1;
     const u32 tid = threadIdx.x, o = tid & JM, rr = tid >> R2;
     const u32 row = (tile << TRL) + rr;
@@ -429,7 +437,8 @@ CKH_DEV void inv_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
 }
 
 template <u32 M, class Ar>
-CKH_DEV void inv_cols(const Ar& ar, u64* __restrict__ io, u64* lds, u32 log_n, u32 tile) {
+CKH_DEV void inv_cols(const Ar& ar, u64* __restrict__ io_p, u64* lds, u32 log_n, u32 tile) {
+    const auto io = glb(io_p);
     constexpr u32 R2 = M - 4, U = 4 - R2, TCL = TL - M, TC = 1u << TCL, JM = (1u << R2) - 1;
     const u32 B = log_n - M;
     const u32 tid = threadIdx.x, cc = tid & (TC - 1), o = tid >> TCL;

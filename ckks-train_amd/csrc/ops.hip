@@ -30,10 +30,11 @@ __global__ __launch_bounds__(256) void k_inner_product(const IpJob* __restrict__
         const double q = C.qf, qinv = C.qinv;
         double sb = 0.0, sa = 0.0;
         for (u32 j = 0; j < J.ndig; ++j) {
-            const u64* src = (J.own && j == J.own_j) ? J.own : J.dig + j * J.dig_stride;
+            const auto src = glb((J.own && j == J.own_j) ? J.own : J.dig + j * J.dig_stride);
+            const auto key = glb(J.key);
             const double d = u2d(src[cs]);
-            const double b = u2d(J.key[(2 * j) * J.key_poly + c]);
-            const double a = u2d(J.key[(2 * j + 1) * J.key_poly + c]);
+            const double b = u2d(key[(2 * j) * J.key_poly + c]);
+            const double a = u2d(key[(2 * j + 1) * J.key_poly + c]);
             sb += fmulmod(d, b, b * qinv, q);
             sa += fmulmod(d, a, a * qinv, q);
             if ((j & 3) == 3) {
@@ -41,21 +42,22 @@ __global__ __launch_bounds__(256) void k_inner_product(const IpJob* __restrict__
                 sa = freduce(sa, q, qinv);
             }
         }
-        J.acc_b[c] = d2u(fcanon(sb, q, qinv));
-        J.acc_a[c] = d2u(fcanon(sa, q, qinv));
+        glb(J.acc_b)[c] = d2u(fcanon(sb, q, qinv));
+        glb(J.acc_a)[c] = d2u(fcanon(sa, q, qinv));
         return;
     }
     Acc128 sb = {0, 0}, sa = {0, 0};
     for (u32 j = 0; j < J.ndig; ++j) {
-        const u64* src = (J.own && j == J.own_j) ? J.own : J.dig + j * J.dig_stride;
+        const auto src = glb((J.own && j == J.own_j) ? J.own : J.dig + j * J.dig_stride);
+        const auto key = glb(J.key);
         const u64 d = src[cs];
-        const u64 b = J.key[(2 * j) * J.key_poly + c];
-        const u64 k = J.key[(2 * j + 1) * J.key_poly + c];
+        const u64 b = key[(2 * j) * J.key_poly + c];
+        const u64 k = key[(2 * j + 1) * J.key_poly + c];
         mac(sb, d, b);
         mac(sa, d, k);
     }
-    J.acc_b[c] = reduce128(sb.hi, sb.lo, C);
-    J.acc_a[c] = reduce128(sa.hi, sa.lo, C);
+    glb(J.acc_b)[c] = reduce128(sb.hi, sb.lo, C);
+    glb(J.acc_a)[c] = reduce128(sa.hi, sa.lo, C);
 }
 
 void inner_product_launch(const IpJob* jobs, u32 njobs, const PrimeConst* pc, u32 log_n, hipStream_t stream) {
@@ -69,7 +71,7 @@ __global__ __launch_bounds__(256) void k_copy(const LimbRef* __restrict__ refs, 
     const u32 c = blockIdx.x * 256 + threadIdx.x;
     if (c >= n) return;
     const LimbRef r = refs[blockIdx.y];
-    r.dst[c] = r.src[c];
+    glb(r.dst)[c] = glb(r.src)[c];
 }
 
 void copy_launch(const LimbRef* refs, u32 nrefs, u32 log_n, hipStream_t stream) {
