@@ -257,7 +257,8 @@ def main():
     if not args.batched_only:
         # the key-switch schedule chosen on the workload itself (untimed)
         tuning["ks_on_workload_ms"] = ctx.autotune_ks(polys, key)
-        tuning["conv_variant"], tuning["fuse_ip"] = ctx.tuning("conv_variant"), ctx.tuning("fuse_ip")
+        for k in ("conv_variant", "conv_tg", "fuse_ip"):
+            tuning[k] = ctx.tuning(k)
         print(f"# autotune on the workload: {json.dumps(tuning['ks_on_workload_ms'])}", file=sys.stderr, flush=True)
     print(f"# workload: {N_CT} polys, limbs 4..24 (mean {np.mean(levels) + 1:.1f}), key {key.info()[2] / 2**20:.0f} MiB",
           file=sys.stderr, flush=True)

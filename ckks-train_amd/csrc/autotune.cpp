@@ -19,6 +19,8 @@ void Context::set_tuning(const std::string& key, int64_t value) {
         conv_variant = (u32)std::min<int64_t>(value, 1);
     else if (key == "fuse_ip")
         fuse_ip = value != 0;
+    else if (key == "conv_tg")
+        conv_tg = (u32)std::min<int64_t>(std::max<int64_t>(value, 1), 64);
     else if (key == "graph_cache")
         graph_cache_max = (u32)std::max<int64_t>(value, 1);
     else
@@ -33,6 +35,7 @@ int64_t Context::get_tuning(const std::string& key) const {
     if (key == "graph_cache") return graph_cache_max;
     if (key == "conv_variant") return conv_variant;
     if (key == "fuse_ip") return fuse_ip;
+    if (key == "conv_tg") return conv_tg;
     fail(CKH_ERR_INVALID_PARAMS, "unknown tuning key " + key);
 }
 
@@ -81,7 +84,7 @@ std::string Context::autotune() {
     return os.str();
 }
 
-// Times every (conversion variant, row-pass fusion) pair on the batched key switch of `ds`
+// Times every (conversion variant, targets per workgroup, row-pass fusion) choice on the batched key switch of `ds`
 // under `key` (direct launches, one warm-up and three timed runs each), keeps the fastest, and
 // returns the timings as a JSON object.
 std::string Context::autotune_ks(const std::vector<const Poly*>& ds, const Ksk& key) {
@@ -108,19 +111,22 @@ std::string Context::autotune_ks(const std::vector<const Poly*>& ds, const Ksk& 
     std::ostringstream os;
     os << "{";
     double best = 1e30;
-    u32 best_var = conv_variant, best_fuse = fuse_ip;
+    u32 best_var = conv_variant, best_fuse = fuse_ip, best_tg = conv_tg;
     bool first = true;
     for (u32 var : {0u, 1u})
-        for (u32 fuse : {0u, 1u}) {
-            if (fuse && !fuse_ip_possible()) continue;
-            conv_variant = var;
-            fuse_ip = fuse;
-            const double t = time_ks();
-            os << (first ? "" : ", ") << "\"conv" << var << (fuse ? "_fused" : "") << "\": " << t;
-            first = false;
-            if (t < best) best = t, best_var = var, best_fuse = fuse;
-        }
+        for (u32 tg : {8u, 16u, 24u})
+            for (u32 fuse : {0u, 1u}) {
+                if (fuse && !fuse_ip_possible()) continue;
+                conv_variant = var;
+                conv_tg = tg;
+                fuse_ip = fuse;
+                const double t = time_ks();
+                os << (first ? "" : ", ") << "\"conv" << var << "_tg" << tg << (fuse ? "_fused" : "") << "\": " << t;
+                first = false;
+                if (t < best) best = t, best_var = var, best_fuse = fuse, best_tg = tg;
+            }
     conv_variant = best_var;
+    conv_tg = best_tg;
     fuse_ip = best_fuse;
     use_graphs = keep_graphs;
     clear_plans();

@@ -73,10 +73,10 @@ struct ConvTables {
 // For each job: coefficient-form input limbs (reduced) -> converted output limbs (reduced,
 // coefficient form): out_t = [x_c]_r with x_c the centered value of the source residues
 // (BaseConverter::convert_exact of [x + h]_Q minus h mod r).
-// variant (a tuning choice): 60-bit targets of 50-bit sources use 128-bit integer sums (0) or
-// f64 partial sums (1).
+// Tuning choices: variant: 60-bit targets of 50-bit sources use 128-bit integer sums (0) or f64
+// partial sums (1); tg: targets per workgroup (the sources are read once per group of tg).
 void conv_launch(const ConvTables& t, const ConvJob* jobs, u32 njobs, u32 max_tgt, u32 max_src, u32 variant,
-                 hipStream_t stream);
+                 u32 tg, hipStream_t stream);
 
 void finish_launch(const FinishJob* jobs, u32 njobs, const PrimeConst* pc, u32 log_n,
                    hipStream_t stream);

@@ -40,21 +40,19 @@ CKH_DEV void add_shl(u64& lo, u64& hi, u64 v, u32 k) {
     hi += vh + (lo < vl ? 1 : 0);
 }
 
-constexpr u32 CONV_TG = 8;  // targets per block (z slice of the grid)
-
-// One block: 256 coefficients of one job, targets [t0, t0 + CONV_TG) of its plan; per-target
+// One block: 256 coefficients of one job, targets [t0, t0 + tg) of its plan; per-target
 // constants are read through the scalar cache. 60-bit targets of f64 sources use (WIDE) six
 // exact f64 partial sums of 25-bit source halves times 20-bit constant pieces combined into 128
 // bits, or 128-bit integer sums.
 template <u32 MAXS, bool WIDE>
-__global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __restrict__ jobs) {
+__global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __restrict__ jobs, u32 tg) {
     const u32 n = 1u << T.log_n;
     const u32 c = blockIdx.x * 256 + threadIdx.x;
     const ConvJob job = jobs[blockIdx.y];
     const ConvPlan P = T.plans[job.plan];
-    const u32 t0 = blockIdx.z * CONV_TG;
+    const u32 t0 = blockIdx.z * tg;
     if (t0 >= P.ntgt || c >= n) return;
-    const u32 t1 = min(t0 + CONV_TG, P.ntgt);
+    const u32 t1 = min(t0 + tg, P.ntgt);
     const auto inp = glb(job.in);
     u64 y[MAXS];
     // f64 source: yd = y, yh = 0; integer source: yd / yh = its low / high 30 bits
@@ -153,31 +151,31 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
 }
 
 template <u32 MAXS>
-static void conv_launch_s(const ConvTables& t, const ConvJob* jobs, u32 njobs, u32 max_tgt, u32 variant,
+static void conv_launch_s(const ConvTables& t, const ConvJob* jobs, u32 njobs, u32 max_tgt, u32 variant, u32 tg,
                           hipStream_t stream) {
     const u32 n = 1u << t.log_n;
-    const u32 zt = (max_tgt + CONV_TG - 1) / CONV_TG;
+    const u32 zt = (max_tgt + tg - 1) / tg;
     if (variant == 1)
-        hipLaunchKernelGGL((k_conv<MAXS, true>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs);
+        hipLaunchKernelGGL((k_conv<MAXS, true>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs, tg);
     else
-        hipLaunchKernelGGL((k_conv<MAXS, false>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs);
-
+        hipLaunchKernelGGL((k_conv<MAXS, false>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs, tg);
 }
 
 void conv_launch(const ConvTables& t, const ConvJob* jobs, u32 njobs, u32 max_tgt, u32 max_src, u32 variant,
-                 hipStream_t stream) {
+                 u32 tg, hipStream_t stream) {
     if (njobs == 0 || max_tgt == 0) return;
+    if (tg == 0) tg = 8;
     switch (max_src) {
         case 0:
-        case 1: conv_launch_s<1>(t, jobs, njobs, max_tgt, variant, stream); break;
-        case 2: conv_launch_s<2>(t, jobs, njobs, max_tgt, variant, stream); break;
-        case 3: conv_launch_s<3>(t, jobs, njobs, max_tgt, variant, stream); break;
-        case 4: conv_launch_s<4>(t, jobs, njobs, max_tgt, variant, stream); break;
-        case 5: conv_launch_s<5>(t, jobs, njobs, max_tgt, variant, stream); break;
-        case 6: conv_launch_s<6>(t, jobs, njobs, max_tgt, variant, stream); break;
-        case 7: conv_launch_s<7>(t, jobs, njobs, max_tgt, variant, stream); break;
-        case 8: conv_launch_s<8>(t, jobs, njobs, max_tgt, variant, stream); break;
-        default: conv_launch_s<16>(t, jobs, njobs, max_tgt, variant, stream); break;
+        case 1: conv_launch_s<1>(t, jobs, njobs, max_tgt, variant, tg, stream); break;
+        case 2: conv_launch_s<2>(t, jobs, njobs, max_tgt, variant, tg, stream); break;
+        case 3: conv_launch_s<3>(t, jobs, njobs, max_tgt, variant, tg, stream); break;
+        case 4: conv_launch_s<4>(t, jobs, njobs, max_tgt, variant, tg, stream); break;
+        case 5: conv_launch_s<5>(t, jobs, njobs, max_tgt, variant, tg, stream); break;
+        case 6: conv_launch_s<6>(t, jobs, njobs, max_tgt, variant, tg, stream); break;
+        case 7: conv_launch_s<7>(t, jobs, njobs, max_tgt, variant, tg, stream); break;
+        case 8: conv_launch_s<8>(t, jobs, njobs, max_tgt, variant, tg, stream); break;
+        default: conv_launch_s<16>(t, jobs, njobs, max_tgt, variant, tg, stream); break;
     }
 }
 

@@ -360,9 +360,9 @@ static void launch_copy(const LimbRef* d, u32 nrefs, u32 log_n, hipStream_t s) {
 }
 
 static void launch_conv(const ConvTables& t, const ConvJob* d, u32 njobs, u32 max_tgt, u32 max_src, u32 variant,
-                        hipStream_t s) {
+                        u32 tg, hipStream_t s) {
     for (u32 i = 0; i < njobs; i += MAX_ROWS)
-        conv_launch(t, d + i, std::min(MAX_ROWS, njobs - i), max_tgt, max_src, variant, s);
+        conv_launch(t, d + i, std::min(MAX_ROWS, njobs - i), max_tgt, max_src, variant, tg, s);
 }
 
 static void launch_ip(const IpJob* d, u32 njobs, const PrimeConst* pc, u32 log_n, hipStream_t s) {
@@ -397,7 +397,7 @@ void Context::run(const Program& p, const uint8_t* d, hipStream_t s) const {
         switch (st.kind) {
             case S_NTT_FWD: launch_ntt(*this, (const LimbRef*)x, st.count, true, s); break;
             case S_NTT_INV: launch_ntt(*this, (const LimbRef*)x, st.count, false, s); break;
-            case S_CONV: launch_conv(conv, (const ConvJob*)x, st.count, st.a, st.b, conv_variant, s); break;
+            case S_CONV: launch_conv(conv, (const ConvJob*)x, st.count, st.a, st.b, conv_variant, conv_tg, s); break;
             case S_COPY: launch_copy((const LimbRef*)x, st.count, log_n, s); break;
             case S_IP: launch_ip((const IpJob*)x, st.count, d_pc, log_n, s); break;
             case S_NTT_FWD_FIN:

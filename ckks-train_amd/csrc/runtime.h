@@ -191,7 +191,7 @@ class Context : public std::enable_shared_from_this<Context> {
     // Tuning (autotune.cpp): limbs per column-pass/row-pass launch pair (0 = all at once). A
     // chunk small enough keeps the pass intermediate in the on-die caches between the passes.
     u32 ntt_chunk_fwd = 0, ntt_chunk_inv = 0;
-    u32 conv_variant = 0;  // base-conversion kernel variant (basis.h, conv_launch)
+    u32 conv_variant = 0, conv_tg = 8;  // base-conversion kernel variant and targets per workgroup
     std::string autotune();
     // The key-switch candidates of autotune() timed on a caller's batch (autotune.cpp).
     std::string autotune_ks(const std::vector<const Poly*>& ds, const Ksk& key);

@@ -92,9 +92,10 @@ ckh_status ckh_autotune_ks(ckh_context *ctx, uint32_t n, const ckh_poly *const *
 /* Forces one tuning value: "ntt_chunk_fwd", "ntt_chunk_inv" (limbs per launch, 0 = all);
  * "graphs" (1 = replay repeated launch sequences from captured hipGraphs, the default; 0 = launch
  * directly); "graph_cache" (captured sequences kept per context); "conv_variant" (0 or 1, the
- * base-conversion kernel); "fuse_ip" (1 = row pass fused into the inner product). */
+ * base-conversion arithmetic for 60-bit targets); "conv_tg" (targets per base-conversion
+ * workgroup); "fuse_ip" (1 = row pass fused into the inner product). */
 ckh_status ckh_set_tuning(ckh_context *ctx, const char *key, int64_t value);
-/* Reads one tuning value (keys as ckh_set_tuning, plus "conv_variant" and "fuse_ip"). */
+/* Reads one tuning value (the keys of ckh_set_tuning). */
 ckh_status ckh_get_tuning(ckh_context *ctx, const char *key, int64_t *value);
 /* Captured hipGraphs so far and their replays (no counterpart in the reference). */
 ckh_status ckh_graph_stats(ckh_context *ctx, uint64_t *captures, uint64_t *launches);
