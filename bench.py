@@ -227,6 +227,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched step (warmup + steps, no autotune) and print its time (for counter runs)")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="with --batched-only: force a tuning value (repeatable)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the dominant-kernel launch and a device copy of the same size (for counter runs)")
     args = ap.parse_args()
@@ -248,6 +250,9 @@ def main():
     levels = workload_levels()
     tuning = {"ntt_chunk_fwd": 0, "ntt_chunk_inv": 128} if args.batched_only else ctx.autotune()
     if args.batched_only:
+        for kv in args.tune:
+            k, v = kv.split("=")
+            tuning[k] = int(v)
         for k, v in tuning.items():
             ctx.set_tuning(k, v)
     print(f"# autotune: {json.dumps(tuning)}", file=sys.stderr, flush=True)

@@ -556,8 +556,10 @@ __global__ __launch_bounds__(NT) void k_rows_ip(const RowIpJob* __restrict__ job
                                                 const FTw* __restrict__ twf, const FTw* __restrict__ rt, u32 log_n,
                                                 u32 njobs) {
     __shared__ u64 lds[LDS_WORDS];
-    u32 ji, tile;
-    xcd_map(blockIdx.x, 1u << (log_n - TL), njobs, ji, tile);
+    // Linear order: tile t of every job runs on XCD t % 8 (workgroups are dealt round-robin), and
+    // jobs are ordered by limb, so the jobs reading one key tile share that XCD's L2.
+    const u32 tiles = 1u << (log_n - TL), ji = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+    (void)njobs;
     const RowIpJob J = jobs[ji];
     const PrimeConst* pc = pcs + J.prime;
     const FpAr ar{twf + ((u64)J.prime << log_n), pc->qf, pc->qinv, pc->n_inv_f, pc->w_n_inv_f,

@@ -239,6 +239,8 @@ class Context : public std::enable_shared_from_this<Context> {
     std::vector<u64> pinv_, pinv_s_;           // P^-1 mod q_i
     std::vector<std::vector<u64>> qlinv_, qlinv_s_;  // [l][i]: q_l^-1 mod q_i
     void build_tables();
+    void build_mf_tables(const std::vector<ConvSrc>& srcs, const std::vector<ConvTgt>& tgts,
+                         const std::vector<ConvPlan>& plans);
     template <class T>
     T* dev_copy(const std::vector<T>& v);
 };

@@ -444,7 +444,7 @@ def test_schedules_give_the_same_switch(case):
     lv = sorted({L, L // 2, 1, 0, max(L - 3, 0)})
     polys = [ctx.uniform(l, seed=600 + i) for i, l in enumerate(lv)]
     want = None
-    for var, tg in ((0, 8), (1, 8), (0, 3), (1, 24)):
+    for var, tg in ((0, 8), (1, 8), (0, 3), (1, 24), (2, 8)):
         for fuse in (0, 1):
             ctx.set_tuning("conv_tg", tg)
             ctx.set_tuning("conv_variant", var)
@@ -456,3 +456,30 @@ def test_schedules_give_the_same_switch(case):
                 assert (wb == gb).all() and (wa == ga).all(), (var, fuse)
     ob, oa = ref.key_switch(polys[-1].download().ravel(), lv[-1], ref.uniform_key(L, 61), L)
     assert (want[-1][0] == ob).all() and (want[-1][1] == oa).all()
+
+
+@pytest.mark.parametrize("case", [SMALL, C2_SHAPE_N4096, SMALL_N8192])
+def test_matrix_core_conversion_equals_oracle(case):
+    """The int8 matrix-core base conversion (variant 2) on every stage that converts: decompose,
+    mod_down with near ties, rescale, and the full switch, word for word with the oracle."""
+    ctx, ref = make(case)
+    ctx.set_tuning("conv_variant", 2)
+    L = len(ref.q) - 1
+    for level in levels_for(len(ref.q)):
+        a = host_uniform(ref, level, 71 + level)
+        digs = ctx.decompose(ctx.upload(a, level))
+        want = ref.decompose(a.ravel(), level)
+        for j, d in enumerate(digs):
+            assert (d.download() == want[j]).all(), (level, j)
+        for ppart in (O.uniform_poly(ref.p, ref.n, 81 + level), near_tie_p_part(ref)):
+            qp = np.concatenate([a, ppart])
+            got = ctx.upload(qp, level, p_limbs=ref.k).mod_down().download()
+            assert (got == ref.mod_down(a.ravel(), level, ppart.ravel())).all(), level
+        if level >= 1:
+            got = ctx.upload(a, level).rescale_by_last().download()
+            assert (got == ref.rescale_by_last(a.ravel(), level)).all(), level
+    key = ctx.ksk_uniform(L, seed=91)
+    d = host_uniform(ref, L, 92)
+    kb, ka = ctx.key_switch(ctx.upload(d, L), key)
+    wb, wa = ref.key_switch(d.ravel(), L, ref.uniform_key(L, 91), L)
+    assert (kb.download() == wb).all() and (ka.download() == wa).all()
