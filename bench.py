@@ -194,6 +194,21 @@ def served_stream(ckh, n_ops=240, rate=4000.0, seed=SEED + 5):
     }
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "ntt_forward_240limbs_n65536.md")
+
+
+def measured_traffic():
+    """HBM bytes per roofline launch from the committed counter summary (rocprofv3 --pmc
+    FETCH_SIZE and WRITE_SIZE runs of --roofline-only, calibrated with a device copy), or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def roofline_only(ctx, ev, reps=3):
     """The roofline launch alone (forward NTT over 8 polynomials x 30 limbs), then a
     device-to-device copy of the same byte count: the copy calibrates the HBM byte counters
@@ -379,7 +394,8 @@ def main():
             "kernel": "forward NTT (column pass + row pass), one launch over 240 limbs at N=2^16 "
                       "(192 x 50-bit, 48 x 60-bit)",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
             "ms_per_launch": ntt_ms, "bytes_per_launch": ntt_bytes,
         },
         "ntt_forward_q_primes_only": ntt_q,
