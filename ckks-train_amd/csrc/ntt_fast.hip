@@ -326,22 +326,30 @@ CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
                       const FinishJob* fin, u64 q, u32 tile) {
     const u32 tid = threadIdx.x;
     const auto out = glb(out_p);
-    fwd_rows_to_lds<M, false>(ar, in, lds, log_n, tile);
     const u64 tbase = (u64)tile << TL;
     if (fin) {
-        // finish of rescale / mod_down: (a - t) * inv (+ add)
-        const u64 inv = fin->inv, inv_s = fin->inv_s;
+        // finish of rescale / mod_down: (a - t) * inv (+ add); the a (and add) words are loaded
+        // before the transform so their latency overlaps it
         const auto fa = glb(fin->a), fadd = glb(fin->add);
         const auto fout = glb(fin->out);
+        const bool has_add = fin->add != nullptr;
+        const u64 inv = fin->inv, inv_s = fin->inv_s;
+        u64 av[16], dv[16];
 #pragma unroll
         for (u32 k = 0; k < 16; ++k) {
-            const u64 i = tbase + tid + k * NT;
-            u64 v = shoup(sub_mod(fa[i], lds[pad(tid + k * NT)], q), inv, inv_s, q);
-            if (fin->add) v = add_mod(v, fadd[i], q);
-            fout[i] = v;
+            av[k] = fa[tbase + tid + k * NT];
+            dv[k] = has_add ? fadd[tbase + tid + k * NT] : 0;
+        }
+        fwd_rows_to_lds<M, false>(ar, in, lds, log_n, tile);
+#pragma unroll
+        for (u32 k = 0; k < 16; ++k) {
+            u64 v = shoup(sub_mod(av[k], lds[pad(tid + k * NT)], q), inv, inv_s, q);
+            if (has_add) v = add_mod(v, dv[k], q);
+            fout[tbase + tid + k * NT] = v;
         }
         return;
     }
+    fwd_rows_to_lds<M, false>(ar, in, lds, log_n, tile);
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) {
         const u32 idx = tid + k * NT;
@@ -362,23 +370,26 @@ CKH_DEV void rows_ip(const FpAr& ar, const RowIpJob& J, u64* lds, u32 log_n, u32
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) sb[k] = sa[k] = 0.0;
     for (u32 j = 0; j < J.ndig; ++j) {
-        double x[16];
+        const auto kb = glb(J.key + (u64)(2 * j) * J.key_poly + tbase);
+        const auto ka = glb(J.key + (u64)(2 * j + 1) * J.key_poly + tbase);
         if (j == J.own_j) {
 #pragma unroll
-            for (u32 k = 0; k < 16; ++k) x[k] = u2d(glb(J.own)[tbase + tid + k * NT]);
+            for (u32 k = 0; k < 16; ++k) {
+                const double x = u2d(glb(J.own)[tbase + tid + k * NT]);
+                const double b = u2d(kb[tid + k * NT]), a = u2d(ka[tid + k * NT]);
+                sb[k] += fmulmod(x, b, b * qinv, q);
+                sa[k] += fmulmod(x, a, a * qinv, q);
+            }
         } else {
             fwd_rows_to_lds<M, true>(ar, J.mid + j * J.mid_stride, lds, log_n, tile);
 #pragma unroll
-            for (u32 k = 0; k < 16; ++k) x[k] = __longlong_as_double((long long)lds[pad(tid + k * NT)]);
+            for (u32 k = 0; k < 16; ++k) {
+                const double x = __longlong_as_double((long long)lds[pad(tid + k * NT)]);
+                const double b = u2d(kb[tid + k * NT]), a = u2d(ka[tid + k * NT]);
+                sb[k] += fmulmod(x, b, b * qinv, q);
+                sa[k] += fmulmod(x, a, a * qinv, q);
+            }
             __syncthreads();  // the next digit's pass reuses the LDS
-        }
-        const auto kb = glb(J.key + (u64)(2 * j) * J.key_poly + tbase);
-        const auto ka = glb(J.key + (u64)(2 * j + 1) * J.key_poly + tbase);
-#pragma unroll
-        for (u32 k = 0; k < 16; ++k) {
-            const double b = u2d(kb[tid + k * NT]), a = u2d(ka[tid + k * NT]);
-            sb[k] += fmulmod(x[k], b, b * qinv, q);
-            sa[k] += fmulmod(x[k], a, a * qinv, q);
         }
         if ((j & 3) == 3) {
 #pragma unroll
@@ -524,13 +535,14 @@ __global__ __launch_bounds__(NT) void k_fwd_cols(const LimbRef* __restrict__ ref
     CKH_AR_DISPATCH((fwd_cols<M>(ar, ref.src, ref.dst, lds, log_n, tile)))
 }
 
-template <u32 M>
+// FIN: the finish variant (a separate instantiation: its prefetched operands cost registers).
+template <u32 M, bool FIN>
 __global__ __launch_bounds__(NT) void k_fwd_rows(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
                                                  const Tw* __restrict__ tw, const FTw* __restrict__ twf,
                                                  const FTw* __restrict__ rt, u32 log_n,
                                                  u32 nrefs, const FinishJob* __restrict__ fins) {
     __shared__ u64 lds[LDS_WORDS];
-    CKH_AR_DISPATCH((fwd_rows<M>(ar, ref.dst, ref.dst, lds, log_n, fins ? fins + ref_i : nullptr, pc->q, tile)))
+    CKH_AR_DISPATCH((fwd_rows<M>(ar, ref.dst, ref.dst, lds, log_n, FIN ? fins + ref_i : nullptr, pc->q, tile)))
 }
 
 template <u32 M>
@@ -577,9 +589,12 @@ static void launch_one(bool fwd, bool cols, const NttTables& t, const LimbRef* r
     dim3 grid((1u << (t.log_n - fast::TL)) * nrefs);
     if (fwd && cols)
         hipLaunchKernelGGL(fast::k_fwd_cols<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.rf, t.log_n, nrefs);
+    else if (fwd && fins)
+        hipLaunchKernelGGL((fast::k_fwd_rows<M, true>), grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.rf,
+                           t.log_n, nrefs, fins);
     else if (fwd)
-        hipLaunchKernelGGL(fast::k_fwd_rows<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.rf, t.log_n, nrefs,
-                           fins);
+        hipLaunchKernelGGL((fast::k_fwd_rows<M, false>), grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.rf,
+                           t.log_n, nrefs, fins);
     else if (cols)
         hipLaunchKernelGGL(fast::k_inv_cols<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.ri, t.log_n, nrefs);
     else
