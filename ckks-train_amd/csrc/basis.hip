@@ -44,7 +44,7 @@ CKH_DEV void add_shl(u64& lo, u64& hi, u64 v, u32 k) {
 // constants are read through the scalar cache. 60-bit targets of f64 sources use (WIDE) six
 // exact f64 partial sums of 25-bit source halves times 20-bit constant pieces combined into 128
 // bits, or 128-bit integer sums.
-template <u32 MAXS, bool WIDE>
+template <u32 MAXS, bool WIDE, bool PIECES>
 __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __restrict__ jobs, u32 tg) {
     const u32 n = 1u << T.log_n;
     const u32 c = blockIdx.x * 256 + threadIdx.x;
@@ -55,14 +55,19 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
     const u32 t1 = min(t0 + tg, P.ntgt);
     const auto inp = glb(job.in);
     u64 y[MAXS];
-    // f64 source: yd = y, yh = 0; integer source: yd / yh = its low / high 30 bits
-    double yd[MAXS], yh[MAXS];
+    // f64 source: yd = y, yh = 0; integer source: yd / yh = its low / high 30 bits, or (PIECES)
+    // yd / yh / yx = its 20-bit pieces from the low end
+    double yd[MAXS], yh[MAXS], yx[MAXS];
     double f = 0.0;
+    // the piece path needs every source integer (at most 7: the bracketed sums below 2^53)
+    bool srcs_int = PIECES && MAXS <= 7;
+    for (u32 j = 0; j < P.nsrc && srcs_int; ++j) srcs_int = !cst(T.src + P.src_first + j)->fp;
 #pragma unroll
     for (u32 j = 0; j < MAXS; ++j) {
         y[j] = 0;
         yd[j] = 0.0;
         yh[j] = 0.0;
+        yx[j] = 0.0;
         if (j < P.nsrc) {
             const ConvSrc* s = T.src + P.src_first + j;
             const u64 x = inp[(u64)(P.src_limb0 + j) * n + c];
@@ -75,8 +80,14 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
                 y[j] = d2u(v);
             } else {
                 y[j] = shoup(add_mod(x, s->h, s->q), s->qhinv, s->qhinv_s, s->q);
-                yd[j] = (double)(u32)(y[j] & ((1u << 30) - 1));  // low 30 bits, exact
-                yh[j] = (double)(u32)(y[j] >> 30);                // high 30 bits, exact
+                if (srcs_int) {
+                    yd[j] = (double)(u32)(y[j] & ((1u << 20) - 1));
+                    yh[j] = (double)(u32)((y[j] >> 20) & ((1u << 20) - 1));
+                    yx[j] = (double)(u32)(y[j] >> 40);
+                } else {
+                    yd[j] = (double)(u32)(y[j] & ((1u << 30) - 1));  // low 30 bits, exact
+                    yh[j] = (double)(u32)(y[j] >> 30);                // high 30 bits, exact
+                }
             }
             if (P.exact) {
                 const double prod = u64_to_f64_rn(y[j]) * s->inv;
@@ -89,7 +100,31 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
     for (u32 t = t0; t < t1; ++t) {
         const auto* g = cst(T.tgt + P.tgt_first + t);
         u64 o;
-        if (g->mode == TGT_SPLIT) {
+        if (PIECES && g->mode == TGT_SPLIT && srcs_int) {
+            // y_j = y0 + y1 2^20 + y2 2^40 (20-bit pieces), c_j = c0 + c1 2^25 (25-bit halves):
+            // S_ab = sum_j y_ja c_jb is exact (every product < 2^45, at most 7 terms), and
+            // sum_j y_j c_j = S00 + (32 S01 + S10) 2^20 + (32 S11 + S20) 2^40 + S21 2^65, the two
+            // bracketed sums staying below 2^53.
+            const double r = g->rf, rinv = g->rinv;
+            double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0, s20 = 0.0, s21 = 0.0;
+#pragma unroll
+            for (u32 j = 0; j < MAXS; ++j) {
+                const double c0 = g->half[0][j], c1 = g->half[1][j];
+                s00 = __builtin_fma(yd[j], c0, s00);
+                s01 = __builtin_fma(yd[j], c1, s01);
+                s10 = __builtin_fma(yh[j], c0, s10);
+                s11 = __builtin_fma(yh[j], c1, s11);
+                s20 = __builtin_fma(yx[j], c0, s20);
+                s21 = __builtin_fma(yx[j], c1, s21);
+            }
+            const double t1 = __builtin_fma(s01, 32.0, s10), t2 = __builtin_fma(s11, 32.0, s20);
+            double acc = s00 + fmulmod(t1, g->p20.w, g->p20.wq, r);
+            acc = acc + fmulmod(t2, g->p40.w, g->p40.wq, r);
+            acc = acc + fmulmod(s21, g->p65.w, g->p65.wq, r);
+            if (P.exact) acc = acc - fmulmod((double)e, g->qmod_f, g->qmod_q, r);
+            acc = acc - g->hmod_f;
+            o = d2u(fcanon(acc, r, rinv));
+        } else if (g->mode == TGT_SPLIT) {
             // y_j = yh 2^30 + yl: y_j c = yh (2^30 c) + yl c (mod r), four terms between reductions.
             const double r = g->rf, rinv = g->rinv;
             double acc = 0.0;
@@ -297,9 +332,11 @@ static void conv_launch_s(const ConvTables& t, const ConvJob* jobs, u32 njobs, u
     const u32 n = 1u << t.log_n;
     const u32 zt = (max_tgt + tg - 1) / tg;
     if (variant == 1)
-        hipLaunchKernelGGL((k_conv<MAXS, true>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs, tg);
+        hipLaunchKernelGGL((k_conv<MAXS, true, false>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs, tg);
+    else if (variant == 3)
+        hipLaunchKernelGGL((k_conv<MAXS, false, true>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs, tg);
     else
-        hipLaunchKernelGGL((k_conv<MAXS, false>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs, tg);
+        hipLaunchKernelGGL((k_conv<MAXS, false, false>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs, tg);
 }
 
 void conv_launch(const ConvTables& t, const ConvJob* jobs, u32 njobs, u32 max_tgt, u32 max_src, u32 variant,

@@ -260,6 +260,12 @@ void Context::build_tables() {
                 t.qmod_f = (double)t.qmod;
                 t.qmod_q = (double)t.qmod / (double)r;
                 t.hmod_f = (double)t.hmod;
+                if (r < FP_PRIME_BOUND) {
+                    for (auto [sh, dst] : {std::pair<u32, FTw*>{20, &t.p20}, {40, &t.p40}, {65, &t.p65}}) {
+                        const u64 v = hm::pow(2, sh, r);
+                        *dst = {(double)v, (double)v / (double)r};
+                    }
+                }
                 for (size_t j = 0; j < src.size(); ++j) {
                     u64 qh = 1 % r;
                     for (size_t i = 0; i < src.size(); ++i)
@@ -271,6 +277,7 @@ void Context::build_tables() {
                     t.qhat30_f[j] = (double)qh30;
                     t.qhat30_q[j] = (double)qh30 / (double)r;
                     for (u32 b = 0; b < 3; ++b) t.wide[b][j] = (double)((qh >> (20 * b)) & ((1u << 20) - 1));
+                    for (u32 b = 0; b < 2; ++b) t.half[b][j] = (double)((qh >> (25 * b)) & ((1u << 25) - 1));
                 }
                 tgts.push_back(t);
             }
