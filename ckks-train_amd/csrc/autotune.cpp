@@ -16,7 +16,7 @@ void Context::set_tuning(const std::string& key, int64_t value) {
     else if (key == "graphs")
         use_graphs = value != 0;
     else if (key == "conv_variant")
-        conv_variant = (u32)std::min<int64_t>(value, 3);
+        conv_variant = (u32)std::min<int64_t>(value, 2);
     else if (key == "fuse_ip")
         fuse_ip = value != 0;
     else if (key == "conv_tg")
@@ -113,7 +113,7 @@ std::string Context::autotune_ks(const std::vector<const Poly*>& ds, const Ksk& 
     double best = 1e30;
     u32 best_var = conv_variant, best_fuse = fuse_ip, best_tg = conv_tg;
     bool first = true;
-    for (u32 var : {0u, 1u, 2u, 3u})
+    for (u32 var : {0u, 1u, 2u})
         for (u32 tg : {8u, 16u, 24u})
             for (u32 fuse : {0u, 1u}) {
                 if (fuse && !fuse_ip_possible()) continue;

@@ -26,18 +26,16 @@ struct ConvTgt {
     u64 qhat[MAX_SRC];                   // (Q / q_j) mod r
     double qhat_f[MAX_SRC], qhat_q[MAX_SRC];      // f64 path: (Q / q_j) mod r and its quotient by r
     double qhat30_f[MAX_SRC], qhat30_q[MAX_SRC];  // split path: 2^30 (Q / q_j) mod r, quotient by r
-    double wide[3][MAX_SRC];  // wide path: (Q / q_j) mod r in 20-bit pieces (low, middle, high)
     double half[2][MAX_SRC];  // piece path: (Q / q_j) mod r in 25-bit halves (low, high)
     FTw p20, p40, p65;        // piece path: 2^20, 2^40, 2^65 mod r (and their quotients by r)
     double qmod_f, qmod_q, hmod_f, rf, rinv;
-    // TGT_INT: 64-bit integer sums; TGT_FP: r and every source below 2^50, f64 products of the
-    // sources; TGT_SPLIT: r below 2^50 with larger sources, each source split into 30-bit halves;
-    // TGT_WIDE: r at least 2^50 with every source below 2^50: exact f64 partial sums of 25-bit
-    // source halves times 20-bit constant pieces, combined into 128 bits and reduced once.
+    // TGT_INT: r at least 2^50, 128-bit integer sums; TGT_FP: r and every source below 2^50, f64
+    // products of the sources; TGT_SPLIT: r below 2^50 with larger sources, each source split
+    // into 30-bit halves (or, variant 1, 20-bit pieces summed exactly).
     u32 mode, pad;
 };
 
-enum : u32 { TGT_INT = 0, TGT_FP = 1, TGT_SPLIT = 2, TGT_WIDE = 3 };
+enum : u32 { TGT_INT = 0, TGT_FP = 1, TGT_SPLIT = 2 };
 
 // A conversion: sources [src_first, src_first + nsrc) of the ConvSrc table (their input limbs
 // are src_limb0 + j of the job's input), targets [tgt_first, tgt_first + ntgt) of the ConvTgt
@@ -102,10 +100,10 @@ struct ConvTables {
 // For each job: coefficient-form input limbs (reduced) -> converted output limbs (reduced,
 // coefficient form): out_t = [x_c]_r with x_c the centered value of the source residues
 // (BaseConverter::convert_exact of [x + h]_Q minus h mod r).
-// Tuning choices: variant: 60-bit targets of 50-bit sources use 128-bit integer sums (0) or f64
-// partial sums (1); 2 = the matrix-core kernel (when ConvTables::mf_ok; else variant 0); 3 =
-// variant 0 with 50-bit targets of 60-bit sources summed as exact f64 pieces (20-bit source
-// pieces times 25-bit constant halves) instead of two modular products per source; tg:
+// Tuning choices: variant: 0 = vector kernel; 1 = vector kernel with 50-bit targets of 60-bit
+// sources summed as exact f64 pieces (20-bit source pieces times 25-bit constant halves) instead
+// of two modular products per source; 2 = the matrix-core kernel (when ConvTables::mf_ok; else
+// variant 0); tg:
 // targets per workgroup of the vector kernels (the sources are read once per group of tg).
 void conv_launch(const ConvTables& t, const ConvJob* jobs, u32 njobs, u32 max_tgt, u32 max_src, u32 variant,
                  u32 tg, hipStream_t stream);

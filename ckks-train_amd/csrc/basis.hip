@@ -23,28 +23,10 @@ CKH_DEV double u64_to_f64_rn(u64 v) {
     return hi + lo;
 }
 
-// (hi, lo) += v << k for v < 2^64, 0 <= k < 128 (k a constant at every call).
-CKH_DEV void add_shl(u64& lo, u64& hi, u64 v, u32 k) {
-    u64 vl, vh;
-    if (k == 0) {
-        vl = v;
-        vh = 0;
-    } else if (k < 64) {
-        vl = v << k;
-        vh = v >> (64 - k);
-    } else {
-        vl = 0;
-        vh = v << (k - 64);
-    }
-    lo += vl;
-    hi += vh + (lo < vl ? 1 : 0);
-}
-
 // One block: 256 coefficients of one job, targets [t0, t0 + tg) of its plan; per-target
-// constants are read through the scalar cache. 60-bit targets of f64 sources use (WIDE) six
-// exact f64 partial sums of 25-bit source halves times 20-bit constant pieces combined into 128
-// bits, or 128-bit integer sums.
-template <u32 MAXS, bool WIDE, bool PIECES>
+// constants are read through the scalar cache. 60-bit targets sum exact 128-bit integer
+// products and reduce once.
+template <u32 MAXS, bool PIECES>
 __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __restrict__ jobs, u32 tg) {
     const u32 n = 1u << T.log_n;
     const u32 c = blockIdx.x * 256 + threadIdx.x;
@@ -150,29 +132,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
             o = d2u(fcanon(acc, r, rinv));
         } else {
             const PrimeConst C = pc_int(T.pc + g->prime);
-            if (WIDE && g->mode == TGT_WIDE) {
-                double s00 = 0.0, s01 = 0.0, s02 = 0.0, s10 = 0.0, s11 = 0.0, s12 = 0.0;
-#pragma unroll
-                for (u32 j = 0; j < MAXS; ++j) {
-                    const double hj = __builtin_floor(yd[j] * 0x1p-25);
-                    const double lj = __builtin_fma(-hj, 0x1p25, yd[j]);
-                    const double c0 = g->wide[0][j], c1 = g->wide[1][j], c2 = g->wide[2][j];
-                    s00 = __builtin_fma(lj, c0, s00);
-                    s01 = __builtin_fma(lj, c1, s01);
-                    s02 = __builtin_fma(lj, c2, s02);
-                    s10 = __builtin_fma(hj, c0, s10);
-                    s11 = __builtin_fma(hj, c1, s11);
-                    s12 = __builtin_fma(hj, c2, s12);
-                }
-                u64 lo = 0, hi = 0;
-                add_shl(lo, hi, d2u(s00), 0);
-                add_shl(lo, hi, d2u(s01), 20);
-                add_shl(lo, hi, d2u(s10), 25);
-                add_shl(lo, hi, d2u(s02), 40);
-                add_shl(lo, hi, d2u(s11), 45);
-                add_shl(lo, hi, d2u(s12), 65);
-                o = reduce128(hi, lo, C);
-            } else {
+            {
                 Acc128 acc = {0, 0};
 #pragma unroll
                 for (u32 j = 0; j < MAXS; ++j) mac(acc, y[j], g->qhat[j]);
@@ -332,11 +292,9 @@ static void conv_launch_s(const ConvTables& t, const ConvJob* jobs, u32 njobs, u
     const u32 n = 1u << t.log_n;
     const u32 zt = (max_tgt + tg - 1) / tg;
     if (variant == 1)
-        hipLaunchKernelGGL((k_conv<MAXS, true, false>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs, tg);
-    else if (variant == 3)
-        hipLaunchKernelGGL((k_conv<MAXS, false, true>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs, tg);
+        hipLaunchKernelGGL((k_conv<MAXS, true>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs, tg);
     else
-        hipLaunchKernelGGL((k_conv<MAXS, false, false>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs, tg);
+        hipLaunchKernelGGL((k_conv<MAXS, false>), dim3((n + 255) / 256, njobs, zt), dim3(256), 0, stream, t, jobs, tg);
 }
 
 void conv_launch(const ConvTables& t, const ConvJob* jobs, u32 njobs, u32 max_tgt, u32 max_src, u32 variant,

@@ -254,7 +254,7 @@ void Context::build_tables() {
                 t.hmod = hm::half_mod(sq.data(), sq.size(), r);
                 bool srcs_fp = true;
                 for (u64 sqj : sq) srcs_fp = srcs_fp && sqj < FP_PRIME_BOUND;
-                t.mode = r >= FP_PRIME_BOUND ? (srcs_fp ? TGT_WIDE : TGT_INT) : (srcs_fp ? TGT_FP : TGT_SPLIT);
+                t.mode = r >= FP_PRIME_BOUND ? TGT_INT : (srcs_fp ? TGT_FP : TGT_SPLIT);
                 t.rf = (double)r;
                 t.rinv = 1.0 / (double)r;
                 t.qmod_f = (double)t.qmod;
@@ -276,7 +276,6 @@ void Context::build_tables() {
                     const u64 qh30 = hm::mul(qh, (1ull << 30) % r, r);
                     t.qhat30_f[j] = (double)qh30;
                     t.qhat30_q[j] = (double)qh30 / (double)r;
-                    for (u32 b = 0; b < 3; ++b) t.wide[b][j] = (double)((qh >> (20 * b)) & ((1u << 20) - 1));
                     for (u32 b = 0; b < 2; ++b) t.half[b][j] = (double)((qh >> (25 * b)) & ((1u << 25) - 1));
                 }
                 tgts.push_back(t);

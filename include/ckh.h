@@ -92,8 +92,8 @@ ckh_status ckh_autotune_ks(ckh_context *ctx, uint32_t n, const ckh_poly *const *
 /* Forces one tuning value: "ntt_chunk_fwd", "ntt_chunk_inv" (limbs per launch, 0 = all);
  * "graphs" (1 = replay repeated launch sequences from captured hipGraphs, the default; 0 = launch
  * directly); "graph_cache" (captured sequences kept per context); "conv_variant" (base conversion:
- * 0 / 1 = vector kernel with integer / f64-piece sums for 60-bit targets, 2 = int8 matrix-core
- * kernel, 3 = vector kernel with f64-piece sums for 60-bit sources); "conv_tg" (targets per base-conversion
+ * 0 = vector kernel, 1 = vector kernel with exact f64-piece sums for 60-bit sources, 2 = int8
+ * matrix-core kernel); "conv_tg" (targets per base-conversion
  * workgroup); "fuse_ip" (1 = row pass fused into the inner product). */
 ckh_status ckh_set_tuning(ckh_context *ctx, const char *key, int64_t value);
 /* Reads one tuning value (the keys of ckh_set_tuning). */
