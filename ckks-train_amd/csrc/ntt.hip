@@ -77,8 +77,8 @@ __global__ __launch_bounds__(NTHREADS) void k_ntt_fwd_cols(const LimbRef* __rest
     const u32 tc_log = TILE_LOG - A;
     const u32 TC = 1u << tc_log;
     const u32 c0 = blockIdx.x * TC;
-    const u64* in = ref.src;
-    u64* out = ref.dst;
+    const auto in = glb(ref.src);
+    const auto out = glb(ref.dst);
     for (u32 e = threadIdx.x; e < TILE; e += NTHREADS) {
         const u32 r = e >> tc_log, cc = e & (TC - 1);
         s[e] = in[((u64)r << B) + c0 + cc];
@@ -115,8 +115,8 @@ __global__ __launch_bounds__(NTHREADS) void k_ntt_fwd_rows(const LimbRef* __rest
     const u32 tr_log = (TILE_LOG - B) < A ? (TILE_LOG - B) : A;  // rows per tile (log)
     const u32 elems = 1u << (tr_log + B);
     const u64 base = (u64)blockIdx.x << (tr_log + B);  // first element of the tile
-    const u64* in = (src_is_dst ? ref.dst : ref.src) + base;
-    u64* out = ref.dst + base;
+    const auto in = glb((src_is_dst ? ref.dst : ref.src) + base);
+    const auto out = glb(ref.dst + base);
     for (u32 e = threadIdx.x; e < elems; e += NTHREADS) s[e] = in[e];
     __syncthreads();
     for (u32 u = 0; u < B; ++u) {
@@ -156,8 +156,8 @@ __global__ __launch_bounds__(NTHREADS) void k_ntt_inv_rows(const LimbRef* __rest
     const u32 tr_log = (TILE_LOG - B) < A ? (TILE_LOG - B) : A;
     const u32 elems = 1u << (tr_log + B);
     const u64 base = (u64)blockIdx.x << (tr_log + B);
-    const u64* in = ref.src + base;
-    u64* out = ref.dst + base;
+    const auto in = glb(ref.src + base);
+    const auto out = glb(ref.dst + base);
     for (u32 e = threadIdx.x; e < elems; e += NTHREADS) s[e] = in[e];
     __syncthreads();
     for (u32 u = 0; u < B; ++u) {
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(NTHREADS) void k_ntt_inv_cols(const LimbRef* __rest
     const u32 tc_log = TILE_LOG - A;
     const u32 TC = 1u << tc_log;
     const u32 c0 = blockIdx.x * TC;
-    u64* io = ref.dst;
+    const auto io = glb(ref.dst);
     for (u32 e = threadIdx.x; e < TILE; e += NTHREADS) {
         const u32 r = e >> tc_log, cc = e & (TC - 1);
         s[e] = io[((u64)r << B) + c0 + cc];

@@ -322,6 +322,27 @@ def test_c2_full_size_key_switch():
     assert (kb.download() == wb).all() and (ka.download() == wa).all()
 
 
+def test_c3_batch_slice_full_size():
+    """48 polynomials of the benchmark workload (N = 2^16, its seeded levels 3..23) under one key:
+    the batched switch, first run and graph replay, equals one switch per call word for word
+    (by digest), and one member equals the oracle."""
+    say("c3 slice full size")
+    ctx, ref = make(C2)
+    rng = np.random.default_rng(20261202)
+    levels = [int(x) - 1 for x in rng.integers(4, 25, size=256)][:48]
+    key = ctx.ksk_uniform(23, seed=515151)
+    polys = [ctx.uniform(lv, seed=5150 + i) for i, lv in enumerate(levels)]
+    single = [(kb.digest(), ka.digest()) for kb, ka in (ctx.key_switch(p, key) for p in polys)]
+    for rep in range(3):  # direct, captured, replayed
+        got = [(kb.digest(), ka.digest()) for kb, ka in ctx.key_switch_batch(polys, key)]
+        assert got == single, rep
+    i = int(np.argmax(levels))
+    kb, ka = ctx.key_switch_batch(polys, key)[i]
+    O.set_threads(16)
+    wb, wa = ref.key_switch(host_uniform(ref, levels[i], 5150 + i).ravel(), levels[i], ref.uniform_key(23, 515151), 23)
+    assert (kb.download() == wb).all() and (ka.download() == wa).all()
+
+
 # ---------------------------------------------------------------- BSGS (hoisted rotations)
 
 def _bsgs_case(case, baby, giant, mask, seed, level=None):
