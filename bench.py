@@ -104,8 +104,8 @@ def cpu_baseline(q, p, levels, budget_s=12.0, budget_1t_s=8.0):
             "sample_1_thread": f"first {done1} ciphertexts, 1 thread, {el1:.1f} s"}
 
 
-def served_stream(ckh, n_ops=240, rate=4000.0, seed=SEED + 5):
-    """The served mixed stream: Poisson arrivals (rate ops/s) over N = 2^16 (24 + 6 primes,
+def served_stream(ckh, n_ops=240, rates=(1500.0, 4000.0), seed=SEED + 5):
+    """The served mixed stream: Poisson arrivals (each of `rates` ops/s) over N = 2^16 (24 + 6 primes,
     dnum 4) and N = 2^15 (20 + 4 primes, dnum 5); 50% relinearize, 30% hoisted 16-rotation
     batch, 20% mul + relinearize + rescale; limbs U{4..24} (at most 20 at 2^15); 64 rotation
     keys and a relinearization key resident per parameter set; 200 GB admission budget.
@@ -120,6 +120,14 @@ def served_stream(ckh, n_ops=240, rate=4000.0, seed=SEED + 5):
         relin = ctx.ksk_uniform(L, seed=seed + 1000 * i)
         rot = [ctx.ksk_uniform(L, seed=seed + 1000 * i + s, galois=ckh.galois_elt(log_n, s)) for s in range(1, 65)]
         envs.append((ctx, L, ckh.Evaluator(ctx, relin, rot)))
+    out = {}
+    for k, rate in enumerate(rates):
+        out[f"at_{int(rate)}_per_s"] = _served_at(ckh, envs, n_ops, rate, seed + 7 * k)
+    return out
+
+
+def _served_at(ckh, envs, n_ops, rate, seed):
+    """One arrival trace at `rate` ops/s through the server and through the plain path."""
     rng = np.random.default_rng(seed)
     ops, t = [], 0.0
     for i in range(n_ops):
