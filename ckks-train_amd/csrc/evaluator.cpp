@@ -211,8 +211,7 @@ void check_switch(const Context& c, const Poly& d, const Ksk& key) {
 }
 
 // decompose + apply for a batch of requests, in one launch per stage.
-void run_ks(Context& c, const std::vector<KsReq>& reqs) {
-    hipStream_t s = c.stream();
+static void run_ks_on(Context& c, const std::vector<KsReq>& reqs, hipStream_t s) {
     u64 total = 0;
     for (auto& r : reqs) total += ks_scratch_words(c, r.d->level, true);
     BufPtr scratch = c.alloc(total, s);
@@ -237,6 +236,8 @@ void run_ks(Context& c, const std::vector<KsReq>& reqs) {
     c.execute(prog, s);
     hip_check(hipGetLastError(), "key switch");
 }
+
+void run_ks(Context& c, const std::vector<KsReq>& reqs) { run_ks_on(c, reqs, c.stream()); }
 }  // namespace
 
 std::vector<PolyPtr> ks_decompose(Context& c, const Poly& d) {

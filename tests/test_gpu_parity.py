@@ -504,3 +504,25 @@ def test_matrix_core_conversion_equals_oracle(case):
     kb, ka = ctx.key_switch(ctx.upload(d, L), key)
     wb, wa = ref.key_switch(d.ravel(), L, ref.uniform_key(L, 91), L)
     assert (kb.download() == wb).all() and (ka.download() == wa).all()
+
+
+@pytest.mark.skip(reason="the stream-split batch path is switched off: its autotuning run (back-to-back "
+                         "split batches on 4 streams at N = 2^16) faulted the GPU with a memory aperture "
+                         "violation whose cause is not shown; the code path was removed")
+def test_batch_split_over_streams_equals_one_stream():
+    """A batched switch split over 2 and 3 streams (concurrent parts joined back into the caller's
+    stream) equals the unsplit batch, on the first run and on graph replays."""
+    ctx, ref = make(C2_SHAPE_N4096, n_streams=3)
+    L = len(ref.q) - 1
+    key = ctx.ksk_uniform(L, seed=71)
+    lv = [L, 3, L // 2, 0, 7, L - 1, 5, 1, L, 2]
+    polys = [ctx.uniform(l, seed=700 + i) for i, l in enumerate(lv)]
+    want = [(kb.digest(), ka.digest()) for kb, ka in ctx.key_switch_batch(polys, key)]
+    for split in (2, 3):
+        ctx.set_tuning("ks_split", split)
+        for rep in range(3):
+            got = [(kb.digest(), ka.digest()) for kb, ka in ctx.key_switch_batch(polys, key)]
+            assert got == want, (split, rep)
+    wb, wa = ref.key_switch(polys[2].download().ravel(), lv[2], ref.uniform_key(L, 71), L)
+    kb, ka = ctx.key_switch_batch(polys, key)[2]
+    assert (kb.download() == wb).all() and (ka.download() == wa).all()
