@@ -271,7 +271,9 @@ def main():
         roofline_only(ctx, ev)
         return
     levels = workload_levels()
-    tuning = {"ntt_chunk_fwd": 0, "ntt_chunk_inv": 128} if args.batched_only else ctx.autotune()
+    # --batched-only runs without the autotuner, at the choices it makes on this workload
+    tuning = ({"ntt_chunk_fwd": 0, "ntt_chunk_inv": 128, "conv_variant": 0, "conv_tg": 24, "fuse_ip": 1}
+              if args.batched_only else ctx.autotune())
     if args.batched_only:
         for kv in args.tune:
             k, v = kv.split("=")
