@@ -45,7 +45,7 @@ PolyPtr poly_upload(Context& c, const u64* host, u32 level, u32 p_limbs, bool nt
             if (host[i * c.n + k] >= q) fail(CKH_ERR_INVALID_PARAMS, "limb entry not reduced");
     }
     hipStream_t s = c.stream();
-    hip_check(hipMemcpyAsync(p->data(), host, (u64)p->limbs() * c.n * 8, hipMemcpyHostToDevice, s), "upload");
+    c.h2d(p->data(), host, (u64)p->limbs() * c.n * 8, s);
     hip_check(hipStreamSynchronize(s), "upload");
     return p;
 }
@@ -171,7 +171,7 @@ KskPtr ksk_upload(Context& c, const u64* host, u32 max_level, u64 galois) {
                     if (row[x] >= q) fail(CKH_ERR_INVALID_PARAMS, "key entry not reduced");
             }
         hipStream_t s = c.stream();
-        hip_check(hipMemcpyAsync(k->buf->ptr, host, words * 8, hipMemcpyHostToDevice, s), "key upload");
+        c.h2d(k->buf->ptr, host, words * 8, s);
         hip_check(hipStreamSynchronize(s), "key upload");
     }
     return k;

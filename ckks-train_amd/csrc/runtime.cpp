@@ -78,7 +78,11 @@ Context::Context(int ordinal, const ckh_params& p, u32 n_streams) {
 Context::~Context() {
     (void)hipSetDevice(device);
     for (auto s : streams_) (void)hipStreamSynchronize(s);
+    for (auto& b : ring_busy_) (void)hipEventDestroy(b.ev);
+    for (auto e : ring_events_) (void)hipEventDestroy(e);
+    if (ring_) (void)hipHostFree(ring_);
     for (auto& pl : plans_) {
+        if (pl->done) (void)hipEventDestroy(pl->done);
         if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
         if (pl->desc) (void)hipFree(pl->desc);
     }
@@ -127,8 +131,56 @@ void Context::free_buffer(Buffer* b) {
 
 BufPtr Context::upload_blob(const void* host, size_t bytes, hipStream_t s) {
     BufPtr b = alloc((bytes + 7) / 8, s);
-    if (bytes) hip_check(hipMemcpyAsync(b->ptr, host, bytes, hipMemcpyHostToDevice, s), "descriptor upload");
+    h2d(b->ptr, host, bytes, s);
     return b;
+}
+
+// A ring slot of len bytes: slots are handed out in address order, wrapping at the end; a slot
+// still read by an unfinished copy (the oldest busy slots, from the previous lap) is waited for.
+size_t Context::ring_reserve(size_t len) {
+    len = (len + 255) & ~(size_t)255;
+    if (ring_head_ + len > ring_cap_) ring_head_ = 0;
+    const size_t off = ring_head_;
+    ring_head_ += len;
+    while (!ring_busy_.empty()) {
+        Staged& b = ring_busy_.front();
+        const bool overlaps = b.off < off + len && off < b.off + b.len;
+        if (!overlaps && hipEventQuery(b.ev) != hipSuccess) break;
+        if (overlaps) hip_check(hipEventSynchronize(b.ev), "staging ring");
+        ring_events_.push_back(b.ev);
+        ring_busy_.pop_front();
+    }
+    for (const Staged& b : ring_busy_)  // a later busy slot can overlap only after a wrap
+        if (b.off < off + len && off < b.off + b.len) hip_check(hipEventSynchronize(b.ev), "staging ring");
+    return off;
+}
+
+void Context::h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return;
+    std::lock_guard<std::mutex> lk(ring_mu_);
+    if (!ring_) {
+        ring_cap_ = (size_t)64 << 20;
+        void* p = nullptr;
+        hip_check(hipHostMalloc(&p, ring_cap_, hipHostMallocDefault), "hipHostMalloc(staging ring)");
+        ring_ = (uint8_t*)p;
+    }
+    const size_t chunk_max = ring_cap_ / 4;
+    for (size_t done = 0; done < bytes;) {
+        const size_t len = std::min(chunk_max, bytes - done);
+        const size_t off = ring_reserve(len);
+        std::memcpy(ring_ + off, (const uint8_t*)src + done, len);
+        hip_check(hipMemcpyAsync((uint8_t*)dst + done, ring_ + off, len, hipMemcpyHostToDevice, s), "upload");
+        hipEvent_t ev;
+        if (ring_events_.empty()) {
+            hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
+        } else {
+            ev = ring_events_.back();
+            ring_events_.pop_back();
+        }
+        hip_check(hipEventRecord(ev, s), "event");
+        ring_busy_.push_back({off, len, ev});
+        done += len;
+    }
 }
 
 template <class T>
@@ -531,6 +583,7 @@ void Context::execute(const Program& p, hipStream_t s) {
 void Context::drop_plan(size_t i) {
     Plan& pl = *plans_[i];
     if (pl.exec || pl.desc) (void)hipStreamSynchronize(pl.s);  // the last replay may still run
+    if (pl.done) (void)hipEventDestroy(pl.done);
     if (pl.exec) (void)hipGraphExecDestroy(pl.exec);
     if (pl.desc) (void)hipFree(pl.desc);
     plans_.erase(plans_.begin() + i);
@@ -575,8 +628,10 @@ bool Context::execute_graph(const Program& p, hipStream_t s) {
             return false;
         }
     }
-    hip_check(hipMemcpyAsync(pl->desc, p.bytes.data(), p.bytes.size(), hipMemcpyHostToDevice, s),
-              "descriptor upload");
+    // The previous replay reads the same descriptor buffer: it must have finished before the
+    // buffer is rewritten (a copy from pageable host memory is not relied on to wait for it).
+    if (pl->done) hip_check(hipEventSynchronize(pl->done), "hipEventSynchronize");
+    h2d(pl->desc, p.bytes.data(), p.bytes.size(), s);
     if (!pl->exec) {
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
@@ -590,6 +645,8 @@ bool Context::execute_graph(const Program& p, hipStream_t s) {
         ++graph_captures;
     }
     hip_check(hipGraphLaunch(pl->exec, s), "hipGraphLaunch");
+    if (!pl->done) hip_check(hipEventCreateWithFlags(&pl->done, hipEventDisableTiming), "event");
+    hip_check(hipEventRecord(pl->done, s), "event");
     ++graph_launches;
     return true;
 }

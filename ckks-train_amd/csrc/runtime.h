@@ -3,6 +3,7 @@
 #pragma once
 
 #include <atomic>
+#include <deque>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -199,6 +200,10 @@ class Context : public std::enable_shared_from_this<Context> {
     int64_t get_tuning(const std::string& key) const;
     // Copies a host blob to a fresh device buffer on stream s.
     BufPtr upload_blob(const void* host, size_t bytes, hipStream_t s);
+    // Host-to-device copy ordered on stream s: the bytes are staged in a pinned ring (so the copy
+    // is an ordinary stream-ordered DMA and `src` may be reused on return). Ring slots are reused
+    // only after the copies that read them have completed.
+    void h2d(void* dst, const void* src, size_t bytes, hipStream_t s);
 
     // Captured programs (hipGraph): a program whose launch steps (kinds, grid sizes, descriptor
     // offsets) were seen before on the same stream is captured once and then replayed, with its
@@ -212,12 +217,23 @@ class Context : public std::enable_shared_from_this<Context> {
     std::atomic<u64> graph_captures{0}, graph_launches{0};
 
    private:
+    struct Staged {
+        size_t off, len;
+        hipEvent_t ev;
+    };
+    std::mutex ring_mu_;
+    uint8_t* ring_ = nullptr;  // pinned host memory
+    size_t ring_cap_ = 0, ring_head_ = 0;
+    std::deque<Staged> ring_busy_;
+    std::vector<hipEvent_t> ring_events_;
+    size_t ring_reserve(size_t len);
     struct Plan {
         hipStream_t s = nullptr;
         std::vector<Program::Step> steps;
         size_t bytes = 0;
         void* desc = nullptr;  // plain device allocation (a pool buffer would keep the context alive)
         hipGraphExec_t exec = nullptr;
+        hipEvent_t done = nullptr;  // recorded after each replay
         u64 last_use = 0;
     };
     std::mutex plan_mu_;

@@ -526,3 +526,22 @@ def test_batch_split_over_streams_equals_one_stream():
     wb, wa = ref.key_switch(polys[2].download().ravel(), lv[2], ref.uniform_key(L, 71), L)
     kb, ka = ctx.key_switch_batch(polys, key)[2]
     assert (kb.download() == wb).all() and (ka.download() == wa).all()
+
+
+def test_graph_replays_back_to_back_without_host_sync():
+    """Replays of one captured plan enqueued back to back (new inputs each time, no host
+    synchronization between them, every output kept) each equal the directly launched switch:
+    a replay's descriptors must not reach the device before the previous replay is done."""
+    ctx, ref = make(C2_SHAPE_N4096)
+    L = len(ref.q) - 1
+    key = ctx.ksk_uniform(L, seed=81)
+    lv = [L, 2, L // 2, 0, 9]
+    rounds = [[ctx.uniform(l, seed=800 + 10 * r + i) for i, l in enumerate(lv)] for r in range(6)]
+    ctx.set_tuning("graphs", 0)
+    want = [[(kb.digest(), ka.digest()) for kb, ka in ctx.key_switch_batch(p, key)] for p in rounds]
+    ctx.set_tuning("graphs", 1)
+    outs = [ctx.key_switch_batch(p, key) for p in rounds]  # first sighting, capture, 4 replays
+    ctx.synchronize()
+    got = [[(kb.digest(), ka.digest()) for kb, ka in o] for o in outs]
+    assert got == want
+    assert ctx.graph_stats()[1] >= 4

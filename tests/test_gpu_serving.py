@@ -23,12 +23,12 @@ def _setup(seed):
     return envs
 
 
-def _requests(envs, n, seed):
+def _requests(envs, n, seed, rate=4000.0):
     rng = np.random.default_rng(seed)
     reqs = []
     t = 0.0
     for i in range(n):
-        t += rng.exponential(1.0 / 4000.0)
+        t += rng.exponential(1.0 / rate)
         e = int(rng.integers(0, len(envs)))
         ctx, ref, ev = envs[e]
         L = len(ref.q) - 1
@@ -66,9 +66,12 @@ def _alone(envs, req):
     return [ev.rescale(ev.relinearize(ev.mul_no_relin(a, b)))]
 
 
-def test_served_mix_equals_alone_and_failure_isolated():
+@pytest.mark.parametrize("rate", [4000.0, 800.0])
+def test_served_mix_equals_alone_and_failure_isolated(rate):
+    """At a high arrival rate (large batches) and a low one (many one-request batches spread
+    over the streams, repeated shapes replayed from captured graphs)."""
     envs = _setup(7000)
-    reqs = _requests(envs, 40, 7)
+    reqs = _requests(envs, 40, 7, rate)
     srv = ckh.Server(hbm_budget_bytes=64 << 30, max_batch=16)
     t0 = srv.now() + 0.05
     bad = 11
