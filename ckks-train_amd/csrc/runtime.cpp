@@ -159,7 +159,7 @@ void Context::h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
     if (bytes == 0) return;
     std::lock_guard<std::mutex> lk(ring_mu_);
     if (!ring_) {
-        ring_cap_ = (size_t)64 << 20;
+        ring_cap_ = (size_t)16 << 20;
         void* p = nullptr;
         hip_check(hipHostMalloc(&p, ring_cap_, hipHostMallocDefault), "hipHostMalloc(staging ring)");
         ring_ = (uint8_t*)p;
