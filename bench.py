@@ -115,7 +115,7 @@ def served_stream(ckh, n_ops=240, rates=(1500.0, 4000.0), seed=SEED + 5):
     envs = []
     for i, (log_n, qb, pb, dnum) in enumerate(sets):
         q, p = ckh.params_primes(log_n, qb, pb)
-        ctx = ckh.Context(log_n, q, p, dnum, n_streams=4)
+        ctx = ckh.Context(log_n, q, p, dnum, n_streams=1)
         L = len(q) - 1
         relin = ctx.ksk_uniform(L, seed=seed + 1000 * i)
         rot = [ctx.ksk_uniform(L, seed=seed + 1000 * i + s, galois=ckh.galois_elt(log_n, s)) for s in range(1, 65)]
@@ -187,8 +187,23 @@ def _served_at(ckh, envs, n_ops, rate, seed):
         lat.append((time.perf_counter() - arrival) * 1e3)
         plain_out.append(out)
     makespan = time.perf_counter() - start
-    identical = all(s[0] == 0 and len(s[2]) == len(p) and all(a.digests() == b.digests() for a, b in zip(s[2], p))
-                    for s, p in zip(served, plain_out))
+    same = [s[0] == 0 and len(s[2]) == len(p) and all(a.digests() == b.digests() for a, b in zip(s[2], p))
+            for s, p in zip(served, plain_out)]
+    identical = all(same)
+    # a mismatch is recomputed once more with direct launches, to say which side differs
+    mismatches = []
+    for i, ok in enumerate(same):
+        if ok:
+            continue
+        op = ops[i]
+        ctx = envs[op[1]][0]
+        ctx.set_tuning("graphs", 0)
+        again = run_plain(op)
+        ctx.set_tuning("graphs", 1)
+        d_again = [a.digests() for a in again]
+        mismatches.append({"op": i, "kind": op[0], "level": op[2].level, "steps": len(op[4]),
+                           "status": served[i][0], "served_ok": [a.digests() for a in served[i][2]] == d_again,
+                           "plain_ok": [a.digests() for a in plain_out[i]] == d_again})
     lat = np.array(lat)
     mix = {k: sum(1 for o in ops if o[0] == k) for k in (0, 1, 2)}
     return {
@@ -199,6 +214,7 @@ def _served_at(ckh, envs, n_ops, rate, seed):
         "plain": {"ops_per_s": n_ops / makespan, "p50_ms": float(np.percentile(lat, 50)),
                   "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean())},
         "identical_alone_and_in_mix": bool(identical),
+        "mismatches": mismatches[:8],
     }
 
 

@@ -76,7 +76,8 @@ class Request(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [(f, u64) for f in ("submitted", "completed", "failed", "rejected", "batches", "peak_bytes")] + \
-               [(f, C.c_double) for f in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "makespan_s", "ops_per_s")]
+               [(f, C.c_double) for f in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "makespan_s", "ops_per_s",
+                                          "launch_ms", "launch_max_ms")]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

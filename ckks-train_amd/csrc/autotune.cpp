@@ -23,6 +23,10 @@ void Context::set_tuning(const std::string& key, int64_t value) {
         conv_tg = (u32)std::min<int64_t>(std::max<int64_t>(value, 1), 64);
     else if (key == "graph_cache")
         graph_cache_max = (u32)std::max<int64_t>(value, 1);
+    else if (key == "pool")
+        pool_mode = value != 0;
+    else if (key == "service_streams")
+        service_streams = (u32)std::min<int64_t>(std::max<int64_t>(value, 1), SERVICE_STREAMS);
     else
         fail(CKH_ERR_INVALID_PARAMS, "unknown tuning key " + key);
     clear_plans();
@@ -36,6 +40,8 @@ int64_t Context::get_tuning(const std::string& key) const {
     if (key == "conv_variant") return conv_variant;
     if (key == "fuse_ip") return fuse_ip;
     if (key == "conv_tg") return conv_tg;
+    if (key == "pool") return pool_mode;
+    if (key == "service_streams") return service_streams;
     fail(CKH_ERR_INVALID_PARAMS, "unknown tuning key " + key);
 }
 

@@ -211,7 +211,7 @@ ckh_status ckh_graph_stats(ckh_context* ctx, uint64_t* captures, uint64_t* launc
 ckh_status ckh_synchronize(ckh_context* ctx) {
     return guarded([&] {
         Context& c = ctx_of(ctx);
-        for (uint32_t i = 0; i < c.n_streams(); ++i) hip_check(hipStreamSynchronize(c.stream_at(i)), "synchronize");
+        for (uint32_t i = 0; i < c.n_all_streams(); ++i) hip_check(hipStreamSynchronize(c.stream_at(i)), "synchronize");
     });
 }
 

@@ -67,7 +67,27 @@ Context::Context(int ordinal, const ckh_params& p, u32 n_streams) {
         (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
     }
     if (n_streams == 0) n_streams = 1;
-    for (u32 i = 0; i < n_streams; ++i) {
+    n_user_streams_ = n_streams;
+    // The context's pool: a block freed on one stream serves another stream only once the free
+    // has completed (never through dependencies the runtime would insert between the streams);
+    // freed memory stays cached in the pool.
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = device;
+    if (hipMemPoolCreate(&ctx_pool_, &props) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx_pool_ = nullptr;
+    } else {
+        uint64_t keep = ~0ull;
+        int on = 1, off = 0;
+        (void)hipMemPoolSetAttribute(ctx_pool_, hipMemPoolAttrReleaseThreshold, &keep);
+        (void)hipMemPoolSetAttribute(ctx_pool_, hipMemPoolReuseFollowEventDependencies, &off);
+        (void)hipMemPoolSetAttribute(ctx_pool_, hipMemPoolReuseAllowInternalDependencies, &off);
+        (void)hipMemPoolSetAttribute(ctx_pool_, hipMemPoolReuseAllowOpportunistic, &on);
+    }
+    for (u32 i = 0; i < n_streams + SERVICE_STREAMS; ++i) {
         hipStream_t s;
         hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
         streams_.push_back(s);
@@ -84,27 +104,37 @@ Context::~Context() {
     for (auto& pl : plans_) {
         if (pl->done) (void)hipEventDestroy(pl->done);
         if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
-        if (pl->desc) (void)hipFree(pl->desc);
+        if (pl->desc) (void)hipFreeAsync(pl->desc, pl->s);
     }
     plans_.clear();
+    for (auto s : streams_) (void)hipStreamSynchronize(s);
     for (void* a : perm_allocs_) (void)hipFree(a);
     for (auto s : streams_) (void)hipStreamDestroy(s);
+    if (ctx_pool_) (void)hipMemPoolDestroy(ctx_pool_);
 }
 
+// A thread's choice is kept per context address; an entry left by a destroyed context at the
+// same address is out of range or harmless.
 hipStream_t Context::stream() const {
     auto it = tl_stream.find(this);
-    return streams_[it == tl_stream.end() ? 0 : it->second];
+    const u32 i = it == tl_stream.end() ? 0 : it->second;
+    return streams_[i < streams_.size() ? i : 0];
 }
 
 void Context::set_stream(u32 index) {
-    if (index >= streams_.size()) fail(CKH_ERR_INVALID_PARAMS, "stream index out of range");
+    if (index >= n_user_streams_) fail(CKH_ERR_INVALID_PARAMS, "stream index out of range");
     tl_stream[this] = index;
+}
+
+void Context::set_service_stream(u32 k) {
+    tl_stream[this] = n_user_streams_ + k % std::min(std::max(service_streams, 1u), SERVICE_STREAMS);
 }
 
 BufPtr Context::alloc(size_t words, hipStream_t s) {
     void* p = nullptr;
     size_t bytes = std::max<size_t>(words, 1) * sizeof(u64);
-    hipError_t e = hipMallocAsync(&p, bytes, s);
+    hipMemPool_t mp = pool_mode == 1 ? ctx_pool_ : nullptr;
+    hipError_t e = mp ? hipMallocFromPoolAsync(&p, bytes, mp, s) : hipMallocAsync(&p, bytes, s);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         std::ostringstream os;
@@ -124,6 +154,22 @@ BufPtr Context::alloc(size_t words, hipStream_t s) {
 void Context::free_buffer(Buffer* b) {
     if (!b->ptr) return;
     (void)hipSetDevice(device);
+    const hipStream_t cur = stream();
+    std::unique_lock<std::mutex> lk(plan_mu_, std::defer_lock);
+    if (cur != b->stream) {
+        // The releasing thread's stream may still read the buffer: the free waits for it. The
+        // allocation stream belongs to another thread; holding the plan lock keeps this off a
+        // graph capture in progress there (captures run under that lock).
+        lk.lock();
+        hipEvent_t ev = nullptr;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
+            if (hipEventRecord(ev, cur) == hipSuccess) (void)hipStreamWaitEvent(b->stream, ev, 0);
+            (void)hipEventDestroy(ev);
+        } else {
+            (void)hipGetLastError();
+            (void)hipStreamSynchronize(cur);
+        }
+    }
     (void)hipFreeAsync(b->ptr, b->stream);
     live_bytes -= std::max<size_t>(b->words, 1) * sizeof(u64);
     b->ptr = nullptr;
@@ -582,10 +628,14 @@ void Context::execute(const Program& p, hipStream_t s) {
 
 void Context::drop_plan(size_t i) {
     Plan& pl = *plans_[i];
-    if (pl.exec || pl.desc) (void)hipStreamSynchronize(pl.s);  // the last replay may still run
-    if (pl.done) (void)hipEventDestroy(pl.done);
+    // Only the plan's own last replay is waited for; its descriptor buffer returns to the pool in
+    // its stream's order (no device-wide synchronization).
+    if (pl.done) {
+        (void)hipEventSynchronize(pl.done);
+        (void)hipEventDestroy(pl.done);
+    }
     if (pl.exec) (void)hipGraphExecDestroy(pl.exec);
-    if (pl.desc) (void)hipFree(pl.desc);
+    if (pl.desc) (void)hipFreeAsync(pl.desc, pl.s);
     plans_.erase(plans_.begin() + i);
 }
 
@@ -621,16 +671,16 @@ bool Context::execute_graph(const Program& p, hipStream_t s) {
     }
     pl->last_use = ++plan_clock_;
     if (!pl->desc) {
-        hipError_t e = hipMalloc(&pl->desc, std::max<size_t>(p.bytes.size(), 16));
+        hipError_t e = hipMallocAsync(&pl->desc, std::max<size_t>(p.bytes.size(), 16), s);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             pl->desc = nullptr;
             return false;
         }
     }
-    // The previous replay reads the same descriptor buffer: it must have finished before the
-    // buffer is rewritten (a copy from pageable host memory is not relied on to wait for it).
-    if (pl->done) hip_check(hipEventSynchronize(pl->done), "hipEventSynchronize");
+    // The previous replay (on this same stream: plans are per stream) reads the descriptor buffer
+    // being rewritten. The rewrite is a DMA from pinned staging memory queued on s, so it runs
+    // after that replay in stream order; the host does not wait.
     h2d(pl->desc, p.bytes.data(), p.bytes.size(), s);
     if (!pl->exec) {
         hipGraph_t g = nullptr;

@@ -158,15 +158,30 @@ class Context : public std::enable_shared_from_this<Context> {
     ConvTables conv{};
     PrimeConst* d_pc = nullptr;
 
-    // Stream of the calling thread.
+    // Stream of the calling thread: one of the n_streams() caller streams (ckh_set_stream) or,
+    // for the serving loop's thread, one of SERVICE_STREAMS streams no caller can select (a
+    // graph is captured on a stream only by the thread working on it).
+    static constexpr u32 SERVICE_STREAMS = 4;
     hipStream_t stream() const;
     void set_stream(u32 index);
-    u32 n_streams() const { return (u32)streams_.size(); }
+    void set_service_stream(u32 k);  // the k-th of service_streams (tuning key, 1..SERVICE_STREAMS)
+    u32 service_streams = SERVICE_STREAMS;
+    u32 n_streams() const { return n_user_streams_; }
+    u32 n_all_streams() const { return (u32)streams_.size(); }
     hipStream_t stream_at(u32 i) const { return streams_[i]; }
 
+    // Device memory is stream-ordered: a buffer is freed on the stream it was allocated on. A
+    // release on another thread's stream first makes the allocation stream wait for the work
+    // enqueued so far on the releasing thread's stream (the buffer's last readers there).
+    // Tuning key "pool": 1 (default) = the context's own pool, where a block freed on one stream
+    // is handed to another stream only once its free has completed on the device (reuse on the
+    // freeing stream follows that stream's order); 0 = the device's default pool with the
+    // runtime's default reuse rules (which may hand a block to another stream before its free
+    // has run, behind a dependency the runtime inserts).
     BufPtr alloc(size_t words, hipStream_t s);
     BufPtr alloc(size_t words) { return alloc(words, stream()); }
     void free_buffer(Buffer* b);
+    u32 pool_mode = 1;
     std::atomic<u64> live_bytes{0};
 
     // Digit ranges of a level-l polynomial (keyswitch.rs:68-74).
@@ -231,7 +246,8 @@ class Context : public std::enable_shared_from_this<Context> {
         hipStream_t s = nullptr;
         std::vector<Program::Step> steps;
         size_t bytes = 0;
-        void* desc = nullptr;  // plain device allocation (a pool buffer would keep the context alive)
+        void* desc = nullptr;  // stream-ordered allocation on s, outside Buffer (which would keep the
+                               // context alive)
         hipGraphExec_t exec = nullptr;
         hipEvent_t done = nullptr;  // recorded after each replay
         u64 last_use = 0;
@@ -246,7 +262,9 @@ class Context : public std::enable_shared_from_this<Context> {
     // Drops every captured plan (a tuning change alters the launch sequences).
     void clear_plans();
 
-    std::vector<hipStream_t> streams_;
+    std::vector<hipStream_t> streams_;  // caller streams, then the service streams
+    hipMemPool_t ctx_pool_ = nullptr;   // pool 1 (null: the default pool serves it)
+    u32 n_user_streams_ = 0;
     std::vector<void*> perm_allocs_;
     std::vector<std::vector<u32>> modup_plan_;  // [level][digit]
     std::vector<u32> modup_maxtgt_;            // [level]

@@ -74,8 +74,7 @@ void Server::launch(std::vector<OpPtr>& ops) {
     for (Evaluator* ev : evs) {
         Context& c = ev->ctx;
         hip_check(hipSetDevice(c.device), "hipSetDevice");
-        const u32 si = rr_[&c]++ % c.n_streams();
-        c.set_stream(si);
+        c.set_service_stream(rr_[&c]++);
         hipStream_t s = c.stream();
         std::vector<OpPtr> mine;
         for (auto& op : ops)
@@ -219,7 +218,12 @@ void Server::loop() {
             }
         }
         try {
+            const double t_l = now();
             launch(admitted);
+            const double dt = (now() - t_l) * 1e3;
+            std::lock_guard<std::mutex> lk(mu_);
+            launch_ms_ += dt;
+            launch_max_ms_ = std::max(launch_max_ms_, dt);
         } catch (const Error& e) {
             std::lock_guard<std::mutex> lk(mu_);
             for (auto& op : admitted)
@@ -257,6 +261,8 @@ ckh_server_stats Server::stats() {
     s.rejected = rejected_;
     s.batches = batches_;
     s.peak_bytes = peak_;
+    s.launch_ms = launch_ms_;
+    s.launch_max_ms = launch_max_ms_;
     if (!lat_ms_.empty()) {
         std::vector<double> v = lat_ms_;
         std::sort(v.begin(), v.end());

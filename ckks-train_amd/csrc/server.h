@@ -63,6 +63,7 @@ class Server {
     u64 next_ticket_ = 1;
     u64 in_use_ = 0, peak_ = 0;
     u64 submitted_ = 0, completed_ = 0, failed_ = 0, rejected_ = 0, batches_ = 0;
+    double launch_ms_ = 0, launch_max_ms_ = 0;
     double first_arrival_ = -1, last_done_ = 0;
     std::vector<double> lat_ms_;
     bool stop_ = false;

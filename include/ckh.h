@@ -12,9 +12,11 @@
  *
  * Threading: a context may be used from several threads; each thread works on its selected
  * stream (ckh_set_stream). Operations are asynchronous on that stream: a returned handle may
- * be used at once by later calls; downloads and ckh_synchronize wait.
+ * be used at once by later calls; downloads and ckh_synchronize wait. A handle used on several
+ * streams must be released from the thread whose stream used it last (or after a synchronize).
  * Ownership: handles are reference counted (`retain` = the reference's cheap `Clone`,
- * ops.rs:17-20); device memory is returned to a stream-ordered pool on the last release.
+ * ops.rs:17-20); device memory is returned to a stream-ordered pool on the last release (on the
+ * stream it was allocated on, after the work queued so far on the releasing thread's stream).
  */
 #ifndef CKH_H
 #define CKH_H
@@ -72,7 +74,8 @@ typedef struct ckh_params {
 /* ------------------------------------------------------------------ context */
 
 /* Opens device `ordinal` for one parameter set and builds its tables (CudaDevice::new,
- * crates/ckks-train-cuda/src/device.rs:262-330). n_streams >= 1 HIP streams (default priority). */
+ * crates/ckks-train-cuda/src/device.rs:262-330). n_streams >= 1 HIP streams (default priority)
+ * for callers, plus the serving loop's own streams. */
 ckh_status ckh_device_open(int ordinal, const ckh_params *params, uint32_t n_streams,
                            ckh_context **out);
 void ckh_device_close(ckh_context *ctx);
@@ -94,7 +97,10 @@ ckh_status ckh_autotune_ks(ckh_context *ctx, uint32_t n, const ckh_poly *const *
  * directly); "graph_cache" (captured sequences kept per context); "conv_variant" (base conversion:
  * 0 = vector kernel, 1 = vector kernel with exact f64-piece sums for 60-bit sources, 2 = int8
  * matrix-core kernel); "conv_tg" (targets per base-conversion
- * workgroup); "fuse_ip" (1 = row pass fused into the inner product). */
+ * workgroup); "fuse_ip" (1 = row pass fused into the inner product); "pool" (1 = the context's
+ * own memory pool, where a block freed on one stream serves another stream only once the free
+ * has completed on the device, the default; 0 = the device's default pool); "service_streams"
+ * (1..4 streams the serving loop spreads its batches over, default 4). */
 ckh_status ckh_set_tuning(ckh_context *ctx, const char *key, int64_t value);
 /* Reads one tuning value (the keys of ckh_set_tuning). */
 ckh_status ckh_get_tuning(ckh_context *ctx, const char *key, int64_t *value);
@@ -245,9 +251,10 @@ ckh_status ckh_relinearize_batch(ckh_evaluator *ev, uint32_t n, const ckh_ct *co
  * keeps evaluators and their keys resident, admits requests in arrival order while the bytes
  * of the admitted work stay within hbm_budget_bytes, and batches whatever is admitted at each
  * turn: relinearizations of one evaluator as one batched key switch, mul+relin+rescale with its
- * relinearization batched likewise, hoisted rotations per request. Batches run on the contexts'
- * streams round-robin, so work admitted while other batches run overlaps with them. Results
- * are bit-identical to the same operations called alone. */
+ * relinearization batched likewise, hoisted rotations per request. Batches run round-robin on
+ * service streams of each context that no caller thread can select (tuning "service_streams"),
+ * so work admitted while other batches run overlaps with them and callers may keep working on
+ * their own streams meanwhile. Results are bit-identical to the same operations called alone. */
 typedef enum ckh_op_kind {
     CKH_OP_RELINEARIZE = 0,       /* a: 3 components -> 1 output */
     CKH_OP_HOISTED_ROTATE = 1,    /* a: 2 components, steps -> n_steps outputs */
@@ -273,6 +280,7 @@ typedef struct ckh_server_stats {
     uint64_t submitted, completed, failed, rejected, batches, peak_bytes;
     double p50_ms, p99_ms, mean_ms, max_ms; /* latency: completion - arrival */
     double makespan_s, ops_per_s;           /* first arrival to last completion */
+    double launch_ms, launch_max_ms;        /* scheduler host time enqueueing batches: total, longest */
 } ckh_server_stats;
 
 ckh_status ckh_server_create(const ckh_server_config *cfg, ckh_server **out);

@@ -110,3 +110,40 @@ def test_request_above_budget_is_rejected():
     assert status == ckh.Device.status and "budget" in msg and outs == []
     assert srv.stats().rejected == 1
     srv.close()
+
+
+def test_caller_keeps_working_while_serving():
+    """The caller's thread runs its own operations and digests on its stream while the scheduler
+    runs batches (the scheduler works on streams of its own, so neither disturbs the other's
+    launches or graph captures), and releases each served result as soon as it has checked it
+    (a release from another stream than the allocation's)."""
+    envs = _setup(9000)
+    reqs = _requests(envs, 40, 9, 1500.0)
+    srv = ckh.Server(hbm_budget_bytes=64 << 30, max_batch=16)
+    t0 = srv.now() + 0.02
+    tickets = [srv.submit(k, envs[e][2], a, b, steps, arrival_s=t0 + t) for (k, e, a, b, steps, t) in reqs]
+    for i, (tk, req) in enumerate(zip(tickets, reqs)):
+        alone = _alone(envs, req)  # caller-stream work while later requests are being served
+        status, msg, outs, _ = srv.poll(tk, wait=True)
+        assert status == 0, msg
+        assert [o.digests() for o in outs] == [w.digests() for w in alone], (i, req[0])
+        del outs, alone
+    assert srv.stats().completed == len(reqs)
+    srv.close()
+
+
+def test_pool_modes_agree():
+    """The context's own pool (default) and the device's default pool give the same results."""
+    envs = _setup(9500)
+    reqs = _requests(envs, 6, 95, 4000.0)
+    ctx = envs[0][0]
+    assert ctx.tuning("pool") == 1
+    first = [[o.digests() for o in _alone(envs, r)] for r in reqs]
+    for c, _, _ in envs:
+        c.set_tuning("pool", 0)
+    try:
+        second = [[o.digests() for o in _alone(envs, r)] for r in reqs]
+    finally:
+        for c, _, _ in envs:
+            c.set_tuning("pool", 1)
+    assert first == second
