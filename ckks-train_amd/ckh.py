@@ -154,7 +154,16 @@ _SIGS = {
     "ckh_conjugate": (C.c_int, [VP, VP, C.POINTER(VP)]),
     "ckh_hoisted_rotate": (C.c_int, [VP, VP, C.POINTER(C.c_int32), u32, C.POINTER(VP)]),
     "ckh_relinearize_batch": (C.c_int, [VP, u32, C.POINTER(VP), C.POINTER(VP)]),
-    "ckh_matvec_plain": (C.c_int, [VP, VP, C.POINTER(VP), C.c_double, u32, u32, u32, C.POINTER(VP)]),
+    "ckh_matvec_plain": (C.c_int, [VP, VP, C.POINTER(VP), C.c_double, u32, u32, u32, C.c_int32, C.POINTER(VP)]),
+    "ckh_matvec_enc": (C.c_int, [VP, VP, C.POINTER(VP), u32, u32, u32, C.c_int32, C.POINTER(VP)]),
+    "ckh_evaluator_set_config": (C.c_int, [VP, u32, u32, u32]),
+    "ckh_evaluator_get_config": (C.c_int, [VP, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
+    "ckh_mul_scalar": (C.c_int, [VP, VP, C.c_double, C.c_double, C.POINTER(VP)]),
+    "ckh_mul_plain_to_scale": (C.c_int, [VP, VP, VP, C.c_double, C.c_double, C.POINTER(VP)]),
+    "ckh_exact_product_scale": (C.c_int, [VP, u32, C.c_double, C.POINTER(C.c_double)]),
+    "ckh_target_plain_scale": (C.c_int, [VP, VP, C.c_double, C.POINTER(C.c_double)]),
+    "ckh_mul_lazy": (C.c_int, [VP, VP, VP, C.POINTER(VP)]),
+    "ckh_relin_lazy": (C.c_int, [VP, VP, C.POINTER(VP)]),
     "ckh_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(VP)]),
     "ckh_server_destroy": (None, [VP]),
     "ckh_server_now": (C.c_double, [VP]),
@@ -505,6 +514,37 @@ class Evaluator:
     def owns(self, ct):
         return bool(lib().ckh_owns(self.h, ct.h))
 
+    def set_config(self, hoist=True, bsgs=True, lazy_relin=True):
+        """Evaluator::with_config (EvalConfig, ops.rs:111-134)."""
+        _check(lib().ckh_evaluator_set_config(self.h, int(hoist), int(bsgs), int(lazy_relin)))
+
+    def config(self):
+        a, b, c = u32(0), u32(0), u32(0)
+        _check(lib().ckh_evaluator_get_config(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"hoist": bool(a.value), "bsgs": bool(b.value), "lazy_relin": bool(c.value)}
+
+    def exact_product_scale(self, level, target_scale):
+        v = C.c_double(0)
+        _check(lib().ckh_exact_product_scale(self.h, level, target_scale, C.byref(v)))
+        return v.value
+
+    def target_plain_scale(self, ct, target_scale):
+        v = C.c_double(0)
+        _check(lib().ckh_target_plain_scale(self.h, ct.h, target_scale, C.byref(v)))
+        return v.value
+
+    def mul_scalar(self, ct, value, target_scale):
+        return self._op(lib().ckh_mul_scalar, ct.h, value, target_scale)
+
+    def mul_plain_to_scale(self, a, pt, pt_scale, target_scale):
+        return self._op(lib().ckh_mul_plain_to_scale, a.h, pt.h, pt_scale, target_scale)
+
+    def mul_lazy(self, a, b):
+        return self._op(lib().ckh_mul_lazy, a.h, b.h)
+
+    def relin_lazy(self, s):
+        return self._op(lib().ckh_relin_lazy, s.h)
+
     def _op(self, fn, *args):
         h = VP()
         _check(fn(self.h, *args, C.byref(h)))
@@ -556,10 +596,16 @@ class Evaluator:
         _check(lib().ckh_hoisted_rotate(self.h, ct.h, st, n, outs))
         return [Ct(self.ctx, VP(outs[i])) for i in range(n)]
 
-    def matvec_plain(self, x, entries, pt_scale, baby, giant, stride=1):
-        """BSGS product; entries[k1 * baby + k2] is a Poly or None (zero entry)."""
+    def matvec_plain(self, x, entries, pt_scale, baby, giant, stride=1, offset=0):
+        """BSGS product; entries[k1 * baby + k2] is a Poly or None (zero entry), pre-rotated by
+        -(offset + baby k1) stride."""
         arr = (VP * len(entries))(*[e.h if e is not None else None for e in entries])
-        return self._op(lib().ckh_matvec_plain, x.h, arr, pt_scale, baby, giant, stride)
+        return self._op(lib().ckh_matvec_plain, x.h, arr, pt_scale, baby, giant, stride, offset)
+
+    def matvec_enc(self, x, entries, baby, giant, stride=1, offset=0):
+        """BSGS product with encrypted diagonals (Ct entries or None)."""
+        arr = (VP * len(entries))(*[e.h if e is not None else None for e in entries])
+        return self._op(lib().ckh_matvec_enc, x.h, arr, baby, giant, stride, offset)
 
     def relinearize_batch(self, cts):
         n = len(cts)

@@ -503,7 +503,29 @@ ckh_status ckh_evaluator_fork(ckh_evaluator* ev, ckh_evaluator** out) {
         auto* h = new ckh_evaluator();
         h->keep = ev->keep;
         h->e.reset(new Evaluator(e.ctx, e.relin, g));
+        h->e->config = e.config;
         *out = h;
+    });
+}
+
+ckh_status ckh_evaluator_set_config(ckh_evaluator* ev, uint32_t hoist, uint32_t bsgs, uint32_t lazy_relin) {
+    return guarded([&] {
+        EvalConfig& c = ev_of(ev).config;
+        c.hoist = hoist != 0;
+        c.bsgs = bsgs != 0;
+        c.lazy_relin = lazy_relin != 0;
+    });
+}
+
+ckh_status ckh_evaluator_get_config(ckh_evaluator* ev, uint32_t* hoist, uint32_t* bsgs, uint32_t* lazy_relin) {
+    return guarded([&] {
+        need(hoist, "hoist");
+        need(bsgs, "bsgs");
+        need(lazy_relin, "lazy_relin");
+        const EvalConfig& c = ev_of(ev).config;
+        *hoist = c.hoist;
+        *bsgs = c.bsgs;
+        *lazy_relin = c.lazy_relin;
     });
 }
 
@@ -552,6 +574,31 @@ ckh_status ckh_add_plain(ckh_evaluator* ev, const ckh_ct* a, const ckh_poly* pt,
 ckh_status ckh_mul_plain(ckh_evaluator* ev, const ckh_ct* a, const ckh_poly* pt, double pt_scale, ckh_ct** out) {
     CT_OP(ev_of(ev).mul_plain(ct_of(a), poly_of(pt), pt_scale));
 }
+ckh_status ckh_mul_scalar(ckh_evaluator* ev, const ckh_ct* ct, double value, double target_scale, ckh_ct** out) {
+    CT_OP(ev_of(ev).mul_scalar(ct_of(ct), value, target_scale));
+}
+ckh_status ckh_mul_plain_to_scale(ckh_evaluator* ev, const ckh_ct* a, const ckh_poly* pt, double pt_scale,
+                                  double target_scale, ckh_ct** out) {
+    CT_OP(ev_of(ev).mul_plain_to_scale(ct_of(a), poly_of(pt), pt_scale, target_scale));
+}
+ckh_status ckh_exact_product_scale(ckh_evaluator* ev, uint32_t level, double target_scale, double* out) {
+    return guarded([&] {
+        need(out, "out");
+        *out = ev_of(ev).exact_product_scale(level, target_scale);
+    });
+}
+ckh_status ckh_target_plain_scale(ckh_evaluator* ev, const ckh_ct* ct, double target_scale, double* out) {
+    return guarded([&] {
+        need(out, "out");
+        *out = ev_of(ev).target_plain_scale(ct_of(ct), target_scale);
+    });
+}
+ckh_status ckh_mul_lazy(ckh_evaluator* ev, const ckh_ct* a, const ckh_ct* b, ckh_ct** out) {
+    CT_OP(ev_of(ev).mul_lazy(ct_of(a), ct_of(b)));
+}
+ckh_status ckh_relin_lazy(ckh_evaluator* ev, const ckh_ct* sum, ckh_ct** out) {
+    CT_OP(ev_of(ev).relin_lazy(ct_of(sum)));
+}
 ckh_status ckh_mul_no_relin(ckh_evaluator* ev, const ckh_ct* a, const ckh_ct* b, ckh_ct** out) {
     CT_OP(ev_of(ev).mul_no_relin(ct_of(a), ct_of(b)));
 }
@@ -586,13 +633,24 @@ ckh_status ckh_hoisted_rotate(ckh_evaluator* ev, const ckh_ct* ct, const int32_t
 }
 
 ckh_status ckh_matvec_plain(ckh_evaluator* ev, const ckh_ct* x, const ckh_poly* const* entries, double pt_scale,
-                            uint32_t baby, uint32_t giant, uint32_t stride, ckh_ct** out) {
+                            uint32_t baby, uint32_t giant, uint32_t stride, int32_t offset, ckh_ct** out) {
     return guarded([&] {
         need(out, "out");
         need(entries, "entries");
         std::vector<const Poly*> v;
         for (uint64_t i = 0; i < (uint64_t)baby * giant; ++i) v.push_back(entries[i] ? &poly_of(entries[i]) : nullptr);
-        *out = wrap(ev_of(ev).matvec_plain(ct_of(x), v, pt_scale, baby, giant, stride));
+        *out = wrap(ev_of(ev).matvec_plain(ct_of(x), v, pt_scale, baby, giant, stride, offset));
+    });
+}
+
+ckh_status ckh_matvec_enc(ckh_evaluator* ev, const ckh_ct* x, const ckh_ct* const* entries, uint32_t baby,
+                          uint32_t giant, uint32_t stride, int32_t offset, ckh_ct** out) {
+    return guarded([&] {
+        need(out, "out");
+        need(entries, "entries");
+        std::vector<const Ct*> v;
+        for (uint64_t i = 0; i < (uint64_t)baby * giant; ++i) v.push_back(entries[i] ? &ct_of(entries[i]) : nullptr);
+        *out = wrap(ev_of(ev).matvec_enc(ct_of(x), v, baby, giant, stride, offset));
     });
 }
 

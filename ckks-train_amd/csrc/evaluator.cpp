@@ -2,6 +2,7 @@
 #include "evaluator.h"
 
 #include <cmath>
+#include <cstring>
 #include <sstream>
 
 #include "hostmath.h"
@@ -125,6 +126,24 @@ PolyPtr poly_rescale(Context& c, const Poly& in) {
     BufPtr scratch = c.alloc((u64)(1 + in.level) * c.n);
     c.rescale({{in.data(), in.level, out->data(), scratch->ptr, scratch->ptr + c.n}}, c.stream());
     hip_check(hipGetLastError(), "rescale");
+    return out;
+}
+
+// RnsPoly::mul_scalar_assign (crates/ckks-train-core/src/poly.rs:184-189; the scalar reduced mod
+// each q_i and applied as a Shoup product), then RnsPoly::neg (poly.rs:157-161) when neg is set.
+PolyPtr poly_scalar_mul(Context& c, const Poly& a, u64 k, bool neg) {
+    if (a.p_limbs) fail(CKH_ERR_INVALID_PARAMS, "scalar products take Q polynomials");
+    PolyPtr out = poly_alloc(c, a.level, 0, a.ntt);
+    std::vector<u64> ws(2 * (a.level + 1));
+    for (u32 i = 0; i <= a.level; ++i) {
+        const u64 q = c.primes[i], w = k % q;
+        ws[2 * i] = w;
+        ws[2 * i + 1] = (u64)(((unsigned __int128)w << 64) / q);
+    }
+    hipStream_t s = c.stream();
+    BufPtr consts = c.upload_blob(ws.data(), ws.size() * 8, s);
+    scalar_mul_launch(a.data(), out->data(), a.level + 1, consts->ptr, neg, c.d_pc, c.log_n, s);
+    hip_check(hipGetLastError(), "scalar product");
     return out;
 }
 
@@ -628,6 +647,12 @@ std::vector<CtPtr> Evaluator::hoisted_rotate(const Ct& ct, const std::vector<int
         for (auto& r : res) r = make_ct(ct.comps, ct.scale);
         return res;
     }
+    if (!config.hoist) {
+        // naive form: every step an individual rotation (same words, same count)
+        for (size_t i = 0; i < steps.size(); ++i)
+            res[i] = keys[i].first ? apply_galois(ct, keys[i].first) : make_ct(ct.comps, ct.scale);
+        return res;
+    }
     const u64 L = level + 1, n = ctx.n, k = ctx.np;
     const u32 nd = ctx.ndigits(level);
     const u64 dw = ctx.digit_words(level);
@@ -704,59 +729,209 @@ std::vector<CtPtr> Evaluator::rotate_batch(const std::vector<std::pair<const Ct*
 // every check before work, hoisted baby rotations for the baby steps a non-zero entry uses,
 // per giant block the sum of mul_plain(rot(x, k2 s), entry) in k2 order, the nonzero giant
 // shifts rotated (in one batch), blocks summed in k1 order.
-CtPtr Evaluator::matvec_plain(const Ct& x, const std::vector<const Poly*>& entries, double pt_scale, u32 baby,
-                              u32 giant, u32 stride) {
-    check_basis(x);
-    check_galois_input(x);
-    if (baby == 0 || giant == 0 || entries.size() != (size_t)baby * giant)
+// check_bsgs and check_stride / check_offset (linalg.rs:523-541,677-684): d * stride divides the
+// slot count, -d < offset <= 0, and every rotation the product will make has its key (the baby
+// steps of the present entries and the nonzero giant shifts of the blocks that hold one).
+void Evaluator::check_bsgs_shape(const Ct& x, u32 baby, u32 giant, u32 stride, int32_t offset,
+                                 const std::vector<bool>& present) const {
+    if (baby == 0 || giant == 0 || present.size() != (size_t)baby * giant)
         fail(CKH_ERR_INVALID_PARAMS, "entries must hold baby * giant diagonals");
-    const u64 slots = ctx.n / 2;
-    if (stride == 0 || ((u64)baby * giant * stride) > slots || slots % ((u64)baby * giant * stride) != 0)
+    const u64 slots = ctx.n / 2, d = (u64)baby * giant;
+    if (stride == 0 || d * stride > slots || slots % (d * stride) != 0)
         fail(CKH_ERR_INVALID_PARAMS, "d * stride must divide the slot count");
-    std::vector<bool> used(baby, false);
+    if (offset > 0 || (int64_t)offset <= -(int64_t)d)
+        fail(CKH_ERR_INVALID_PARAMS, "diagonal offset outside -d < offset <= 0");
     bool any = false;
+    std::vector<bool> used(baby, false), block(giant, false);
     for (u32 k1 = 0; k1 < giant; ++k1)
-        for (u32 k2 = 0; k2 < baby; ++k2) {
-            const Poly* w = entries[k1 * baby + k2];
-            if (!w) continue;
-            check_plain(ctx, *w);
-            check_level(x.level(), w->level);
-            used[k2] = any = true;
-        }
+        for (u32 k2 = 0; k2 < baby; ++k2)
+            if (present[k1 * baby + k2]) used[k2] = block[k1] = any = true;
     if (!any) fail(CKH_ERR_INVALID_PARAMS, "every diagonal entry is zero");
+    for (u32 k2 = 1; k2 < baby; ++k2)
+        if (used[k2] && rotation_elt((int32_t)(k2 * stride))) galois_key(rotation_elt((int32_t)(k2 * stride)), x.level());
+    for (u32 k1 = 0; k1 < giant; ++k1) {
+        const int32_t shift = (offset + (int32_t)(baby * k1)) * (int32_t)stride;
+        if (block[k1] && rotation_elt(shift)) galois_key(rotation_elt(shift), x.level());
+    }
+}
+
+CtPtr Evaluator::bsgs(const Ct& x, u32 baby, u32 giant, u32 stride, int32_t offset, const std::vector<bool>& present,
+                      const std::function<CtPtr(size_t, const Ct&)>& mul,
+                      const std::function<CtPtr(const CtPtr&)>& finish) {
+    // xs[k2] = rot(x, k2 stride) for the baby steps a present entry uses (one hoisted call).
     std::vector<int32_t> steps;
     std::vector<u32> used_k2;
-    for (u32 k2 = 0; k2 < baby; ++k2)
-        if (used[k2]) {
+    for (u32 k2 = 0; k2 < baby; ++k2) {
+        bool used = false;
+        for (u32 k1 = 0; k1 < giant && !used; ++k1) used = present[k1 * baby + k2];
+        if (used) {
             steps.push_back((int32_t)(k2 * stride));
             used_k2.push_back(k2);
         }
-    for (int32_t st : steps)
-        if (rotation_elt(st)) galois_key(rotation_elt(st), x.level());
-    for (u32 k1 = 1; k1 < giant; ++k1)
-        if (rotation_elt((int32_t)(baby * k1 * stride))) galois_key(rotation_elt((int32_t)(baby * k1 * stride)), x.level());
+    }
     std::vector<CtPtr> rot = hoisted_rotate(x, steps);
     std::vector<CtPtr> xs(baby);
     for (size_t i = 0; i < used_k2.size(); ++i) xs[used_k2[i]] = rot[i];
-    std::vector<CtPtr> blocks(giant);
-    for (u32 k1 = 0; k1 < giant; ++k1)
+    // Per block: with bsgs, the products summed, finished once and rotated by the block's giant
+    // shift (all blocks' giant rotations in one batched switch); naive, every product finished
+    // and rotated by itself. Sums are exact, so their order does not change the words.
+    std::vector<CtPtr> parts;
+    std::vector<std::pair<const Ct*, int32_t>> to_rotate;
+    for (u32 k1 = 0; k1 < giant; ++k1) {
+        const int32_t shift = (offset + (int32_t)(baby * k1)) * (int32_t)stride;
+        if (!config.bsgs && shift != 0) {
+            for (u32 k2 = 0; k2 < baby; ++k2) {
+                const size_t i = (size_t)k1 * baby + k2;
+                if (!present[i]) continue;
+                parts.push_back(finish(mul(i, *xs[k2])));
+                to_rotate.push_back({parts.back().get(), shift});
+            }
+            continue;
+        }
+        CtPtr inner;
         for (u32 k2 = 0; k2 < baby; ++k2) {
-            const Poly* w = entries[k1 * baby + k2];
-            if (!w) continue;
-            CtPtr t = mul_plain(*xs[k2], *w, pt_scale);
-            blocks[k1] = blocks[k1] ? add(*blocks[k1], *t, false) : t;
+            const size_t i = (size_t)k1 * baby + k2;
+            if (!present[i]) continue;
+            CtPtr t = mul(i, *xs[k2]);
+            inner = inner ? add(*inner, *t, false) : t;
         }
-    std::vector<std::pair<const Ct*, int32_t>> giant_rot;
-    std::vector<u32> idx;
-    for (u32 k1 = 0; k1 < giant; ++k1)
-        if (blocks[k1]) {
-            giant_rot.push_back({blocks[k1].get(), (int32_t)(baby * k1 * stride)});
-            idx.push_back(k1);
-        }
-    std::vector<CtPtr> rotated = rotate_batch(giant_rot);
+        if (!inner) continue;
+        parts.push_back(finish(inner));
+        to_rotate.push_back({parts.back().get(), shift});
+    }
+    std::vector<CtPtr> rotated = rotate_batch(to_rotate);
     CtPtr acc;
-    for (size_t i = 0; i < rotated.size(); ++i) acc = acc ? add(*acc, *rotated[i], false) : rotated[i];
+    for (auto& r : rotated) acc = acc ? add(*acc, *r, false) : r;
     return acc;
+}
+
+// Evaluator::matvec_plain (crates/ckks-train-ckks/src/linalg.rs:608-621): entries[k1 * baby +
+// k2] = rot(diag_{offset + baby k1 + k2}, -(offset + baby k1) stride) (NULL: a zero entry,
+// skipped), all at x's level with scale pt_scale.
+CtPtr Evaluator::matvec_plain(const Ct& x, const std::vector<const Poly*>& entries, double pt_scale, u32 baby,
+                              u32 giant, u32 stride, int32_t offset) {
+    check_basis(x);
+    check_galois_input(x);
+    std::vector<bool> present(entries.size());
+    for (size_t i = 0; i < entries.size(); ++i) {
+        if (!entries[i]) continue;
+        check_plain(ctx, *entries[i]);
+        check_level(x.level(), entries[i]->level);
+        present[i] = true;
+    }
+    check_bsgs_shape(x, baby, giant, stride, offset, present);
+    return bsgs(
+        x, baby, giant, stride, offset, present,
+        [&](size_t i, const Ct& xr) { return mul_plain(xr, *entries[i], pt_scale); },
+        [](const CtPtr& c) { return c; });
+}
+
+// Evaluator::matvec_enc (linalg.rs:638-665): encrypted 2-component diagonals; lazy_relin: per
+// block the tensor products summed at degree 2 and relinearized once, else one mul per product.
+CtPtr Evaluator::matvec_enc(const Ct& x, const std::vector<const Ct*>& entries, u32 baby, u32 giant, u32 stride,
+                            int32_t offset) {
+    check_basis(x);
+    check_galois_input(x);
+    std::vector<bool> present(entries.size());
+    const Ct* first = nullptr;
+    for (size_t i = 0; i < entries.size(); ++i) {
+        if (!entries[i]) continue;
+        check_basis(*entries[i]);
+        check_galois_input(*entries[i]);
+        present[i] = true;
+        if (!first) first = entries[i];
+    }
+    // check_entries (linalg.rs:545-556): one level, then one exact scale; then x's level
+    if (first) {
+        for (const Ct* e : entries)
+            if (e && e->level() != first->level()) fail(CKH_ERR_LEVEL_MISMATCH, "level mismatch");
+        for (const Ct* e : entries)
+            if (e && e->scale != first->scale) fail(CKH_ERR_SCALE_MISMATCH, "diagonal entries differ in scale");
+        check_level(x.level(), first->level());
+    }
+    check_bsgs_shape(x, baby, giant, stride, offset, present);
+    if (x.level() == 0) fail(CKH_ERR_NO_LEVELS_LEFT, "no levels left");
+    relin_key_checked(x.level());
+    if (config.lazy_relin)
+        return bsgs(
+            x, baby, giant, stride, offset, present,
+            [&](size_t i, const Ct& xr) { return mul_no_relin(*entries[i], xr); },
+            [&](const CtPtr& c) { return relinearize(*c); });
+    return bsgs(
+        x, baby, giant, stride, offset, present, [&](size_t i, const Ct& xr) { return mul(*entries[i], xr); },
+        [](const CtPtr& c) { return c; });
+}
+
+// HeOps::mul_lazy / relin_lazy (ops.rs:345-363).
+CtPtr Evaluator::mul_lazy(const Ct& a, const Ct& b) { return config.lazy_relin ? mul_no_relin(a, b) : mul(a, b); }
+
+CtPtr Evaluator::relin_lazy(const Ct& sum) {
+    if (config.lazy_relin) return relinearize(sum);
+    return make_ct(sum.comps, sum.scale);
+}
+
+// exact_product_scale (evaluator.rs:738-772): the f64 label L with L / q_l == target_scale, the
+// nearest of the candidates within 8 ulps of target_scale * q_l (nearest first, above before below).
+double Evaluator::exact_product_scale(u32 level, double target_scale) const {
+    if (level > ctx.nq - 1) fail(CKH_ERR_INVALID_PARAMS, "level exceeds max_level");
+    if (!(std::isfinite(target_scale) && target_scale > 0.0))
+        fail(CKH_ERR_INVALID_PARAMS, "target scale must be positive and finite");
+    if (level == 0) fail(CKH_ERR_NO_LEVELS_LEFT, "no levels left");
+    const double q = (double)ctx.primes[level];
+    const double l0 = target_scale * q;
+    if (!std::isfinite(l0)) fail(CKH_ERR_INVALID_PARAMS, "target scale times q_l overflows");
+    u64 bits;
+    std::memcpy(&bits, &l0, 8);
+    for (u64 k = 0; k <= 8; ++k)
+        for (u64 cand : {bits + k, bits - k}) {
+            double l;
+            std::memcpy(&l, &cand, 8);
+            if (l / q == target_scale) return l;
+        }
+    fail(CKH_ERR_SCALE_MISMATCH, "no label within 8 ulps divides to the target scale");
+}
+
+// check_plain_scale (evaluator.rs:108-119).
+static void check_plain_scale(double sp) {
+    if (sp < 1048576.0) fail(CKH_ERR_INVALID_PARAMS, "plaintext scale below 2^20");
+}
+
+// Evaluator::target_plain_scale (evaluator.rs:341-345).
+double Evaluator::target_plain_scale(const Ct& ct, double target_scale) const {
+    const double sp = exact_product_scale(ct.level(), target_scale) / ct.scale;
+    check_plain_scale(sp);
+    return sp;
+}
+
+// Evaluator::mul_scalar (evaluator.rs:369-383) with scalar_product_parts (775-797): every
+// component times the integer k = round(value * L / scale) (negated for k < 0), labelled L.
+CtPtr Evaluator::mul_scalar(const Ct& ct, double value, double target_scale) {
+    check_basis(ct);
+    if (!std::isfinite(value)) fail(CKH_ERR_INVALID_PARAMS, "scalar must be finite");
+    const double label = exact_product_scale(ct.level(), target_scale);
+    check_plain_scale(label / ct.scale);
+    const double k = std::round(value * (label / ct.scale));
+    if (std::fabs(k) >= 4611686018427387904.0) fail(CKH_ERR_INVALID_PARAMS, "scalar integer at or above 2^62");
+    std::vector<PolyPtr> out;
+    for (auto& p : ct.comps) out.push_back(poly_scalar_mul(ctx, *p, (u64)std::fabs(k), k < 0.0));
+    CtPtr r = make_ct(std::move(out), label);
+    count(&ckh_op_counter::ct_pt_mults);
+    return r;
+}
+
+// Evaluator::mul_plain_to_scale (evaluator.rs:399-418).
+CtPtr Evaluator::mul_plain_to_scale(const Ct& a, const Poly& pt, double pt_scale, double target_scale) {
+    check_basis(a);
+    check_plain(ctx, pt);
+    check_level(a.level(), pt.level);
+    const double label = exact_product_scale(a.level(), target_scale);
+    check_plain_scale(label / a.scale);
+    if (pt_scale != label / a.scale) fail(CKH_ERR_SCALE_MISMATCH, "plaintext scale is not the target plaintext scale");
+    std::vector<PolyPtr> out;
+    for (auto& c : a.comps) out.push_back(poly_pointwise(ctx, PW_MUL, *c, &pt));
+    CtPtr r = make_ct(std::move(out), label);
+    count(&ckh_op_counter::ct_pt_mults);
+    return r;
 }
 
 }  // namespace ckh

@@ -2,6 +2,7 @@
 // the checks, errors and counts of the CPU Evaluator (crates/ckks-train-ckks/src/evaluator.rs).
 #pragma once
 
+#include <functional>
 #include <map>
 #include <mutex>
 
@@ -21,6 +22,7 @@ PolyPtr poly_rescale(Context& c, const Poly& in);
 double time_ntt(Context& c, const std::vector<const Poly*>& polys, bool forward, u32 reps);
 PolyPtr poly_mod_down(Context& c, const Poly& qp);
 PolyPtr poly_pointwise(Context& c, PwOp op, const Poly& a, const Poly* b);
+PolyPtr poly_scalar_mul(Context& c, const Poly& a, u64 k, bool neg);
 // keyswitch::decompose: one buffer holding all digits; returns one QpPoly handle per digit.
 std::vector<PolyPtr> ks_decompose(Context& c, const Poly& d);
 // keyswitch::apply_key for digits laid out as ks_decompose returns them (or copied so).
@@ -34,12 +36,20 @@ KskPtr ksk_uniform(Context& c, u32 max_level, u64 galois, u64 seed);
 
 u64 galois_elt(u64 n, int32_t step);
 
+// EvalConfig (crates/ckks-train-ckks/src/ops.rs:111-134): the optimised (all true, the default)
+// or naive form of hoisted rotations (one shared decomposition), BSGS products (one giant
+// rotation per block) and relinearization of product sums (once per sum).
+struct EvalConfig {
+    bool hoist = true, bsgs = true, lazy_relin = true;
+};
+
 class Evaluator {
    public:
     Evaluator(Context& c, KskPtr relin, std::vector<KskPtr> galois);
     Context& ctx;
     KskPtr relin;
     std::map<u64, KskPtr> galois;
+    EvalConfig config;
 
     ckh_op_counter counter();
     void reset_counter();
@@ -64,7 +74,15 @@ class Evaluator {
     // Rotations of several ciphertexts (their own steps) switched in one batch.
     std::vector<CtPtr> rotate_batch(const std::vector<std::pair<const Ct*, int32_t>>& items);
     CtPtr matvec_plain(const Ct& x, const std::vector<const Poly*>& entries, double pt_scale, u32 baby, u32 giant,
-                       u32 stride);
+                       u32 stride, int32_t offset);
+    CtPtr matvec_enc(const Ct& x, const std::vector<const Ct*>& entries, u32 baby, u32 giant, u32 stride,
+                     int32_t offset);
+    CtPtr mul_scalar(const Ct& ct, double value, double target_scale);
+    CtPtr mul_plain_to_scale(const Ct& a, const Poly& pt, double pt_scale, double target_scale);
+    double exact_product_scale(u32 level, double target_scale) const;
+    double target_plain_scale(const Ct& ct, double target_scale) const;
+    CtPtr mul_lazy(const Ct& a, const Ct& b);
+    CtPtr relin_lazy(const Ct& sum);
 
    private:
     std::mutex mu_;
@@ -76,6 +94,14 @@ class Evaluator {
     const Ksk& galois_key(u64 elt, u32 level) const;
     u64 rotation_elt(int32_t steps) const;  // 0 for a multiple of the slot count
     CtPtr apply_galois(const Ct& ct, u64 elt);
+    // The BSGS shape checks shared by both products, before any work.
+    void check_bsgs_shape(const Ct& x, u32 baby, u32 giant, u32 stride, int32_t offset,
+                          const std::vector<bool>& present) const;
+    // The BSGS driver (linalg.rs:800-888): present[k1 * baby + k2] entries are multiplied by
+    // rot(x, k2 stride) with mul(index, rotated x) and each block (or, naive, each product) is
+    // finished with finish() before its giant rotation.
+    CtPtr bsgs(const Ct& x, u32 baby, u32 giant, u32 stride, int32_t offset, const std::vector<bool>& present,
+               const std::function<CtPtr(size_t, const Ct&)>& mul, const std::function<CtPtr(const CtPtr&)>& finish);
 };
 
 }  // namespace ckh

@@ -33,6 +33,11 @@ enum PwOp : u32 { PW_ADD = 0, PW_SUB = 1, PW_NEG = 2, PW_MUL = 3, PW_COPY = 4 };
 void pointwise_launch(PwOp op, const u64* a, const u64* b, u64* out, u32 limbs, u32 prime0,
                       const PrimeConst* pc, u32 log_n, hipStream_t stream);
 
+// out limb i <- (a limb i) * w_i mod q_i (Shoup pair (w_i, ws_i) at consts[2 i], consts[2 i + 1]),
+// negated mod q_i when neg is set; Q primes 0..limbs.
+void scalar_mul_launch(const u64* a, u64* out, u32 limbs, const u64* consts, bool neg, const PrimeConst* pc,
+                       u32 log_n, hipStream_t stream);
+
 // out limb i <- sigma_g(in limb i) in NTT form (index permutation, prime independent).
 void automorphism_ntt_launch(const u64* in, u64* out, u32 limbs, u64 g, u32 log_n, hipStream_t stream);
 

@@ -107,6 +107,27 @@ void pointwise_launch(PwOp op, const u64* a, const u64* b, u64* out, u32 limbs, 
                        pc, log_n);
 }
 
+__global__ __launch_bounds__(256) void k_scalar_mul(const u64* __restrict__ a, u64* __restrict__ out,
+                                                    const u64* __restrict__ consts, u32 neg,
+                                                    const PrimeConst* __restrict__ pc, u32 log_n) {
+    const u32 n = 1u << log_n;
+    const u32 c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const u64 i = (u64)blockIdx.y * n + c;
+    const u64 q = pc[blockIdx.y].q, w = consts[2 * blockIdx.y], ws = consts[2 * blockIdx.y + 1];
+    u64 v = shoup(a[i], w, ws, q);
+    if (neg) v = sub_mod(0, v, q);
+    out[i] = v;
+}
+
+void scalar_mul_launch(const u64* a, u64* out, u32 limbs, const u64* consts, bool neg, const PrimeConst* pc,
+                       u32 log_n, hipStream_t stream) {
+    if (limbs == 0) return;
+    const u32 n = 1u << log_n;
+    hipLaunchKernelGGL(k_scalar_mul, dim3((n + 255) / 256, limbs), dim3(256), 0, stream, a, out, consts, (u32)neg, pc,
+                       log_n);
+}
+
 __global__ __launch_bounds__(256) void k_automorphism_ntt(const u64* __restrict__ in, u64* __restrict__ out, u64 g,
                                                           u32 log_n) {
     const u32 n = 1u << log_n;

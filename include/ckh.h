@@ -198,9 +198,15 @@ typedef struct ckh_op_counter {
 ckh_status ckh_evaluator_create(ckh_context *ctx, const ckh_ksk *relin,
                                 const ckh_ksk *const *galois, uint32_t n_galois,
                                 ckh_evaluator **out);
-/* HeOps::fork (ops.rs:243-246): same keys, own counter from zero. */
+/* HeOps::fork (ops.rs:243-246): same keys and configuration, own counter from zero. */
 ckh_status ckh_evaluator_fork(ckh_evaluator *ev, ckh_evaluator **out);
 void ckh_evaluator_destroy(ckh_evaluator *ev);
+/* Evaluator::with_config / config (EvalConfig, ops.rs:111-134; default all 1): hoist = one
+ * shared decomposition per hoisted rotation (0: individual rotations, same words and counts);
+ * bsgs = one giant rotation per BSGS block (0: one per product, d - 1 rotations); lazy_relin =
+ * mul_lazy is mul_no_relin and relin_lazy relinearizes (0: mul, and relin_lazy passes through). */
+ckh_status ckh_evaluator_set_config(ckh_evaluator *ev, uint32_t hoist, uint32_t bsgs, uint32_t lazy_relin);
+ckh_status ckh_evaluator_get_config(ckh_evaluator *ev, uint32_t *hoist, uint32_t *bsgs, uint32_t *lazy_relin);
 
 ckh_status ckh_counter(ckh_evaluator *ev, ckh_op_counter *out);
 ckh_status ckh_reset_counter(ckh_evaluator *ev);
@@ -219,6 +225,21 @@ ckh_status ckh_add_plain(ckh_evaluator *ev, const ckh_ct *a, const ckh_poly *pt,
                          ckh_ct **out);
 ckh_status ckh_mul_plain(ckh_evaluator *ev, const ckh_ct *a, const ckh_poly *pt, double pt_scale,
                          ckh_ct **out);
+/* HeOps::mul_scalar (evaluator.rs:369-383): ct times the integer round(value * L / scale),
+ * labelled L = exact_product_scale(level, target_scale), so a rescale lands on target_scale. */
+ckh_status ckh_mul_scalar(ckh_evaluator *ev, const ckh_ct *ct, double value, double target_scale,
+                          ckh_ct **out);
+/* HeOps::mul_plain_to_scale (evaluator.rs:399-418): pt_scale must equal
+ * target_plain_scale(a, target_scale) exactly; the product is labelled L. */
+ckh_status ckh_mul_plain_to_scale(ckh_evaluator *ev, const ckh_ct *a, const ckh_poly *pt, double pt_scale,
+                                  double target_scale, ckh_ct **out);
+/* HeOps::exact_product_scale / target_plain_scale (ops.rs:322-337; evaluator.rs:330-345,
+ * 738-772): the label L with L / q_level == target_scale exactly, and L / ct scale. */
+ckh_status ckh_exact_product_scale(ckh_evaluator *ev, uint32_t level, double target_scale, double *out);
+ckh_status ckh_target_plain_scale(ckh_evaluator *ev, const ckh_ct *ct, double target_scale, double *out);
+/* HeOps::mul_lazy / relin_lazy (ops.rs:339-356), following the lazy_relin flag. */
+ckh_status ckh_mul_lazy(ckh_evaluator *ev, const ckh_ct *a, const ckh_ct *b, ckh_ct **out);
+ckh_status ckh_relin_lazy(ckh_evaluator *ev, const ckh_ct *sum, ckh_ct **out);
 /* HeOps::mul_no_relin (evaluator.rs:422-438), relinearize (441-458), mul (463-467). */
 ckh_status ckh_mul_no_relin(ckh_evaluator *ev, const ckh_ct *a, const ckh_ct *b, ckh_ct **out);
 ckh_status ckh_relinearize(ckh_evaluator *ev, const ckh_ct *ct, ckh_ct **out);
@@ -233,14 +254,19 @@ ckh_status ckh_rotate(ckh_evaluator *ev, const ckh_ct *ct, int32_t steps, ckh_ct
 ckh_status ckh_conjugate(ckh_evaluator *ev, const ckh_ct *ct, ckh_ct **out);
 ckh_status ckh_hoisted_rotate(ckh_evaluator *ev, const ckh_ct *ct, const int32_t *steps,
                               uint32_t n_steps, ckh_ct **outs);
-/* Evaluator::matvec_plain's BSGS product (crates/ckks-train-ckks/src/linalg.rs:608-621,800-888):
- * entries[k1 * baby + k2] is the pre-rotated plaintext diagonal of giant block k1, baby step k2
- * (NULL = a zero entry, skipped), all at x's level with scale pt_scale. The baby rotations
- * k2 * stride share one decomposition; block k1 is rotated by baby * k1 * stride (the giant
- * rotations of all blocks are switched in one batch). Returns the unrescaled sum. */
+/* Evaluator::matvec_plain's BSGS product (crates/ckks-train-ckks/src/linalg.rs:608-621,800-888)
+ * for Diagonals::from_prerotated_with_offset (linalg.rs:230-260): entries[k1 * baby + k2] is
+ * rot(diag_{offset + baby k1 + k2}, -(offset + baby k1) stride) (NULL = a zero entry, skipped),
+ * all at x's level with scale pt_scale; -baby * giant < offset <= 0. The baby rotations
+ * k2 * stride share one decomposition; block k1 is rotated by (offset + baby k1) stride (the
+ * giant rotations of all blocks are switched in one batch). Returns the unrescaled sum. */
 ckh_status ckh_matvec_plain(ckh_evaluator *ev, const ckh_ct *x, const ckh_poly *const *entries,
                             double pt_scale, uint32_t baby, uint32_t giant, uint32_t stride,
-                            ckh_ct **out);
+                            int32_t offset, ckh_ct **out);
+/* Evaluator::matvec_enc (linalg.rs:638-665): the same product with encrypted 2-component
+ * diagonals; lazy_relin: per block the tensor products are summed and relinearized once. */
+ckh_status ckh_matvec_enc(ckh_evaluator *ev, const ckh_ct *x, const ckh_ct *const *entries, uint32_t baby,
+                          uint32_t giant, uint32_t stride, int32_t offset, ckh_ct **out);
 /* n relinearizations in batched launches (bit-identical to n ckh_relinearize calls). */
 ckh_status ckh_relinearize_batch(ckh_evaluator *ev, uint32_t n, const ckh_ct *const *in,
                                  ckh_ct **out);

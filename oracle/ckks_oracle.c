@@ -802,6 +802,19 @@ void or_poly_mul(const or_params *pp, u64 *a, const u64 *b, u64 limbs) {
         for (u64 c = 0; c < pp->n; ++c) a[i * pp->n + c] = or_mod_mul(&pp->mq[i], a[i * pp->n + c], b[i * pp->n + c]);
 }
 
+/* RnsPoly::mul_scalar_assign (poly.rs:184-189): limb i times k mod q_i as a Shoup product;
+ * then RnsPoly::neg (poly.rs:157-161), x -> 0 - x mod q_i, when neg is set. */
+void or_poly_mul_scalar(const or_params *pp, u64 *a, u64 limbs, u64 k, int neg) {
+    for (u64 i = 0; i < limbs; ++i) {
+        const or_mod *m = &pp->mq[i];
+        shoup_t s = mk_shoup(m, k);
+        for (u64 c = 0; c < pp->n; ++c) {
+            u64 v = msh(m, a[i * pp->n + c], s);
+            a[i * pp->n + c] = neg ? or_mod_sub(m, 0, v) : v;
+        }
+    }
+}
+
 /* Evaluator::relinearize (evaluator.rs:441-458). */
 void or_relinearize(const or_params *pp, const u64 *c0, const u64 *c1, const u64 *c2, u64 l,
                     const u64 *key, u64 kml, u64 *out0, u64 *out1) {

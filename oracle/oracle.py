@@ -4,7 +4,9 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the CPU-
 leg of bench.py, always as the checker. The product library never imports it.
 """
 import ctypes as C
+import math
 import os
+import struct
 import subprocess
 
 import numpy as np
@@ -77,6 +79,7 @@ def lib():
         L.or_mul_no_relin.argtypes = [C.c_void_p, P64, P64, P64, P64, u64, P64, P64, P64]
         L.or_poly_add.argtypes = [C.c_void_p, P64, P64, u64]
         L.or_poly_mul.argtypes = [C.c_void_p, P64, P64, u64]
+        L.or_poly_mul_scalar.argtypes = [C.c_void_p, P64, u64, u64, C.c_int]
         L.or_uniform_word.restype = u64
         L.or_uniform_word.argtypes = [u64, u64, u64, u64]
         L.or_fill_uniform.argtypes = [P64, u64, u64, u64, u64]
@@ -214,6 +217,60 @@ def get_threads():
     return lib().or_get_threads()
 
 
+MIN_PLAIN_SCALE = 1048576.0  # evaluator.rs:108
+MAX_SCALAR_INT = 4611686018427387904.0  # evaluator.rs:103, 2^62
+LABEL_SEARCH_ULPS = 8  # evaluator.rs:100
+
+
+def _bits(x):
+    return struct.unpack("<Q", struct.pack("<d", x))[0]
+
+
+def _from_bits(b):
+    return struct.unpack("<d", struct.pack("<Q", b & ((1 << 64) - 1)))[0]
+
+
+def exact_product_scale(q, level, target_scale):
+    """exact_product_scale (crates/ckks-train-ckks/src/evaluator.rs:738-772): the f64 label L
+    with L / q_level == target_scale, searched from target_scale * q_level outwards (candidates
+    bits + k, bits - k for k = 0..=8). Raises ValueError(kind) with the HekitError variant."""
+    if level > len(q) - 1:
+        raise ValueError("InvalidParams")
+    if not (math.isfinite(target_scale) and target_scale > 0.0):
+        raise ValueError("InvalidParams")
+    if level == 0:
+        raise ValueError("NoLevelsLeft")
+    ql = float(q[level])
+    l0 = target_scale * ql
+    if not math.isfinite(l0):
+        raise ValueError("InvalidParams")
+    b = _bits(l0)
+    for k in range(LABEL_SEARCH_ULPS + 1):
+        for cand in (b + k, b - k):
+            lab = _from_bits(cand)
+            if lab / ql == target_scale:
+                return lab
+    raise ValueError("ScaleMismatch")
+
+
+def scalar_product_parts(q, level, source_scale, value, target_scale):
+    """scalar_product_parts (evaluator.rs:775-797): (label, plaintext scale, integer k)."""
+    if not math.isfinite(value):
+        raise ValueError("InvalidParams")
+    label = exact_product_scale(q, level, target_scale)
+    if label / source_scale < MIN_PLAIN_SCALE:
+        raise ValueError("InvalidParams")
+    x = value * (label / source_scale)
+    a = abs(x)  # f64::round: half away from zero (a - floor(a) is exact)
+    r = float(math.floor(a))
+    if a - r >= 0.5:
+        r += 1.0
+    k = r if x >= 0 else -r
+    if abs(k) >= MAX_SCALAR_INT:
+        raise ValueError("InvalidParams")
+    return label, label / source_scale, k
+
+
 class Params:
     """or_params: a CkksParams with its Q and P bases."""
 
@@ -318,6 +375,17 @@ class Params:
         out = arr(a).copy()
         lib().or_poly_add(self.h, ptr(out), ptr(arr(b)), level + 1)
         return out.reshape(level + 1, self.n)
+
+    def poly_mul_scalar(self, a, level, k, neg):
+        out = arr(a).copy()
+        lib().or_poly_mul_scalar(self.h, ptr(out), level + 1, k, int(neg))
+        return out.reshape(level + 1, self.n)
+
+    def exact_product_scale(self, level, target_scale):
+        return exact_product_scale(self.q, level, target_scale)
+
+    def mul_scalar_parts(self, level, source_scale, value, target_scale):
+        return scalar_product_parts(self.q, level, source_scale, value, target_scale)
 
     def poly_mul(self, a, b, level):
         out = arr(a).copy()

@@ -71,14 +71,16 @@ def near_tie_p_part(ref):
     return O.ntt_forward(n, ref.p, limbs.ravel()).reshape(len(ref.p), n)
 
 
-def oracle_matvec(ref, c0, c1, level, entries, baby, giant, stride, key_of_step, kml):
-    """The BSGS product composed from oracle operations, in the reference's order
-    (crates/ckks-train-ckks/src/linalg.rs:800-888): hoisted baby rotations of the used steps,
-    per giant block the sum over k2 of mul_plain(rot(x, k2 s), entry), the block rotated by
-    baby k1 s, blocks summed in k1 order. entries[k1 * baby + k2] is a host array or None;
-    key_of_step(step) returns the host key of that rotation step."""
+def oracle_bsgs(ref, c0, c1, level, present, baby, giant, stride, offset, mul, finish, share_giant,
+                key_of_step, kml):
+    """The BSGS driver composed from oracle operations, in the reference's order
+    (crates/ckks-train-ckks/src/linalg.rs:800-888): hoisted baby rotations of the used steps;
+    per giant block k1 with shift (offset + baby k1) stride: with share_giant (EvalConfig::bsgs),
+    finish(sum over k2 of mul(i, rot(x, k2 s))) rotated by the shift, else (shift != 0) every
+    product finished and rotated by itself; blocks summed in k1 order. mul(i, (x0, x1)) returns
+    the components of entry i times the rotated x; key_of_step(step) the host key of a step."""
     slots = ref.n // 2
-    used = [k2 for k2 in range(baby) if any(entries[k1 * baby + k2] is not None for k1 in range(giant))]
+    used = [k2 for k2 in range(baby) if any(present[k1 * baby + k2] for k1 in range(giant))]
     nontriv = [k2 for k2 in used if (k2 * stride) % slots]
     elts = [galois_elt_host(ref.n, k2 * stride) for k2 in nontriv]
     hoisted = ref.hoisted_galois(c0, c1, level, elts, [key_of_step(k2 * stride) for k2 in nontriv], kml)
@@ -89,23 +91,38 @@ def oracle_matvec(ref, c0, c1, level, entries, baby, giant, stride, key_of_step,
             xs[k2] = (hoisted[i, 0], hoisted[i, 1])
         else:
             xs[k2] = (c0.reshape(level + 1, ref.n), c1.reshape(level + 1, ref.n))
+
+    def add(a, b):
+        return b if a is None else tuple(ref.poly_add(u, v, level) for u, v in zip(a, b))
+
+    def rot(t, shift):
+        if shift % slots == 0:
+            return t
+        return ref.apply_galois(t[0], t[1], level, galois_elt_host(ref.n, shift), key_of_step(shift % slots), kml)
+
     acc = None
     for k1 in range(giant):
+        shift = (offset + baby * k1) * stride
+        if not share_giant and shift % slots:
+            for k2 in range(baby):
+                if present[k1 * baby + k2]:
+                    acc = add(acc, rot(finish(mul(k1 * baby + k2, xs[k2])), shift))
+            continue
         blk = None
         for k2 in range(baby):
-            w = entries[k1 * baby + k2]
-            if w is None:
-                continue
-            t = tuple(ref.poly_mul(x, w, level) for x in xs[k2])
-            blk = t if blk is None else tuple(ref.poly_add(a, b, level) for a, b in zip(blk, t))
-        if blk is None:
-            continue
-        shift = (baby * k1 * stride) % slots
-        if shift:
-            blk = ref.apply_galois(blk[0], blk[1], level, galois_elt_host(ref.n, baby * k1 * stride),
-                                   key_of_step(baby * k1 * stride), kml)
-        acc = blk if acc is None else tuple(ref.poly_add(a, b, level) for a, b in zip(acc, blk))
+            if present[k1 * baby + k2]:
+                blk = add(blk, mul(k1 * baby + k2, xs[k2]))
+        if blk is not None:
+            acc = add(acc, rot(finish(blk), shift))
     return acc
+
+
+def oracle_matvec(ref, c0, c1, level, entries, baby, giant, stride, key_of_step, kml, offset=0, share_giant=True):
+    """matvec_plain composed from oracle operations: mul = mul_plain by the host entry."""
+    present = [e is not None for e in entries]
+    return oracle_bsgs(ref, c0, c1, level, present, baby, giant, stride, offset,
+                       lambda i, x: tuple(ref.poly_mul(xc, entries[i], level) for xc in x),
+                       lambda t: t, share_giant, key_of_step, kml)
 
 
 def galois_elt_host(n, step):

@@ -1,0 +1,249 @@
+"""The HeOps surface beyond key switching against the oracle, bit for bit, through the C ABI:
+mul_scalar / mul_plain_to_scale with their exact labels (evaluator.rs:330-418,738-797), the
+EvalConfig flags (ops.rs:111-134: hoist, bsgs, lazy_relin), mul_lazy / relin_lazy
+(ops.rs:339-356), and the BSGS products with a diagonal offset and encrypted diagonals
+(linalg.rs:608-665,800-888)."""
+import numpy as np
+import pytest
+
+import ckh
+import oracle as O
+from gpu_cases import C4_SHAPE_SMALL_N, SMALL, galois_elt_host, host_uniform, make, oracle_bsgs, oracle_matvec
+
+pytestmark = pytest.mark.gpu
+
+DELTA = 2.0 ** 40
+
+
+def _ct(ctx, ref, level, seed, ncomp=2, scale=DELTA):
+    polys = np.stack([host_uniform(ref, level, seed + i) for i in range(ncomp)])
+    return ctx.ct_upload(polys, level, scale), polys
+
+
+@pytest.fixture(scope="module", params=[SMALL, C4_SHAPE_SMALL_N], ids=lambda c: c[0])
+def env(request):
+    print("case", request.param[0], flush=True)
+    ctx, ref = make(request.param)
+    yield ctx, ref
+    ctx.synchronize()
+
+
+def test_exact_product_scale_matches_oracle(env):
+    ctx, ref = env
+    ev = ckh.Evaluator(ctx)
+    rng = np.random.default_rng(3)
+    for level in range(1, len(ref.q)):
+        for t in [DELTA, 1.5 * DELTA, DELTA * DELTA / ref.q[level], 3.0] + list(2.0 ** rng.uniform(20, 60, 20)):
+            try:
+                want = ref.exact_product_scale(level, float(t))
+            except ValueError as e:
+                with pytest.raises(getattr(ckh, str(e))):
+                    ev.exact_product_scale(level, float(t))
+                continue
+            assert ev.exact_product_scale(level, float(t)) == want
+    with pytest.raises(ckh.NoLevelsLeft):
+        ev.exact_product_scale(0, DELTA)
+    with pytest.raises(ckh.InvalidParams):
+        ev.exact_product_scale(len(ref.q), DELTA)
+    with pytest.raises(ckh.InvalidParams):
+        ev.exact_product_scale(1, -1.0)
+
+
+def test_mul_scalar_matches_oracle_and_lands_on_the_target(env):
+    ctx, ref = env
+    L = len(ref.q) - 1
+    ev = ckh.Evaluator(ctx)
+    for level in (L, 1):
+        for ncomp in (2, 3):
+            ct, h = _ct(ctx, ref, level, 700 + 10 * level + ncomp, ncomp)
+            for target in (DELTA, 1.5 * DELTA):
+                for value in (1.0, 0.5, -1.25, 1e-3, 0.0125, 3.0, 0.0):
+                    ev.reset_counter()
+                    got = ev.mul_scalar(ct, value, target)
+                    label, _, k = ref.mul_scalar_parts(level, DELTA, value, target)
+                    assert got.scale == label and got.level == level and got.size() == ncomp
+                    d = got.download()
+                    for c in range(ncomp):
+                        want = ref.poly_mul_scalar(h[c].ravel(), level, int(abs(k)), k < 0)
+                        assert (d[c] == want).all(), (level, ncomp, target, value)
+                    assert ev.counter().ct_pt_mults == 1
+                    if ncomp == 2:
+                        assert ev.rescale(got).scale == target
+    ct0, _ = _ct(ctx, ref, 0, 790)
+    with pytest.raises(ckh.NoLevelsLeft):
+        ev.mul_scalar(ct0, 1.0, DELTA)
+    ct, _ = _ct(ctx, ref, L, 791)
+    for bad in (float("nan"), float("inf")):
+        with pytest.raises(ckh.InvalidParams):
+            ev.mul_scalar(ct, bad, DELTA)
+    for bad in (0.0, -DELTA, float("nan"), float("inf")):
+        with pytest.raises(ckh.InvalidParams):
+            ev.mul_scalar(ct, 1.0, bad)
+    with pytest.raises(ckh.InvalidParams):  # an integer at or above 2^62
+        ev.mul_scalar(ct, 1e9, DELTA * 1e9)
+
+
+def test_mul_plain_to_scale(env):
+    ctx, ref = env
+    L = len(ref.q) - 1
+    ev = ckh.Evaluator(ctx)
+    ct, h = _ct(ctx, ref, L, 800)
+    target = DELTA
+    sp = ev.target_plain_scale(ct, target)
+    assert sp == ref.exact_product_scale(L, target) / DELTA
+    w = host_uniform(ref, L, 810)
+    pt = ctx.upload(w, L)
+    got = ev.mul_plain_to_scale(ct, pt, sp, target)
+    assert got.scale == ref.exact_product_scale(L, target)
+    d = got.download()
+    for c in range(2):
+        assert (d[c] == ref.poly_mul(h[c], w, L)).all()
+    assert ev.rescale(got).scale == target
+    with pytest.raises(ckh.ScaleMismatch):
+        ev.mul_plain_to_scale(ct, pt, sp * 2.0, target)
+    with pytest.raises(ckh.LevelMismatch):
+        ev.mul_plain_to_scale(ct, ctx.upload(host_uniform(ref, L - 1, 811), L - 1), sp, target)
+    with pytest.raises(ckh.InvalidParams):  # plaintext scale below 2^20
+        ev.target_plain_scale(ct, 1.0)
+
+
+def test_config_flags_and_lazy_products(env):
+    ctx, ref = env
+    L = len(ref.q) - 1
+    relin = ctx.ksk_uniform(L, seed=14)
+    steps = [1, 3, -2]
+    gkeys = [ctx.ksk_uniform(L, seed=60 + i, galois=ckh.galois_elt(ref.log_n, s)) for i, s in enumerate(steps)]
+    ev = ckh.Evaluator(ctx, relin, gkeys)
+    assert ev.config() == {"hoist": True, "bsgs": True, "lazy_relin": True}
+    a, ha = _ct(ctx, ref, L, 900)
+    b, hb = _ct(ctx, ref, L, 910)
+    # hoist off: individual rotations, same words and counts
+    hoisted = [o.download() for o in ev.hoisted_rotate(a, steps)]
+    c_h = ev.counter().rotations
+    ev.set_config(hoist=False)
+    assert ev.config()["hoist"] is False
+    ev.reset_counter()
+    single = [o.download() for o in ev.hoisted_rotate(a, steps)]
+    assert ev.counter().rotations == c_h == 3
+    assert all((x == y).all() for x, y in zip(hoisted, single))
+    # lazy_relin on: mul_lazy is the tensor product, relin_lazy relinearizes
+    ev.set_config()
+    t = ev.mul_lazy(a, b)
+    assert t.size() == 3
+    d0, d1, d2 = ref.mul_no_relin(ha[0], ha[1], hb[0], hb[1], L)
+    got = t.download()
+    assert (got[0] == d0).all() and (got[1] == d1).all() and (got[2] == d2).all()
+    r = ev.relin_lazy(t).download()
+    w0, w1 = ref.relinearize(d0, d1, d2, L, ref.uniform_key(L, 14), L)
+    assert (r[0] == w0).all() and (r[1] == w1).all()
+    # lazy_relin off: mul_lazy is mul, relin_lazy passes its input through
+    ev.set_config(lazy_relin=False)
+    m = ev.mul_lazy(a, b)
+    assert m.size() == 2
+    got = m.download()
+    assert (got[0] == w0).all() and (got[1] == w1).all()
+    assert (ev.relin_lazy(m).download() == got).all()
+    # fork keeps the configuration
+    assert ev.fork().config() == {"hoist": True, "bsgs": True, "lazy_relin": False}
+
+
+def _keys_for(ctx, ref, L, steps, seed):
+    slots = ref.n // 2
+    norm = sorted({s % slots for s in steps if s % slots})
+    keys, hseed = [], {}
+    for i, st in enumerate(norm):
+        keys.append(ctx.ksk_uniform(L, seed=seed + 10 * i, galois=ckh.galois_elt(ref.log_n, st)))
+        hseed[st] = seed + 10 * i
+    cache = {}
+
+    def key_of_step(st):
+        st %= slots
+        if st not in cache:
+            cache[st] = ref.uniform_key(L, hseed[st])
+        return cache[st]
+
+    return keys, key_of_step
+
+
+@pytest.mark.parametrize("offset,bsgs", [(0, True), (-3, True), (-3, False), (0, False)])
+def test_matvec_plain_offset_and_naive(offset, bsgs):
+    ctx, ref = make(C4_SHAPE_SMALL_N)
+    L = len(ref.q) - 1
+    baby, giant, stride = 4, 4, 2
+    steps = [k2 * stride for k2 in range(1, baby)] + [(offset + baby * k1) * stride for k1 in range(giant)]
+    keys, key_of_step = _keys_for(ctx, ref, L, steps, 1200)
+    ev = ckh.Evaluator(ctx, None, keys)
+    ev.set_config(bsgs=bsgs)
+    level = L - 1
+    c0, c1 = host_uniform(ref, level, 1201), host_uniform(ref, level, 1202)
+    x = ctx.ct_upload(np.stack([c0, c1]), level, DELTA)
+    mask = [(i % 5) != 2 for i in range(baby * giant)]
+    entries_h = [host_uniform(ref, level, 1300 + i) if m else None for i, m in enumerate(mask)]
+    entries_d = [ctx.upload(w, level) if w is not None else None for w in entries_h]
+    got = ev.matvec_plain(x, entries_d, DELTA, baby, giant, stride, offset).download()
+    want = oracle_matvec(ref, c0, c1, level, entries_h, baby, giant, stride, key_of_step, L, offset=offset,
+                         share_giant=bsgs)
+    assert (got[0] == want[0]).all() and (got[1] == want[1]).all()
+    with pytest.raises(ckh.InvalidParams):
+        ev.matvec_plain(x, entries_d, DELTA, baby, giant, stride, 1)
+    with pytest.raises(ckh.InvalidParams):
+        ev.matvec_plain(x, entries_d, DELTA, baby, giant, stride, -baby * giant)
+
+
+@pytest.mark.parametrize("lazy,bsgs", [(True, True), (False, True), (True, False)])
+def test_matvec_enc_matches_oracle(lazy, bsgs):
+    ctx, ref = make(C4_SHAPE_SMALL_N)
+    L = len(ref.q) - 1
+    baby, giant, stride, offset = 4, 2, 1, -1
+    steps = [k2 * stride for k2 in range(1, baby)] + [(offset + baby * k1) * stride for k1 in range(giant)]
+    keys, key_of_step = _keys_for(ctx, ref, L, steps, 1400)
+    relin = ctx.ksk_uniform(L, seed=1450)
+    hrelin = ref.uniform_key(L, 1450)
+    ev = ckh.Evaluator(ctx, relin, keys)
+    ev.set_config(bsgs=bsgs, lazy_relin=lazy)
+    level = L - 2
+    c0, c1 = host_uniform(ref, level, 1401), host_uniform(ref, level, 1402)
+    x = ctx.ct_upload(np.stack([c0, c1]), level, DELTA)
+    ents_h, ents_d = [], []
+    for i in range(baby * giant):
+        if i == 5:
+            ents_h.append(None)
+            ents_d.append(None)
+            continue
+        e = np.stack([host_uniform(ref, level, 1500 + 2 * i), host_uniform(ref, level, 1501 + 2 * i)])
+        ents_h.append(e)
+        ents_d.append(ctx.ct_upload(e, level, DELTA))
+    ev.reset_counter()
+    got = ev.matvec_enc(x, ents_d, baby, giant, stride, offset)
+    assert got.size() == 2 and got.level == level and got.scale == DELTA * DELTA
+
+    def mul(i, xr):
+        d = ref.mul_no_relin(ents_h[i][0], ents_h[i][1], xr[0], xr[1], level)
+        if lazy:
+            return tuple(d)
+        return ref.relinearize(d[0], d[1], d[2], level, hrelin, L)
+
+    def finish(t):
+        return ref.relinearize(t[0], t[1], t[2], level, hrelin, L) if lazy else t
+
+    present = [e is not None for e in ents_h]
+    want = oracle_bsgs(ref, c0, c1, level, present, baby, giant, stride, offset, mul, finish, bsgs, key_of_step, L)
+    g = got.download()
+    assert (g[0] == want[0]).all() and (g[1] == want[1]).all()
+    c = ev.counter()
+    n_products = sum(present)
+    assert c.ct_ct_tensor == n_products
+    if lazy and bsgs:
+        assert c.relins == giant  # one per block
+    elif not lazy:
+        assert c.relins == n_products
+    x0 = ctx.ct_upload(np.stack([host_uniform(ref, 0, 1), host_uniform(ref, 0, 2)]), 0, DELTA)
+    ents0 = [ctx.ct_upload(np.stack([host_uniform(ref, 0, 3), host_uniform(ref, 0, 4)]), 0, DELTA)] * (baby * giant)
+    with pytest.raises(ckh.NoLevelsLeft):
+        ev.matvec_enc(x0, ents0, baby, giant, stride, offset)
+    with pytest.raises(ckh.InvalidParams):  # no relinearization key
+        ckh.Evaluator(ctx, None, keys).matvec_enc(x, ents_d, baby, giant, stride, offset)
+    bad = list(ents_d)
+    bad[0] = ctx.ct_upload(ents_h[0], level, 2 * DELTA)
+    with pytest.raises(ckh.ScaleMismatch):
+        ev.matvec_enc(x, bad, baby, giant, stride, offset)

@@ -486,3 +486,79 @@ def test_uniform_generator_definition():
     assert (big < (1 << 60) - 93).all()
     # roughly uniform: mean within 1% of q/2
     assert abs(float(big.astype(np.float64).mean()) / ((1 << 60) / 2) - 1) < 0.01
+
+
+# ---------------- scalar products and exact labels (evaluator.rs:330-418,738-797) ----------------
+
+def test_poly_mul_scalar_is_the_big_integer_product():
+    p = O.Params(8, [60, 50, 40], [60], 3)
+    level = 2
+    a = O.uniform_poly(p.q, p.n, 81)
+    a[0, :3] = [0, 1, p.q[0] - 1]
+    for k in [0, 1, 3, (1 << 62) - 1, p.q[1], p.q[1] + 5, 123456789123456789]:
+        for neg in (False, True):
+            got = p.poly_mul_scalar(a.ravel(), level, k, neg)
+            for i, q in enumerate(p.q):
+                want = [(int(x) * k) % q for x in a[i]]
+                if neg:
+                    want = [(q - w) % q for w in want]
+                assert got[i].tolist() == want, (k, neg, i)
+
+
+def _ulp_neighbours(x, k):
+    import struct
+    b = struct.unpack("<Q", struct.pack("<d", x))[0]
+    return [struct.unpack("<d", struct.pack("<Q", b + k))[0], struct.unpack("<d", struct.pack("<Q", b - k))[0]]
+
+
+def test_exact_product_scale_divides_to_the_target_and_is_the_nearest_such_label():
+    p = O.Params(10, [60, 40, 40, 40, 40, 40], [60, 60], 3)
+    rng = random.Random(5)
+    delta = 2.0 ** 40
+    targets = [delta, 1.5 * delta, delta * delta / p.q[4], 3.0, 2.0 ** 70] + \
+              [rng.uniform(2.0 ** 20, 2.0 ** 60) for _ in range(200)]
+    for t in targets:
+        for level in range(1, len(p.q)):
+            q = float(p.q[level])
+            try:
+                lab = p.exact_product_scale(level, t)
+            except ValueError as e:
+                # then no candidate within 8 ulps of t * q divides back to t
+                assert str(e) == "ScaleMismatch"
+                assert all(c / q != t for k in range(9) for c in _ulp_neighbours(t * q, k))
+                continue
+            assert lab / q == t
+            # no candidate nearer to t * q (in ulps) divides back to t
+            k = next(k for k in range(9) if lab in _ulp_neighbours(t * q, k))
+            assert all(c / q != t for j in range(k) for c in _ulp_neighbours(t * q, j))
+            if lab != _ulp_neighbours(t * q, k)[0]:
+                assert _ulp_neighbours(t * q, k)[0] / q != t  # above is tried before below
+
+
+def test_exact_product_scale_errors():
+    p = O.Params(10, [60, 40, 40], [60], 3)
+    for args, kind in [((3, 2.0 ** 40), "InvalidParams"), ((0, 2.0 ** 40), "NoLevelsLeft"),
+                       ((1, 0.0), "InvalidParams"), ((1, -1.0), "InvalidParams"),
+                       ((1, float("nan")), "InvalidParams"), ((1, float("inf")), "InvalidParams"),
+                       ((1, 1e300), "InvalidParams")]:
+        with pytest.raises(ValueError, match=kind):
+            p.exact_product_scale(*args)
+
+
+def test_scalar_rounding_is_half_away_from_zero():
+    from decimal import Decimal, ROUND_HALF_UP
+    p = O.Params(10, [60, 40, 40], [60], 3)
+    lab = p.exact_product_scale(2, 2.0 ** 40)
+    src = lab / 2.0 ** 30  # plaintext scale 2^30
+    for value in [0.5, -0.5, 1.5, -2.5, 1e-3, -1.25, 3.0, 1.0 / 3.0, 2.0 ** -30 * 2.5]:
+        label, sp, k = p.mul_scalar_parts(2, src, value, 2.0 ** 40)
+        x = value * sp
+        want = float(Decimal(x).quantize(Decimal(1), rounding=ROUND_HALF_UP))
+        assert label == lab and k == want, value
+    # the reference's bound: an integer at or above 2^62 is rejected (evaluator.rs:1754-1758 shape)
+    with pytest.raises(ValueError, match="InvalidParams"):
+        p.mul_scalar_parts(2, 2.0 ** 40, 1e9, 2.0 ** 40 * 1e9)
+    with pytest.raises(ValueError, match="InvalidParams"):
+        p.mul_scalar_parts(2, 2.0 ** 40, float("nan"), 2.0 ** 40)
+    with pytest.raises(ValueError, match="InvalidParams"):  # plaintext scale below 2^20
+        p.mul_scalar_parts(2, 2.0 ** 40 * p.q[2], 1.0, 2.0 ** 40)
