@@ -83,6 +83,10 @@ class ServerStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class BsgsShape(C.Structure):
+    _fields_ = [("baby", u32), ("giant", u32), ("stride", u32), ("offset", C.c_int32)]
+
+
 OP_RELINEARIZE, OP_HOISTED_ROTATE, OP_MUL_RELIN_RESCALE = 0, 1, 2
 
 # name: (restype, argtypes)
@@ -163,6 +167,12 @@ _SIGS = {
     "ckh_exact_product_scale": (C.c_int, [VP, u32, C.c_double, C.POINTER(C.c_double)]),
     "ckh_target_plain_scale": (C.c_int, [VP, VP, C.c_double, C.POINTER(C.c_double)]),
     "ckh_mul_lazy": (C.c_int, [VP, VP, VP, C.POINTER(VP)]),
+    "ckh_pool_used_memory": (C.c_int, [VP, C.POINTER(u64)]),
+    "ckh_matvec_enc_batch": (C.c_int, [VP, u32, C.POINTER(VP), C.POINTER(C.POINTER(VP)), C.c_void_p,
+                                       C.POINTER(VP)]),
+    "ckh_matvec_enc_workspace_bytes": (C.c_int, [VP, u32, u32, u32, C.POINTER(u64)]),
+    "ckh_set_pool_release_threshold": (C.c_int, [VP, u64]),
+    "ckh_trim_pool": (C.c_int, [VP, u64]),
     "ckh_relin_lazy": (C.c_int, [VP, VP, C.POINTER(VP)]),
     "ckh_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(VP)]),
     "ckh_server_destroy": (None, [VP]),
@@ -263,6 +273,17 @@ class Context:
         arr = (VP * len(polys))(*[p.h for p in polys])
         _check(lib().ckh_autotune_ks(self.h, len(polys), arr, key.h, buf, len(buf)))
         return json.loads(buf.value.decode())
+
+    def pool_used_memory(self):
+        v = u64(0)
+        _check(lib().ckh_pool_used_memory(self.h, C.byref(v)))
+        return v.value
+
+    def set_pool_release_threshold(self, nbytes):
+        _check(lib().ckh_set_pool_release_threshold(self.h, nbytes))
+
+    def trim_pool(self, keep_bytes=0):
+        _check(lib().ckh_trim_pool(self.h, keep_bytes))
 
     def set_tuning(self, key, value):
         _check(lib().ckh_set_tuning(self.h, key.encode(), value))
@@ -606,6 +627,22 @@ class Evaluator:
         """BSGS product with encrypted diagonals (Ct entries or None)."""
         arr = (VP * len(entries))(*[e.h if e is not None else None for e in entries])
         return self._op(lib().ckh_matvec_enc, x.h, arr, baby, giant, stride, offset)
+
+    def matvec_enc_batch(self, xs, entries, shapes):
+        """jobs (x, entries, (baby, giant, stride, offset)); results in job order."""
+        n = len(xs)
+        xa = (VP * max(n, 1))(*[x.h for x in xs])
+        rows = [(VP * len(e))(*[c.h if c is not None else None for c in e]) for e in entries]
+        ea = (C.POINTER(VP) * max(n, 1))(*[C.cast(r, C.POINTER(VP)) for r in rows])
+        sa = (BsgsShape * max(n, 1))(*[BsgsShape(*sh) for sh in shapes])
+        outs = (VP * max(n, 1))()
+        _check(lib().ckh_matvec_enc_batch(self.h, n, xa, ea, C.cast(sa, C.c_void_p), outs))
+        return [Ct(self.ctx, VP(outs[i])) for i in range(n)]
+
+    def matvec_enc_workspace_bytes(self, level, baby, giant):
+        v = u64(0)
+        _check(lib().ckh_matvec_enc_workspace_bytes(self.h, level, baby, giant, C.byref(v)))
+        return v.value
 
     def relinearize_batch(self, cts):
         n = len(cts)

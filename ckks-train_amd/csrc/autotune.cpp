@@ -8,6 +8,10 @@
 namespace ckh {
 
 void Context::set_tuning(const std::string& key, int64_t value) {
+    if (key == "fail_alloc_after") {  // a test hook (negative = off); captured plans stay valid
+        fail_alloc_after = value;
+        return;
+    }
     if (value < 0) fail(CKH_ERR_INVALID_PARAMS, "tuning values are non-negative");
     if (key == "ntt_chunk_fwd")
         ntt_chunk_fwd = (u32)value;
@@ -42,6 +46,7 @@ int64_t Context::get_tuning(const std::string& key) const {
     if (key == "conv_tg") return conv_tg;
     if (key == "pool") return pool_mode;
     if (key == "service_streams") return service_streams;
+    if (key == "fail_alloc_after") return fail_alloc_after.load();
     fail(CKH_ERR_INVALID_PARAMS, "unknown tuning key " + key);
 }
 

@@ -208,6 +208,21 @@ ckh_status ckh_graph_stats(ckh_context* ctx, uint64_t* captures, uint64_t* launc
     });
 }
 
+ckh_status ckh_pool_used_memory(ckh_context* ctx, uint64_t* bytes) {
+    return guarded([&] {
+        need(bytes, "bytes");
+        *bytes = ctx_of(ctx).pool_used_memory();
+    });
+}
+
+ckh_status ckh_set_pool_release_threshold(ckh_context* ctx, uint64_t bytes) {
+    return guarded([&] { ctx_of(ctx).set_pool_release_threshold(bytes); });
+}
+
+ckh_status ckh_trim_pool(ckh_context* ctx, uint64_t keep_bytes) {
+    return guarded([&] { ctx_of(ctx).trim_pool(keep_bytes); });
+}
+
 ckh_status ckh_synchronize(ckh_context* ctx) {
     return guarded([&] {
         Context& c = ctx_of(ctx);
@@ -651,6 +666,40 @@ ckh_status ckh_matvec_enc(ckh_evaluator* ev, const ckh_ct* x, const ckh_ct* cons
         std::vector<const Ct*> v;
         for (uint64_t i = 0; i < (uint64_t)baby * giant; ++i) v.push_back(entries[i] ? &ct_of(entries[i]) : nullptr);
         *out = wrap(ev_of(ev).matvec_enc(ct_of(x), v, baby, giant, stride, offset));
+    });
+}
+
+ckh_status ckh_matvec_enc_batch(ckh_evaluator* ev, uint32_t n, const ckh_ct* const* xs,
+                                const ckh_ct* const* const* entries, const ckh_bsgs_shape* shapes, ckh_ct** outs) {
+    return guarded([&] {
+        need(outs, "outs");
+        if (n) {
+            need(xs, "xs");
+            need(entries, "entries");
+            need(shapes, "shapes");
+        }
+        std::vector<const Ct*> x;
+        std::vector<std::vector<const Ct*>> e(n);
+        std::vector<BsgsShape> sh;
+        for (uint32_t i = 0; i < n; ++i) {
+            x.push_back(&ct_of(xs[i]));
+            sh.push_back({shapes[i].baby, shapes[i].giant, shapes[i].stride, shapes[i].offset});
+            need(entries[i], "entries");
+            for (uint64_t j = 0; j < (uint64_t)shapes[i].baby * shapes[i].giant; ++j)
+                e[i].push_back(entries[i][j] ? &ct_of(entries[i][j]) : nullptr);
+        }
+        auto r = ev_of(ev).matvec_enc_batch(x, e, sh);
+        for (uint32_t i = 0; i < n; ++i) outs[i] = wrap(r[i]);
+    });
+}
+
+ckh_status ckh_matvec_enc_workspace_bytes(ckh_evaluator* ev, uint32_t level, uint32_t baby, uint32_t giant,
+                                          uint64_t* bytes) {
+    return guarded([&] {
+        need(bytes, "bytes");
+        if (baby == 0 || giant == 0) fail(CKH_ERR_INVALID_PARAMS, "baby and giant must be positive");
+        if (level >= ev_of(ev).ctx.nq) fail(CKH_ERR_INVALID_PARAMS, "level exceeds max_level");
+        *bytes = ev_of(ev).matvec_enc_workspace_bytes(level, baby, giant);
     });
 }
 

@@ -358,7 +358,54 @@ Evaluator::Evaluator(Context& c, KskPtr r, std::vector<KskPtr> g) : ctx(c), reli
     for (auto& k : g) galois[k->galois] = k;
 }
 
+namespace {
+// Counts of a composite operation in progress on this thread (innermost last): its nested
+// operations count here, and the total reaches the evaluator only when it succeeds
+// (crates/ckks-train-cuda/src/matvec.rs `counted`: a device error part-way leaves the counter
+// unchanged).
+struct PendingCounts {
+    const Evaluator* ev;
+    ckh_op_counter c;
+};
+thread_local std::vector<PendingCounts> tl_pending;
+
+void add_into(ckh_op_counter& a, const ckh_op_counter& b) {
+    a.rotations += b.rotations;
+    a.relins += b.relins;
+    a.ct_ct_tensor += b.ct_ct_tensor;
+    a.ct_pt_mults += b.ct_pt_mults;
+    a.rescales += b.rescales;
+    a.sse_switches += b.sse_switches;
+    a.bootstraps += b.bootstraps;
+}
+}  // namespace
+
+template <class F>
+auto Evaluator::counted(F&& body) -> decltype(body()) {
+    tl_pending.push_back({this, ckh_op_counter{}});
+    try {
+        auto r = body();
+        const ckh_op_counter c = tl_pending.back().c;
+        tl_pending.pop_back();
+        for (auto it = tl_pending.rbegin(); it != tl_pending.rend(); ++it)
+            if (it->ev == this) {
+                add_into(it->c, c);
+                return r;
+            }
+        add_counts(c);
+        return r;
+    } catch (...) {
+        tl_pending.pop_back();
+        throw;
+    }
+}
+
 void Evaluator::count(u64 ckh_op_counter::*field, u64 v) {
+    for (auto it = tl_pending.rbegin(); it != tl_pending.rend(); ++it)
+        if (it->ev == this) {
+            it->c.*field += v;
+            return;
+        }
     std::lock_guard<std::mutex> lk(mu_);
     cnt_.*field += v;
 }
@@ -557,7 +604,7 @@ std::vector<CtPtr> Evaluator::relinearize_batch(const std::vector<const Ct*>& ct
 CtPtr Evaluator::mul(const Ct& a, const Ct& b) {
     check_mul_operands(a, b);
     relin_key_checked(a.level());
-    return relinearize(*mul_no_relin(a, b));
+    return counted([&] { return relinearize(*mul_no_relin(a, b)); });
 }
 
 // Evaluator::rescale (evaluator.rs:498-509): scale / q_l as f64.
@@ -649,9 +696,11 @@ std::vector<CtPtr> Evaluator::hoisted_rotate(const Ct& ct, const std::vector<int
     }
     if (!config.hoist) {
         // naive form: every step an individual rotation (same words, same count)
-        for (size_t i = 0; i < steps.size(); ++i)
-            res[i] = keys[i].first ? apply_galois(ct, keys[i].first) : make_ct(ct.comps, ct.scale);
-        return res;
+        return counted([&] {
+            for (size_t i = 0; i < steps.size(); ++i)
+                res[i] = keys[i].first ? apply_galois(ct, keys[i].first) : make_ct(ct.comps, ct.scale);
+            return res;
+        });
     }
     const u64 L = level + 1, n = ctx.n, k = ctx.np;
     const u32 nd = ctx.ndigits(level);
@@ -820,16 +869,19 @@ CtPtr Evaluator::matvec_plain(const Ct& x, const std::vector<const Poly*>& entri
         present[i] = true;
     }
     check_bsgs_shape(x, baby, giant, stride, offset, present);
-    return bsgs(
-        x, baby, giant, stride, offset, present,
-        [&](size_t i, const Ct& xr) { return mul_plain(xr, *entries[i], pt_scale); },
-        [](const CtPtr& c) { return c; });
+    return counted([&] {
+        return bsgs(
+            x, baby, giant, stride, offset, present,
+            [&](size_t i, const Ct& xr) { return mul_plain(xr, *entries[i], pt_scale); },
+            [](const CtPtr& c) { return c; });
+    });
 }
 
 // Evaluator::matvec_enc (linalg.rs:638-665): encrypted 2-component diagonals; lazy_relin: per
 // block the tensor products summed at degree 2 and relinearized once, else one mul per product.
-CtPtr Evaluator::matvec_enc(const Ct& x, const std::vector<const Ct*>& entries, u32 baby, u32 giant, u32 stride,
-                            int32_t offset) {
+// The checks of matvec_enc, before any work; returns which entries are present.
+std::vector<bool> Evaluator::check_matvec_enc(const Ct& x, const std::vector<const Ct*>& entries, u32 baby, u32 giant,
+                                              u32 stride, int32_t offset) const {
     check_basis(x);
     check_galois_input(x);
     std::vector<bool> present(entries.size());
@@ -852,6 +904,11 @@ CtPtr Evaluator::matvec_enc(const Ct& x, const std::vector<const Ct*>& entries, 
     check_bsgs_shape(x, baby, giant, stride, offset, present);
     if (x.level() == 0) fail(CKH_ERR_NO_LEVELS_LEFT, "no levels left");
     relin_key_checked(x.level());
+    return present;
+}
+
+CtPtr Evaluator::matvec_enc_run(const Ct& x, const std::vector<const Ct*>& entries, u32 baby, u32 giant, u32 stride,
+                                int32_t offset, const std::vector<bool>& present) {
     if (config.lazy_relin)
         return bsgs(
             x, baby, giant, stride, offset, present,
@@ -860,6 +917,61 @@ CtPtr Evaluator::matvec_enc(const Ct& x, const std::vector<const Ct*>& entries, 
     return bsgs(
         x, baby, giant, stride, offset, present, [&](size_t i, const Ct& xr) { return mul(*entries[i], xr); },
         [](const CtPtr& c) { return c; });
+}
+
+// Evaluator::matvec_enc (linalg.rs:638-665): encrypted 2-component diagonals; lazy_relin: per
+// block the tensor products summed at degree 2 and relinearized once, else one mul per product.
+CtPtr Evaluator::matvec_enc(const Ct& x, const std::vector<const Ct*>& entries, u32 baby, u32 giant, u32 stride,
+                            int32_t offset) {
+    const std::vector<bool> present = check_matvec_enc(x, entries, baby, giant, stride, offset);
+    return counted([&] { return matvec_enc_run(x, entries, baby, giant, stride, offset, present); });
+}
+
+// Device bytes one matvec_enc of this engine holds at its peak, an upper estimate: the shared
+// decomposition, the hoisted steps' scratch and rotated copies, a degree-2 product and its sum,
+// one key switch's scratch with its output, and the blocks awaiting their giant rotation.
+u64 Evaluator::matvec_enc_workspace_bytes(u32 level, u32 baby, u32 giant) const {
+    const u64 q = level + 1, k = ctx.np, nd = ctx.ndigits(level), qp = q + k;
+    const u64 limbs = nd * qp + q + (baby - 1) * (2 * qp + 2 * k + 5 * q) + 6 * q +
+                      (q + nd * qp + 2 * qp + 2 * k + 2 * q) + 2 * q + 4 * q * giant;
+    return limbs * ctx.n * 8;
+}
+
+// CudaEvaluator::matvec_enc_batch (crates/ckks-train-cuda/src/evaluator.rs:437-519): every job
+// checked first (in order), the largest job's working set compared with the device memory
+// available now (Device error, nothing done or counted), then the products one after the other
+// on the calling thread's stream (one product already fills the device); counts merged only
+// when every job succeeded.
+std::vector<CtPtr> Evaluator::matvec_enc_batch(const std::vector<const Ct*>& xs,
+                                               const std::vector<std::vector<const Ct*>>& entries,
+                                               const std::vector<BsgsShape>& shapes) {
+    if (xs.size() != entries.size() || xs.size() != shapes.size())
+        fail(CKH_ERR_INVALID_PARAMS, "matvec_enc_batch: diagonal sets and vectors differ in number");
+    std::vector<std::vector<bool>> present;
+    u64 need = 0;
+    for (size_t i = 0; i < xs.size(); ++i) {
+        const BsgsShape& sh = shapes[i];
+        present.push_back(check_matvec_enc(*xs[i], entries[i], sh.baby, sh.giant, sh.stride, sh.offset));
+        need = std::max(need, matvec_enc_workspace_bytes(xs[i]->level(), sh.baby, sh.giant));
+    }
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    const u64 avail = (u64)free_b + ctx.pool_unused_reserve();
+    if (need > total_b || need > avail) {
+        std::ostringstream os;
+        os << "matvec_enc_batch: a product needs " << need / (1 << 20) << " MiB of working memory, more than the "
+           << (need > total_b ? "device's total " : "available ") << (need > total_b ? total_b : avail) / (1 << 20)
+           << " MiB";
+        fail(CKH_ERR_DEVICE, os.str());
+    }
+    return counted([&] {
+        std::vector<CtPtr> out;
+        for (size_t i = 0; i < xs.size(); ++i) {
+            const BsgsShape& sh = shapes[i];
+            out.push_back(matvec_enc_run(*xs[i], entries[i], sh.baby, sh.giant, sh.stride, sh.offset, present[i]));
+        }
+        return out;
+    });
 }
 
 // HeOps::mul_lazy / relin_lazy (ops.rs:345-363).

@@ -43,6 +43,13 @@ struct EvalConfig {
     bool hoist = true, bsgs = true, lazy_relin = true;
 };
 
+// A BSGS product's shape: entries[k1 * baby + k2] hold rot(diag_{offset + baby k1 + k2},
+// -(offset + baby k1) stride).
+struct BsgsShape {
+    u32 baby, giant, stride;
+    int32_t offset;
+};
+
 class Evaluator {
    public:
     Evaluator(Context& c, KskPtr relin, std::vector<KskPtr> galois);
@@ -77,6 +84,10 @@ class Evaluator {
                        u32 stride, int32_t offset);
     CtPtr matvec_enc(const Ct& x, const std::vector<const Ct*>& entries, u32 baby, u32 giant, u32 stride,
                      int32_t offset);
+    std::vector<CtPtr> matvec_enc_batch(const std::vector<const Ct*>& xs,
+                                        const std::vector<std::vector<const Ct*>>& entries,
+                                        const std::vector<BsgsShape>& shapes);
+    u64 matvec_enc_workspace_bytes(u32 level, u32 baby, u32 giant) const;
     CtPtr mul_scalar(const Ct& ct, double value, double target_scale);
     CtPtr mul_plain_to_scale(const Ct& a, const Poly& pt, double pt_scale, double target_scale);
     double exact_product_scale(u32 level, double target_scale) const;
@@ -88,12 +99,19 @@ class Evaluator {
     std::mutex mu_;
     ckh_op_counter cnt_{};
     void count(u64 ckh_op_counter::*field, u64 v = 1);
+    // Runs a composite operation whose counts are merged only if it succeeds.
+    template <class F>
+    auto counted(F&& body) -> decltype(body());
     void check_basis(const Ct& ct) const;
     void check_mul_operands(const Ct& a, const Ct& b) const;
     const Ksk& relin_key_checked(u32 level) const;
     const Ksk& galois_key(u64 elt, u32 level) const;
     u64 rotation_elt(int32_t steps) const;  // 0 for a multiple of the slot count
     CtPtr apply_galois(const Ct& ct, u64 elt);
+    std::vector<bool> check_matvec_enc(const Ct& x, const std::vector<const Ct*>& entries, u32 baby, u32 giant,
+                                       u32 stride, int32_t offset) const;
+    CtPtr matvec_enc_run(const Ct& x, const std::vector<const Ct*>& entries, u32 baby, u32 giant, u32 stride,
+                         int32_t offset, const std::vector<bool>& present);
     // The BSGS shape checks shared by both products, before any work.
     void check_bsgs_shape(const Ct& x, u32 baby, u32 giant, u32 stride, int32_t offset,
                           const std::vector<bool>& present) const;

@@ -130,9 +130,45 @@ void Context::set_service_stream(u32 k) {
     tl_stream[this] = n_user_streams_ + k % std::min(std::max(service_streams, 1u), SERVICE_STREAMS);
 }
 
+u64 Context::pool_used_memory() {
+    hipMemPool_t mp = ctx_pool_;
+    if (pool_mode != 1 || !mp) hip_check(hipDeviceGetDefaultMemPool(&mp, device), "hipDeviceGetDefaultMemPool");
+    uint64_t v = 0;
+    hip_check(hipMemPoolGetAttribute(mp, hipMemPoolAttrUsedMemCurrent, &v), "hipMemPoolGetAttribute");
+    return v;
+}
+
+u64 Context::pool_unused_reserve() {
+    hipMemPool_t mp = ctx_pool_;
+    if (pool_mode != 1 || !mp) hip_check(hipDeviceGetDefaultMemPool(&mp, device), "hipDeviceGetDefaultMemPool");
+    uint64_t used = 0, reserved = 0;
+    hip_check(hipMemPoolGetAttribute(mp, hipMemPoolAttrUsedMemCurrent, &used), "hipMemPoolGetAttribute");
+    hip_check(hipMemPoolGetAttribute(mp, hipMemPoolAttrReservedMemCurrent, &reserved), "hipMemPoolGetAttribute");
+    return reserved > used ? reserved - used : 0;
+}
+
+void Context::set_pool_release_threshold(u64 bytes) {
+    uint64_t v = bytes;
+    if (ctx_pool_) hip_check(hipMemPoolSetAttribute(ctx_pool_, hipMemPoolAttrReleaseThreshold, &v), "pool threshold");
+    hipMemPool_t dp;
+    hip_check(hipDeviceGetDefaultMemPool(&dp, device), "hipDeviceGetDefaultMemPool");
+    hip_check(hipMemPoolSetAttribute(dp, hipMemPoolAttrReleaseThreshold, &v), "pool threshold");
+}
+
+void Context::trim_pool(u64 keep_bytes) {
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    for (auto st : streams_) hip_check(hipStreamSynchronize(st), "synchronize");
+    if (ctx_pool_) hip_check(hipMemPoolTrimTo(ctx_pool_, keep_bytes), "hipMemPoolTrimTo");
+    hipMemPool_t dp;
+    hip_check(hipDeviceGetDefaultMemPool(&dp, device), "hipDeviceGetDefaultMemPool");
+    hip_check(hipMemPoolTrimTo(dp, keep_bytes), "hipMemPoolTrimTo");
+}
+
 BufPtr Context::alloc(size_t words, hipStream_t s) {
     void* p = nullptr;
     size_t bytes = std::max<size_t>(words, 1) * sizeof(u64);
+    if (fail_alloc_after.load() >= 0 && fail_alloc_after.fetch_sub(1) == 0)
+        fail(CKH_ERR_DEVICE, "out of device memory (injected allocation failure)");
     hipMemPool_t mp = pool_mode == 1 ? ctx_pool_ : nullptr;
     hipError_t e = mp ? hipMallocFromPoolAsync(&p, bytes, mp, s) : hipMallocAsync(&p, bytes, s);
     if (e != hipSuccess) {

@@ -182,6 +182,16 @@ class Context : public std::enable_shared_from_this<Context> {
     BufPtr alloc(size_t words) { return alloc(words, stream()); }
     void free_buffer(Buffer* b);
     u32 pool_mode = 1;
+    // The pool's bytes in use, its release threshold (bytes kept reserved after a
+    // synchronization; default: all) and a trim to keep_bytes after every stream has finished
+    // (crates/ckks-train-cuda/src/device.rs:478-527).
+    u64 pool_used_memory();
+    u64 pool_unused_reserve();  // reserved by the pool but not in use
+    void set_pool_release_threshold(u64 bytes);
+    void trim_pool(u64 keep_bytes);
+    // Test hook (tuning key "fail_alloc_after"): the n-th allocation from now fails with a device
+    // error, as an out-of-memory would; negative = off.
+    std::atomic<int64_t> fail_alloc_after{-1};
     std::atomic<u64> live_bytes{0};
 
     // Digit ranges of a level-l polynomial (keyswitch.rs:68-74).

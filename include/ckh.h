@@ -100,8 +100,18 @@ ckh_status ckh_autotune_ks(ckh_context *ctx, uint32_t n, const ckh_poly *const *
  * workgroup); "fuse_ip" (1 = row pass fused into the inner product); "pool" (1 = the context's
  * own memory pool, where a block freed on one stream serves another stream only once the free
  * has completed on the device, the default; 0 = the device's default pool); "service_streams"
- * (1..4 streams the serving loop spreads its batches over, default 4). */
+ * (1..4 streams the serving loop spreads its batches over, default 4); test hook "fail_alloc_after"
+ * (the n-th device allocation from now fails with CKH_ERR_DEVICE, as an out-of-memory would;
+ * negative = off). */
 ckh_status ckh_set_tuning(ckh_context *ctx, const char *key, int64_t value);
+/* CudaDevice::pool_used_memory / set_pool_release_threshold / trim_pool
+ * (crates/ckks-train-cuda/src/device.rs:478-527) for the context's memory pool (and the
+ * device's default pool): bytes in use; bytes kept reserved after a synchronization (default:
+ * all, so temporaries are reused); and, after every stream has finished, the reserve beyond
+ * keep_bytes returned to the driver (call after one-off peaks on a shared GPU). */
+ckh_status ckh_pool_used_memory(ckh_context *ctx, uint64_t *bytes);
+ckh_status ckh_set_pool_release_threshold(ckh_context *ctx, uint64_t bytes);
+ckh_status ckh_trim_pool(ckh_context *ctx, uint64_t keep_bytes);
 /* Reads one tuning value (the keys of ckh_set_tuning). */
 ckh_status ckh_get_tuning(ckh_context *ctx, const char *key, int64_t *value);
 /* Captured hipGraphs so far and their replays (no counterpart in the reference). */
@@ -267,6 +277,22 @@ ckh_status ckh_matvec_plain(ckh_evaluator *ev, const ckh_ct *x, const ckh_poly *
  * diagonals; lazy_relin: per block the tensor products are summed and relinearized once. */
 ckh_status ckh_matvec_enc(ckh_evaluator *ev, const ckh_ct *x, const ckh_ct *const *entries, uint32_t baby,
                           uint32_t giant, uint32_t stride, int32_t offset, ckh_ct **out);
+/* CudaEvaluator::matvec_enc_batch (crates/ckks-train-cuda/src/evaluator.rs:412-519): n
+ * independent products (job i: xs[i], entries[i][0..baby * giant), shapes[i]); every job is
+ * checked first, in order; a product whose working set exceeds the device memory available now
+ * (free plus the pool's unused reserve) fails with CKH_ERR_DEVICE before any work; the products
+ * then run one after the other on the calling thread's stream, and their counts are merged
+ * only when every job succeeded. */
+typedef struct ckh_bsgs_shape {
+    uint32_t baby, giant, stride;
+    int32_t offset;
+} ckh_bsgs_shape;
+ckh_status ckh_matvec_enc_batch(ckh_evaluator *ev, uint32_t n, const ckh_ct *const *xs,
+                                const ckh_ct *const *const *entries, const ckh_bsgs_shape *shapes,
+                                ckh_ct **outs);
+/* CudaEvaluator::matvec_enc_workspace_bytes (evaluator.rs:403-410) for this engine. */
+ckh_status ckh_matvec_enc_workspace_bytes(ckh_evaluator *ev, uint32_t level, uint32_t baby, uint32_t giant,
+                                          uint64_t *bytes);
 /* n relinearizations in batched launches (bit-identical to n ckh_relinearize calls). */
 ckh_status ckh_relinearize_batch(ckh_evaluator *ev, uint32_t n, const ckh_ct *const *in,
                                  ckh_ct **out);

@@ -247,3 +247,83 @@ def test_matvec_enc_matches_oracle(lazy, bsgs):
     bad[0] = ctx.ct_upload(ents_h[0], level, 2 * DELTA)
     with pytest.raises(ckh.ScaleMismatch):
         ev.matvec_enc(x, bad, baby, giant, stride, offset)
+
+
+def _enc_job(ctx, ref, level, baby, giant, seed, missing=()):
+    c0, c1 = host_uniform(ref, level, seed), host_uniform(ref, level, seed + 1)
+    x = ctx.ct_upload(np.stack([c0, c1]), level, DELTA)
+    ents = []
+    for i in range(baby * giant):
+        if i in missing:
+            ents.append(None)
+            continue
+        e = np.stack([host_uniform(ref, level, seed + 10 + 2 * i), host_uniform(ref, level, seed + 11 + 2 * i)])
+        ents.append(ctx.ct_upload(e, level, DELTA))
+    return x, ents
+
+
+def test_matvec_enc_batch_equals_individual_and_counts_only_on_success():
+    """CudaEvaluator::matvec_enc_batch (crates/ckks-train-cuda/src/evaluator.rs:412-519) and the
+    success-only counts of composite operations (src/matvec.rs `counted`): an allocation failure
+    part-way, forced from the host, is a Device error that leaves the counter unchanged."""
+    ctx, ref = make(C4_SHAPE_SMALL_N)
+    L = len(ref.q) - 1
+    shapes = [(4, 2, 1, 0), (2, 2, 2, -1), (4, 2, 1, -3)]
+    steps = set()
+    for baby, giant, stride, offset in shapes:
+        steps |= {k2 * stride for k2 in range(1, baby)} | {(offset + baby * k1) * stride for k1 in range(giant)}
+    keys, _ = _keys_for(ctx, ref, L, sorted(steps), 1600)
+    ev = ckh.Evaluator(ctx, ctx.ksk_uniform(L, seed=1650), keys)
+    jobs = [_enc_job(ctx, ref, lv, sh[0], sh[1], 1700 + 100 * j, missing=(1,) if j == 1 else ())
+            for j, (lv, sh) in enumerate(zip((L - 1, 3, L), shapes))]
+    single, counts = [], []
+    for (x, ents), sh in zip(jobs, shapes):
+        ev.reset_counter()
+        single.append(ev.matvec_enc(x, ents, *sh).download())
+        counts.append(ev.counter())
+    ev.reset_counter()
+    batch = ev.matvec_enc_batch([j[0] for j in jobs], [j[1] for j in jobs], shapes)
+    for b, s_ in zip(batch, single):
+        assert (b.download() == s_).all()
+    c = ev.counter()
+    for f in ("rotations", "relins", "ct_ct_tensor"):
+        assert getattr(c, f) == sum(getattr(k, f) for k in counts), f
+    assert ev.matvec_enc_workspace_bytes(L, 4, 2) > 8 * ref.n * (L + 1) * 2
+    # an allocation failure part-way: Device error, nothing counted
+    for hook_at in (3, 40):
+        ev.reset_counter()
+        ctx.set_tuning("fail_alloc_after", hook_at)
+        with pytest.raises(ckh.Device):
+            ev.matvec_enc_batch([j[0] for j in jobs], [j[1] for j in jobs], shapes)
+        ctx.set_tuning("fail_alloc_after", -1)
+        c = ev.counter()
+        assert (c.rotations, c.relins, c.ct_ct_tensor) == (0, 0, 0), hook_at
+    ev.reset_counter()
+    ctx.set_tuning("fail_alloc_after", 25)
+    with pytest.raises(ckh.Device):
+        ev.matvec_enc(jobs[0][0], jobs[0][1], *shapes[0])
+    ctx.set_tuning("fail_alloc_after", -1)
+    assert ev.counter().rotations == 0 and ev.counter().relins == 0
+    # and everything works again afterwards
+    assert (ev.matvec_enc(jobs[0][0], jobs[0][1], *shapes[0]).download() == single[0]).all()
+    with pytest.raises(ckh.InvalidParams):
+        ev.matvec_enc_batch([jobs[0][0]], [], [shapes[0]])
+
+
+def test_pool_accounting_and_trim():
+    """CudaDevice::pool_used_memory / set_pool_release_threshold / trim_pool (device.rs:478-527)."""
+    ctx, ref = make(SMALL)
+    L = len(ref.q) - 1
+    base = ctx.pool_used_memory()
+    polys = [ctx.uniform(L, seed=i) for i in range(8)]
+    ctx.synchronize()
+    used = ctx.pool_used_memory()
+    assert used >= base + 8 * (L + 1) * ref.n * 8
+    del polys
+    ctx.synchronize()
+    assert ctx.pool_used_memory() < used
+    ctx.set_pool_release_threshold(256 << 20)
+    ctx.trim_pool(0)
+    ctx.set_pool_release_threshold((1 << 64) - 1)
+    p = ctx.uniform(L, seed=99)  # allocation after a trim works
+    assert p.download().shape == (L + 1, ref.n)
