@@ -168,6 +168,9 @@ _SIGS = {
     "ckh_target_plain_scale": (C.c_int, [VP, VP, C.c_double, C.POINTER(C.c_double)]),
     "ckh_mul_lazy": (C.c_int, [VP, VP, VP, C.POINTER(VP)]),
     "ckh_pool_used_memory": (C.c_int, [VP, C.POINTER(u64)]),
+    "ckh_raise_from_level0": (C.c_int, [VP, VP, u32, C.POINTER(VP)]),
+    "ckh_mod_raise": (C.c_int, [VP, VP, VP, VP, C.POINTER(VP)]),
+    "ckh_mod_raise_main_key": (C.c_int, [VP, VP, C.POINTER(VP)]),
     "ckh_matvec_enc_batch": (C.c_int, [VP, u32, C.POINTER(VP), C.POINTER(C.POINTER(VP)), C.c_void_p,
                                        C.POINTER(VP)]),
     "ckh_matvec_enc_workspace_bytes": (C.c_int, [VP, u32, u32, u32, C.POINTER(u64)]),
@@ -437,6 +440,11 @@ class Poly:
         _check(lib().ckh_mod_down(self.ctx.h, self.h, C.byref(h)))
         return Poly(self.ctx, h)
 
+    def raise_from_level0(self, top):
+        h = VP()
+        _check(lib().ckh_raise_from_level0(self.ctx.h, self.h, top, C.byref(h)))
+        return Poly(self.ctx, h)
+
 
 class Ksk:
     def __init__(self, ctx, h):
@@ -559,6 +567,12 @@ class Evaluator:
 
     def mul_plain_to_scale(self, a, pt, pt_scale, target_scale):
         return self._op(lib().ckh_mul_plain_to_scale, a.h, pt.h, pt_scale, target_scale)
+
+    def mod_raise(self, ct, to_sparse, to_dense):
+        return self._op(lib().ckh_mod_raise, ct.h, to_sparse.h, to_dense.h)
+
+    def mod_raise_main_key(self, ct):
+        return self._op(lib().ckh_mod_raise_main_key, ct.h)
 
     def mul_lazy(self, a, b):
         return self._op(lib().ckh_mul_lazy, a.h, b.h)

@@ -327,6 +327,13 @@ ckh_status ckh_rescale_poly(ckh_context* ctx, const ckh_poly* in, ckh_poly** out
     });
 }
 
+ckh_status ckh_raise_from_level0(ckh_context* ctx, const ckh_poly* in, uint32_t top, ckh_poly** out) {
+    return guarded([&] {
+        need(out, "out");
+        *out = wrap(poly_raise(ctx_of(ctx), poly_of(in), top));
+    });
+}
+
 ckh_status ckh_mod_down(ckh_context* ctx, const ckh_poly* qp, ckh_poly** out) {
     return guarded([&] {
         need(out, "out");
@@ -607,6 +614,13 @@ ckh_status ckh_target_plain_scale(ckh_evaluator* ev, const ckh_ct* ct, double ta
         need(out, "out");
         *out = ev_of(ev).target_plain_scale(ct_of(ct), target_scale);
     });
+}
+ckh_status ckh_mod_raise(ckh_evaluator* ev, const ckh_ct* ct, const ckh_ksk* to_sparse, const ckh_ksk* to_dense,
+                         ckh_ct** out) {
+    CT_OP(ev_of(ev).mod_raise(ct_of(ct), key_of(to_sparse), key_of(to_dense)));
+}
+ckh_status ckh_mod_raise_main_key(ckh_evaluator* ev, const ckh_ct* ct, ckh_ct** out) {
+    CT_OP(ev_of(ev).mod_raise_main_key(ct_of(ct)));
 }
 ckh_status ckh_mul_lazy(ckh_evaluator* ev, const ckh_ct* a, const ckh_ct* b, ckh_ct** out) {
     CT_OP(ev_of(ev).mul_lazy(ct_of(a), ct_of(b)));

@@ -147,6 +147,17 @@ PolyPtr poly_scalar_mul(Context& c, const Poly& a, u64 k, bool neg) {
     return out;
 }
 
+// RnsPoly::raise_from_level0 (crates/ckks-train-core/src/poly.rs:298-345).
+PolyPtr poly_raise(Context& c, const Poly& a, u32 top) {
+    if (a.level != 0 || a.p_limbs) fail(CKH_ERR_INVALID_PARAMS, "raise_from_level0 needs a level-0 Q polynomial");
+    if (top >= c.nq) fail(CKH_ERR_INVALID_PARAMS, "level out of range");
+    PolyPtr out = poly_alloc(c, top, 0, a.ntt);
+    BufPtr x = c.alloc(c.n);
+    c.raise_level0({{a.data(), top, out->data(), x->ptr, a.ntt}}, c.stream());
+    hip_check(hipGetLastError(), "raise");
+    return out;
+}
+
 PolyPtr poly_pointwise(Context& c, PwOp op, const Poly& a, const Poly* b) {
     if (b && (b->level != a.level || b->p_limbs != a.p_limbs || b->ntt != a.ntt))
         fail(CKH_ERR_LEVEL_MISMATCH, "operands at different levels or forms");
@@ -972,6 +983,58 @@ std::vector<CtPtr> Evaluator::matvec_enc_batch(const std::vector<const Ct*>& xs,
         }
         return out;
     });
+}
+
+// check_mod_raise_input (crates/ckks-train-ckks/src/bootstrap/modraise.rs:94-106).
+void Evaluator::check_mod_raise_input(const Ct& ct) const {
+    check_basis(ct);
+    if (ct.comps.size() != 2) fail(CKH_ERR_INVALID_PARAMS, "mod_raise needs a 2-component ciphertext");
+    if (ct.level() != 0) fail(CKH_ERR_LEVEL_MISMATCH, "mod_raise needs a level-0 ciphertext");
+}
+
+// Both components raised to max_level, as one program (modraise.rs:131-139).
+CtPtr Evaluator::raise_ct(const Ct& ct) {
+    const u32 top = ctx.nq - 1;
+    const u64 L = top + 1;
+    BufPtr out = ctx.alloc(2 * L * ctx.n);
+    BufPtr x = ctx.alloc(2 * ctx.n);
+    std::vector<RaiseTask> tasks;
+    for (u32 i = 0; i < 2; ++i)
+        tasks.push_back({ct.comps[i]->data(), top, out->ptr + i * L * ctx.n, x->ptr + i * ctx.n, true});
+    ctx.raise_level0(tasks, ctx.stream());
+    hip_check(hipGetLastError(), "raise");
+    return make_ct({view(out, 0, top, 0, true), view(out, L * ctx.n, top, 0, true)}, ct.scale);
+}
+
+// Evaluator::mod_raise_main_key (modraise.rs:77-80).
+CtPtr Evaluator::mod_raise_main_key(const Ct& ct) {
+    check_mod_raise_input(ct);
+    return raise_ct(ct);
+}
+
+// Evaluator::mod_raise (modraise.rs:53-69): c_1 switched to the ephemeral key at level 0
+// ((c_0 + kb, ka)), both components raised to max_level, c_1 switched back under to_dense;
+// sse_switches += 2 on success.
+CtPtr Evaluator::mod_raise(const Ct& ct, const Ksk& to_sparse, const Ksk& to_dense) {
+    check_mod_raise_input(ct);
+    // check_sse_key (modraise.rs:110-128): to_sparse one part at level 0, to_dense dnum parts at
+    // max_level, both over this context
+    if (to_sparse.ctx != &ctx || to_sparse.max_level != 0 || to_sparse.parts != 1)
+        fail(CKH_ERR_INVALID_PARAMS, "to_sparse is not an SSE key of these parameters");
+    if (to_dense.ctx != &ctx || to_dense.max_level != ctx.nq - 1 || to_dense.parts != ctx.dnum)
+        fail(CKH_ERR_INVALID_PARAMS, "to_dense is not an SSE key of these parameters");
+    const u64 n = ctx.n;
+    BufPtr s1 = ctx.alloc(2 * n);
+    run_ks(ctx, {{ct.comps[1].get(), &to_sparse, ct.comps[0]->data(), nullptr, s1->ptr, s1->ptr + n}});
+    const Ct sparse{{view(s1, 0, 0, 0, true), view(s1, n, 0, 0, true)}, ct.scale};
+    CtPtr raised = raise_ct(sparse);
+    const u32 top = ctx.nq - 1;
+    const u64 L = top + 1;
+    BufPtr out = ctx.alloc(2 * L * n);
+    run_ks(ctx, {{raised->comps[1].get(), &to_dense, raised->comps[0]->data(), nullptr, out->ptr, out->ptr + L * n}});
+    CtPtr r = make_ct({view(out, 0, top, 0, true), view(out, L * n, top, 0, true)}, ct.scale);
+    count(&ckh_op_counter::sse_switches, 2);
+    return r;
 }
 
 // HeOps::mul_lazy / relin_lazy (ops.rs:345-363).

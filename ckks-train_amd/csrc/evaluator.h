@@ -23,6 +23,7 @@ double time_ntt(Context& c, const std::vector<const Poly*>& polys, bool forward,
 PolyPtr poly_mod_down(Context& c, const Poly& qp);
 PolyPtr poly_pointwise(Context& c, PwOp op, const Poly& a, const Poly* b);
 PolyPtr poly_scalar_mul(Context& c, const Poly& a, u64 k, bool neg);
+PolyPtr poly_raise(Context& c, const Poly& a, u32 top);
 // keyswitch::decompose: one buffer holding all digits; returns one QpPoly handle per digit.
 std::vector<PolyPtr> ks_decompose(Context& c, const Poly& d);
 // keyswitch::apply_key for digits laid out as ks_decompose returns them (or copied so).
@@ -92,6 +93,8 @@ class Evaluator {
     CtPtr mul_plain_to_scale(const Ct& a, const Poly& pt, double pt_scale, double target_scale);
     double exact_product_scale(u32 level, double target_scale) const;
     double target_plain_scale(const Ct& ct, double target_scale) const;
+    CtPtr mod_raise(const Ct& ct, const Ksk& to_sparse, const Ksk& to_dense);
+    CtPtr mod_raise_main_key(const Ct& ct);
     CtPtr mul_lazy(const Ct& a, const Ct& b);
     CtPtr relin_lazy(const Ct& sum);
 
@@ -108,6 +111,8 @@ class Evaluator {
     const Ksk& galois_key(u64 elt, u32 level) const;
     u64 rotation_elt(int32_t steps) const;  // 0 for a multiple of the slot count
     CtPtr apply_galois(const Ct& ct, u64 elt);
+    void check_mod_raise_input(const Ct& ct) const;
+    CtPtr raise_ct(const Ct& ct);
     std::vector<bool> check_matvec_enc(const Ct& x, const std::vector<const Ct*>& entries, u32 baby, u32 giant,
                                        u32 stride, int32_t offset) const;
     CtPtr matvec_enc_run(const Ct& x, const std::vector<const Ct*>& entries, u32 baby, u32 giant, u32 stride,

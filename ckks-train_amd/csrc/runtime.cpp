@@ -469,6 +469,19 @@ void Context::build_tables() {
             qlinv_s_[l].push_back(hm::shoup(v, primes[i]));
         }
     }
+    // Raise from level 0: q_0 -> q_1..q_top (the targets of top are the first top of one list;
+    // a single source, so the centered lift needs no overflow estimate).
+    raise_plan_.assign(nq, 0);
+    if (nq > 1) {
+        std::vector<std::pair<u32, u32>> all;
+        for (u32 i = 1; i < nq; ++i) all.push_back({i, i});
+        const u32 full = add_plan({0}, all, 0, false, 0);
+        const u32 first = plans[full].tgt_first;
+        for (u32 l = 1; l < nq; ++l) {
+            std::vector<std::pair<u32, u32>> tg(all.begin(), all.begin() + l);
+            raise_plan_[l] = add_plan({0}, tg, 0, true, first);
+        }
+    }
     conv.src = dev_copy(srcs);
     conv.tgt = dev_copy(tgts);
     conv.plans = dev_copy(plans);
@@ -883,6 +896,39 @@ void Context::prog_rescale(Program& p, const std::vector<RescaleTask>& tasks) co
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_t, 1);
     const size_t o_ntt = p.add(nttr), o_fin = p.add(fin);
     p.step(S_NTT_FWD_FIN, o_ntt, (u32)nttr.size(), o_fin);
+}
+
+// RnsPoly::raise_from_level0 (crates/ckks-train-core/src/poly.rs:298-345; the device form of
+// crates/ckks-train-cuda/src/basis.rs:513-533): limb 0 copied as it is; its coefficients (after an
+// inverse NTT for NTT-form input) converted to q_1..q_top by the single-source centered lift,
+// then transformed forward again.
+void Context::prog_raise(Program& p, const std::vector<RaiseTask>& tasks) const {
+    std::vector<LimbRef> copy, intt, nttr;
+    std::vector<ConvJob> jobs;
+    u32 max_t = 0;
+    for (const RaiseTask& t : tasks) {
+        if (t.top >= nq) fail(CKH_ERR_INVALID_PARAMS, "level out of range");
+        copy.push_back({t.in, t.out, 0, 0});
+        if (t.top == 0) continue;
+        const u64* src = t.in;
+        if (t.ntt) {
+            intt.push_back({t.in, t.x, 0, 0});
+            src = t.x;
+            for (u32 i = 1; i <= t.top; ++i) nttr.push_back({t.out + i * n, t.out + i * n, i, 0});
+        }
+        jobs.push_back({src, t.out, raise_plan_[t.top], 0});
+        max_t = std::max(max_t, t.top);
+    }
+    p.step(S_COPY, p.add(copy), (u32)copy.size());
+    p.step(S_NTT_INV, p.add(intt), (u32)intt.size());
+    p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_t, 1);
+    p.step(S_NTT_FWD, p.add(nttr), (u32)nttr.size());
+}
+
+void Context::raise_level0(const std::vector<RaiseTask>& tasks, hipStream_t s) {
+    Program p;
+    prog_raise(p, tasks);
+    execute(p, s);
 }
 
 void Context::decompose(const std::vector<DecTask>& tasks, hipStream_t s) {

@@ -122,6 +122,16 @@ struct RescaleTask {
     u64* z;
 };
 
+// One raise from level 0 (RnsPoly::raise_from_level0): limb 0 of in (NTT form when ntt) copied
+// to out, limbs 1..=top of out the centered lift of its coefficients; scratch x (1 limb).
+struct RaiseTask {
+    const u64* in;
+    u32 top;
+    u64* out;
+    u64* x;
+    bool ntt;
+};
+
 // Descriptor bytes plus the launch steps that read them (see runtime.cpp, "engine").
 enum StepKind : uint8_t {
     S_NTT_FWD, S_NTT_INV, S_CONV, S_COPY, S_IP, S_NTT_FWD_FIN, S_FINISH, S_NTT_COLS, S_NTT_ROWS, S_ROWS_IP
@@ -210,6 +220,8 @@ class Context : public std::enable_shared_from_this<Context> {
     void apply(const std::vector<ApplyTask>& tasks, hipStream_t s);
     void mod_down(const std::vector<MdTask>& tasks, hipStream_t s);
     void rescale(const std::vector<RescaleTask>& tasks, hipStream_t s);
+    void prog_raise(Program& p, const std::vector<RaiseTask>& tasks) const;
+    void raise_level0(const std::vector<RaiseTask>& tasks, hipStream_t s);
     void ntt_poly(const u64* in, u64* out, u32 limbs, u32 level, u32 p_limbs, bool forward, hipStream_t s);
 
     void launch_ntt_finish(const LimbRef* refs, const FinishJob* fin, u32 n_refs, hipStream_t s) const;
@@ -280,6 +292,7 @@ class Context : public std::enable_shared_from_this<Context> {
     std::vector<u32> modup_maxtgt_;            // [level]
     std::vector<u32> moddown_plan_;            // [level]
     std::vector<u32> rescale_plan_;            // [level], level >= 1
+    std::vector<u32> raise_plan_;              // [top], top >= 1: q_0 -> q_1..q_top
     std::vector<u64> pinv_, pinv_s_;           // P^-1 mod q_i
     std::vector<std::vector<u64>> qlinv_, qlinv_s_;  // [l][i]: q_l^-1 mod q_i
     void build_tables();

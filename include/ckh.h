@@ -154,6 +154,10 @@ ckh_status ckh_time_ntt(ckh_context *ctx, ckh_poly *const *polys, uint32_t n, in
 ckh_status ckh_rescale_poly(ckh_context *ctx, const ckh_poly *in, ckh_poly **out);
 /* RnsPoly::mod_down of a QpPoly (baseconv.rs:459-511): Q part level l -> level l. */
 ckh_status ckh_mod_down(ckh_context *ctx, const ckh_poly *qp, ckh_poly **out);
+/* RnsPoly::raise_from_level0 (crates/ckks-train-core/src/poly.rs:298-345; device form
+ * crates/ckks-train-cuda/src/basis.rs:513-533): a level-0 Q polynomial to level top, limb 0 as it
+ * is and limbs 1..=top the centered lift of its coefficients (same form as the input). */
+ckh_status ckh_raise_from_level0(ckh_context *ctx, const ckh_poly *in, uint32_t top, ckh_poly **out);
 
 /* ------------------------------------------------------------------ key switching */
 
@@ -247,6 +251,14 @@ ckh_status ckh_mul_plain_to_scale(ckh_evaluator *ev, const ckh_ct *a, const ckh_
  * 738-772): the label L with L / q_level == target_scale exactly, and L / ct scale. */
 ckh_status ckh_exact_product_scale(ckh_evaluator *ev, uint32_t level, double target_scale, double *out);
 ckh_status ckh_target_plain_scale(ckh_evaluator *ev, const ckh_ct *ct, double target_scale, double *out);
+/* Evaluator::mod_raise (crates/ckks-train-ckks/src/bootstrap/modraise.rs:53-69): a level-0
+ * ciphertext switched to the ephemeral key at level 0 (to_sparse: one part at level 0), both
+ * components raised to max_level, and switched back (to_dense: dnum parts at max_level), the
+ * SseKeys of keys.rs:425-467 in the key layout of ckh_ksk_upload; sse_switches += 2.
+ * mod_raise_main_key (modraise.rs:77-80): the raise alone. */
+ckh_status ckh_mod_raise(ckh_evaluator *ev, const ckh_ct *ct, const ckh_ksk *to_sparse, const ckh_ksk *to_dense,
+                         ckh_ct **out);
+ckh_status ckh_mod_raise_main_key(ckh_evaluator *ev, const ckh_ct *ct, ckh_ct **out);
 /* HeOps::mul_lazy / relin_lazy (ops.rs:339-356), following the lazy_relin flag. */
 ckh_status ckh_mul_lazy(ckh_evaluator *ev, const ckh_ct *a, const ckh_ct *b, ckh_ct **out);
 ckh_status ckh_relin_lazy(ckh_evaluator *ev, const ckh_ct *sum, ckh_ct **out);

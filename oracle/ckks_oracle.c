@@ -802,6 +802,32 @@ void or_poly_mul(const or_params *pp, u64 *a, const u64 *b, u64 limbs) {
         for (u64 c = 0; c < pp->n; ++c) a[i * pp->n + c] = or_mod_mul(&pp->mq[i], a[i * pp->n + c], b[i * pp->n + c]);
 }
 
+/* RnsPoly::raise_from_level0 (poly.rs:298-345): limb 0 (level-0 input `a`, NTT form when ntt)
+ * copied; limb i in 1..=top the coefficient x of limb 0 (inverse-transformed first) lifted to
+ * x or -(q_0 - x) by x > (q_0 - 1) / 2, reduced mod q_i, transformed forward again. */
+void or_raise_from_level0(const or_params *pp, const u64 *a, u64 top, int ntt, u64 *out) {
+    const u64 n = pp->n, q0 = pp->mq[0].q, half = q0 / 2;
+    u64 *src = malloc(n * sizeof(u64));
+    memcpy(src, a, n * sizeof(u64));
+    if (ntt) or_ntt_inverse(&pp->tq[0], src);
+    memcpy(out, a, n * sizeof(u64));
+    for (u64 i = 1; i <= top; ++i) {
+        const u64 q = pp->mq[i].q;
+        u64 *o = out + i * n;
+        for (u64 c = 0; c < n; ++c) {
+            const u64 x = src[c];
+            if (x > half) {
+                const u64 r = (q0 - x) % q;
+                o[c] = r == 0 ? 0 : q - r;
+            } else {
+                o[c] = x % q;
+            }
+        }
+        if (ntt) or_ntt_forward(&pp->tq[i], o);
+    }
+    free(src);
+}
+
 /* RnsPoly::mul_scalar_assign (poly.rs:184-189): limb i times k mod q_i as a Shoup product;
  * then RnsPoly::neg (poly.rs:157-161), x -> 0 - x mod q_i, when neg is set. */
 void or_poly_mul_scalar(const or_params *pp, u64 *a, u64 limbs, u64 k, int neg) {

@@ -327,3 +327,58 @@ def test_pool_accounting_and_trim():
     ctx.set_pool_release_threshold((1 << 64) - 1)
     p = ctx.uniform(L, seed=99)  # allocation after a trim works
     assert p.download().shape == (L + 1, ref.n)
+
+
+# ---------------------------------------------------------------- mod-raise (bootstrapping)
+
+@pytest.mark.parametrize("case", [SMALL, C4_SHAPE_SMALL_N], ids=lambda c: c[0])
+def test_raise_from_level0_matches_oracle(case):
+    ctx, ref = make(case)
+    L = len(ref.q) - 1
+    a = host_uniform(ref, 0, 2100)
+    a[0, :4] = [0, 1, ref.q[0] // 2, ref.q[0] // 2 + 1]
+    for ntt in (True, False):
+        p = ctx.upload(a, 0, ntt=ntt)
+        for top in sorted({0, 1, L // 2, L}):
+            got = p.raise_from_level0(top).download()
+            want = ref.raise_from_level0(a.ravel(), top, ntt=ntt)
+            assert (got == want).all(), (ntt, top)
+    with pytest.raises(ckh.InvalidParams):
+        ctx.upload(host_uniform(ref, 1, 2101), 1).raise_from_level0(L)
+    with pytest.raises(ckh.InvalidParams):
+        ctx.upload(a, 0).raise_from_level0(L + 1)
+
+
+@pytest.mark.parametrize("case", [SMALL, C4_SHAPE_SMALL_N], ids=lambda c: c[0])
+def test_mod_raise_with_sse_matches_oracle(case):
+    """Evaluator::mod_raise (bootstrap/modraise.rs:53-69): switch to the ephemeral key at level 0,
+    raise both components, switch back at the top; and mod_raise_main_key."""
+    ctx, ref = make(case)
+    L = len(ref.q) - 1
+    to_sparse, to_dense = ctx.ksk_uniform(0, seed=2200), ctx.ksk_uniform(L, seed=2201)
+    h_sparse, h_dense = ref.uniform_key(0, 2200), ref.uniform_key(L, 2201)
+    ev = ckh.Evaluator(ctx)
+    ct, h = _ct(ctx, ref, 0, 2210)
+    got = ev.mod_raise(ct, to_sparse, to_dense)
+    assert got.level == L and got.scale == DELTA and got.size() == 2
+    kb, ka = ref.key_switch(h[1].ravel(), 0, h_sparse, 0)
+    c0 = ref.poly_add(h[0], kb, 0)
+    r0, r1 = ref.raise_from_level0(c0.ravel(), L), ref.raise_from_level0(ka.ravel(), L)
+    kb2, ka2 = ref.key_switch(r1.ravel(), L, h_dense, L)
+    g = got.download()
+    assert (g[0] == ref.poly_add(r0, kb2, L)).all() and (g[1] == ka2).all()
+    c = ev.counter()
+    assert c.sse_switches == 2 and c.key_switches() == 2
+    m = ev.mod_raise_main_key(ct).download()
+    assert (m[0] == ref.raise_from_level0(h[0].ravel(), L)).all()
+    assert (m[1] == ref.raise_from_level0(h[1].ravel(), L)).all()
+    assert ev.counter().sse_switches == 2
+    ct1, _ = _ct(ctx, ref, 1, 2220)
+    with pytest.raises(ckh.LevelMismatch):
+        ev.mod_raise(ct1, to_sparse, to_dense)
+    ct3, _ = _ct(ctx, ref, 0, 2230, ncomp=3)
+    with pytest.raises(ckh.InvalidParams):
+        ev.mod_raise_main_key(ct3)
+    with pytest.raises(ckh.InvalidParams):  # the keys swapped
+        ev.mod_raise(ct, to_dense, to_sparse)
+    assert ev.counter().sse_switches == 2

@@ -562,3 +562,27 @@ def test_scalar_rounding_is_half_away_from_zero():
         p.mul_scalar_parts(2, 2.0 ** 40, float("nan"), 2.0 ** 40)
     with pytest.raises(ValueError, match="InvalidParams"):  # plaintext scale below 2^20
         p.mul_scalar_parts(2, 2.0 ** 40 * p.q[2], 1.0, 2.0 ** 40)
+
+
+# ---------------- raise from level 0 (poly.rs:298-345, 637-745 tests) ----------------
+
+def test_raise_from_level0_is_the_centered_lift():
+    for qb in ([50, 40, 40, 30], [60, 50, 50, 40]):
+        p = O.Params(6, qb, [60], 2)
+        q0 = p.q[0]
+        a = O.uniform_poly(p.q[:1], p.n, 17)[0]
+        a[:6] = [0, 1, q0 // 2, q0 // 2 + 1, q0 - 1, q0 - 2]
+        top = len(p.q) - 1
+        coeff = p.raise_from_level0(a, top, ntt=False)
+        for i, q in enumerate(p.q):
+            for c in range(p.n):
+                x = int(a[c])
+                centered = x if x <= (q0 - 1) // 2 else x - q0
+                assert int(coeff[i, c]) == centered % q, (i, c)
+        # NTT form: limb 0 untouched, the rest = forward(centered lift of inverse(limb 0))
+        a_ntt = O.ntt_forward(p.n, p.q[:1], a.copy())
+        got = p.raise_from_level0(a_ntt, top, ntt=True)
+        assert (got[0] == a_ntt).all()
+        want = O.ntt_forward(p.n, p.q[: top + 1], coeff.ravel().copy()).reshape(top + 1, p.n)
+        assert (got[1:] == want[1:]).all()
+        assert (p.raise_from_level0(a, 0, ntt=False) == a).all()
