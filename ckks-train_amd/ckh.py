@@ -174,6 +174,8 @@ _SIGS = {
     "ckh_matvec_enc_batch": (C.c_int, [VP, u32, C.POINTER(VP), C.POINTER(C.POINTER(VP)), C.c_void_p,
                                        C.POINTER(VP)]),
     "ckh_matvec_enc_workspace_bytes": (C.c_int, [VP, u32, u32, u32, C.POINTER(u64)]),
+    "ckh_matvec_plain_sum": (C.c_int, [VP, u32, C.POINTER(VP), C.POINTER(C.POINTER(VP)), C.c_double, u32, u32,
+                                       u32, C.c_int32, C.POINTER(VP)]),
     "ckh_set_pool_release_threshold": (C.c_int, [VP, u64]),
     "ckh_trim_pool": (C.c_int, [VP, u64]),
     "ckh_relin_lazy": (C.c_int, [VP, VP, C.POINTER(VP)]),
@@ -641,6 +643,14 @@ class Evaluator:
         """BSGS product with encrypted diagonals (Ct entries or None)."""
         arr = (VP * len(entries))(*[e.h if e is not None else None for e in entries])
         return self._op(lib().ckh_matvec_enc, x.h, arr, baby, giant, stride, offset)
+
+    def matvec_plain_sum(self, xs, entries, pt_scale, baby, giant, stride=1, offset=0):
+        """sum_t W_t x_t; entries[t] is a list of Poly or None."""
+        n = len(xs)
+        xa = (VP * max(n, 1))(*[x.h for x in xs])
+        rows = [(VP * len(e))(*[p.h if p is not None else None for p in e]) for e in entries]
+        ea = (C.POINTER(VP) * max(n, 1))(*[C.cast(r, C.POINTER(VP)) for r in rows])
+        return self._op(lib().ckh_matvec_plain_sum, n, xa, ea, pt_scale, baby, giant, stride, offset)
 
     def matvec_enc_batch(self, xs, entries, shapes):
         """jobs (x, entries, (baby, giant, stride, offset)); results in job order."""

@@ -683,6 +683,27 @@ ckh_status ckh_matvec_enc(ckh_evaluator* ev, const ckh_ct* x, const ckh_ct* cons
     });
 }
 
+ckh_status ckh_matvec_plain_sum(ckh_evaluator* ev, uint32_t n_terms, const ckh_ct* const* xs,
+                                const ckh_poly* const* const* entries, double pt_scale, uint32_t baby,
+                                uint32_t giant, uint32_t stride, int32_t offset, ckh_ct** out) {
+    return guarded([&] {
+        need(out, "out");
+        if (n_terms) {
+            need(xs, "xs");
+            need(entries, "entries");
+        }
+        std::vector<const Ct*> x;
+        std::vector<std::vector<const Poly*>> e(n_terms);
+        for (uint32_t t = 0; t < n_terms; ++t) {
+            x.push_back(&ct_of(xs[t]));
+            need(entries[t], "entries");
+            for (uint64_t j = 0; j < (uint64_t)baby * giant; ++j)
+                e[t].push_back(entries[t][j] ? &poly_of(entries[t][j]) : nullptr);
+        }
+        *out = wrap(ev_of(ev).matvec_plain_sum(x, e, pt_scale, baby, giant, stride, offset));
+    });
+}
+
 ckh_status ckh_matvec_enc_batch(ckh_evaluator* ev, uint32_t n, const ckh_ct* const* xs,
                                 const ckh_ct* const* const* entries, const ckh_bsgs_shape* shapes, ckh_ct** outs) {
     return guarded([&] {

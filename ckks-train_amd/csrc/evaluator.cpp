@@ -815,46 +815,52 @@ void Evaluator::check_bsgs_shape(const Ct& x, u32 baby, u32 giant, u32 stride, i
     }
 }
 
-CtPtr Evaluator::bsgs(const Ct& x, u32 baby, u32 giant, u32 stride, int32_t offset, const std::vector<bool>& present,
-                      const std::function<CtPtr(size_t, const Ct&)>& mul,
+CtPtr Evaluator::bsgs(const std::vector<BsgsTerm>& terms, u32 baby, u32 giant, u32 stride, int32_t offset,
+                      const std::function<CtPtr(size_t, size_t, const Ct&)>& mul,
                       const std::function<CtPtr(const CtPtr&)>& finish) {
-    // xs[k2] = rot(x, k2 stride) for the baby steps a present entry uses (one hoisted call).
-    std::vector<int32_t> steps;
-    std::vector<u32> used_k2;
-    for (u32 k2 = 0; k2 < baby; ++k2) {
-        bool used = false;
-        for (u32 k1 = 0; k1 < giant && !used; ++k1) used = present[k1 * baby + k2];
-        if (used) {
-            steps.push_back((int32_t)(k2 * stride));
-            used_k2.push_back(k2);
+    // xs[t][k2] = rot(x_t, k2 stride) for the baby steps a present entry of term t uses (one
+    // hoisted call per term).
+    std::vector<std::vector<CtPtr>> xs(terms.size(), std::vector<CtPtr>(baby));
+    for (size_t t = 0; t < terms.size(); ++t) {
+        std::vector<int32_t> steps;
+        std::vector<u32> used_k2;
+        for (u32 k2 = 0; k2 < baby; ++k2) {
+            bool used = false;
+            for (u32 k1 = 0; k1 < giant && !used; ++k1) used = terms[t].present[k1 * baby + k2];
+            if (used) {
+                steps.push_back((int32_t)(k2 * stride));
+                used_k2.push_back(k2);
+            }
         }
+        std::vector<CtPtr> rot = hoisted_rotate(*terms[t].x, steps);
+        for (size_t i = 0; i < used_k2.size(); ++i) xs[t][used_k2[i]] = rot[i];
     }
-    std::vector<CtPtr> rot = hoisted_rotate(x, steps);
-    std::vector<CtPtr> xs(baby);
-    for (size_t i = 0; i < used_k2.size(); ++i) xs[used_k2[i]] = rot[i];
-    // Per block: with bsgs, the products summed, finished once and rotated by the block's giant
-    // shift (all blocks' giant rotations in one batched switch); naive, every product finished
-    // and rotated by itself. Sums are exact, so their order does not change the words.
+    // Per block: with bsgs, the products of every term summed, finished once and rotated by the
+    // block's giant shift (all blocks' giant rotations in one batched switch); naive, every
+    // product finished and rotated by itself. Sums are exact, so their order does not change
+    // the words; a block without a present entry costs nothing.
     std::vector<CtPtr> parts;
     std::vector<std::pair<const Ct*, int32_t>> to_rotate;
     for (u32 k1 = 0; k1 < giant; ++k1) {
         const int32_t shift = (offset + (int32_t)(baby * k1)) * (int32_t)stride;
         if (!config.bsgs && shift != 0) {
-            for (u32 k2 = 0; k2 < baby; ++k2) {
-                const size_t i = (size_t)k1 * baby + k2;
-                if (!present[i]) continue;
-                parts.push_back(finish(mul(i, *xs[k2])));
-                to_rotate.push_back({parts.back().get(), shift});
-            }
+            for (size_t t = 0; t < terms.size(); ++t)
+                for (u32 k2 = 0; k2 < baby; ++k2) {
+                    const size_t i = (size_t)k1 * baby + k2;
+                    if (!terms[t].present[i]) continue;
+                    parts.push_back(finish(mul(t, i, *xs[t][k2])));
+                    to_rotate.push_back({parts.back().get(), shift});
+                }
             continue;
         }
         CtPtr inner;
-        for (u32 k2 = 0; k2 < baby; ++k2) {
-            const size_t i = (size_t)k1 * baby + k2;
-            if (!present[i]) continue;
-            CtPtr t = mul(i, *xs[k2]);
-            inner = inner ? add(*inner, *t, false) : t;
-        }
+        for (size_t t = 0; t < terms.size(); ++t)
+            for (u32 k2 = 0; k2 < baby; ++k2) {
+                const size_t i = (size_t)k1 * baby + k2;
+                if (!terms[t].present[i]) continue;
+                CtPtr v = mul(t, i, *xs[t][k2]);
+                inner = inner ? add(*inner, *v, false) : v;
+            }
         if (!inner) continue;
         parts.push_back(finish(inner));
         to_rotate.push_back({parts.back().get(), shift});
@@ -863,6 +869,45 @@ CtPtr Evaluator::bsgs(const Ct& x, u32 baby, u32 giant, u32 stride, int32_t offs
     CtPtr acc;
     for (auto& r : rotated) acc = acc ? add(*acc, *r, false) : r;
     return acc;
+}
+
+// Evaluator::matvec_plain_sum_sparse (crates/ckks-train-ckks/src/linalg.rs:694-790): sum_t W_t x_t
+// over terms of one shape with the giant rotations shared; NULL entries are skipped (with the
+// baby rotations and giant blocks only they would use). Checks in the reference's order, all
+// before any work; every rotation step of the shape needs its key.
+CtPtr Evaluator::matvec_plain_sum(const std::vector<const Ct*>& xs, const std::vector<std::vector<const Poly*>>& entries,
+                                  double pt_scale, u32 baby, u32 giant, u32 stride, int32_t offset) {
+    if (xs.empty()) fail(CKH_ERR_INVALID_PARAMS, "matvec_plain_sum needs at least one term");
+    if (entries.size() != xs.size()) fail(CKH_ERR_INVALID_PARAMS, "one diagonal set per term");
+    std::vector<BsgsTerm> terms;
+    for (size_t t = 0; t < xs.size(); ++t) {
+        const Ct& x = *xs[t];
+        check_basis(x);
+        check_galois_input(x);
+        if (entries[t].size() != (size_t)baby * giant)
+            fail(CKH_ERR_INVALID_PARAMS, "entries must hold baby * giant diagonals");
+        BsgsTerm term{&x, std::vector<bool>(entries[t].size())};
+        bool any = false;
+        for (size_t i = 0; i < entries[t].size(); ++i)
+            if (entries[t][i]) term.present[i] = any = true;
+        if (!any) fail(CKH_ERR_INVALID_PARAMS, "a term of matvec_plain_sum has no nonzero diagonal");
+        for (const Poly* w : entries[t]) {
+            if (!w) continue;
+            check_plain(ctx, *w);
+            check_level(x.level(), w->level);
+        }
+        check_level(x.level(), xs[0]->level());
+        if (x.scale != xs[0]->scale) fail(CKH_ERR_SCALE_MISMATCH, "terms differ in scale");
+        std::vector<bool> all(entries[t].size(), true);
+        check_bsgs_shape(x, baby, giant, stride, offset, all);
+        terms.push_back(std::move(term));
+    }
+    return counted([&] {
+        return bsgs(
+            terms, baby, giant, stride, offset,
+            [&](size_t t, size_t i, const Ct& xr) { return mul_plain(xr, *entries[t][i], pt_scale); },
+            [](const CtPtr& c) { return c; });
+    });
 }
 
 // Evaluator::matvec_plain (crates/ckks-train-ckks/src/linalg.rs:608-621): entries[k1 * baby +
@@ -882,8 +927,8 @@ CtPtr Evaluator::matvec_plain(const Ct& x, const std::vector<const Poly*>& entri
     check_bsgs_shape(x, baby, giant, stride, offset, present);
     return counted([&] {
         return bsgs(
-            x, baby, giant, stride, offset, present,
-            [&](size_t i, const Ct& xr) { return mul_plain(xr, *entries[i], pt_scale); },
+            {{&x, present}}, baby, giant, stride, offset,
+            [&](size_t, size_t i, const Ct& xr) { return mul_plain(xr, *entries[i], pt_scale); },
             [](const CtPtr& c) { return c; });
     });
 }
@@ -922,12 +967,12 @@ CtPtr Evaluator::matvec_enc_run(const Ct& x, const std::vector<const Ct*>& entri
                                 int32_t offset, const std::vector<bool>& present) {
     if (config.lazy_relin)
         return bsgs(
-            x, baby, giant, stride, offset, present,
-            [&](size_t i, const Ct& xr) { return mul_no_relin(*entries[i], xr); },
+            {{&x, present}}, baby, giant, stride, offset,
+            [&](size_t, size_t i, const Ct& xr) { return mul_no_relin(*entries[i], xr); },
             [&](const CtPtr& c) { return relinearize(*c); });
     return bsgs(
-        x, baby, giant, stride, offset, present, [&](size_t i, const Ct& xr) { return mul(*entries[i], xr); },
-        [](const CtPtr& c) { return c; });
+        {{&x, present}}, baby, giant, stride, offset,
+        [&](size_t, size_t i, const Ct& xr) { return mul(*entries[i], xr); }, [](const CtPtr& c) { return c; });
 }
 
 // Evaluator::matvec_enc (linalg.rs:638-665): encrypted 2-component diagonals; lazy_relin: per

@@ -85,6 +85,8 @@ class Evaluator {
                        u32 stride, int32_t offset);
     CtPtr matvec_enc(const Ct& x, const std::vector<const Ct*>& entries, u32 baby, u32 giant, u32 stride,
                      int32_t offset);
+    CtPtr matvec_plain_sum(const std::vector<const Ct*>& xs, const std::vector<std::vector<const Poly*>>& entries,
+                           double pt_scale, u32 baby, u32 giant, u32 stride, int32_t offset);
     std::vector<CtPtr> matvec_enc_batch(const std::vector<const Ct*>& xs,
                                         const std::vector<std::vector<const Ct*>>& entries,
                                         const std::vector<BsgsShape>& shapes);
@@ -120,11 +122,16 @@ class Evaluator {
     // The BSGS shape checks shared by both products, before any work.
     void check_bsgs_shape(const Ct& x, u32 baby, u32 giant, u32 stride, int32_t offset,
                           const std::vector<bool>& present) const;
-    // The BSGS driver (linalg.rs:800-888): present[k1 * baby + k2] entries are multiplied by
-    // rot(x, k2 stride) with mul(index, rotated x) and each block (or, naive, each product) is
-    // finished with finish() before its giant rotation.
-    CtPtr bsgs(const Ct& x, u32 baby, u32 giant, u32 stride, int32_t offset, const std::vector<bool>& present,
-               const std::function<CtPtr(size_t, const Ct&)>& mul, const std::function<CtPtr(const CtPtr&)>& finish);
+    // The BSGS driver (linalg.rs:800-888): for every term t, the present[k1 * baby + k2]
+    // entries are multiplied by rot(x_t, k2 stride) with mul(t, index, rotated x_t) and each
+    // block (or, naive, each product) is finished with finish() before its giant rotation.
+    struct BsgsTerm {
+        const Ct* x;
+        std::vector<bool> present;
+    };
+    CtPtr bsgs(const std::vector<BsgsTerm>& terms, u32 baby, u32 giant, u32 stride, int32_t offset,
+               const std::function<CtPtr(size_t, size_t, const Ct&)>& mul,
+               const std::function<CtPtr(const CtPtr&)>& finish);
 };
 
 }  // namespace ckh

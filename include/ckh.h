@@ -289,6 +289,13 @@ ckh_status ckh_matvec_plain(ckh_evaluator *ev, const ckh_ct *x, const ckh_poly *
  * diagonals; lazy_relin: per block the tensor products are summed and relinearized once. */
 ckh_status ckh_matvec_enc(ckh_evaluator *ev, const ckh_ct *x, const ckh_ct *const *entries, uint32_t baby,
                           uint32_t giant, uint32_t stride, int32_t offset, ckh_ct **out);
+/* Evaluator::matvec_plain_sum_sparse (crates/ckks-train-ckks/src/linalg.rs:729-790): sum over
+ * n_terms of W_t x_t, one shape for every term, entries[t][k1 * baby + k2] pre-rotated as for
+ * ckh_matvec_plain (NULL = zero, skipped with the rotations only it would use), every entry at
+ * the terms' common level with scale pt_scale; the giant rotations are shared by all terms. */
+ckh_status ckh_matvec_plain_sum(ckh_evaluator *ev, uint32_t n_terms, const ckh_ct *const *xs,
+                                const ckh_poly *const *const *entries, double pt_scale, uint32_t baby,
+                                uint32_t giant, uint32_t stride, int32_t offset, ckh_ct **out);
 /* CudaEvaluator::matvec_enc_batch (crates/ckks-train-cuda/src/evaluator.rs:412-519): n
  * independent products (job i: xs[i], entries[i][0..baby * giant), shapes[i]); every job is
  * checked first, in order; a product whose working set exceeds the device memory available now

@@ -117,6 +117,56 @@ def oracle_bsgs(ref, c0, c1, level, present, baby, giant, stride, offset, mul, f
     return acc
 
 
+def oracle_matvec_sum(ref, xs, level, entries, baby, giant, stride, offset, share_giant, key_of_step, kml):
+    """matvec_plain_sum composed from oracle operations (linalg.rs:800-888 with several terms):
+    every term's baby rotations hoisted; per block the products of every term summed (or, naive
+    and a nonzero shift, each rotated by itself); xs[t] = (c0, c1), entries[t][i] a host array
+    or None."""
+    slots = ref.n // 2
+    rx = []
+    for (c0, c1), ents in zip(xs, entries):
+        used = [k2 for k2 in range(baby) if any(ents[k1 * baby + k2] is not None for k1 in range(giant))]
+        nontriv = [k2 for k2 in used if (k2 * stride) % slots]
+        hoisted = ref.hoisted_galois(c0, c1, level, [galois_elt_host(ref.n, k2 * stride) for k2 in nontriv],
+                                     [key_of_step(k2 * stride) for k2 in nontriv], kml)
+        row = {}
+        for k2 in used:
+            if k2 in nontriv:
+                i = nontriv.index(k2)
+                row[k2] = (hoisted[i, 0], hoisted[i, 1])
+            else:
+                row[k2] = (c0.reshape(level + 1, ref.n), c1.reshape(level + 1, ref.n))
+        rx.append(row)
+
+    def add(a, b):
+        return b if a is None else tuple(ref.poly_add(u, v, level) for u, v in zip(a, b))
+
+    def rot(t, shift):
+        if shift % slots == 0:
+            return t
+        return ref.apply_galois(t[0], t[1], level, galois_elt_host(ref.n, shift), key_of_step(shift % slots), kml)
+
+    def mul(t, i, x):
+        return tuple(ref.poly_mul(xc, entries[t][i], level) for xc in x)
+
+    acc = None
+    for k1 in range(giant):
+        shift = (offset + baby * k1) * stride
+        blk = None
+        for t in range(len(xs)):
+            for k2 in range(baby):
+                i = k1 * baby + k2
+                if entries[t][i] is None:
+                    continue
+                if not share_giant and shift % slots:
+                    acc = add(acc, rot(mul(t, i, rx[t][k2]), shift))
+                else:
+                    blk = add(blk, mul(t, i, rx[t][k2]))
+        if blk is not None:
+            acc = add(acc, rot(blk, shift))
+    return acc
+
+
 def oracle_matvec(ref, c0, c1, level, entries, baby, giant, stride, key_of_step, kml, offset=0, share_giant=True):
     """matvec_plain composed from oracle operations: mul = mul_plain by the host entry."""
     present = [e is not None for e in entries]

@@ -382,3 +382,40 @@ def test_mod_raise_with_sse_matches_oracle(case):
     with pytest.raises(ckh.InvalidParams):  # the keys swapped
         ev.mod_raise(ct, to_dense, to_sparse)
     assert ev.counter().sse_switches == 2
+
+
+@pytest.mark.parametrize("bsgs", [True, False])
+def test_matvec_plain_sum_matches_oracle(bsgs):
+    """Evaluator::matvec_plain_sum_sparse (linalg.rs:729-790): two terms sharing the giant
+    rotations, zero entries skipped (a column unused by one term costs that term no baby
+    rotation)."""
+    from gpu_cases import oracle_matvec_sum
+    ctx, ref = make(C4_SHAPE_SMALL_N)
+    L = len(ref.q) - 1
+    baby, giant, stride, offset = 4, 4, 1, -2
+    steps = [k2 * stride for k2 in range(1, baby)] + [(offset + baby * k1) * stride for k1 in range(giant)]
+    keys, key_of_step = _keys_for(ctx, ref, L, steps, 2400)
+    ev = ckh.Evaluator(ctx, None, keys)
+    ev.set_config(bsgs=bsgs)
+    level = L - 1
+    xs_h, xs_d, ents_h, ents_d = [], [], [], []
+    masks = [[(i % 3) != 1 for i in range(16)], [(i % 4) != 2 and i < 12 for i in range(16)]]
+    for t, mask in enumerate(masks):
+        c0, c1 = host_uniform(ref, level, 2500 + 10 * t), host_uniform(ref, level, 2501 + 10 * t)
+        xs_h.append((c0, c1))
+        xs_d.append(ctx.ct_upload(np.stack([c0, c1]), level, DELTA))
+        eh = [host_uniform(ref, level, 2600 + 50 * t + i) if m else None for i, m in enumerate(mask)]
+        ents_h.append(eh)
+        ents_d.append([ctx.upload(w, level) if w is not None else None for w in eh])
+    ev.reset_counter()
+    got = ev.matvec_plain_sum(xs_d, ents_d, DELTA, baby, giant, stride, offset).download()
+    want = oracle_matvec_sum(ref, xs_h, level, ents_h, baby, giant, stride, offset, bsgs, key_of_step, L)
+    assert (got[0] == want[0]).all() and (got[1] == want[1]).all()
+    assert ev.counter().ct_pt_mults == sum(map(sum, masks))
+    with pytest.raises(ckh.InvalidParams):  # a term without any entry
+        ev.matvec_plain_sum(xs_d, [ents_d[0], [None] * 16], DELTA, baby, giant, stride, offset)
+    with pytest.raises(ckh.ScaleMismatch):
+        other = ctx.ct_upload(np.stack(xs_h[1]), level, 2 * DELTA)
+        ev.matvec_plain_sum([xs_d[0], other], ents_d, DELTA, baby, giant, stride, offset)
+    with pytest.raises(ckh.InvalidParams):
+        ev.matvec_plain_sum([], [], DELTA, baby, giant, stride, offset)
