@@ -772,6 +772,13 @@ bool Context::fuse_ip_possible() const {
     return true;
 }
 
+// Limb descriptors grouped by prime (stable): a pass over many limbs then works on one prime at a
+// time, and the integer path's row-pass twiddle table (1 MiB per prime at N = 2^16, read once per
+// limb) stays in the L2 of the XCD running that prime instead of being fetched again per limb.
+void group_by_prime(std::vector<LimbRef>& r) {
+    std::stable_sort(r.begin(), r.end(), [](const LimbRef& a, const LimbRef& b) { return a.prime < b.prime; });
+}
+
 void Context::prog_decompose(Program& p, const std::vector<DecTask>& tasks) const {
     std::vector<LimbRef> intt, copies, nttr, cols, rows;
     std::vector<ConvJob> jobs;
@@ -800,6 +807,8 @@ void Context::prog_decompose(Program& p, const std::vector<DecTask>& tasks) cons
         }
         max_tgt = std::max(max_tgt, modup_maxtgt_[t.level]);
     }
+    group_by_prime(nttr);
+    group_by_prime(rows);
     p.step(S_NTT_INV, p.add(intt), (u32)intt.size());
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_tgt, alpha);
     p.step(S_COPY, p.add(copies), (u32)copies.size());
@@ -870,6 +879,7 @@ void Context::prog_mod_down(Program& p, const std::vector<MdTask>& tasks) const 
             fin.push_back({t.acc + i * n, zi, t.out + i * n, t.add ? t.add + i * n : nullptr, pinv_[i], pinv_s_[i], i, 0});
         }
     }
+    group_by_prime(intt);
     p.step(S_NTT_INV, p.add(intt), (u32)intt.size());
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_l, np);
     const size_t o_ntt = p.add(nttr), o_fin = p.add(fin);
