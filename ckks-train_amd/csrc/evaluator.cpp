@@ -101,7 +101,6 @@ double time_ntt(Context& c, const std::vector<const Poly*>& polys, bool forward,
             refs.push_back({p->data() + i * c.n, p->data() + i * c.n, prime, 0});
         }
     if (refs.empty() || reps == 0) fail(CKH_ERR_INVALID_PARAMS, "nothing to time");
-    group_by_prime(refs);  // the engine's schedule for multi-limb passes
     hipStream_t s = c.stream();
     BufPtr d = c.upload_blob(refs.data(), refs.size() * sizeof(LimbRef), s);
     const LimbRef* dr = (const LimbRef*)d->ptr;

@@ -775,7 +775,7 @@ bool Context::fuse_ip_possible() const {
 // Limb descriptors grouped by prime (stable): a pass over many limbs then works on one prime at a
 // time, and the integer path's row-pass twiddle table (1 MiB per prime at N = 2^16, read once per
 // limb) stays in the L2 of the XCD running that prime instead of being fetched again per limb.
-void group_by_prime(std::vector<LimbRef>& r) {
+static void group_by_prime(std::vector<LimbRef>& r) {
     std::stable_sort(r.begin(), r.end(), [](const LimbRef& a, const LimbRef& b) { return a.prime < b.prime; });
 }
 

@@ -29,8 +29,6 @@ void hip_check(hipError_t e, const char* what);
 
 class Context;
 
-// Orders limb descriptors by prime (stable), as the engine schedules every multi-limb pass.
-void group_by_prime(std::vector<LimbRef>& r);
 
 // A device allocation from the context's stream-ordered pool; freed on its stream when the
 // last reference goes.

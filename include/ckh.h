@@ -145,10 +145,9 @@ ckh_status ckh_intt(ckh_context *ctx, const ckh_poly *in, ckh_poly **out);
 ckh_status ckh_automorphism(ckh_context *ctx, const ckh_poly *in, uint64_t galois_elt,
                             ckh_poly **out);
 /* Measurement hook (no counterpart in the reference): transforms every limb of the n polys in
- * place (forward or inverse), as one batched launch (limbs grouped by prime, as the engine
- * schedules its multi-limb passes), reps times between two HIP events on the calling thread's
- * stream, and writes the mean milliseconds per launch. Overwrites the polynomials' contents
- * (residues stay reduced). */
+ * place (forward or inverse), as one batched launch, reps times between two HIP events on the
+ * calling thread's stream, and writes the mean milliseconds per launch. Overwrites the
+ * polynomials' contents (residues stay reduced). */
 ckh_status ckh_time_ntt(ckh_context *ctx, ckh_poly *const *polys, uint32_t n, int forward,
                         uint32_t reps, double *ms_per_launch);
 /* RnsPoly::rescale_by_last (crates/ckks-train-core/src/baseconv.rs:258-291). */
