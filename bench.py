@@ -222,6 +222,115 @@ def _served_at(ckh, envs, n_ops, rate, seed):
     }
 
 
+def config_table(ckh, O, ev):
+    """The per-configuration figures (one JSON object): C1 NTT round trip at N = 2^13 (4 x 50-bit),
+    C2 one key switch at N = 2^16 and 24 limbs, C4 the hoisted BSGS product at N = 2^15 (20 + 4
+    primes, dnum 5, 32 x 32 blocks, two diagonals per block), each on the GPU and with the CPU
+    oracle (1 thread and the host's threads, at most 16), and whether the GPU result equals the
+    oracle's word for word."""
+    from gpu_cases import C4, c4_entries_mask, host_uniform, make, oracle_matvec
+    threads = min(16, os.cpu_count() or 1)
+    out = {}
+
+    def cpu_time(fn, nthreads, reps=1):
+        O.set_threads(nthreads)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        O.set_threads(threads)
+        return (time.perf_counter() - t0) / reps, r
+
+    # C1: NTT round trip, 4 limbs at N = 2^13
+    n1 = 1 << 13
+    q1 = O.ntt_primes(50, n1, 4)
+    ctx1 = ckh.Context(13, q1, O.ntt_primes(60, n1, 1), 1)
+    a = O.uniform_poly(q1, n1, 77)
+    one = [ctx1.upload(a, 3)]
+    batch = [ctx1.uniform(3, seed=200 + i) for i in range(256)]
+    ctx1.synchronize()
+    fwd1 = ctx1.time_ntt(one, forward=True, reps=50)
+    inv1 = ctx1.time_ntt(one, forward=False, reps=50)
+    fwdb = ctx1.time_ntt(batch, forward=True, reps=20)
+    invb = ctx1.time_ntt(batch, forward=False, reps=20)
+    p1 = ctx1.upload(a, 3, ntt=False)
+    f1 = p1.ntt()
+    rt = f1.intt().download()
+    want_f = O.ntt_forward(n1, q1, a.ravel().copy())
+    exact1 = bool((f1.download().ravel() == want_f.ravel()).all() and (rt == a).all())
+    r1 = O.Params(13, q=q1, p=O.ntt_primes(60, n1, 1), dnum=1)
+    assert (r1.ntt(r1.ntt(a, 4, True), 4, False) == a).all()
+    c1_1, _ = cpu_time(lambda: r1.ntt(r1.ntt(a, 4, True), 4, False), 1, reps=20)
+    c1_n, _ = cpu_time(lambda: r1.ntt(r1.ntt(a, 4, True), 4, False), threads, reps=20)
+    out["C1_ntt_round_trip_n8192_4limbs"] = {
+        "gpu_ms": fwd1 + inv1, "gpu_batch256_us_per_poly": (fwdb + invb) * 1e3 / 256,
+        "cpu_1_thread_ms": c1_1 * 1e3, "cpu_threads_ms": c1_n * 1e3, "bit_exact": exact1}
+    del ctx1, one, batch, p1, f1
+
+    # C2: one key switch at 24 limbs
+    q, p = ckh.params_primes(LOG_N, Q_BITS, P_BITS)
+    ctx = ckh.Context(LOG_N, q, p, DNUM)
+    ref = O.Params(LOG_N, q=q, p=p, dnum=DNUM)
+    key = ctx.ksk_uniform(23, seed=SEED + 1)
+    hkey = ref.uniform_key(23, SEED + 1)
+    d = O.uniform_poly(q, N, 4242)
+    dd = ctx.upload(d, 23)
+    ctx.synchronize()
+    lat = []
+    for i in range(60):
+        t0 = time.perf_counter()
+        ctx.key_switch(dd, key)
+        ctx.synchronize()
+        if i >= 10:
+            lat.append((time.perf_counter() - t0) * 1e3)
+    kb, ka = ctx.key_switch(dd, key)
+    c2_n, (wb, wa) = cpu_time(lambda: ref.key_switch(d.ravel(), 23, hkey, 23), threads)
+    c2_1, _ = cpu_time(lambda: ref.key_switch(d.ravel(), 23, hkey, 23), 1)
+    out["C2_key_switch_n65536_24limbs"] = {
+        "gpu_ms_p50": float(np.percentile(lat, 50)), "gpu_ms_p99": float(np.percentile(lat, 99)),
+        "cpu_1_thread_ms": c2_1 * 1e3, "cpu_threads_ms": c2_n * 1e3,
+        "bit_exact": bool((kb.download() == wb).all() and (ka.download() == wa).all())}
+    del ctx, key, dd, kb, ka, hkey
+
+    # C4: the hoisted BSGS product
+    ctx4, ref4 = make(C4)
+    L = len(ref4.q) - 1
+    baby = giant = 32
+    mask = c4_entries_mask(baby, giant)
+    steps = sorted({k2 for k2 in range(1, baby)} | {baby * k1 for k1 in range(1, giant)})
+    seeds = {st: 7000 + 10 * i for i, st in enumerate(steps)}
+    keys = [ctx4.ksk_uniform(L, seed=seeds[st], galois=ckh.galois_elt(ref4.log_n, st)) for st in steps]
+    e4 = ckh.Evaluator(ctx4, None, keys)
+    c0, c1 = host_uniform(ref4, L, 7001), host_uniform(ref4, L, 7002)
+    x = ctx4.ct_upload(np.stack([c0, c1]), L, 2.0 ** 40)
+    ents_h = [host_uniform(ref4, L, 8000 + k1 * baby + k2) if mask[k1][k2] else None
+              for k1 in range(giant) for k2 in range(baby)]
+    ents_d = [ctx4.upload(w, L) if w is not None else None for w in ents_h]
+    ctx4.synchronize()
+    e4.matvec_plain(x, ents_d, 2.0 ** 40, baby, giant)
+    ctx4.synchronize()
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        got = e4.matvec_plain(x, ents_d, 2.0 ** 40, baby, giant)
+        ctx4.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    cache = {}
+
+    def key_of_step(st):
+        if st not in cache:
+            cache[st] = ref4.uniform_key(L, seeds[st])
+        return cache[st]
+
+    c4_n, want = cpu_time(lambda: oracle_matvec(ref4, c0, c1, L, ents_h, baby, giant, 1, key_of_step, L), threads)
+    g = got.download()
+    out["C4_hoisted_bsgs_n32768_32x32"] = {
+        "gpu_ms": float(np.median(times)), "cpu_threads_ms": c4_n * 1e3, "cpu_1_thread_ms": None,
+        "rotations": len(steps), "bit_exact": bool((g[0] == want[0]).all() and (g[1] == want[1]).all()),
+        "note": "the 1-thread oracle run of this product (62 key switches at 21 limbs) is left out for time"}
+    out["cpu_threads"] = threads
+    return out
+
+
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "ntt_forward_240limbs_n65536.md")
 
 
@@ -272,6 +381,9 @@ def main():
                     help="run only the batched step (warmup + steps, no autotune) and print its time (for counter runs)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="with --batched-only: force a tuning value (repeatable)")
+    ap.add_argument("--configs", action="store_true",
+                    help="print the per-configuration figures (NTT round trip, one key switch, the hoisted "
+                         "BSGS product; GPU and CPU oracle) as one JSON line and exit")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the dominant-kernel launch and a device copy of the same size (for counter runs)")
     args = ap.parse_args()
@@ -283,6 +395,9 @@ def main():
     import ckh
     import oracle as O
 
+    if args.configs:
+        print(json.dumps(config_table(ckh, O, None)), flush=True)
+        return
     q, p = ckh.params_primes(LOG_N, Q_BITS, P_BITS)
     ctx = ckh.Context(LOG_N, q, p, DNUM)
     stream = ctx.stream_handle()

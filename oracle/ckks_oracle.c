@@ -802,6 +802,16 @@ void or_poly_mul(const or_params *pp, u64 *a, const u64 *b, u64 limbs) {
         for (u64 c = 0; c < pp->n; ++c) a[i * pp->n + c] = or_mod_mul(&pp->mq[i], a[i * pp->n + c], b[i * pp->n + c]);
 }
 
+/* RnsPoly::to_ntt / to_coeff (poly.rs:191-215) on Q limbs 0..limbs with the parameters' tables. */
+void or_params_ntt(const or_params *pp, u64 *a, u64 limbs, int forward) {
+    for (u64 i = 0; i < limbs; ++i) {
+        if (forward)
+            or_ntt_forward(&pp->tq[i], a + i * pp->n);
+        else
+            or_ntt_inverse(&pp->tq[i], a + i * pp->n);
+    }
+}
+
 /* RnsPoly::raise_from_level0 (poly.rs:298-345): limb 0 (level-0 input `a`, NTT form when ntt)
  * copied; limb i in 1..=top the coefficient x of limb 0 (inverse-transformed first) lifted to
  * x or -(q_0 - x) by x > (q_0 - 1) / 2, reduced mod q_i, transformed forward again. */

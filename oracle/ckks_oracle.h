@@ -145,6 +145,7 @@ void or_poly_add(const or_params *p, u64 *a, const u64 *b, u64 limbs);
 void or_poly_mul(const or_params *p, u64 *a, const u64 *b, u64 limbs);
 void or_poly_mul_scalar(const or_params *p, u64 *a, u64 limbs, u64 k, int neg);
 void or_raise_from_level0(const or_params *p, const u64 *a, u64 top, int ntt, u64 *out);
+void or_params_ntt(const or_params *p, u64 *a, u64 limbs, int forward);
 
 /* ---- seeded inputs (shared definition with the device generator) ---- */
 u64 or_uniform_word(u64 seed, u64 stream, u64 index, u64 q);

@@ -81,6 +81,7 @@ def lib():
         L.or_poly_mul.argtypes = [C.c_void_p, P64, P64, u64]
         L.or_poly_mul_scalar.argtypes = [C.c_void_p, P64, u64, u64, C.c_int]
         L.or_raise_from_level0.argtypes = [C.c_void_p, P64, u64, C.c_int, P64]
+        L.or_params_ntt.argtypes = [C.c_void_p, P64, u64, C.c_int]
         L.or_uniform_word.restype = u64
         L.or_uniform_word.argtypes = [u64, u64, u64, u64]
         L.or_fill_uniform.argtypes = [P64, u64, u64, u64, u64]
@@ -376,6 +377,12 @@ class Params:
         out = arr(a).copy()
         lib().or_poly_add(self.h, ptr(out), ptr(arr(b)), level + 1)
         return out.reshape(level + 1, self.n)
+
+    def ntt(self, a, limbs, forward=True):
+        """Q limbs 0..limbs transformed with the parameters' tables (a copy)."""
+        out = arr(a).copy()
+        lib().or_params_ntt(self.h, ptr(out), limbs, int(forward))
+        return out.reshape(limbs, self.n)
 
     def raise_from_level0(self, a, top, ntt=True):
         out = np.zeros((top + 1, self.n), dtype=np.uint64)

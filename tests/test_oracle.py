@@ -586,3 +586,11 @@ def test_raise_from_level0_is_the_centered_lift():
         want = O.ntt_forward(p.n, p.q[: top + 1], coeff.ravel().copy()).reshape(top + 1, p.n)
         assert (got[1:] == want[1:]).all()
         assert (p.raise_from_level0(a, 0, ntt=False) == a).all()
+
+
+def test_params_ntt_matches_the_standalone_transform():
+    p = O.Params(9, [50, 40, 60], [60], 1)
+    a = O.uniform_poly(p.q, p.n, 5)
+    f = p.ntt(a, 3, True)
+    assert (f.ravel() == O.ntt_forward(p.n, p.q, a.ravel().copy())).all()
+    assert (p.ntt(f, 3, False) == a).all()
