@@ -241,7 +241,8 @@ void check_switch(const Context& c, const Poly& d, const Ksk& key) {
 }
 
 // decompose + apply for a batch of requests, in one launch per stage.
-static void run_ks_on(Context& c, const std::vector<KsReq>& reqs, hipStream_t s) {
+static void run_ks_on(Context& c, const std::vector<KsReq>& reqs, hipStream_t s,
+                      const std::vector<AutJob>* pre = nullptr) {
     u64 total = 0;
     for (auto& r : reqs) total += ks_scratch_words(c, r.d->level, true);
     BufPtr scratch = c.alloc(total, s);
@@ -261,6 +262,7 @@ static void run_ks_on(Context& c, const std::vector<KsReq>& reqs, hipStream_t s)
         app.push_back({dig, r.d->level, 0, r.key, r.out_b, r.out_a, r.add_b, r.add_a, acc, y, z, r.d->data(), fuse});
     }
     Program prog;  // one program (one graph when captured) per batched switch
+    if (pre) prog.step(S_AUTOMORPH, prog.add(*pre), (u32)pre->size());
     c.prog_decompose(prog, dec);
     c.prog_apply(prog, app);
     c.execute(prog, s);
@@ -723,6 +725,7 @@ std::vector<CtPtr> Evaluator::hoisted_rotate(const Ct& ct, const std::vector<int
     BufPtr scratch = ctx.alloc(per * nontrivial);
     BufPtr out = ctx.alloc(2 * L * n * nontrivial);
     std::vector<ApplyTask> tasks;
+    std::vector<AutJob> aut;  // sigma_g(c0) of every step, in the same program
     u32 t = 0;
     for (size_t i = 0; i < steps.size(); ++i) {
         if (!keys[i].first) {
@@ -733,14 +736,17 @@ std::vector<CtPtr> Evaluator::hoisted_rotate(const Ct& ct, const std::vector<int
         u64* y = acc + 2 * (L + k) * n;
         u64* z = y + 2 * k * n;
         u64* r0 = z + 2 * L * n;
-        automorphism_ntt_launch(ct.comps[0]->data(), r0, level + 1, keys[i].first, ctx.log_n, s);
+        for (u32 li = 0; li < L; ++li) aut.push_back({ct.comps[0]->data() + li * n, r0 + li * n, keys[i].first});
         u64* ob = out->ptr + 2 * L * n * t;
         tasks.push_back({digs->ptr, level, keys[i].first, keys[i].second, ob, ob + L * n, r0, nullptr, acc, y, z});
         res[i] = make_ct({view(out, 2 * L * n * t, level, 0, true), view(out, 2 * L * n * t + L * n, level, 0, true)},
                          ct.scale);
         ++t;
     }
-    ctx.apply(tasks, s);
+    Program prog;
+    prog.step(S_AUTOMORPH, prog.add(aut), (u32)aut.size());
+    ctx.prog_apply(prog, tasks);
+    ctx.execute(prog, s);
     hip_check(hipGetLastError(), "hoisted rotate");
     count(&ckh_op_counter::rotations, nontrivial);
     return res;
@@ -767,20 +773,22 @@ std::vector<CtPtr> Evaluator::rotate_batch(const std::vector<std::pair<const Ct*
     std::vector<Poly> dpolys;
     dpolys.reserve(todo.size());
     u64 off = 0;
-    hipStream_t s = ctx.stream();
+    std::vector<AutJob> aut;  // sigma_g of both components, in the switch's program
     for (auto& [i, e, k] : todo) {
         const Ct& ct = *items[i].first;
         const u32 level = ct.level();
         const u64 Lw = (u64)(level + 1) * ctx.n;
         u64* d = buf->ptr + off;
-        automorphism_ntt_launch(ct.comps[1]->data(), d, level + 1, e, ctx.log_n, s);
-        automorphism_ntt_launch(ct.comps[0]->data(), d + Lw, level + 1, e, ctx.log_n, s);
+        for (u32 li = 0; li <= level; ++li) {
+            aut.push_back({ct.comps[1]->data() + li * ctx.n, d + li * ctx.n, e});
+            aut.push_back({ct.comps[0]->data() + li * ctx.n, d + Lw + li * ctx.n, e});
+        }
         dpolys.push_back(Poly{buf, off, level, 0, true});
         reqs.push_back({&dpolys.back(), k, d + Lw, nullptr, d + 2 * Lw, d + 3 * Lw});
         res[i] = make_ct({view(buf, off + 2 * Lw, level, 0, true), view(buf, off + 3 * Lw, level, 0, true)}, ct.scale);
         off += 4 * Lw;
     }
-    run_ks(ctx, reqs);
+    run_ks_on(ctx, reqs, ctx.stream(), &aut);
     count(&ckh_op_counter::rotations, todo.size());
     return res;
 }
@@ -815,11 +823,41 @@ void Evaluator::check_bsgs_shape(const Ct& x, u32 baby, u32 giant, u32 stride, i
     }
 }
 
-CtPtr Evaluator::bsgs(const std::vector<BsgsTerm>& terms, u32 baby, u32 giant, u32 stride, int32_t offset,
-                      const std::function<CtPtr(size_t, size_t, const Ct&)>& mul,
-                      const std::function<CtPtr(const CtPtr&)>& finish) {
-    // xs[t][k2] = rot(x_t, k2 stride) for the baby steps a present entry of term t uses (one
-    // hoisted call per term).
+// The sum of ciphertexts of one level, component count and scale (add applied in turn: the same
+// words), every limb of every component in one launch.
+CtPtr Evaluator::sum_cts(const std::vector<CtPtr>& cts) {
+    if (cts.size() == 1) return cts[0];
+    const Ct& c0 = *cts[0];
+    for (auto& c : cts) {
+        check_basis(*c);
+        check_level(c->level(), c0.level());
+        check_scale(c->scale, c0.scale);
+        if (c->comps.size() != c0.comps.size()) fail(CKH_ERR_INVALID_PARAMS, "sums need one component count");
+    }
+    const u32 level = c0.level();
+    const u64 L = level + 1, n = ctx.n, nc = c0.comps.size();
+    BufPtr out = ctx.alloc(nc * L * n);
+    std::vector<MulSumJob> jobs;
+    std::vector<const u64*> xs;
+    for (u64 comp = 0; comp < nc; ++comp)
+        for (u32 i = 0; i < L; ++i) {
+            jobs.push_back({out->ptr + (comp * L + i) * n, (u32)xs.size(), (u32)cts.size(), i, 0});
+            for (auto& c : cts) xs.push_back(c->comps[comp]->data() + i * n);
+        }
+    Program prog;
+    const size_t oj = prog.add(jobs), ox = prog.add(xs);
+    prog.step(S_ADD_SUM, oj, (u32)jobs.size(), ox);
+    ctx.execute(prog, ctx.stream());
+    hip_check(hipGetLastError(), "sum");
+    std::vector<PolyPtr> comps;
+    for (u64 comp = 0; comp < nc; ++comp) comps.push_back(view(out, comp * L * n, level, 0, true));
+    return make_ct(std::move(comps), c0.scale);
+}
+
+// xs[t][k2] = rot(x_t, k2 stride) for the baby steps a present entry of term t uses (one hoisted
+// call per term).
+std::vector<std::vector<CtPtr>> Evaluator::baby_rotations(const std::vector<BsgsTerm>& terms, u32 baby, u32 giant,
+                                                          u32 stride) {
     std::vector<std::vector<CtPtr>> xs(terms.size(), std::vector<CtPtr>(baby));
     for (size_t t = 0; t < terms.size(); ++t) {
         std::vector<int32_t> steps;
@@ -835,6 +873,69 @@ CtPtr Evaluator::bsgs(const std::vector<BsgsTerm>& terms, u32 baby, u32 giant, u
         std::vector<CtPtr> rot = hoisted_rotate(*terms[t].x, steps);
         for (size_t i = 0; i < used_k2.size(); ++i) xs[t][used_k2[i]] = rot[i];
     }
+    return xs;
+}
+
+// Plaintext BSGS products with shared giant rotations: every block sum
+// sum_t sum_k2 entry_{t,k1,k2} * rot(x_t, k2 stride), both components, in one launch of the
+// fused product-sum kernel (the same words as mul_plain and add, which they replace; counted as
+// one ct_pt_mults per product), then the blocks' giant rotations in one batched switch.
+CtPtr Evaluator::plain_bsgs(const std::vector<BsgsTerm>& terms, const std::vector<std::vector<const Poly*>>& entries,
+                            double pt_scale, u32 baby, u32 giant, u32 stride, int32_t offset) {
+    std::vector<std::vector<CtPtr>> xs = baby_rotations(terms, baby, giant, stride);
+    const Ct& x0 = *terms[0].x;
+    const u32 level = x0.level();
+    const u64 L = level + 1, n = ctx.n;
+    std::vector<u32> blocks;
+    u64 products = 0;
+    for (u32 k1 = 0; k1 < giant; ++k1) {
+        bool any = false;
+        for (size_t t = 0; t < terms.size(); ++t)
+            for (u32 k2 = 0; k2 < baby; ++k2)
+                if (terms[t].present[k1 * baby + k2]) {
+                    any = true;
+                    ++products;
+                }
+        if (any) blocks.push_back(k1);
+    }
+    BufPtr out = ctx.alloc(blocks.size() * 2 * L * n);
+    std::vector<MulSumJob> jobs;
+    std::vector<MulSumTerm> mterms;
+    for (size_t b = 0; b < blocks.size(); ++b) {
+        const u32 k1 = blocks[b];
+        for (u32 c = 0; c < 2; ++c)
+            for (u32 i = 0; i < L; ++i) {
+                MulSumJob J{out->ptr + ((2 * b + c) * L + i) * n, (u32)mterms.size(), 0, i, 0};
+                for (size_t t = 0; t < terms.size(); ++t)
+                    for (u32 k2 = 0; k2 < baby; ++k2) {
+                        const size_t e = (size_t)k1 * baby + k2;
+                        if (!terms[t].present[e]) continue;
+                        mterms.push_back({xs[t][k2]->comps[c]->data() + i * n, entries[t][e]->data() + i * n});
+                        ++J.count;
+                    }
+                jobs.push_back(J);
+            }
+    }
+    Program prog;
+    const size_t oj = prog.add(jobs), ot = prog.add(mterms);
+    prog.step(S_MUL_SUM, oj, (u32)jobs.size(), ot);
+    ctx.execute(prog, ctx.stream());
+    hip_check(hipGetLastError(), "bsgs block sums");
+    count(&ckh_op_counter::ct_pt_mults, products);
+    std::vector<CtPtr> sums;
+    std::vector<std::pair<const Ct*, int32_t>> to_rotate;
+    for (size_t b = 0; b < blocks.size(); ++b) {
+        sums.push_back(make_ct({view(out, 2 * b * L * n, level, 0, true), view(out, (2 * b + 1) * L * n, level, 0, true)},
+                               x0.scale * pt_scale));
+        to_rotate.push_back({sums.back().get(), (offset + (int32_t)(baby * blocks[b])) * (int32_t)stride});
+    }
+    return sum_cts(rotate_batch(to_rotate));
+}
+
+CtPtr Evaluator::bsgs(const std::vector<BsgsTerm>& terms, u32 baby, u32 giant, u32 stride, int32_t offset,
+                      const std::function<CtPtr(size_t, size_t, const Ct&)>& mul,
+                      const std::function<CtPtr(const CtPtr&)>& finish) {
+    std::vector<std::vector<CtPtr>> xs = baby_rotations(terms, baby, giant, stride);
     // Per block: with bsgs, the products of every term summed, finished once and rotated by the
     // block's giant shift (all blocks' giant rotations in one batched switch); naive, every
     // product finished and rotated by itself. Sums are exact, so their order does not change
@@ -903,6 +1004,7 @@ CtPtr Evaluator::matvec_plain_sum(const std::vector<const Ct*>& xs, const std::v
         terms.push_back(std::move(term));
     }
     return counted([&] {
+        if (config.bsgs) return plain_bsgs(terms, entries, pt_scale, baby, giant, stride, offset);
         return bsgs(
             terms, baby, giant, stride, offset,
             [&](size_t t, size_t i, const Ct& xr) { return mul_plain(xr, *entries[t][i], pt_scale); },
@@ -926,6 +1028,7 @@ CtPtr Evaluator::matvec_plain(const Ct& x, const std::vector<const Poly*>& entri
     }
     check_bsgs_shape(x, baby, giant, stride, offset, present);
     return counted([&] {
+        if (config.bsgs) return plain_bsgs({{&x, present}}, {entries}, pt_scale, baby, giant, stride, offset);
         return bsgs(
             {{&x, present}}, baby, giant, stride, offset,
             [&](size_t, size_t i, const Ct& xr) { return mul_plain(xr, *entries[i], pt_scale); },

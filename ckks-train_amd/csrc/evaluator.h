@@ -129,6 +129,11 @@ class Evaluator {
         const Ct* x;
         std::vector<bool> present;
     };
+    CtPtr sum_cts(const std::vector<CtPtr>& cts);
+    std::vector<std::vector<CtPtr>> baby_rotations(const std::vector<BsgsTerm>& terms, u32 baby, u32 giant,
+                                                   u32 stride);
+    CtPtr plain_bsgs(const std::vector<BsgsTerm>& terms, const std::vector<std::vector<const Poly*>>& entries,
+                     double pt_scale, u32 baby, u32 giant, u32 stride, int32_t offset);
     CtPtr bsgs(const std::vector<BsgsTerm>& terms, u32 baby, u32 giant, u32 stride, int32_t offset,
                const std::function<CtPtr(size_t, size_t, const Ct&)>& mul,
                const std::function<CtPtr(const CtPtr&)>& finish);

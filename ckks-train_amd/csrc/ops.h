@@ -9,6 +9,23 @@ namespace ckh {
 // at key + (2 j) * key_poly (b) and key + (2 j + 1) * key_poly (a). galois != 0 reads the digit
 // through the NTT-form automorphism permutation (hoisted rotations). When own is set, digit
 // own_j's limb (the digit's own, unconverted limb) is read from own instead.
+// out (one limb, prime `prime`) <- sum over the job's terms [first, first + count) of x * w mod q:
+// the BSGS block sums of plaintext products in one launch (MulSumTerm: two limb pointers).
+struct MulSumJob {
+    u64* out;
+    u32 first, count, prime, pad;
+};
+struct MulSumTerm {
+    const u64* x;
+    const u64* w;
+};
+void mul_sum_launch(const MulSumJob* jobs, const MulSumTerm* terms, u32 njobs, const PrimeConst* pc, u32 log_n,
+                    hipStream_t stream);
+
+// out (one limb, prime `prime`) <- sum of the limbs xs[first, first + count) mod q.
+void add_sum_launch(const MulSumJob* jobs, const u64* const* xs, u32 njobs, const PrimeConst* pc, u32 log_n,
+                    hipStream_t stream);
+
 struct IpJob {
     const u64* dig;
     u64 dig_stride;
@@ -40,6 +57,13 @@ void scalar_mul_launch(const u64* a, u64* out, u32 limbs, const u64* consts, boo
 
 // out limb i <- sigma_g(in limb i) in NTT form (index permutation, prime independent).
 void automorphism_ntt_launch(const u64* in, u64* out, u32 limbs, u64 g, u32 log_n, hipStream_t stream);
+// The same for many (limb, Galois element) jobs in one launch.
+struct AutJob {
+    const u64* in;
+    u64* out;
+    u64 galois;
+};
+void automorphism_batch_launch(const AutJob* jobs, u32 njobs, u32 log_n, hipStream_t stream);
 
 // Seeded uniform residues: limb i of the layout gets prime primes_map(i) and generator stream
 // stream0 + i; q_limbs limbs use Q primes 0.., the rest P primes (QpPoly layout).

@@ -107,6 +107,48 @@ void pointwise_launch(PwOp op, const u64* a, const u64* b, u64* out, u32 limbs, 
                        pc, log_n);
 }
 
+__global__ __launch_bounds__(256) void k_mul_sum(const MulSumJob* __restrict__ jobs,
+                                                 const MulSumTerm* __restrict__ terms,
+                                                 const PrimeConst* __restrict__ pc, u32 log_n) {
+    const u32 n = 1u << log_n;
+    const u32 c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const MulSumJob J = jobs[blockIdx.y];
+    const PrimeConst C = pc[J.prime];
+    u64 acc = 0;
+    for (u32 t = 0; t < J.count; ++t) {
+        const MulSumTerm T = terms[J.first + t];
+        acc = add_mod(acc, mul_mod(glb(T.x)[c], glb(T.w)[c], C), C.q);
+    }
+    glb(J.out)[c] = acc;
+}
+
+void mul_sum_launch(const MulSumJob* jobs, const MulSumTerm* terms, u32 njobs, const PrimeConst* pc, u32 log_n,
+                    hipStream_t stream) {
+    if (njobs == 0) return;
+    const u32 n = 1u << log_n;
+    hipLaunchKernelGGL(k_mul_sum, dim3((n + 255) / 256, njobs), dim3(256), 0, stream, jobs, terms, pc, log_n);
+}
+
+__global__ __launch_bounds__(256) void k_add_sum(const MulSumJob* __restrict__ jobs, const u64* const* __restrict__ xs,
+                                                 const PrimeConst* __restrict__ pc, u32 log_n) {
+    const u32 n = 1u << log_n;
+    const u32 c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const MulSumJob J = jobs[blockIdx.y];
+    const u64 q = pc[J.prime].q;
+    u64 acc = 0;
+    for (u32 t = 0; t < J.count; ++t) acc = add_mod(acc, glb(xs[J.first + t])[c], q);
+    glb(J.out)[c] = acc;
+}
+
+void add_sum_launch(const MulSumJob* jobs, const u64* const* xs, u32 njobs, const PrimeConst* pc, u32 log_n,
+                    hipStream_t stream) {
+    if (njobs == 0) return;
+    const u32 n = 1u << log_n;
+    hipLaunchKernelGGL(k_add_sum, dim3((n + 255) / 256, njobs), dim3(256), 0, stream, jobs, xs, pc, log_n);
+}
+
 __global__ __launch_bounds__(256) void k_scalar_mul(const u64* __restrict__ a, u64* __restrict__ out,
                                                     const u64* __restrict__ consts, u32 neg,
                                                     const PrimeConst* __restrict__ pc, u32 log_n) {
@@ -141,6 +183,20 @@ void automorphism_ntt_launch(const u64* in, u64* out, u32 limbs, u64 g, u32 log_
     if (limbs == 0) return;
     const u32 n = 1u << log_n;
     hipLaunchKernelGGL(k_automorphism_ntt, dim3((n + 255) / 256, limbs), dim3(256), 0, stream, in, out, g, log_n);
+}
+
+__global__ __launch_bounds__(256) void k_automorphism_batch(const AutJob* __restrict__ jobs, u32 log_n) {
+    const u32 n = 1u << log_n;
+    const u32 c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const AutJob J = jobs[blockIdx.y];
+    glb(J.out)[c] = glb(J.in)[galois_index(c, J.galois, log_n)];
+}
+
+void automorphism_batch_launch(const AutJob* jobs, u32 njobs, u32 log_n, hipStream_t stream) {
+    if (njobs == 0) return;
+    const u32 n = 1u << log_n;
+    hipLaunchKernelGGL(k_automorphism_batch, dim3((n + 255) / 256, njobs), dim3(256), 0, stream, jobs, log_n);
 }
 
 // Stateless seeded generator: word (seed, stream, index) is uniform below q by rejection on

@@ -636,14 +636,6 @@ static void launch_finish(const FinishJob* d, u32 njobs, const PrimeConst* pc, u
 // bytes to the device in one transfer and launches the steps. A Program's steps depend only on
 // the shapes of its tasks, so a captured launch sequence can be replayed with new bytes.
 
-template <class T>
-size_t Program::add(const std::vector<T>& v) {
-    size_t off = (bytes.size() + 15) & ~(size_t)15;
-    bytes.resize(off + v.size() * sizeof(T));
-    if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
-    return off;
-}
-
 void Program::step(StepKind k, size_t off, u32 count, size_t off2, u32 a, u32 b) {
     if (count) steps.push_back({k, count, off, off2, a, b});
 }
@@ -664,6 +656,20 @@ void Context::run(const Program& p, const uint8_t* d, hipStream_t s) const {
             case S_NTT_COLS: ntt_fast_forward_cols(ntt, (const LimbRef*)x, st.count, s); break;
             case S_NTT_ROWS: ntt_fast_forward_rows(ntt, (const LimbRef*)x, st.count, s); break;
             case S_ROWS_IP: rows_ip_launch(ntt, (const RowIpJob*)x, st.count, s); break;
+            case S_ADD_SUM:
+                for (u32 i = 0; i < st.count; i += MAX_ROWS)
+                    add_sum_launch((const MulSumJob*)x + i, (const u64* const*)(d + st.off2), std::min(MAX_ROWS, st.count - i),
+                                   d_pc, log_n, s);
+                break;
+            case S_AUTOMORPH:
+                for (u32 i = 0; i < st.count; i += MAX_ROWS)
+                    automorphism_batch_launch((const AutJob*)x + i, std::min(MAX_ROWS, st.count - i), log_n, s);
+                break;
+            case S_MUL_SUM:
+                for (u32 i = 0; i < st.count; i += MAX_ROWS)
+                    mul_sum_launch((const MulSumJob*)x + i, (const MulSumTerm*)(d + st.off2), std::min(MAX_ROWS, st.count - i),
+                                   d_pc, log_n, s);
+                break;
         }
     }
 }

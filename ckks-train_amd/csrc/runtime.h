@@ -3,6 +3,7 @@
 #pragma once
 
 #include <atomic>
+#include <cstring>
 #include <deque>
 #include <memory>
 #include <mutex>
@@ -135,7 +136,8 @@ struct RaiseTask {
 
 // Descriptor bytes plus the launch steps that read them (see runtime.cpp, "engine").
 enum StepKind : uint8_t {
-    S_NTT_FWD, S_NTT_INV, S_CONV, S_COPY, S_IP, S_NTT_FWD_FIN, S_FINISH, S_NTT_COLS, S_NTT_ROWS, S_ROWS_IP
+    S_NTT_FWD, S_NTT_INV, S_CONV, S_COPY, S_IP, S_NTT_FWD_FIN, S_FINISH, S_NTT_COLS, S_NTT_ROWS, S_ROWS_IP,
+    S_MUL_SUM, S_AUTOMORPH, S_ADD_SUM
 };
 struct Program {
     struct Step {
@@ -149,8 +151,14 @@ struct Program {
     };
     std::vector<uint8_t> bytes;
     std::vector<Step> steps;
+    // Appends v at a 16-byte aligned offset of the descriptor bytes and returns the offset.
     template <class T>
-    size_t add(const std::vector<T>& v);
+    size_t add(const std::vector<T>& v) {
+        const size_t off = (bytes.size() + 15) & ~(size_t)15;
+        bytes.resize(off + v.size() * sizeof(T));
+        if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+        return off;
+    }
     void step(StepKind k, size_t off, u32 count, size_t off2 = 0, u32 a = 0, u32 b = 0);
 };
 
