@@ -87,6 +87,31 @@ class BsgsShape(C.Structure):
     _fields_ = [("baby", u32), ("giant", u32), ("stride", u32), ("offset", C.c_int32)]
 
 
+class LinearStageC(C.Structure):
+    _fields_ = [("slot_to_coeff", C.c_int32), ("level", u32), ("scale", C.c_double), ("shape", BsgsShape),
+                ("diags", C.POINTER(VP)), ("pt_scale", C.c_double), ("second_shape", BsgsShape),
+                ("second", C.POINTER(VP)), ("second_pt_scale", C.c_double)]
+
+
+COEFF_TO_SLOT, SLOT_TO_COEFF = 0, 1
+
+
+class LinearStage:
+    """One stage of CoeffToSlot / SlotToCoeff (LinearTransform,
+    crates/ckks-train-ckks/src/bootstrap/linear.rs:467-502): `diags` (Poly or None, baby * giant
+    of them, pre-rotated) at `level` with scale `pt_scale`, in `shape` = (baby, giant, stride,
+    offset); `second` the diagonals of a two-input stage (None: one input); `scale` the exact
+    scale after the stage's rescale."""
+
+    def __init__(self, direction, level, scale, diags, pt_scale, shape, second=None, second_pt_scale=None,
+                 second_shape=None):
+        self.direction, self.level, self.scale = direction, level, scale
+        self.diags, self.pt_scale, self.shape = list(diags), pt_scale, tuple(shape)
+        self.second = None if second is None else list(second)
+        self.second_pt_scale = pt_scale if second_pt_scale is None else second_pt_scale
+        self.second_shape = self.shape if second_shape is None else tuple(second_shape)
+
+
 OP_RELINEARIZE, OP_HOISTED_ROTATE, OP_MUL_RELIN_RESCALE = 0, 1, 2
 
 # name: (restype, argtypes)
@@ -174,6 +199,8 @@ _SIGS = {
     "ckh_matvec_enc_batch": (C.c_int, [VP, u32, C.POINTER(VP), C.POINTER(C.POINTER(VP)), C.c_void_p,
                                        C.POINTER(VP)]),
     "ckh_matvec_enc_workspace_bytes": (C.c_int, [VP, u32, u32, u32, C.POINTER(u64)]),
+    "ckh_coeff_to_slot": (C.c_int, [VP, VP, u32, C.POINTER(LinearStageC), C.POINTER(VP), C.POINTER(VP)]),
+    "ckh_slot_to_coeff": (C.c_int, [VP, VP, VP, u32, C.POINTER(LinearStageC), C.POINTER(VP)]),
     "ckh_matvec_plain_sum": (C.c_int, [VP, u32, C.POINTER(VP), C.POINTER(C.POINTER(VP)), C.c_double, u32, u32,
                                        u32, C.c_int32, C.POINTER(VP)]),
     "ckh_set_pool_release_threshold": (C.c_int, [VP, u64]),
@@ -651,6 +678,38 @@ class Evaluator:
         rows = [(VP * len(e))(*[p.h if p is not None else None for p in e]) for e in entries]
         ea = (C.POINTER(VP) * max(n, 1))(*[C.cast(r, C.POINTER(VP)) for r in rows])
         return self._op(lib().ckh_matvec_plain_sum, n, xa, ea, pt_scale, baby, giant, stride, offset)
+
+    @staticmethod
+    def _stages(stages):
+        """The stages as C structs, and the arrays they point into (kept alive by the caller)."""
+        keep = []
+        arr = (LinearStageC * max(len(stages), 1))()
+        for i, st in enumerate(stages):
+            d = (VP * max(len(st.diags), 1))(*[p.h if p is not None else None for p in st.diags])
+            keep.append(d)
+            sec = None
+            if st.second is not None:
+                sec = (VP * max(len(st.second), 1))(*[p.h if p is not None else None for p in st.second])
+                keep.append(sec)
+            arr[i] = LinearStageC(st.direction, st.level, st.scale, BsgsShape(*st.shape), C.cast(d, C.POINTER(VP)),
+                                  st.pt_scale, BsgsShape(*st.second_shape),
+                                  C.cast(sec, C.POINTER(VP)) if sec is not None else None, st.second_pt_scale)
+        return arr, keep
+
+    def coeff_to_slot(self, ct, stages):
+        """Evaluator::coeff_to_slot (linear.rs:776-806): (re, im)."""
+        arr, keep = self._stages(stages)
+        re, im = VP(), VP()
+        _check(lib().ckh_coeff_to_slot(self.h, ct.h, len(stages), arr, C.byref(re), C.byref(im)))
+        del keep
+        return Ct(self.ctx, re), Ct(self.ctx, im)
+
+    def slot_to_coeff(self, re, im, stages):
+        """Evaluator::slot_to_coeff (linear.rs:824-859)."""
+        arr, keep = self._stages(stages)
+        out = self._op(lib().ckh_slot_to_coeff, re.h, im.h, len(stages), arr)
+        del keep
+        return out
 
     def matvec_enc_batch(self, xs, entries, shapes):
         """jobs (x, entries, (baby, giant, stride, offset)); results in job order."""

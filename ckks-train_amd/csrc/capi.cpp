@@ -704,6 +704,50 @@ ckh_status ckh_matvec_plain_sum(ckh_evaluator* ev, uint32_t n_terms, const ckh_c
     });
 }
 
+static std::vector<LinearStage> linear_stages(uint32_t n, const ckh_linear_stage* st) {
+    if (n) need(st, "stages");
+    std::vector<LinearStage> out;
+    for (uint32_t k = 0; k < n; ++k) {
+        const ckh_linear_stage& s = st[k];
+        LinearStage lt;
+        lt.slot_to_coeff = s.slot_to_coeff != 0;
+        lt.level = s.level;
+        lt.scale = s.scale;
+        lt.shape = {s.shape.baby, s.shape.giant, s.shape.stride, s.shape.offset};
+        lt.pt_scale = s.pt_scale;
+        need(s.diags, "diags");
+        for (uint64_t j = 0; j < (uint64_t)s.shape.baby * s.shape.giant; ++j)
+            lt.diags.push_back(s.diags[j] ? &poly_of(s.diags[j]) : nullptr);
+        lt.has_second = s.second != nullptr;
+        lt.second_shape = {s.second_shape.baby, s.second_shape.giant, s.second_shape.stride, s.second_shape.offset};
+        lt.second_pt_scale = s.second_pt_scale;
+        if (lt.has_second)
+            for (uint64_t j = 0; j < (uint64_t)s.second_shape.baby * s.second_shape.giant; ++j)
+                lt.second.push_back(s.second[j] ? &poly_of(s.second[j]) : nullptr);
+        out.push_back(std::move(lt));
+    }
+    return out;
+}
+
+ckh_status ckh_coeff_to_slot(ckh_evaluator* ev, const ckh_ct* ct, uint32_t n_stages, const ckh_linear_stage* stages,
+                             ckh_ct** re, ckh_ct** im) {
+    return guarded([&] {
+        need(re, "re");
+        need(im, "im");
+        auto r = ev_of(ev).coeff_to_slot(ct_of(ct), linear_stages(n_stages, stages));
+        *re = wrap(r.first);
+        *im = wrap(r.second);
+    });
+}
+
+ckh_status ckh_slot_to_coeff(ckh_evaluator* ev, const ckh_ct* re, const ckh_ct* im, uint32_t n_stages,
+                             const ckh_linear_stage* stages, ckh_ct** out) {
+    return guarded([&] {
+        need(out, "out");
+        *out = wrap(ev_of(ev).slot_to_coeff(ct_of(re), ct_of(im), linear_stages(n_stages, stages)));
+    });
+}
+
 ckh_status ckh_matvec_enc_batch(ckh_evaluator* ev, uint32_t n, const ckh_ct* const* xs,
                                 const ckh_ct* const* const* entries, const ckh_bsgs_shape* shapes, ckh_ct** outs) {
     return guarded([&] {

@@ -800,7 +800,7 @@ std::vector<CtPtr> Evaluator::rotate_batch(const std::vector<std::pair<const Ct*
 // check_bsgs and check_stride / check_offset (linalg.rs:523-541,677-684): d * stride divides the
 // slot count, -d < offset <= 0, and every rotation the product will make has its key (the baby
 // steps of the present entries and the nonzero giant shifts of the blocks that hold one).
-void Evaluator::check_bsgs_shape(const Ct& x, u32 baby, u32 giant, u32 stride, int32_t offset,
+void Evaluator::check_bsgs_shape(u32 level, u32 baby, u32 giant, u32 stride, int32_t offset,
                                  const std::vector<bool>& present) const {
     if (baby == 0 || giant == 0 || present.size() != (size_t)baby * giant)
         fail(CKH_ERR_INVALID_PARAMS, "entries must hold baby * giant diagonals");
@@ -816,10 +816,10 @@ void Evaluator::check_bsgs_shape(const Ct& x, u32 baby, u32 giant, u32 stride, i
             if (present[k1 * baby + k2]) used[k2] = block[k1] = any = true;
     if (!any) fail(CKH_ERR_INVALID_PARAMS, "every diagonal entry is zero");
     for (u32 k2 = 1; k2 < baby; ++k2)
-        if (used[k2] && rotation_elt((int32_t)(k2 * stride))) galois_key(rotation_elt((int32_t)(k2 * stride)), x.level());
+        if (used[k2] && rotation_elt((int32_t)(k2 * stride))) galois_key(rotation_elt((int32_t)(k2 * stride)), level);
     for (u32 k1 = 0; k1 < giant; ++k1) {
         const int32_t shift = (offset + (int32_t)(baby * k1)) * (int32_t)stride;
-        if (block[k1] && rotation_elt(shift)) galois_key(rotation_elt(shift), x.level());
+        if (block[k1] && rotation_elt(shift)) galois_key(rotation_elt(shift), level);
     }
 }
 
@@ -1000,7 +1000,7 @@ CtPtr Evaluator::matvec_plain_sum(const std::vector<const Ct*>& xs, const std::v
         check_level(x.level(), xs[0]->level());
         if (x.scale != xs[0]->scale) fail(CKH_ERR_SCALE_MISMATCH, "terms differ in scale");
         std::vector<bool> all(entries[t].size(), true);
-        check_bsgs_shape(x, baby, giant, stride, offset, all);
+        check_bsgs_shape(x.level(), baby, giant, stride, offset, all);
         terms.push_back(std::move(term));
     }
     return counted([&] {
@@ -1026,7 +1026,7 @@ CtPtr Evaluator::matvec_plain(const Ct& x, const std::vector<const Poly*>& entri
         check_level(x.level(), entries[i]->level);
         present[i] = true;
     }
-    check_bsgs_shape(x, baby, giant, stride, offset, present);
+    check_bsgs_shape(x.level(), baby, giant, stride, offset, present);
     return counted([&] {
         if (config.bsgs) return plain_bsgs({{&x, present}}, {entries}, pt_scale, baby, giant, stride, offset);
         return bsgs(
@@ -1060,7 +1060,7 @@ std::vector<bool> Evaluator::check_matvec_enc(const Ct& x, const std::vector<con
             if (e && e->scale != first->scale) fail(CKH_ERR_SCALE_MISMATCH, "diagonal entries differ in scale");
         check_level(x.level(), first->level());
     }
-    check_bsgs_shape(x, baby, giant, stride, offset, present);
+    check_bsgs_shape(x.level(), baby, giant, stride, offset, present);
     if (x.level() == 0) fail(CKH_ERR_NO_LEVELS_LEFT, "no levels left");
     relin_key_checked(x.level());
     return present;
@@ -1255,6 +1255,134 @@ CtPtr Evaluator::mul_plain_to_scale(const Ct& a, const Poly& pt, double pt_scale
     CtPtr r = make_ct(std::move(out), label);
     count(&ckh_op_counter::ct_pt_mults);
     return r;
+}
+
+// ------------------------------------------------------------------ CoeffToSlot / SlotToCoeff
+
+// Evaluator::check_stages (linear.rs:861-898), before any work: stage k at level - k, every
+// diagonal scale exactly the one that takes the planned input scale (scale, then each stage's
+// scale) to the stage's pre-rescale label, a second diagonal set of the same layout and scale,
+// and every rotation key (the shape checks of the BSGS product are made here too, so that no
+// stage fails after an earlier one ran).
+void Evaluator::check_stages(const std::vector<LinearStage>& stages, u32 level, double scale) const {
+    if (stages.empty()) fail(CKH_ERR_INVALID_PARAMS, "no stages");
+    double s_in = scale;
+    for (size_t k = 0; k < stages.size(); ++k) {
+        const LinearStage& lt = stages[k];
+        if ((u64)lt.level + k != level) fail(CKH_ERR_LEVEL_MISMATCH, "stage levels must descend by one from the input's");
+        const double label = exact_product_scale(lt.level, lt.scale);
+        bool any = false;
+        for (const Poly* w : lt.diags) any = any || w;
+        if (!any) fail(CKH_ERR_INVALID_PARAMS, "a stage without diagonals");
+        if (lt.pt_scale != label / s_in) fail(CKH_ERR_SCALE_MISMATCH, "diagonal scale does not give the stage scale");
+        const BsgsShape& a = lt.shape;
+        if (lt.has_second) {
+            const BsgsShape& b = lt.second_shape;
+            if (b.baby != a.baby || b.giant != a.giant || b.stride != a.stride || b.offset != a.offset)
+                fail(CKH_ERR_INVALID_PARAMS, "second diagonals of a different layout");
+            bool any2 = false;
+            for (const Poly* w : lt.second) any2 = any2 || w;
+            if (!any2 || lt.second_pt_scale != lt.pt_scale)
+                fail(CKH_ERR_SCALE_MISMATCH, "second diagonals of a different scale");
+        }
+        const std::vector<bool> all((size_t)a.baby * a.giant, true);
+        check_bsgs_shape(lt.level, a.baby, a.giant, a.stride, a.offset, all);
+        if (lt.diags.size() != all.size()) fail(CKH_ERR_INVALID_PARAMS, "entries must hold baby * giant diagonals");
+        if (lt.has_second && lt.second.size() != all.size())
+            fail(CKH_ERR_INVALID_PARAMS, "entries must hold baby * giant diagonals");
+        for (const auto* set : {&lt.diags, &lt.second})
+            for (const Poly* w : *set) {
+                if (!w) continue;
+                check_plain(ctx, *w);
+                check_level(lt.level, w->level);
+            }
+        s_in = lt.scale;
+    }
+}
+
+// Evaluator::apply_stage (linear.rs:900-919): the BSGS sum of diags * x (and second * x2),
+// labelled with the exact pre-rescale scale (within 1e-15 of the f64 product of the scales) and
+// rescaled to the stage's scale.
+CtPtr Evaluator::apply_stage(const LinearStage& lt, const Ct& x, const Ct* x2) {
+    std::vector<const Ct*> xs{&x};
+    std::vector<std::vector<const Poly*>> entries{lt.diags};
+    if (lt.has_second && x2) {
+        xs.push_back(x2);
+        entries.push_back(lt.second);
+    }
+    const BsgsShape& a = lt.shape;
+    CtPtr y = matvec_plain_sum(xs, entries, lt.pt_scale, a.baby, a.giant, a.stride, a.offset);
+    const double label = exact_product_scale(lt.level, lt.scale);
+    if (std::fabs((y->scale - label) / label) > 1e-15) fail(CKH_ERR_SCALE_MISMATCH, "stage product scale is not its label");
+    return rescale(*relabel(*y, label));
+}
+
+// Evaluator::mul_by_x_half (linear.rs:921-945): every component times X^(N/2) (NTT form,
+// pointwise), which multiplies slot j by i^(3^j). No key, no level, not counted.
+CtPtr Evaluator::mul_by_x_half(const Ct& ct) {
+    check_basis(ct);
+    std::call_once(mono_once_, [&] {
+        std::vector<u64> host((u64)ctx.nq * ctx.n, 0);
+        for (u32 i = 0; i < ctx.nq; ++i) host[(u64)i * ctx.n + ctx.n / 2] = 1;
+        PolyPtr coeff = poly_upload(ctx, host.data(), ctx.nq - 1, 0, false);
+        PolyPtr m = poly_transform(ctx, *coeff, true);
+        hip_check(hipStreamSynchronize(ctx.stream()), "monomial");  // used from any stream afterwards
+        mono_ = m;
+    });
+    std::vector<PolyPtr> out;
+    for (auto& p : ct.comps) {
+        PolyPtr mono = view(mono_->buf, mono_->off, p->level, 0, true);
+        out.push_back(poly_pointwise(ctx, PW_MUL, *p, mono.get()));
+    }
+    return make_ct(std::move(out), ct.scale);
+}
+
+// Evaluator::coeff_to_slot (linear.rs:776-806): the stages in order (the last also applied to
+// the conjugate of its input), then re = x + conj(x), im = (conj(x) - x) X^(N/2).
+std::pair<CtPtr, CtPtr> Evaluator::coeff_to_slot(const Ct& ct, const std::vector<LinearStage>& stages) {
+    check_basis(ct);
+    check_galois_input(ct);
+    if (stages.empty()) fail(CKH_ERR_INVALID_PARAMS, "no stages");
+    for (auto& lt : stages)
+        if (lt.slot_to_coeff) fail(CKH_ERR_INVALID_PARAMS, "coeff_to_slot: a stage of the SlotToCoeff direction");
+    bool order_ok = stages.back().has_second;
+    for (size_t k = 0; k + 1 < stages.size(); ++k) order_ok = order_ok && !stages[k].has_second;
+    if (!order_ok) fail(CKH_ERR_INVALID_PARAMS, "coeff_to_slot: only the last stage (and it must) takes the conjugate");
+    check_stages(stages, ct.level(), ct.scale);
+    galois_key(2 * ctx.n - 1, stages.back().level);  // the conjugation key, at the highest level it is used
+    return counted([&] {
+        CtPtr x = make_ct(ct.comps, ct.scale);
+        for (auto& lt : stages) {
+            CtPtr xc = lt.has_second ? conjugate(*x) : nullptr;
+            x = apply_stage(lt, *x, xc.get());
+        }
+        CtPtr xc = conjugate(*x);
+        CtPtr re = add(*x, *xc, false);
+        CtPtr im = mul_by_x_half(*add(*xc, *x, true));
+        return std::make_pair(re, im);
+    });
+}
+
+// Evaluator::slot_to_coeff (linear.rs:824-859): the first stage takes re and im, the others the
+// previous stage's output.
+CtPtr Evaluator::slot_to_coeff(const Ct& re, const Ct& im, const std::vector<LinearStage>& stages) {
+    check_basis(re);
+    check_galois_input(re);
+    check_basis(im);
+    check_galois_input(im);
+    for (auto& lt : stages)
+        if (!lt.slot_to_coeff) fail(CKH_ERR_INVALID_PARAMS, "slot_to_coeff: a stage of the CoeffToSlot direction");
+    bool order_ok = !stages.empty() && stages[0].has_second;
+    for (size_t k = 1; k < stages.size(); ++k) order_ok = order_ok && !stages[k].has_second;
+    if (!order_ok) fail(CKH_ERR_INVALID_PARAMS, "slot_to_coeff: only the first stage (and it must) takes two inputs");
+    if (re.level() != im.level()) fail(CKH_ERR_LEVEL_MISMATCH, "re and im at different levels");
+    if (re.scale != im.scale) fail(CKH_ERR_SCALE_MISMATCH, "re and im of different scales");
+    check_stages(stages, re.level(), re.scale);
+    return counted([&] {
+        CtPtr x = apply_stage(stages[0], re, &im);
+        for (size_t k = 1; k < stages.size(); ++k) x = apply_stage(stages[k], *x, nullptr);
+        return x;
+    });
 }
 
 }  // namespace ckh

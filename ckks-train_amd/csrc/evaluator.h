@@ -51,6 +51,23 @@ struct BsgsShape {
     int32_t offset;
 };
 
+// One stage of CoeffToSlot or SlotToCoeff (LinearTransform,
+// crates/ckks-train-ckks/src/bootstrap/linear.rs:467-502): plaintext diagonals at the stage's
+// level in a BSGS shape (null entries are zero and skipped), the diagonals of a two-input stage
+// (CoeffToSlot's last, SlotToCoeff's first), and the exact scale after the stage's rescale.
+struct LinearStage {
+    bool slot_to_coeff;  // the direction of the stage list it was built for
+    u32 level;           // level of the stage input and of its diagonals
+    double scale;        // exact scale after the stage's rescale (at level - 1)
+    BsgsShape shape;
+    std::vector<const Poly*> diags;  // baby * giant entries
+    double pt_scale;                 // the common scale of the diagonals
+    bool has_second;
+    BsgsShape second_shape;
+    std::vector<const Poly*> second;
+    double second_pt_scale;
+};
+
 class Evaluator {
    public:
     Evaluator(Context& c, KskPtr relin, std::vector<KskPtr> galois);
@@ -99,6 +116,9 @@ class Evaluator {
     CtPtr mod_raise_main_key(const Ct& ct);
     CtPtr mul_lazy(const Ct& a, const Ct& b);
     CtPtr relin_lazy(const Ct& sum);
+    // CoeffToSlot / SlotToCoeff (linear.rs:776-859) over the BSGS products.
+    std::pair<CtPtr, CtPtr> coeff_to_slot(const Ct& ct, const std::vector<LinearStage>& stages);
+    CtPtr slot_to_coeff(const Ct& re, const Ct& im, const std::vector<LinearStage>& stages);
 
    private:
     std::mutex mu_;
@@ -120,7 +140,7 @@ class Evaluator {
     CtPtr matvec_enc_run(const Ct& x, const std::vector<const Ct*>& entries, u32 baby, u32 giant, u32 stride,
                          int32_t offset, const std::vector<bool>& present);
     // The BSGS shape checks shared by both products, before any work.
-    void check_bsgs_shape(const Ct& x, u32 baby, u32 giant, u32 stride, int32_t offset,
+    void check_bsgs_shape(u32 level, u32 baby, u32 giant, u32 stride, int32_t offset,
                           const std::vector<bool>& present) const;
     // The BSGS driver (linalg.rs:800-888): for every term t, the present[k1 * baby + k2]
     // entries are multiplied by rot(x_t, k2 stride) with mul(t, index, rotated x_t) and each
@@ -130,6 +150,11 @@ class Evaluator {
         std::vector<bool> present;
     };
     CtPtr sum_cts(const std::vector<CtPtr>& cts);
+    void check_stages(const std::vector<LinearStage>& stages, u32 level, double scale) const;
+    CtPtr apply_stage(const LinearStage& lt, const Ct& x, const Ct* x2);
+    CtPtr mul_by_x_half(const Ct& ct);
+    std::once_flag mono_once_;
+    PolyPtr mono_;  // X^(N/2) in NTT form over every Q prime, built at first use
     std::vector<std::vector<CtPtr>> baby_rotations(const std::vector<BsgsTerm>& terms, u32 baby, u32 giant,
                                                    u32 stride);
     CtPtr plain_bsgs(const std::vector<BsgsTerm>& terms, const std::vector<std::vector<const Poly*>>& entries,

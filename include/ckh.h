@@ -306,6 +306,33 @@ typedef struct ckh_bsgs_shape {
     uint32_t baby, giant, stride;
     int32_t offset;
 } ckh_bsgs_shape;
+/* One stage of CoeffToSlot / SlotToCoeff (LinearTransform, crates/ckks-train-ckks/src/bootstrap/
+ * linear.rs:467-502): baby * giant plaintext diagonals at `level` (NULL = zero), pre-rotated as
+ * for ckh_matvec_plain, with the common scale pt_scale; `second` (NULL: a one-input stage) the
+ * diagonals applied to the second input, in their own shape and scale, which must equal the
+ * first's; `scale` the exact ciphertext scale after the stage's rescale (to level - 1). */
+typedef struct ckh_linear_stage {
+    int32_t slot_to_coeff; /* Direction: 0 CoeffToSlot, 1 SlotToCoeff */
+    uint32_t level;
+    double scale;
+    ckh_bsgs_shape shape;
+    const ckh_poly *const *diags;
+    double pt_scale;
+    ckh_bsgs_shape second_shape;
+    const ckh_poly *const *second;
+    double second_pt_scale;
+} ckh_linear_stage;
+/* Evaluator::coeff_to_slot (linear.rs:776-806): the stages in order, the last one (and only it)
+ * also applied to the conjugate of its input; then re = x + conj(x), im = (conj(x) - x) X^(N/2).
+ * Every check (directions, stage order, levels descending from ct's, diagonal scales against
+ * each stage's exact pre-rescale label, layouts, rotation keys, the conjugation key) is made
+ * before any work. */
+ckh_status ckh_coeff_to_slot(ckh_evaluator *ev, const ckh_ct *ct, uint32_t n_stages, const ckh_linear_stage *stages,
+                             ckh_ct **re, ckh_ct **im);
+/* Evaluator::slot_to_coeff (linear.rs:824-859): the first stage (and only it) takes re and im,
+ * the others the previous output; re and im at one level with exactly one scale. */
+ckh_status ckh_slot_to_coeff(ckh_evaluator *ev, const ckh_ct *re, const ckh_ct *im, uint32_t n_stages,
+                             const ckh_linear_stage *stages, ckh_ct **out);
 ckh_status ckh_matvec_enc_batch(ckh_evaluator *ev, uint32_t n, const ckh_ct *const *xs,
                                 const ckh_ct *const *const *entries, const ckh_bsgs_shape *shapes,
                                 ckh_ct **outs);

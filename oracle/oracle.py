@@ -405,6 +405,28 @@ class Params:
         lib().or_poly_mul(self.h, ptr(out), ptr(arr(b)), level + 1)
         return out.reshape(level + 1, self.n)
 
+    def poly_sub(self, a, b, level):
+        """(a - b) mod q_i per limb (Q limbs 0..=level)."""
+        a = arr(a).reshape(level + 1, self.n)
+        b = arr(b).reshape(level + 1, self.n)
+        q = np.array(self.q[: level + 1], dtype=np.uint64)[:, None]
+        return np.where(a >= b, a - b, a + (q - b))
+
+    def mul_by_x_half(self, a, level):
+        """X^(N/2) a(X) mod (X^N + 1) for an NTT-form polynomial over Q limbs 0..=level
+        (Evaluator::mul_by_x_half, crates/ckks-train-ckks/src/bootstrap/linear.rs:921-945, which
+        multiplies by the NTT of the monomial), restated from the ring's definition: back to
+        coefficients, coefficient i moves to i + N/2 and the upper half wraps around negated,
+        then forward again."""
+        c = self.ntt(a, level + 1, forward=False)
+        h = self.n // 2
+        q = np.array(self.q[: level + 1], dtype=np.uint64)[:, None]
+        out = np.empty_like(c)
+        out[:, h:] = c[:, :h]
+        up = c[:, h:]
+        out[:, :h] = np.where(up == 0, up, q - up)
+        return self.ntt(out, level + 1, forward=True)
+
     def key_shape(self, key_max_level):
         """(parts, limbs per key polynomial) of a key serving levels 0..=key_max_level."""
         return len(self.digit_ranges(key_max_level)), key_max_level + 1 + self.k

@@ -182,3 +182,52 @@ def galois_elt_host(n, step):
 def c4_entries_mask(baby=32, giant=32):
     """The C4 product: in giant block k1 the diagonals at baby steps k1 and k1 + 16 (mod 32)."""
     return [[(k2 == k1 % baby) or (k2 == (k1 + baby // 2) % baby) for k2 in range(baby)] for k1 in range(giant)]
+
+
+def oracle_linear_stage(ref, xs, level, entries, shape, key_of_step, kml):
+    """apply_stage (linear.rs:900-919) from oracle operations: the BSGS sum of the terms
+    (xs[t] = (c0, c1) with entries[t]), then the rescale of both components."""
+    baby, giant, stride, offset = shape
+    y = oracle_matvec_sum(ref, xs, level, entries, baby, giant, stride, offset, True, key_of_step, kml)
+    r = ref.rescale_ct(np.stack([np.asarray(c).reshape(level + 1, ref.n) for c in y]), level)
+    return r[0], r[1]
+
+
+def oracle_coeff_to_slot(ref, c0, c1, stages, key_of_step, conj_key, kml):
+    """coeff_to_slot (linear.rs:776-806) from oracle operations; stages are (level, entries,
+    second entries or None, shape)."""
+    conj = 2 * ref.n - 1
+    x = (c0, c1)
+    for level, ents, sec, shape in stages:
+        xs, es = [x], [ents]
+        if sec is not None:
+            xs.append(ref.apply_galois(x[0], x[1], level, conj, conj_key, kml))
+            es.append(sec)
+        x = oracle_linear_stage(ref, xs, level, es, shape, key_of_step, kml)
+    level = stages[-1][0] - 1
+    xc = ref.apply_galois(x[0], x[1], level, conj, conj_key, kml)
+    re = tuple(ref.poly_add(a, b, level) for a, b in zip(x, xc))
+    im = tuple(ref.mul_by_x_half(ref.poly_sub(b, a, level), level) for a, b in zip(x, xc))
+    return re, im
+
+
+def oracle_slot_to_coeff(ref, re, im, stages, key_of_step, kml):
+    """slot_to_coeff (linear.rs:824-859) from oracle operations."""
+    level, ents, sec, shape = stages[0]
+    x = oracle_linear_stage(ref, [re, im], level, [ents, sec], shape, key_of_step, kml)
+    for level, ents, _, shape in stages[1:]:
+        x = oracle_linear_stage(ref, [x], level, [ents], shape, key_of_step, kml)
+    return x
+
+
+def bsgs_rotation_count(masks, baby, giant, stride, offset, slots):
+    """Rotations of one BSGS sum (linalg.rs:695-790 docs): per term, a baby rotation for each
+    nonzero baby step a present entry of that term uses; a giant rotation for each block with a
+    present entry of any term and a nonzero shift."""
+    n = 0
+    for m in masks:
+        n += sum(1 for k2 in range(1, baby)
+                 if (k2 * stride) % slots and any(m[k1 * baby + k2] for k1 in range(giant)))
+    n += sum(1 for k1 in range(giant)
+             if ((offset + baby * k1) * stride) % slots and any(m[k1 * baby + k2] for m in masks for k2 in range(baby)))
+    return n

@@ -594,3 +594,36 @@ def test_params_ntt_matches_the_standalone_transform():
     f = p.ntt(a, 3, True)
     assert (f.ravel() == O.ntt_forward(p.n, p.q, a.ravel().copy())).all()
     assert (p.ntt(f, 3, False) == a).all()
+
+
+# ---------------- bootstrap/linear.rs:921-945 (mul_by_x_half) ----------------
+
+def test_mul_by_x_half_is_the_monomial_product():
+    """X^(N/2) a: equal to the schoolbook negacyclic product with the monomial (coefficient
+    form), to the NTT-form product with NTT(X^(N/2)) (the reference's own method), applying it
+    twice negates (X^N = -1), and every NTT slot is multiplied by a square root of -1."""
+    p = O.Params(5, [50, 40], [60], 1)
+    level = 1
+    a_coef = O.uniform_poly(p.q, p.n, 31)
+    a = p.ntt(a_coef, 2, True)
+    got = p.mul_by_x_half(a, level)
+    mono = np.zeros((2, p.n), dtype=np.uint64)
+    mono[:, p.n // 2] = 1
+    for i, q in enumerate(p.q):
+        want = negacyclic([int(x) for x in a_coef[i]], [int(x) for x in mono[i]], q)
+        assert [int(x) for x in p.ntt(got, 2, False)[i]] == want
+    assert (got == p.poly_mul(a, p.ntt(mono, 2, True), level)).all()
+    neg = p.poly_sub(np.zeros_like(a), a, level)
+    assert (p.mul_by_x_half(got, level) == neg).all()
+    for i, q in enumerate(p.q):
+        for x, y in zip(a[i], got[i]):
+            r = int(y) * pow(int(x), -1, q) % q
+            assert r * r % q == q - 1
+
+
+def test_poly_sub_wraps():
+    p = O.Params(4, [30, 30], [40], 1)
+    a = np.zeros((2, p.n), dtype=np.uint64)
+    b = np.ones((2, p.n), dtype=np.uint64)
+    assert (p.poly_sub(a, b, 1) == np.array(p.q, dtype=np.uint64)[:, None] - 1).all()
+    assert (p.poly_sub(b, a, 1) == 1).all()
