@@ -90,14 +90,14 @@ std::string Context::autotune() {
     for (u32 i = 0; i < 12; ++i) ds.push_back(poly_uniform(*this, top - (i * top) / 12, 0, true, 0x7eed + i, 0));
     for (auto& p : ds) draw.push_back(p.get());
     os << "}, \"ks_candidates_ms\": " << autotune_ks(draw, *key);
-    os << ", \"ntt_chunk_fwd\": " << arg_f << ", \"ntt_chunk_inv\": " << arg_i << ", \"conv_variant\": " << conv_variant
+    os << ", \"ntt_chunk_fwd\": " << ntt_chunk_fwd << ", \"ntt_chunk_inv\": " << ntt_chunk_inv << ", \"conv_variant\": " << conv_variant
        << ", \"fuse_ip\": " << fuse_ip << "}";
     return os.str();
 }
 
 // Times every (conversion variant, targets per workgroup, row-pass fusion) choice on the batched key switch of `ds`
-// under `key` (direct launches, one warm-up and three timed runs each), keeps the fastest, and
-// returns the timings as a JSON object.
+// under `key` (direct launches, one warm-up and three timed runs each), keeps the fastest, then
+// the inverse-NTT chunk against none, and returns the timings as a JSON object.
 std::string Context::autotune_ks(const std::vector<const Poly*>& ds, const Ksk& key) {
     if (ds.empty()) fail(CKH_ERR_INVALID_PARAMS, "nothing to tune on");
     const u32 keep_graphs = use_graphs;
@@ -140,6 +140,18 @@ std::string Context::autotune_ks(const std::vector<const Poly*>& ds, const Ksk& 
     conv_variant = best_var;
     conv_tg = best_tg;
     fuse_ip = best_fuse;
+    // The inverse-NTT chunk picked on plain transforms, against no chunking, on the switch
+    // itself (its mod_down INTT runs over P limbs grouped by prime, which changes what pays).
+    u32 best_chunk = ntt_chunk_inv;
+    if (ntt_chunk_inv != 0) {
+        for (u32 chunk : {ntt_chunk_inv, 0u}) {
+            ntt_chunk_inv = chunk;
+            const double t = chunk == best_chunk ? best : time_ks();
+            if (chunk != best_chunk) os << ", \"chunk_inv" << chunk << "\": " << t;
+            if (t < best) best = t, best_chunk = chunk;
+        }
+    }
+    ntt_chunk_inv = best_chunk;
     use_graphs = keep_graphs;
     clear_plans();
     os << "}";

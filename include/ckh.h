@@ -87,8 +87,8 @@ void *ckh_stream_handle(ckh_context *ctx);
  * on this device with synthetic limbs of the context's primes and keeps the fastest; writes a
  * JSON report (candidates and choices) to report (may be NULL). */
 ckh_status ckh_autotune(ckh_context *ctx, char *report, size_t report_len);
-/* The autotuner's key-switch candidates (conversion kernel variant, row-pass fusion) timed on
- * the caller's own batch: n NTT-form polynomials switched under key (no counterpart in the
+/* The autotuner's key-switch candidates (conversion kernel variant, row-pass fusion, then the
+ * inverse-NTT chunk against none) timed on the caller's own batch: n NTT-form polynomials switched under key (no counterpart in the
  * reference). Keeps the fastest; writes the timings as JSON to report (may be NULL). */
 ckh_status ckh_autotune_ks(ckh_context *ctx, uint32_t n, const ckh_poly *const *d, const ckh_ksk *key,
                            char *report, size_t report_len);

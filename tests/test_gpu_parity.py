@@ -545,3 +545,38 @@ def test_graph_replays_back_to_back_without_host_sync():
     got = [[(kb.digest(), ka.digest()) for kb, ka in o] for o in outs]
     assert got == want
     assert ctx.graph_stats()[1] >= 4
+
+
+
+P_F64_N4096 = ("p45_n4096", 12, [40] * 4, [45] * 2, 2)  # special primes on the f64 path
+
+
+@pytest.mark.parametrize("case", [C2_SHAPE_N4096, SMALL_N8192, P_F64_N4096], ids=lambda c: c[0])
+def test_inverse_chunks_and_special_prime_sizes_give_the_same_switch(case):
+    """Batched switches and hoisted rotations (digits read through sigma_g) with the mod_down
+    INTT in chunks of 3 limbs or in one launch, with 60-bit and with 45-bit special primes (the
+    latter on the f64 path of the register-radix NTT), equal to each other and to the oracle."""
+    say("inverse chunks", case[0])
+    ctx, ref = make(case)
+    L = len(ref.q) - 1
+    key = ctx.ksk_uniform(L, seed=71)
+    lv = sorted({L, L // 2, 1, 0})
+    polys = [ctx.uniform(l, seed=700 + i) for i, l in enumerate(lv)]
+    steps = [1, 5]
+    gkeys = [ctx.ksk_uniform(L, seed=80 + s, galois=ckh.galois_elt(ref.log_n, s)) for s in steps]
+    ev = ckh.Evaluator(ctx, None, gkeys)
+    ct = ctx.ct_from_polys([ctx.uniform(L - 1, seed=750), ctx.uniform(L - 1, seed=751)], 2.0 ** 30)
+    res = []
+    for chunk in (3, 0):
+        ctx.set_tuning("ntt_chunk_inv", chunk)
+        ks = [(kb.digest(), ka.digest()) for kb, ka in ctx.key_switch_batch(polys, key)]
+        hr = [tuple(r.digests()) for r in ev.hoisted_rotate(ct, steps)]
+        res.append((ks, hr))
+    assert res[0] == res[1]
+    kb, ka = ctx.key_switch_batch(polys, key)[-1]
+    wb, wa = ref.key_switch(polys[-1].download().ravel(), lv[-1], ref.uniform_key(L, 71), L)
+    assert (kb.download() == wb).all() and (ka.download() == wa).all()
+    c0, c1 = ct.download()
+    o = ref.apply_galois(c0.ravel(), c1.ravel(), L - 1, ckh.galois_elt(ref.log_n, 5), ref.uniform_key(L, 85), L)
+    got = ev.hoisted_rotate(ct, [5])[0].download()
+    assert (got[0] == o[0]).all() and (got[1] == o[1]).all()
