@@ -289,11 +289,12 @@ CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in_p, u64* __restric
 // The row pass of one tile, its canonical outputs left in LDS in natural order (word i of the
 // tile at pad(i)): as u64 residues, or (F64) as the f64 bits of the residue.
 template <u32 M, bool F64, class Ar>
-CKH_DEV void fwd_rows_to_lds(const Ar& ar, const u64* __restrict__ in_p, u64* lds, u32 log_n, u32 tile) {
+CKH_DEV void fwd_rows_to_lds(const Ar& ar, const u64* __restrict__ in_p, u64* lds, u32 log_n, u32 tile,
+                             u32 tid = threadIdx.x) {
     const auto in = glb(in_p);
     constexpr u32 R2 = M - 4, U = 4 - R2, TRL = TL - M, JM = (1u << R2) - 1;
     const u32 A = log_n - M;
-    const u32 tid = threadIdx.x, o = tid & JM, rr = tid >> R2;
+    const u32 o = tid & JM, rr = tid >> R2;
     const u32 row = (tile << TRL) + rr;
     const u64 base = (u64)row << M;
     typename Ar::V v[16];
@@ -363,13 +364,18 @@ CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
 // written; the transformed digits never reach memory (keyswitch.rs:254-303).
 template <u32 M>
 CKH_DEV void rows_ip(const FpAr& ar, const RowIpJob& J, u64* lds, u32 log_n, u32 tile) {
-    const u32 tid = threadIdx.x;
     const u64 tbase = (u64)tile << TL;
     const double q = ar.q, qinv = ar.qinv;
     double sb[16], sa[16];
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) sb[k] = sa[k] = 0.0;
     for (u32 j = 0; j < J.ndig; ++j) {
+        // The thread index, opaque to the compiler per digit: otherwise it hoists every address
+        // derived from it (16 words x the row pass's and the products' operands) out of the
+        // digit loop and keeps them live throughout, 244 VGPRs (2 waves per SIMD) instead of 166
+        // (3 waves); recomputing them costs a few integer ops per digit.
+        u32 tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
         const auto kb = glb(J.key + (u64)(2 * j) * J.key_poly + tbase);
         const auto ka = glb(J.key + (u64)(2 * j + 1) * J.key_poly + tbase);
         if (j == J.own_j) {
@@ -381,7 +387,7 @@ CKH_DEV void rows_ip(const FpAr& ar, const RowIpJob& J, u64* lds, u32 log_n, u32
                 sa[k] += fmulmod(x, a, a * qinv, q);
             }
         } else {
-            fwd_rows_to_lds<M, true>(ar, J.mid + j * J.mid_stride, lds, log_n, tile);
+            fwd_rows_to_lds<M, true>(ar, J.mid + j * J.mid_stride, lds, log_n, tile, tid);
 #pragma unroll
             for (u32 k = 0; k < 16; ++k) {
                 const double x = __longlong_as_double((long long)lds[pad(tid + k * NT)]);
@@ -399,6 +405,7 @@ CKH_DEV void rows_ip(const FpAr& ar, const RowIpJob& J, u64* lds, u32 log_n, u32
             }
         }
     }
+    const u32 tid = threadIdx.x;
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) {
         glb(J.acc_b)[tbase + tid + k * NT] = d2u(fcanon(sb[k], q, qinv));
