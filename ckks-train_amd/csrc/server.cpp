@@ -148,8 +148,12 @@ void Server::launch(std::vector<OpPtr>& ops) {
         }
         Batch b{&c, nullptr, mine};
         hip_check(hipEventCreateWithFlags(&b.done, hipEventDisableTiming), "event");
-        hip_check(hipEventRecord(b.done, s), "event");
+        if (hipEventRecord(b.done, s) != hipSuccess) {
+            (void)hipEventDestroy(b.done);
+            fail(CKH_ERR_DEVICE, "event record failed");
+        }
         std::lock_guard<std::mutex> lk(mu_);
+        for (auto& op : mine) op->launched = true;
         inflight_.push_back(b);
         ++batches_;
     }
@@ -164,9 +168,14 @@ void Server::loop() {
             for (size_t i = 0; i < inflight_.size();) {
                 Batch& b = inflight_[i];
                 (void)hipSetDevice(b.ctx->device);
-                if (hipEventQuery(b.done) == hipSuccess) {
+                const hipError_t q = hipEventQuery(b.done);
+                if (q != hipErrorNotReady) {
+                    // done, or the device reported an error: then the batch's requests fail
+                    // instead of waiting for ever
                     const double t = now();
                     for (auto& op : b.ops) {
+                        if (q != hipSuccess && op->status == CKH_OK)
+                            fail_op(op, CKH_ERR_DEVICE, std::string("device error: ") + hipGetErrorString(q));
                         op->t_done = t;
                         op->done = true;
                         in_use_ -= op->footprint;
@@ -225,17 +234,26 @@ void Server::loop() {
             launch_ms_ += dt;
             launch_max_ms_ = std::max(launch_max_ms_, dt);
         } catch (const Error& e) {
-            std::lock_guard<std::mutex> lk(mu_);
-            for (auto& op : admitted)
-                if (!op->done) {
-                    fail_op(op, e.status, e.msg);
-                    op->done = true;
-                    in_use_ -= op->footprint;
-                    ++failed_;
-                }
-            done_cv_.notify_all();
+            fail_admitted(admitted, e.status, e.msg);
+        } catch (const std::exception& e) {  // host out of memory and the like: the loop keeps running
+            fail_admitted(admitted, CKH_ERR_DEVICE, e.what());
+        } catch (...) {
+            fail_admitted(admitted, CKH_ERR_DEVICE, "unknown internal error");
         }
     }
+}
+
+// A turn whose launch threw: every request of it not yet completed fails with the error.
+void Server::fail_admitted(std::vector<OpPtr>& admitted, ckh_status st, const std::string& msg) {
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto& op : admitted)
+        if (!op->done && !op->launched) {  // launched ones complete with their batch
+            fail_op(op, st, msg);
+            op->done = true;
+            in_use_ -= op->footprint;
+            ++failed_;
+        }
+    done_cv_.notify_all();
 }
 
 bool Server::poll(u64 ticket, bool wait, OpPtr& op) {

@@ -26,6 +26,7 @@ struct ServedOp {
     std::string err;
     std::vector<CtPtr> outs;
     bool done = false;
+    bool launched = false;  // its batch is in flight (completion marks it done)
 };
 using OpPtr = std::shared_ptr<ServedOp>;
 
@@ -49,6 +50,7 @@ class Server {
     void loop();
     void launch(std::vector<OpPtr>& ops);
     void fail_op(const OpPtr& op, ckh_status st, const std::string& msg);
+    void fail_admitted(std::vector<OpPtr>& admitted, ckh_status st, const std::string& msg);
     u64 footprint(const ServedOp& op) const;
 
     ckh_server_config cfg_;
