@@ -23,6 +23,14 @@ void Context::set_tuning(const std::string& key, int64_t value) {
         conv_variant = (u32)std::min<int64_t>(value, 2);
     else if (key == "fuse_ip")
         fuse_ip = value != 0;
+    else if (key == "small_limbs")
+        small_limbs = (u32)std::min<int64_t>(value, 1u << 30);
+    else if (key == "conv_variant_small")
+        conv_variant_s = (u32)std::min<int64_t>(value, 2);
+    else if (key == "conv_tg_small")
+        conv_tg_s = (u32)std::min<int64_t>(std::max<int64_t>(value, 1), 64);
+    else if (key == "fuse_ip_small")
+        fuse_ip_s = value != 0;
     else if (key == "conv_tg")
         conv_tg = (u32)std::min<int64_t>(std::max<int64_t>(value, 1), 64);
     else if (key == "graph_cache")
@@ -43,6 +51,10 @@ int64_t Context::get_tuning(const std::string& key) const {
     if (key == "graph_cache") return graph_cache_max;
     if (key == "conv_variant") return conv_variant;
     if (key == "fuse_ip") return fuse_ip;
+    if (key == "small_limbs") return small_limbs;
+    if (key == "conv_variant_small") return conv_variant_s;
+    if (key == "conv_tg_small") return conv_tg_s;
+    if (key == "fuse_ip_small") return fuse_ip_s;
     if (key == "conv_tg") return conv_tg;
     if (key == "pool") return pool_mode;
     if (key == "service_streams") return service_streams;
@@ -90,8 +102,59 @@ std::string Context::autotune() {
     for (u32 i = 0; i < 12; ++i) ds.push_back(poly_uniform(*this, top - (i * top) / 12, 0, true, 0x7eed + i, 0));
     for (auto& p : ds) draw.push_back(p.get());
     os << "}, \"ks_candidates_ms\": " << autotune_ks(draw, *key);
+    os << ", \"small_candidates_ms\": " << autotune_small(*ds[0], *key);
     os << ", \"ntt_chunk_fwd\": " << ntt_chunk_fwd << ", \"ntt_chunk_inv\": " << ntt_chunk_inv << ", \"conv_variant\": " << conv_variant
-       << ", \"fuse_ip\": " << fuse_ip << "}";
+       << ", \"fuse_ip\": " << fuse_ip << ", \"small\": [" << conv_variant_s << ", " << conv_tg_s << ", " << fuse_ip_s
+       << "]}";
+    return os.str();
+}
+
+// The small-batch schedule (conversion variant, targets per workgroup, row-pass fusion), timed on
+// one switch of d under key with captured launches, as a caller's single switch replays them.
+std::string Context::autotune_small(const Poly& d, const Ksk& key) {
+    const u32 keep_small = small_limbs;
+    small_limbs = std::max(small_limbs, d.level + 1);  // d's switch takes the small schedule
+    hipStream_t s = stream();
+    auto time_one = [&]() {
+        clear_plans();
+        for (u32 r = 0; r < 3; ++r) (void)ks_switch(*this, d, key);  // direct run, capture, replay
+        hipEvent_t a, b;
+        hip_check(hipEventCreate(&a), "event");
+        hip_check(hipEventCreate(&b), "event");
+        hip_check(hipEventRecord(a, s), "event");
+        for (u32 r = 0; r < 10; ++r) (void)ks_switch(*this, d, key);
+        hip_check(hipEventRecord(b, s), "event");
+        hip_check(hipEventSynchronize(b), "event");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, a, b), "event");
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return (double)ms / 10;
+    };
+    std::ostringstream os;
+    os << "{";
+    double best = 1e30;
+    u32 bv = conv_variant_s, bt = conv_tg_s, bf = fuse_ip_s;
+    bool first = true;
+    for (u32 var : {0u, 1u, 2u})
+        for (u32 tg : {4u, 8u, 16u, 24u})
+            for (u32 fuse : {0u, 1u}) {
+                if (fuse && !fuse_ip_possible()) continue;
+                if (var == 2 && (tg != 8 || !conv.mf_ok)) continue;
+                conv_variant_s = var;
+                conv_tg_s = tg;
+                fuse_ip_s = fuse;
+                const double t = time_one();
+                os << (first ? "" : ", ") << "\"conv" << var << "_tg" << tg << (fuse ? "_fused" : "") << "\": " << t;
+                first = false;
+                if (t < best) best = t, bv = var, bt = tg, bf = fuse;
+            }
+    conv_variant_s = bv;
+    conv_tg_s = bt;
+    fuse_ip_s = bf;
+    small_limbs = keep_small;
+    clear_plans();
+    os << "}";
     return os.str();
 }
 

@@ -246,7 +246,10 @@ static void run_ks_on(Context& c, const std::vector<KsReq>& reqs, hipStream_t s,
     u64 total = 0;
     for (auto& r : reqs) total += ks_scratch_words(c, r.d->level, true);
     BufPtr scratch = c.alloc(total, s);
-    const bool fuse = c.fuse_ip && c.fuse_ip_possible();
+    u64 limbs = 0;
+    for (auto& r : reqs) limbs += r.d->level + 1;
+    const bool small = limbs <= c.small_limbs;
+    const bool fuse = (small ? c.fuse_ip_s : c.fuse_ip) && c.fuse_ip_possible();
     std::vector<DecTask> dec;
     std::vector<ApplyTask> app;
     u64* p = scratch->ptr;
@@ -265,6 +268,7 @@ static void run_ks_on(Context& c, const std::vector<KsReq>& reqs, hipStream_t s,
     if (pre) prog.step(S_AUTOMORPH, prog.add(*pre), (u32)pre->size());
     c.prog_decompose(prog, dec);
     c.prog_apply(prog, app);
+    if (small) prog.set_conv_schedule(c.conv_variant_s, c.conv_tg_s);
     c.execute(prog, s);
     hip_check(hipGetLastError(), "key switch");
 }

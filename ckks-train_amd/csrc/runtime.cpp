@@ -646,7 +646,10 @@ void Context::run(const Program& p, const uint8_t* d, hipStream_t s) const {
         switch (st.kind) {
             case S_NTT_FWD: launch_ntt(*this, (const LimbRef*)x, st.count, true, s); break;
             case S_NTT_INV: launch_ntt(*this, (const LimbRef*)x, st.count, false, s); break;
-            case S_CONV: launch_conv(conv, (const ConvJob*)x, st.count, st.a, st.b, conv_variant, conv_tg, s); break;
+            case S_CONV:
+                launch_conv(conv, (const ConvJob*)x, st.count, st.a, st.b, st.sched ? (st.sched >> 1) & 127 : conv_variant,
+                            st.sched ? st.sched >> 8 : conv_tg, s);
+                break;
             case S_COPY: launch_copy((const LimbRef*)x, st.count, log_n, s); break;
             case S_IP: launch_ip((const IpJob*)x, st.count, d_pc, log_n, s); break;
             case S_NTT_FWD_FIN:

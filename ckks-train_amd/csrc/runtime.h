@@ -145,8 +145,10 @@ struct Program {
         u32 count;
         size_t off, off2;
         u32 a, b;
+        u32 sched = 0;  // S_CONV: 0 = the context's schedule, else 1 | variant << 1 | tg << 8
         bool operator==(const Step& o) const {
-            return kind == o.kind && count == o.count && off == o.off && off2 == o.off2 && a == o.a && b == o.b;
+            return kind == o.kind && count == o.count && off == o.off && off2 == o.off2 && a == o.a && b == o.b &&
+                   sched == o.sched;
         }
     };
     std::vector<uint8_t> bytes;
@@ -160,6 +162,11 @@ struct Program {
         return off;
     }
     void step(StepKind k, size_t off, u32 count, size_t off2 = 0, u32 a = 0, u32 b = 0);
+    // Every base conversion of the program with this kernel variant and targets per workgroup.
+    void set_conv_schedule(u32 variant, u32 tg) {
+        for (Step& st : steps)
+            if (st.kind == S_CONV) st.sched = 1u | (variant << 1) | (tg << 8);
+    }
 };
 
 class Context : public std::enable_shared_from_this<Context> {
@@ -239,9 +246,14 @@ class Context : public std::enable_shared_from_this<Context> {
     // chunk small enough keeps the pass intermediate in the on-die caches between the passes.
     u32 ntt_chunk_fwd = 0, ntt_chunk_inv = 0;
     u32 conv_variant = 0, conv_tg = 8;  // base-conversion kernel variant and targets per workgroup
+    // Batched key switches of at most small_limbs input limbs in all (a single switch, a few
+    // small ones) have too few workgroups per launch to fill the device; they run their own
+    // schedule (tuning keys "small_limbs", "conv_variant_small", "conv_tg_small", "fuse_ip_small").
+    u32 small_limbs = 48, conv_variant_s = 0, conv_tg_s = 8, fuse_ip_s = 0;
     std::string autotune();
     // The key-switch candidates of autotune() timed on a caller's batch (autotune.cpp).
     std::string autotune_ks(const std::vector<const Poly*>& ds, const Ksk& key);
+    std::string autotune_small(const Poly& d, const Ksk& key);
     void set_tuning(const std::string& key, int64_t value);
     int64_t get_tuning(const std::string& key) const;
     // Copies a host blob to a fresh device buffer on stream s.

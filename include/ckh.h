@@ -84,8 +84,9 @@ ckh_status ckh_set_stream(ckh_context *ctx, uint32_t index);
 /* The calling thread's hipStream_t (for event timing by callers). */
 void *ckh_stream_handle(ckh_context *ctx);
 /* The project's autotuner (no counterpart in the reference): times candidate launch schedules
- * on this device with synthetic limbs of the context's primes and keeps the fastest; writes a
- * JSON report (candidates and choices) to report (may be NULL). */
+ * on this device with synthetic limbs of the context's primes and keeps the fastest (NTT chunks,
+ * the batched key-switch schedule, and the small-batch schedule on one top-level switch); writes
+ * a JSON report (candidates and choices) to report (may be NULL). */
 ckh_status ckh_autotune(ckh_context *ctx, char *report, size_t report_len);
 /* The autotuner's key-switch candidates (conversion kernel variant, row-pass fusion, then the
  * inverse-NTT chunk against none) timed on the caller's own batch: n NTT-form polynomials switched under key (no counterpart in the
@@ -97,7 +98,9 @@ ckh_status ckh_autotune_ks(ckh_context *ctx, uint32_t n, const ckh_poly *const *
  * directly); "graph_cache" (captured sequences kept per context); "conv_variant" (base conversion:
  * 0 = vector kernel, 1 = vector kernel with exact f64-piece sums for 60-bit sources, 2 = int8
  * matrix-core kernel); "conv_tg" (targets per base-conversion
- * workgroup); "fuse_ip" (1 = row pass fused into the inner product); "pool" (1 = the context's
+ * workgroup); "fuse_ip" (1 = row pass fused into the inner product); "small_limbs" (batched
+ * switches of at most this many input limbs in all, a single switch for one, use their own
+ * schedule: "conv_variant_small", "conv_tg_small", "fuse_ip_small"); "pool" (1 = the context's
  * own memory pool, where a block freed on one stream serves another stream only once the free
  * has completed on the device, the default; 0 = the device's default pool); "service_streams"
  * (1..4 streams the serving loop spreads its batches over, default 4); test hook "fail_alloc_after"

@@ -477,6 +477,17 @@ def test_schedules_give_the_same_switch(case):
                 assert (wb == gb).all() and (wa == ga).all(), (var, fuse)
     ob, oa = ref.key_switch(polys[-1].download().ravel(), lv[-1], ref.uniform_key(L, 61), L)
     assert (want[-1][0] == ob).all() and (want[-1][1] == oa).all()
+    # the small-batch schedule: the whole batch (and each single switch) under every choice
+    ctx.set_tuning("small_limbs", 1 << 20)
+    for var, tg, fuse in ((0, 8, 0), (1, 4, 1), (2, 8, 0), (0, 24, 1)):
+        ctx.set_tuning("conv_variant_small", var)
+        ctx.set_tuning("conv_tg_small", tg)
+        ctx.set_tuning("fuse_ip_small", fuse)
+        got = [(kb.download(), ka.download()) for kb, ka in ctx.key_switch_batch(polys, key)]
+        for (wb, wa), (gb, ga) in zip(want, got):
+            assert (wb == gb).all() and (wa == ga).all(), ("small", var, tg, fuse)
+        kb, ka = ctx.key_switch(polys[0], key)
+        assert (kb.download() == want[0][0]).all() and (ka.download() == want[0][1]).all()
 
 
 @pytest.mark.parametrize("case", [SMALL, C2_SHAPE_N4096, SMALL_N8192])
