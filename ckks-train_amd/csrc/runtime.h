@@ -249,7 +249,7 @@ class Context : public std::enable_shared_from_this<Context> {
     // Batched key switches of at most small_limbs input limbs in all (a single switch, a few
     // small ones) have too few workgroups per launch to fill the device; they run their own
     // schedule (tuning keys "small_limbs", "conv_variant_small", "conv_tg_small", "fuse_ip_small").
-    u32 small_limbs = 48, conv_variant_s = 0, conv_tg_s = 8, fuse_ip_s = 0;
+    u32 small_limbs = 48, conv_variant_s = 2, conv_tg_s = 8, fuse_ip_s = 0;
     std::string autotune();
     // The key-switch candidates of autotune() timed on a caller's batch (autotune.cpp).
     std::string autotune_ks(const std::vector<const Poly*>& ds, const Ksk& key);
