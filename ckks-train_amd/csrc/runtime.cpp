@@ -107,6 +107,11 @@ Context::~Context() {
         if (pl->desc) (void)hipFreeAsync(pl->desc, pl->s);
     }
     plans_.clear();
+    for (auto& pl : retired_) {  // every stream was synchronized above
+        (void)hipEventDestroy(pl->done);
+        if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+    }
+    retired_.clear();
     for (auto s : streams_) (void)hipStreamSynchronize(s);
     for (void* a : perm_allocs_) (void)hipFree(a);
     for (auto s : streams_) (void)hipStreamDestroy(s);
@@ -684,28 +689,48 @@ void Context::execute(const Program& p, hipStream_t s) {
     run(p, (const uint8_t*)d->ptr, s);
 }
 
+// Evicts plan i without waiting: its descriptor buffer returns to the pool in its stream's
+// order (after the last replay), and the executable graph and event are destroyed once that
+// replay has finished (retired until then, swept by later calls).
 void Context::drop_plan(size_t i) {
-    Plan& pl = *plans_[i];
-    // Only the plan's own last replay is waited for; its descriptor buffer returns to the pool in
-    // its stream's order (no device-wide synchronization).
-    if (pl.done) {
-        (void)hipEventSynchronize(pl.done);
-        (void)hipEventDestroy(pl.done);
-    }
-    if (pl.exec) (void)hipGraphExecDestroy(pl.exec);
-    if (pl.desc) (void)hipFreeAsync(pl.desc, pl.s);
+    std::unique_ptr<Plan> pl = std::move(plans_[i]);
     plans_.erase(plans_.begin() + i);
+    if (pl->desc) (void)hipFreeAsync(pl->desc, pl->s);
+    pl->desc = nullptr;
+    if (pl->done && hipEventQuery(pl->done) == hipErrorNotReady) {
+        retired_.push_back(std::move(pl));
+        return;
+    }
+    if (pl->done) (void)hipEventDestroy(pl->done);
+    if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+}
+
+// Destroys the retired plans whose last replay has finished (wait: all of them, after it).
+void Context::sweep_retired(bool wait) {
+    for (size_t i = 0; i < retired_.size();) {
+        Plan& pl = *retired_[i];
+        if (wait) (void)hipEventSynchronize(pl.done);
+        if (wait || hipEventQuery(pl.done) != hipErrorNotReady) {
+            (void)hipEventDestroy(pl.done);
+            if (pl.exec) (void)hipGraphExecDestroy(pl.exec);
+            retired_.erase(retired_.begin() + i);
+        } else {
+            ++i;
+        }
+    }
 }
 
 void Context::clear_plans() {
     std::lock_guard<std::mutex> lk(plan_mu_);
     while (!plans_.empty()) drop_plan(plans_.size() - 1);
+    sweep_retired(true);
 }
 
 // Returns false when the program ran on no graph (first sighting of its shape: the caller runs
 // it directly). The graph holds only kernel launches; descriptors travel in the plan's buffer.
 bool Context::execute_graph(const Program& p, hipStream_t s) {
     std::lock_guard<std::mutex> lk(plan_mu_);
+    if (!retired_.empty()) sweep_retired(false);
     Plan* pl = nullptr;
     for (auto& q : plans_)
         if (q->s == s && q->bytes == p.bytes.size() && q->steps == p.steps) {

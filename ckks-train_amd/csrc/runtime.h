@@ -297,6 +297,8 @@ class Context : public std::enable_shared_from_this<Context> {
     };
     std::mutex plan_mu_;
     std::vector<std::unique_ptr<Plan>> plans_;
+    std::vector<std::unique_ptr<Plan>> retired_;  // evicted plans whose last replay may still run
+    void sweep_retired(bool wait);
     u64 plan_clock_ = 0;
     bool execute_graph(const Program& p, hipStream_t s);
     void drop_plan(size_t i);

@@ -591,3 +591,29 @@ def test_inverse_chunks_and_special_prime_sizes_give_the_same_switch(case):
     o = ref.apply_galois(c0.ravel(), c1.ravel(), L - 1, ckh.galois_elt(ref.log_n, 5), ref.uniform_key(L, 85), L)
     got = ev.hoisted_rotate(ct, [5])[0].download()
     assert (got[0] == o[0]).all() and (got[1] == o[1]).all()
+
+
+def test_plan_eviction_under_churn_keeps_results():
+    """A plan cache of two entries under five alternating batch shapes, with no host wait between
+    calls: plans are evicted while their last replay may still run (destroyed once it finishes),
+    and every result equals the directly launched one."""
+    say("plan eviction churn")
+    ctx, ref = make(C2_SHAPE_N4096)
+    L = len(ref.q) - 1
+    key = ctx.ksk_uniform(L, seed=91)
+    shapes = [[L], [L, 3], [5, 5, 5], [L // 2, 1], [0, L, 2]]
+    batches = [[ctx.uniform(l, seed=900 + 10 * i + j) for j, l in enumerate(sh)] for i, sh in enumerate(shapes)]
+    ctx.set_tuning("graphs", 0)
+    want = [[(kb.digest(), ka.digest()) for kb, ka in ctx.key_switch_batch(b, key)] for b in batches]
+    ctx.set_tuning("graphs", 1)
+    ctx.set_tuning("graph_cache", 2)
+    outs = []
+    for r in range(3):
+        for i, b in enumerate(batches):
+            for _ in range(3):  # seen, captured, replayed; then evicted by the next shapes
+                outs.append((i, ctx.key_switch_batch(b, key)))
+    ctx.synchronize()
+    for i, o in outs:
+        assert [(kb.digest(), ka.digest()) for kb, ka in o] == want[i], i
+    caps, launches = ctx.graph_stats()
+    assert caps >= 15 and launches >= 15
