@@ -132,7 +132,27 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
             o = d2u(fcanon(acc, r, rinv));
         } else {
             const PrimeConst C = pc_int(T.pc + g->prime);
-            {
+            if constexpr (MAXS <= 7) {
+                // sum_j y_j c_j from 32-bit halves (y_j, c_j < 2^60): y c = y0 c0 + (y0 c1 +
+                // y1 c0) 2^32 + y1 c1 2^64 with y1, c1 < 2^28, so the middle sum stays below
+                // 7 * 2^61 < 2^64 and the high one below 2^59; one 32 x 32 -> 64 multiply per
+                // partial product instead of full 64 x 64 products. The same exact sum.
+                u64 l = 0, m = 0, h = 0;
+                u32 carry = 0;
+#pragma unroll
+                for (u32 j = 0; j < MAXS; ++j) {
+                    const u64 cj = g->qhat[j];
+                    const u32 y0 = (u32)y[j], y1 = (u32)(y[j] >> 32), c0 = (u32)cj, c1 = (u32)(cj >> 32);
+                    const u64 p = (u64)y0 * c0;
+                    l += p;
+                    carry += l < p ? 1u : 0u;
+                    m += (u64)y0 * c1 + (u64)y1 * c0;
+                    h += (u64)y1 * c1;
+                }
+                const u64 lo = l + (m << 32);
+                const u64 hi = h + carry + (m >> 32) + (lo < l ? 1 : 0);
+                o = reduce128(hi, lo, C);
+            } else {
                 Acc128 acc = {0, 0};
 #pragma unroll
                 for (u32 j = 0; j < MAXS; ++j) mac(acc, y[j], g->qhat[j]);
