@@ -153,10 +153,12 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
                 const u64 hi = h + carry + (m >> 32) + (lo < l ? 1 : 0);
                 o = reduce128(hi, lo, C);
             } else {
-                Acc128 acc = {0, 0};
+                Dot60 acc;
 #pragma unroll
-                for (u32 j = 0; j < MAXS; ++j) mac(acc, y[j], g->qhat[j]);
-                o = reduce128(acc.hi, acc.lo, C);
+                for (u32 j = 0; j < MAXS; ++j) acc.add(y[j], g->qhat[j]);
+                u64 hi, lo;
+                acc.get(hi, lo);
+                o = reduce128(hi, lo, C);
             }
             if (P.exact) o = sub_mod(o, shoup(e, g->qmod, g->qmod_s, C.q), C.q);
             o = sub_mod(o, g->hmod, C.q);

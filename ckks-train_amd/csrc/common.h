@@ -136,17 +136,29 @@ CKH_DEV __attribute__((address_space(1))) T* glb(T* p) {
     return (__attribute__((address_space(1))) T*)p;
 }
 
-// 128-bit accumulator.
-struct Acc128 {
-    u64 lo, hi;
+// Exact sum of products of values below 2^60 (every residue here), from 32-bit halves:
+// y c = y0 c0 + (y0 c1 + y1 c0) 2^32 + y1 c1 2^64 with y1, c1 < 2^28, one 32 x 32 -> 64 multiply
+// per partial product instead of full 64 x 64 products; the low and middle sums carry into
+// counters, the high one (below 2^56 per term) cannot overflow for fewer than 256 terms.
+struct Dot60 {
+    u64 l = 0, m = 0, h = 0;
+    u32 cl = 0, cm = 0;
+    CKH_DEV void add(u64 y, u64 c) {
+        const u32 y0 = (u32)y, y1 = (u32)(y >> 32), c0 = (u32)c, c1 = (u32)(c >> 32);
+        const u64 p = (u64)y0 * c0;
+        l += p;
+        cl += l < p ? 1u : 0u;
+        const u64 t = (u64)y0 * c1 + (u64)y1 * c0;
+        m += t;
+        cm += m < t ? 1u : 0u;
+        h += (u64)y1 * c1;
+    }
+    // The sum as hi 2^64 + lo.
+    CKH_DEV void get(u64& hi, u64& lo) const {
+        lo = l + (m << 32);
+        hi = h + cl + (m >> 32) + ((u64)cm << 32) + (lo < l ? 1 : 0);
+    }
 };
-
-CKH_DEV void mac(Acc128& acc, u64 a, u64 b) {
-    u64 lo = a * b;
-    u64 hi = mulhi(a, b);
-    acc.lo += lo;
-    acc.hi += hi + (acc.lo < lo ? 1 : 0);
-}
 
 // blockIdx.x remapped so that consecutive logical tiles of one launch land on one XCD
 // (workgroups are dealt round-robin over the 8 XCDs, each with its own L2).

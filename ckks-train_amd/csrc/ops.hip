@@ -46,18 +46,19 @@ __global__ __launch_bounds__(256) void k_inner_product(const IpJob* __restrict__
         glb(J.acc_a)[c] = d2u(fcanon(sa, q, qinv));
         return;
     }
-    Acc128 sb = {0, 0}, sa = {0, 0};
+    Dot60 sb, sa;
     for (u32 j = 0; j < J.ndig; ++j) {
         const auto src = glb((J.own && j == J.own_j) ? J.own : J.dig + j * J.dig_stride);
         const auto key = glb(J.key);
         const u64 d = src[cs];
-        const u64 b = key[(2 * j) * J.key_poly + c];
-        const u64 k = key[(2 * j + 1) * J.key_poly + c];
-        mac(sb, d, b);
-        mac(sa, d, k);
+        sb.add(d, key[(2 * j) * J.key_poly + c]);
+        sa.add(d, key[(2 * j + 1) * J.key_poly + c]);
     }
-    glb(J.acc_b)[c] = reduce128(sb.hi, sb.lo, C);
-    glb(J.acc_a)[c] = reduce128(sa.hi, sa.lo, C);
+    u64 hi, lo;
+    sb.get(hi, lo);
+    glb(J.acc_b)[c] = reduce128(hi, lo, C);
+    sa.get(hi, lo);
+    glb(J.acc_a)[c] = reduce128(hi, lo, C);
 }
 
 void inner_product_launch(const IpJob* jobs, u32 njobs, const PrimeConst* pc, u32 log_n, hipStream_t stream) {

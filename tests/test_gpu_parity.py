@@ -617,3 +617,23 @@ def test_plan_eviction_under_churn_keeps_results():
         assert [(kb.digest(), ka.digest()) for kb, ka in o] == want[i], i
     caps, launches = ctx.graph_stats()
     assert caps >= 15 and launches >= 15
+
+
+@pytest.mark.parametrize("dnum", [2, 1], ids=["alpha8", "alpha16"])
+def test_wide_digits_key_switch(dnum):
+    """Digits of 8 and 16 primes (the conversion's 8- and 16-source kernels, whose 60-bit special
+    targets take the exact 128-bit dot product) against the oracle, batched and single."""
+    say("wide digits", dnum)
+    case = ("wide", 10, [40] * 16, [60] * 3, dnum)
+    ctx, ref = make(case)
+    L = len(ref.q) - 1
+    key = ctx.ksk_uniform(L, seed=95)
+    hkey = ref.uniform_key(L, 95)
+    lv = [L, 9, 3]
+    polys = [ctx.uniform(l, seed=950 + l) for l in lv]
+    got = ctx.key_switch_batch(polys, key)
+    for p_, l, (kb, ka) in zip(polys, lv, got):
+        wb, wa = ref.key_switch(p_.download().ravel(), l, hkey, L)
+        assert (kb.download() == wb).all() and (ka.download() == wa).all(), l
+    kb, ka = ctx.key_switch(polys[0], key)
+    assert kb.digest() == got[0][0].digest() and ka.digest() == got[0][1].digest()
