@@ -68,25 +68,6 @@ Context::Context(int ordinal, const ckh_params& p, u32 n_streams) {
     }
     if (n_streams == 0) n_streams = 1;
     n_user_streams_ = n_streams;
-    // The context's pool: a block freed on one stream serves another stream only once the free
-    // has completed (never through dependencies the runtime would insert between the streams);
-    // freed memory stays cached in the pool.
-    hipMemPoolProps props{};
-    props.allocType = hipMemAllocationTypePinned;
-    props.handleTypes = hipMemHandleTypeNone;
-    props.location.type = hipMemLocationTypeDevice;
-    props.location.id = device;
-    if (hipMemPoolCreate(&ctx_pool_, &props) != hipSuccess) {
-        (void)hipGetLastError();
-        ctx_pool_ = nullptr;
-    } else {
-        uint64_t keep = ~0ull;
-        int on = 1, off = 0;
-        (void)hipMemPoolSetAttribute(ctx_pool_, hipMemPoolAttrReleaseThreshold, &keep);
-        (void)hipMemPoolSetAttribute(ctx_pool_, hipMemPoolReuseFollowEventDependencies, &off);
-        (void)hipMemPoolSetAttribute(ctx_pool_, hipMemPoolReuseAllowInternalDependencies, &off);
-        (void)hipMemPoolSetAttribute(ctx_pool_, hipMemPoolReuseAllowOpportunistic, &on);
-    }
     for (u32 i = 0; i < n_streams + SERVICE_STREAMS; ++i) {
         hipStream_t s;
         hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
@@ -113,9 +94,10 @@ Context::~Context() {
     }
     retired_.clear();
     for (auto s : streams_) (void)hipStreamSynchronize(s);
+    cache_release(0);
+    for (hipEvent_t e : cache_events_) (void)hipEventDestroy(e);
     for (void* a : perm_allocs_) (void)hipFree(a);
     for (auto s : streams_) (void)hipStreamDestroy(s);
-    if (ctx_pool_) (void)hipMemPoolDestroy(ctx_pool_);
 }
 
 // A thread's choice is kept per context address; an entry left by a destroyed context at the
@@ -135,17 +117,90 @@ void Context::set_service_stream(u32 k) {
     tl_stream[this] = n_user_streams_ + k % std::min(std::max(service_streams, 1u), SERVICE_STREAMS);
 }
 
+// Block size classes: 4 KiB up to 4 KiB, then four per power of two (at most 25 % slack).
+static size_t size_class(size_t b) {
+    if (b <= 4096) return 4096;
+    const int k = 63 - __builtin_clzll((unsigned long long)(b - 1));  // 2^k < b <= 2^(k+1)
+    const size_t base = (size_t)1 << k, step = base / 4;
+    const size_t c = base + ((b - base + step - 1) / step) * step;
+    return (c + 4095) & ~(size_t)4095;
+}
+
+// A cached block of class cls for stream s: one last used on s (stream order makes it free for
+// the work queued after), else one whose event has completed; null if none.
+void* Context::cache_take(size_t cls, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(cache_mu_);
+    auto it = cache_.find(cls);
+    if (it == cache_.end() || it->second.empty()) return nullptr;
+    std::vector<Cached>& v = it->second;
+    size_t pick = v.size();
+    for (size_t i = 0; i < v.size() && pick == v.size(); ++i)
+        if (v[i].s == s) pick = i;
+    for (size_t i = 0; i < v.size() && pick == v.size(); ++i)
+        if (hipEventQuery(v[i].ready) == hipSuccess) pick = i;
+    if (pick == v.size()) return nullptr;
+    void* p = v[pick].p;
+    cache_events_.push_back(v[pick].ready);
+    v[pick] = v.back();
+    v.pop_back();
+    cache_bytes_ -= cls;
+    return p;
+}
+
+// Caches a block whose last use is queued on s (over the release threshold it goes back to the
+// driver once that use has finished).
+void Context::cache_put(void* p, size_t cls, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(cache_mu_);
+    hipEvent_t ev = nullptr;
+    if (!cache_events_.empty()) {
+        ev = cache_events_.back();
+        cache_events_.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        ev = nullptr;
+    }
+    if (cache_bytes_ + cls > cache_limit_ || !ev || hipEventRecord(ev, s) != hipSuccess) {
+        (void)hipGetLastError();
+        if (ev) cache_events_.push_back(ev);
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(p);
+        return;
+    }
+    cache_[cls].push_back({p, s, ev});
+    cache_bytes_ += cls;
+}
+
+// Returns cached blocks to the driver, largest first, until at most keep_bytes stay cached.
+void Context::cache_release(u64 keep_bytes) {
+    std::lock_guard<std::mutex> lk(cache_mu_);
+    for (auto it = cache_.rbegin(); it != cache_.rend() && cache_bytes_ > keep_bytes; ++it) {
+        std::vector<Cached>& v = it->second;
+        while (!v.empty() && cache_bytes_ > keep_bytes) {
+            (void)hipEventSynchronize(v.back().ready);
+            (void)hipFree(v.back().p);
+            cache_events_.push_back(v.back().ready);
+            v.pop_back();
+            cache_bytes_ -= it->first;
+        }
+    }
+}
+
 u64 Context::pool_used_memory() {
-    hipMemPool_t mp = ctx_pool_;
-    if (pool_mode != 1 || !mp) hip_check(hipDeviceGetDefaultMemPool(&mp, device), "hipDeviceGetDefaultMemPool");
+    if (pool_mode == 1) return live_bytes.load();
+    hipMemPool_t mp;
+    hip_check(hipDeviceGetDefaultMemPool(&mp, device), "hipDeviceGetDefaultMemPool");
     uint64_t v = 0;
     hip_check(hipMemPoolGetAttribute(mp, hipMemPoolAttrUsedMemCurrent, &v), "hipMemPoolGetAttribute");
     return v;
 }
 
 u64 Context::pool_unused_reserve() {
-    hipMemPool_t mp = ctx_pool_;
-    if (pool_mode != 1 || !mp) hip_check(hipDeviceGetDefaultMemPool(&mp, device), "hipDeviceGetDefaultMemPool");
+    if (pool_mode == 1) {
+        std::lock_guard<std::mutex> lk(cache_mu_);
+        return cache_bytes_;
+    }
+    hipMemPool_t mp;
+    hip_check(hipDeviceGetDefaultMemPool(&mp, device), "hipDeviceGetDefaultMemPool");
     uint64_t used = 0, reserved = 0;
     hip_check(hipMemPoolGetAttribute(mp, hipMemPoolAttrUsedMemCurrent, &used), "hipMemPoolGetAttribute");
     hip_check(hipMemPoolGetAttribute(mp, hipMemPoolAttrReservedMemCurrent, &reserved), "hipMemPoolGetAttribute");
@@ -153,8 +208,11 @@ u64 Context::pool_unused_reserve() {
 }
 
 void Context::set_pool_release_threshold(u64 bytes) {
+    {
+        std::lock_guard<std::mutex> lk(cache_mu_);
+        cache_limit_ = bytes;
+    }
     uint64_t v = bytes;
-    if (ctx_pool_) hip_check(hipMemPoolSetAttribute(ctx_pool_, hipMemPoolAttrReleaseThreshold, &v), "pool threshold");
     hipMemPool_t dp;
     hip_check(hipDeviceGetDefaultMemPool(&dp, device), "hipDeviceGetDefaultMemPool");
     hip_check(hipMemPoolSetAttribute(dp, hipMemPoolAttrReleaseThreshold, &v), "pool threshold");
@@ -163,7 +221,7 @@ void Context::set_pool_release_threshold(u64 bytes) {
 void Context::trim_pool(u64 keep_bytes) {
     hip_check(hipSetDevice(device), "hipSetDevice");
     for (auto st : streams_) hip_check(hipStreamSynchronize(st), "synchronize");
-    if (ctx_pool_) hip_check(hipMemPoolTrimTo(ctx_pool_, keep_bytes), "hipMemPoolTrimTo");
+    cache_release(keep_bytes);
     hipMemPool_t dp;
     hip_check(hipDeviceGetDefaultMemPool(&dp, device), "hipDeviceGetDefaultMemPool");
     hip_check(hipMemPoolTrimTo(dp, keep_bytes), "hipMemPoolTrimTo");
@@ -171,11 +229,27 @@ void Context::trim_pool(u64 keep_bytes) {
 
 BufPtr Context::alloc(size_t words, hipStream_t s) {
     void* p = nullptr;
-    size_t bytes = std::max<size_t>(words, 1) * sizeof(u64);
+    size_t bytes = std::max<size_t>(words, 1) * sizeof(u64), cls = 0;
     if (fail_alloc_after.load() >= 0 && fail_alloc_after.fetch_sub(1) == 0)
         fail(CKH_ERR_DEVICE, "out of device memory (injected allocation failure)");
-    hipMemPool_t mp = pool_mode == 1 ? ctx_pool_ : nullptr;
-    hipError_t e = mp ? hipMallocFromPoolAsync(&p, bytes, mp, s) : hipMallocAsync(&p, bytes, s);
+    hipError_t e;
+    if (pool_mode == 1) {
+        cls = size_class(bytes);
+        p = cache_take(cls, s);
+        e = hipSuccess;
+        if (!p) {
+            e = hipMalloc(&p, cls);
+            if (e != hipSuccess) {  // the cache back to the driver, then once more
+                (void)hipGetLastError();
+                for (auto st : streams_) (void)hipStreamSynchronize(st);
+                cache_release(0);
+                e = hipMalloc(&p, cls);
+            }
+        }
+        bytes = cls;
+    } else {
+        e = hipMallocAsync(&p, bytes, s);
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         std::ostringstream os;
@@ -189,6 +263,7 @@ BufPtr Context::alloc(size_t words, hipStream_t s) {
     b->ptr = (u64*)p;
     b->words = words;
     b->stream = s;
+    b->cls = cls;
     return b;
 }
 
@@ -211,8 +286,11 @@ void Context::free_buffer(Buffer* b) {
             (void)hipStreamSynchronize(cur);
         }
     }
-    (void)hipFreeAsync(b->ptr, b->stream);
-    live_bytes -= std::max<size_t>(b->words, 1) * sizeof(u64);
+    if (b->cls)
+        cache_put(b->ptr, b->cls, b->stream);
+    else
+        (void)hipFreeAsync(b->ptr, b->stream);
+    live_bytes -= b->cls ? b->cls : std::max<size_t>(b->words, 1) * sizeof(u64);
     b->ptr = nullptr;
 }
 

@@ -5,6 +5,7 @@
 #include <atomic>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -39,6 +40,7 @@ struct Buffer {
     u64* ptr;
     size_t words;
     hipStream_t stream;
+    size_t cls = 0;  // size class of a block from the context's cache (0: from the default pool)
     ~Buffer();
 };
 using BufPtr = std::shared_ptr<Buffer>;
@@ -199,11 +201,13 @@ class Context : public std::enable_shared_from_this<Context> {
     // Device memory is stream-ordered: a buffer is freed on the stream it was allocated on. A
     // release on another thread's stream first makes the allocation stream wait for the work
     // enqueued so far on the releasing thread's stream (the buffer's last readers there).
-    // Tuning key "pool": 1 (default) = the context's own pool, where a block freed on one stream
-    // is handed to another stream only once its free has completed on the device (reuse on the
-    // freeing stream follows that stream's order); 0 = the device's default pool with the
-    // runtime's default reuse rules (which may hand a block to another stream before its free
-    // has run, behind a dependency the runtime inserts).
+    // Tuning key "pool": 1 (default) = the context's block cache: device blocks in size classes
+    // (four per power of two), kept for reuse instead of returned to the driver (whose pools
+    // released memory at synchronizations and stalled the host for the whole device queue when
+    // growing under load); a freed block serves its own stream at once (stream order) and any
+    // other stream once an event recorded after its last use has completed; 0 = the device's
+    // default stream-ordered pool with the runtime's reuse rules (which may hand a block to
+    // another stream before its free has run, behind a dependency the runtime inserts).
     BufPtr alloc(size_t words, hipStream_t s);
     BufPtr alloc(size_t words) { return alloc(words, stream()); }
     void free_buffer(Buffer* b);
@@ -308,7 +312,20 @@ class Context : public std::enable_shared_from_this<Context> {
     void clear_plans();
 
     std::vector<hipStream_t> streams_;  // caller streams, then the service streams
-    hipMemPool_t ctx_pool_ = nullptr;   // pool 1 (null: the default pool serves it)
+    // The block cache of pool mode 1: free blocks per size class, each with the stream of its
+    // last use and an event recorded there after it.
+    struct Cached {
+        void* p;
+        hipStream_t s;
+        hipEvent_t ready;
+    };
+    std::mutex cache_mu_;
+    std::map<size_t, std::vector<Cached>> cache_;
+    std::vector<hipEvent_t> cache_events_;
+    u64 cache_bytes_ = 0, cache_limit_ = ~0ull;
+    void* cache_take(size_t cls, hipStream_t s);
+    void cache_put(void* p, size_t cls, hipStream_t s);
+    void cache_release(u64 keep_bytes);  // after every stream has finished
     u32 n_user_streams_ = 0;
     std::vector<void*> perm_allocs_;
     std::vector<std::vector<u32>> modup_plan_;  // [level][digit]

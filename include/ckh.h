@@ -101,17 +101,19 @@ ckh_status ckh_autotune_ks(ckh_context *ctx, uint32_t n, const ckh_poly *const *
  * workgroup); "fuse_ip" (1 = row pass fused into the inner product); "small_limbs" (batched
  * switches of at most this many input limbs in all, a single switch for one, use their own
  * schedule: "conv_variant_small", "conv_tg_small", "fuse_ip_small"); "pool" (1 = the context's
- * own memory pool, where a block freed on one stream serves another stream only once the free
- * has completed on the device, the default; 0 = the device's default pool); "service_streams"
+ * block cache, the default: freed device blocks stay cached per size class and serve their own
+ * stream at once and another stream once their last use has completed on the device; 0 = the
+ * device's default stream-ordered pool); "service_streams"
  * (1..4 streams the serving loop spreads its batches over, default 4); test hook "fail_alloc_after"
  * (the n-th device allocation from now fails with CKH_ERR_DEVICE, as an out-of-memory would;
  * negative = off). */
 ckh_status ckh_set_tuning(ckh_context *ctx, const char *key, int64_t value);
 /* CudaDevice::pool_used_memory / set_pool_release_threshold / trim_pool
- * (crates/ckks-train-cuda/src/device.rs:478-527) for the context's memory pool (and the
- * device's default pool): bytes in use; bytes kept reserved after a synchronization (default:
- * all, so temporaries are reused); and, after every stream has finished, the reserve beyond
- * keep_bytes returned to the driver (call after one-off peaks on a shared GPU). */
+ * (crates/ckks-train-cuda/src/device.rs:478-527) for the context's block cache (and the
+ * device's default pool): bytes in use; bytes kept cached when freed (default: all, so
+ * temporaries are reused; beyond it a freed block goes back to the driver once its last use has
+ * finished); and, after every stream has finished, the cache beyond keep_bytes returned to the
+ * driver (call after one-off peaks on a shared GPU). */
 ckh_status ckh_pool_used_memory(ckh_context *ctx, uint64_t *bytes);
 ckh_status ckh_set_pool_release_threshold(ckh_context *ctx, uint64_t bytes);
 ckh_status ckh_trim_pool(ckh_context *ctx, uint64_t keep_bytes);
