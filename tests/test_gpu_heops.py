@@ -322,10 +322,17 @@ def test_pool_accounting_and_trim():
     del polys
     ctx.synchronize()
     assert ctx.pool_used_memory() < used
-    ctx.set_pool_release_threshold(256 << 20)
-    ctx.trim_pool(0)
+    ctx.trim_pool(0)  # the cached blocks back to the driver
+    again = [ctx.uniform(L, seed=i) for i in range(8)]  # allocations after a trim work
+    assert all(a.download().shape == (L + 1, ref.n) for a in again)
+    del again
+    # under a release threshold of zero freed blocks go straight back to the driver
+    ctx.set_pool_release_threshold(0)
+    q = ctx.uniform(L, seed=98)
+    del q
+    ctx.synchronize()
     ctx.set_pool_release_threshold((1 << 64) - 1)
-    p = ctx.uniform(L, seed=99)  # allocation after a trim works
+    p = ctx.uniform(L, seed=99)
     assert p.download().shape == (L + 1, ref.n)
 
 
