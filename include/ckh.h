@@ -15,8 +15,9 @@
  * be used at once by later calls; downloads and ckh_synchronize wait. A handle used on several
  * streams must be released from the thread whose stream used it last (or after a synchronize).
  * Ownership: handles are reference counted (`retain` = the reference's cheap `Clone`,
- * ops.rs:17-20); device memory is returned to a stream-ordered pool on the last release (on the
- * stream it was allocated on, after the work queued so far on the releasing thread's stream).
+ * ops.rs:17-20); device memory goes back to the context's block cache on the last release, in
+ * the order of the stream it was allocated on, after the work queued so far on the releasing
+ * thread's stream; it is reused from there, not returned to the driver until ckh_trim_pool.
  */
 #ifndef CKH_H
 #define CKH_H
