@@ -364,6 +364,31 @@ def measured_traffic():
     return None
 
 
+def copy_peak(ev, stream, nbytes=2 << 30, reps=5):
+    """A device-to-device copy of 2 GiB (eight times the 256 MiB last-level cache) on the
+    library's stream, timed with HIP events: the measured HBM rate (bytes read + written)."""
+    h = ev.h
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    h.hipFree.argtypes = [C.c_void_p]
+    h.hipStreamSynchronize.argtypes = [C.c_void_p]
+    a, b = C.c_void_p(), C.c_void_p()
+    assert h.hipMalloc(C.byref(a), nbytes) == 0 and h.hipMalloc(C.byref(b), nbytes) == 0
+    try:
+        assert h.hipMemcpyAsync(b, a, nbytes, 3, stream) == 0  # warm (hipMemcpyDeviceToDevice)
+        e0, e1 = ev.event(), ev.event()
+        ev.record(e0, stream)
+        for _ in range(reps):
+            assert h.hipMemcpyAsync(b, a, nbytes, 3, stream) == 0
+        ev.record(e1, stream)
+        ms = ev.elapsed_ms(e0, e1) / reps
+    finally:
+        assert h.hipStreamSynchronize(stream) == 0
+        h.hipFree(a)
+        h.hipFree(b)
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
 def roofline_only(ctx, ev, reps=3):
     """The roofline launch alone (forward NTT over 8 polynomials x 30 limbs), then a
     device-to-device copy of the same byte count: the copy calibrates the HBM byte counters
@@ -526,6 +551,8 @@ def main():
     ntt_bytes = 16 * N * limbs
     achieved = ntt_bytes / (ntt_ms * 1e-3) / 1e9
     del tpolys
+    ctx.synchronize()
+    copy_gbs = copy_peak(ev, stream)
     # the same launch over data primes only (240 x 50-bit limbs, the f64 path)
     qpolys = [ctx.uniform(23, seed=SEED + 19 + i, ntt=False) for i in range(10)]
     ntt_q_ms = ctx.time_ntt(qpolys, forward=True, reps=20)
@@ -574,6 +601,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
             "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
             "ms_per_launch": ntt_ms, "bytes_per_launch": ntt_bytes,
+            "measured_copy_GBps": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs,
         },
         "ntt_forward_q_primes_only": ntt_q,
         "tuning": tuning,
