@@ -310,6 +310,29 @@ def config_table(ckh, O, ev):
     del ctx, key, dd, kb, ka, hkey
 
     # C4: the hoisted BSGS product
+    ref4, L, baby, giant, steps, seeds, c0, c1, ents_h, times, got = bsgs_c4(ckh)
+    cache = {}
+
+    def key_of_step(st):
+        if st not in cache:
+            cache[st] = ref4.uniform_key(L, seeds[st])
+        return cache[st]
+
+    c4_n, want = cpu_time(lambda: oracle_matvec(ref4, c0, c1, L, ents_h, baby, giant, 1, key_of_step, L), threads)
+    g = got.download()
+    out["C4_hoisted_bsgs_n32768_32x32"] = {
+        "gpu_ms": float(np.median(times)), "cpu_threads_ms": c4_n * 1e3, "cpu_1_thread_ms": None,
+        "rotations": len(steps), "bit_exact": bool((g[0] == want[0]).all() and (g[1] == want[1]).all()),
+        "note": "the 1-thread oracle run of this product (62 key switches at 21 limbs) is left out for time"}
+    out["cpu_threads"] = threads
+    return out
+
+
+def bsgs_c4(ckh, reps=5):
+    """The C4 product: N = 2^15 (20 x 50-bit + 4 x 60-bit, dnum 5), 32 x 32 BSGS blocks with the
+    diagonals at baby steps k1 and k1 + 16 of block k1 (64 entries), 62 rotation keys; one
+    warm-up product, then reps products timed by wall clock around a synchronize."""
+    from gpu_cases import C4, c4_entries_mask, host_uniform, make
     ctx4, ref4 = make(C4)
     L = len(ref4.q) - 1
     baby = giant = 32
@@ -327,26 +350,12 @@ def config_table(ckh, O, ev):
     e4.matvec_plain(x, ents_d, 2.0 ** 40, baby, giant)
     ctx4.synchronize()
     times = []
-    for _ in range(5):
+    for _ in range(reps):
         t0 = time.perf_counter()
         got = e4.matvec_plain(x, ents_d, 2.0 ** 40, baby, giant)
         ctx4.synchronize()
         times.append((time.perf_counter() - t0) * 1e3)
-    cache = {}
-
-    def key_of_step(st):
-        if st not in cache:
-            cache[st] = ref4.uniform_key(L, seeds[st])
-        return cache[st]
-
-    c4_n, want = cpu_time(lambda: oracle_matvec(ref4, c0, c1, L, ents_h, baby, giant, 1, key_of_step, L), threads)
-    g = got.download()
-    out["C4_hoisted_bsgs_n32768_32x32"] = {
-        "gpu_ms": float(np.median(times)), "cpu_threads_ms": c4_n * 1e3, "cpu_1_thread_ms": None,
-        "rotations": len(steps), "bit_exact": bool((g[0] == want[0]).all() and (g[1] == want[1]).all()),
-        "note": "the 1-thread oracle run of this product (62 key switches at 21 limbs) is left out for time"}
-    out["cpu_threads"] = threads
-    return out
+    return ref4, L, baby, giant, steps, seeds, c0, c1, ents_h, times, got
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "ntt_forward_240limbs_n65536.md")
@@ -427,6 +436,8 @@ def main():
     ap.add_argument("--configs", action="store_true",
                     help="print the per-configuration figures (NTT round trip, one key switch, the hoisted "
                          "BSGS product; GPU and CPU oracle) as one JSON line and exit")
+    ap.add_argument("--bsgs-only", action="store_true",
+                    help="run only the hoisted BSGS product at N=2^15 (3 timed products, for traces)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the dominant-kernel launch and a device copy of the same size (for counter runs)")
     args = ap.parse_args()
@@ -440,6 +451,10 @@ def main():
 
     if args.configs:
         print(json.dumps(config_table(ckh, O, None)), flush=True)
+        return
+    if args.bsgs_only:
+        times = bsgs_c4(ckh, reps=3)[-2]
+        print(json.dumps({"bsgs_c4_ms": times}), flush=True)
         return
     q, p = ckh.params_primes(LOG_N, Q_BITS, P_BITS)
     ctx = ckh.Context(LOG_N, q, p, DNUM)
