@@ -241,7 +241,12 @@ BufPtr Context::alloc(size_t words, hipStream_t s) {
             e = hipMalloc(&p, cls);
             if (e != hipSuccess) {  // the cache back to the driver, then once more
                 (void)hipGetLastError();
-                for (auto st : streams_) (void)hipStreamSynchronize(st);
+                {
+                    // no graph capture runs meanwhile (captures hold the plan lock, and a
+                    // synchronize would end one)
+                    std::lock_guard<std::mutex> lk(plan_mu_);
+                    for (auto st : streams_) (void)hipStreamSynchronize(st);
+                }
                 cache_release(0);
                 e = hipMalloc(&p, cls);
             }
