@@ -225,8 +225,7 @@ ckh_status ckh_trim_pool(ckh_context* ctx, uint64_t keep_bytes) {
 
 ckh_status ckh_synchronize(ckh_context* ctx) {
     return guarded([&] {
-        Context& c = ctx_of(ctx);
-        for (uint32_t i = 0; i < c.n_all_streams(); ++i) hip_check(hipStreamSynchronize(c.stream_at(i)), "synchronize");
+        ctx_of(ctx).synchronize_all();
     });
 }
 

@@ -218,9 +218,14 @@ void Context::set_pool_release_threshold(u64 bytes) {
     hip_check(hipMemPoolSetAttribute(dp, hipMemPoolAttrReleaseThreshold, &v), "pool threshold");
 }
 
+void Context::synchronize_all() {
+    std::lock_guard<std::mutex> lk(plan_mu_);
+    for (auto st : streams_) hip_check(hipStreamSynchronize(st), "synchronize");
+}
+
 void Context::trim_pool(u64 keep_bytes) {
     hip_check(hipSetDevice(device), "hipSetDevice");
-    for (auto st : streams_) hip_check(hipStreamSynchronize(st), "synchronize");
+    synchronize_all();
     cache_release(keep_bytes);
     hipMemPool_t dp;
     hip_check(hipDeviceGetDefaultMemPool(&dp, device), "hipDeviceGetDefaultMemPool");
@@ -242,9 +247,7 @@ BufPtr Context::alloc(size_t words, hipStream_t s) {
             if (e != hipSuccess) {  // the cache back to the driver, then once more
                 (void)hipGetLastError();
                 {
-                    // no graph capture runs meanwhile (captures hold the plan lock, and a
-                    // synchronize would end one)
-                    std::lock_guard<std::mutex> lk(plan_mu_);
+                    std::lock_guard<std::mutex> lk(plan_mu_);  // as in synchronize_all
                     for (auto st : streams_) (void)hipStreamSynchronize(st);
                 }
                 cache_release(0);

@@ -196,6 +196,9 @@ class Context : public std::enable_shared_from_this<Context> {
     u32 service_streams = SERVICE_STREAMS;
     u32 n_streams() const { return n_user_streams_; }
     u32 n_all_streams() const { return (u32)streams_.size(); }
+    // Waits for every stream, the serving loop's included, outside any graph capture (captures
+    // hold the plan lock; synchronizing a capturing stream would end its capture).
+    void synchronize_all();
     hipStream_t stream_at(u32 i) const { return streams_[i]; }
 
     // Device memory is stream-ordered: a buffer is freed on the stream it was allocated on. A
