@@ -7,6 +7,23 @@
 
 namespace ckh {
 
+// Milliseconds per call of f over reps calls queued back to back on s (HIP events).
+template <class F>
+static double time_on(hipStream_t s, u32 reps, F&& f) {
+    hipEvent_t a, b;
+    hip_check(hipEventCreate(&a), "event");
+    hip_check(hipEventCreate(&b), "event");
+    hip_check(hipEventRecord(a, s), "event");
+    for (u32 r = 0; r < reps; ++r) f();
+    hip_check(hipEventRecord(b, s), "event");
+    hip_check(hipEventSynchronize(b), "event");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, a, b), "event");
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return (double)ms / reps;
+}
+
 void Context::set_tuning(const std::string& key, int64_t value) {
     if (key == "fail_alloc_after") {  // a test hook (negative = off); captured plans stay valid
         fail_alloc_after = value;
@@ -118,18 +135,7 @@ std::string Context::autotune_small(const Poly& d, const Ksk& key) {
     auto time_one = [&]() {
         clear_plans();
         for (u32 r = 0; r < 3; ++r) (void)ks_switch(*this, d, key);  // direct run, capture, replay
-        hipEvent_t a, b;
-        hip_check(hipEventCreate(&a), "event");
-        hip_check(hipEventCreate(&b), "event");
-        hip_check(hipEventRecord(a, s), "event");
-        for (u32 r = 0; r < 10; ++r) (void)ks_switch(*this, d, key);
-        hip_check(hipEventRecord(b, s), "event");
-        hip_check(hipEventSynchronize(b), "event");
-        float ms = 0;
-        hip_check(hipEventElapsedTime(&ms, a, b), "event");
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-        return (double)ms / 10;
+        return time_on(s, 10, [&] { (void)ks_switch(*this, d, key); });
     };
     std::ostringstream os;
     os << "{";
@@ -169,18 +175,7 @@ std::string Context::autotune_ks(const std::vector<const Poly*>& ds, const Ksk& 
     auto time_ks = [&]() {
         clear_plans();
         (void)ks_switch_batch(*this, ds, key);  // warm
-        hipEvent_t a, b;
-        hip_check(hipEventCreate(&a), "event");
-        hip_check(hipEventCreate(&b), "event");
-        hip_check(hipEventRecord(a, s), "event");
-        for (u32 r = 0; r < 3; ++r) (void)ks_switch_batch(*this, ds, key);
-        hip_check(hipEventRecord(b, s), "event");
-        hip_check(hipEventSynchronize(b), "event");
-        float ms = 0;
-        hip_check(hipEventElapsedTime(&ms, a, b), "event");
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-        return (double)ms / 3;
+        return time_on(s, 3, [&] { (void)ks_switch_batch(*this, ds, key); });
     };
     std::ostringstream os;
     os << "{";
