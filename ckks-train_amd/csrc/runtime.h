@@ -32,8 +32,8 @@ void hip_check(hipError_t e, const char* what);
 class Context;
 
 
-// A device allocation from the context's stream-ordered pool; freed on its stream when the
-// last reference goes.
+// A device allocation from the context's block cache (or the default pool); released in its
+// stream's order when the last reference goes.
 struct Buffer {
     Context* ctx;
     std::shared_ptr<Context> keep;  // a buffer keeps its context (and streams) alive
