@@ -50,3 +50,43 @@ def test_galois_elements_match_oracle():
         for step in (0, 1, 3, 5, -1, -7, n // 2, n // 2 + 3, 12345):
             assert ckh.galois_elt(log_n, step) == O.galois_elt(n, step)
         assert ckh.conjugation_elt(log_n) == O.conjugation_elt(n)
+
+
+STRUCTS = [("ckh_params", "Params"), ("ckh_op_counter", "OpCounter"), ("ckh_bsgs_shape", "BsgsShape"),
+           ("ckh_linear_stage", "LinearStageC"), ("ckh_server_config", "ServerConfig"),
+           ("ckh_request", "Request"), ("ckh_server_stats", "ServerStats")]
+
+
+def test_binding_structs_match_the_header_layout(tmp_path):
+    """Every struct the binding mirrors has the size and field offsets the C compiler gives the
+    header's struct (a mismatch would pass wrong values through the C ABI silently)."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "ckh.h"', "int main(void) {"]
+    for cname, pname in STRUCTS:
+        cls = getattr(ckh, pname)
+        lines.append(f'  printf("%zu", sizeof({cname}));')
+        lines += [f'  printf(" %zu", offsetof({cname}, {f}));' for f, _ in cls._fields_]
+        lines.append('  printf("\\n");')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines()
+    for (cname, pname), line in zip(STRUCTS, out):
+        cls = getattr(ckh, pname)
+        got = [C.sizeof(cls)] + [getattr(cls, f).offset for f, _ in cls._fields_]
+        assert got == [int(v) for v in line.split()], cname
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
