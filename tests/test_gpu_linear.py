@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import ckh
-from gpu_cases import (C2_SHAPE_N4096, C4_SHAPE_SMALL_N, bsgs_rotation_count, host_uniform, make,
+from gpu_cases import (C2, C2_SHAPE_N4096, C4_SHAPE_SMALL_N, bsgs_rotation_count, host_uniform, make,
                        oracle_coeff_to_slot, oracle_slot_to_coeff)
 
 pytestmark = pytest.mark.gpu
@@ -65,7 +65,7 @@ def _plan(ev, levels, s_in, target=DELTA):
     return out
 
 
-@pytest.mark.parametrize("case", [C4_SHAPE_SMALL_N, C2_SHAPE_N4096])
+@pytest.mark.parametrize("case", [C4_SHAPE_SMALL_N, C2_SHAPE_N4096, C2], ids=lambda c: c[0])
 def test_coeff_to_slot_matches_oracle(case):
     say("coeff_to_slot", case[0])
     shapes = [(2, 2, 1, 0), (4, 2, 2, -3)]
@@ -99,7 +99,7 @@ def test_coeff_to_slot_matches_oracle(case):
     assert c.ct_pt_mults == sum(masks[0]) + sum(masks[1]) + sum(masks2[1])
 
 
-@pytest.mark.parametrize("case", [C4_SHAPE_SMALL_N, C2_SHAPE_N4096])
+@pytest.mark.parametrize("case", [C4_SHAPE_SMALL_N, C2_SHAPE_N4096], ids=lambda c: c[0])
 def test_slot_to_coeff_matches_oracle(case):
     say("slot_to_coeff", case[0])
     shapes = [(4, 2, 1, -1), (2, 4, 4, 0)]
