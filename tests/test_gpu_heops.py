@@ -8,7 +8,7 @@ import pytest
 
 import ckh
 import oracle as O
-from gpu_cases import C4_SHAPE_SMALL_N, SMALL, galois_elt_host, host_uniform, make, oracle_bsgs, oracle_matvec
+from gpu_cases import C2, C4_SHAPE_SMALL_N, SMALL, galois_elt_host, host_uniform, make, oracle_bsgs, oracle_matvec
 
 pytestmark = pytest.mark.gpu
 
@@ -338,7 +338,7 @@ def test_pool_accounting_and_trim():
 
 # ---------------------------------------------------------------- mod-raise (bootstrapping)
 
-@pytest.mark.parametrize("case", [SMALL, C4_SHAPE_SMALL_N], ids=lambda c: c[0])
+@pytest.mark.parametrize("case", [SMALL, C4_SHAPE_SMALL_N, C2], ids=lambda c: c[0])
 def test_raise_from_level0_matches_oracle(case):
     ctx, ref = make(case)
     L = len(ref.q) - 1
@@ -356,7 +356,7 @@ def test_raise_from_level0_matches_oracle(case):
         ctx.upload(a, 0).raise_from_level0(L + 1)
 
 
-@pytest.mark.parametrize("case", [SMALL, C4_SHAPE_SMALL_N], ids=lambda c: c[0])
+@pytest.mark.parametrize("case", [SMALL, C4_SHAPE_SMALL_N, C2], ids=lambda c: c[0])
 def test_mod_raise_with_sse_matches_oracle(case):
     """Evaluator::mod_raise (bootstrap/modraise.rs:53-69): switch to the ephemeral key at level 0,
     raise both components, switch back at the top; and mod_raise_main_key."""
