@@ -52,8 +52,13 @@ void Context::set_tuning(const std::string& key, int64_t value) {
         conv_tg = (u32)std::min<int64_t>(std::max<int64_t>(value, 1), 64);
     else if (key == "graph_cache")
         graph_cache_max = (u32)std::max<int64_t>(value, 1);
-    else if (key == "pool")
-        pool_mode = value != 0;
+    else if (key == "pool") {
+        // The device's default stream-ordered pool (mode 0) gave wrong results in the served
+        // stream with batches on several streams at once, for a cause not yet shown; only the
+        // context's block cache may be selected.
+        if (value != 1) fail(CKH_ERR_INVALID_PARAMS, "pool mode 0 (the device's default pool) is disabled");
+        pool_mode = 1;
+    }
     else if (key == "service_streams")
         service_streams = (u32)std::min<int64_t>(std::max<int64_t>(value, 1), SERVICE_STREAMS);
     else

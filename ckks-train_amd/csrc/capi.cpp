@@ -28,7 +28,9 @@ struct ckh_ct {
 };
 struct ckh_evaluator {
     std::shared_ptr<Context> keep;
-    std::unique_ptr<Evaluator> e;
+    // Shared with the serving loop's requests: an evaluator destroyed by its handle while one of
+    // its requests is pending or in flight lives (with its keys) until that request completes.
+    std::shared_ptr<Evaluator> e;
 };
 struct ckh_server {
     std::unique_ptr<Server> s;
@@ -510,7 +512,7 @@ ckh_status ckh_evaluator_create(ckh_context* ctx, const ckh_ksk* relin, const ck
         }
         auto* h = new ckh_evaluator();
         h->keep = ctx->c;
-        h->e.reset(new Evaluator(c, r, g));
+        h->e = std::make_shared<Evaluator>(c, r, g);
         *out = h;
     });
 }
@@ -523,7 +525,7 @@ ckh_status ckh_evaluator_fork(ckh_evaluator* ev, ckh_evaluator** out) {
         for (auto& kv : e.galois) g.push_back(kv.second);
         auto* h = new ckh_evaluator();
         h->keep = ev->keep;
-        h->e.reset(new Evaluator(e.ctx, e.relin, g));
+        h->e = std::make_shared<Evaluator>(e.ctx, e.relin, g);
         h->e->config = e.config;
         *out = h;
     });
@@ -813,7 +815,8 @@ ckh_status ckh_server_submit(ckh_server* srv, const ckh_request* req, uint64_t* 
         need(ticket, "ticket");
         auto op = std::make_shared<ServedOp>();
         op->kind = req->kind;
-        op->ev = &ev_of(req->ev);
+        ev_of(req->ev);  // a valid handle
+        op->ev = req->ev->e;
         op->a = req->a ? req->a->c : nullptr;
         op->b = req->b ? req->b->c : nullptr;
         if (req->steps && req->n_steps) op->steps.assign(req->steps, req->steps + req->n_steps);

@@ -11,6 +11,11 @@ namespace ckh {
 
 // ------------------------------------------------------------------ polynomials
 
+// The calling thread's stream is about to queue work on this buffer: recorded so that the
+// buffer's free is ordered after it (Buffer::note_use, Context::free_buffer).
+static void use(const Context& c, const Poly& p) { p.buf->note_use(c.stream()); }
+static void use(const Context& c, const Ksk& k) { k.buf->note_use(c.stream()); }
+
 PolyPtr poly_alloc(Context& c, u32 level, u32 p_limbs, bool ntt) {
     if (level >= c.nq) fail(CKH_ERR_INVALID_PARAMS, "level above max_level");
     if (p_limbs != 0 && p_limbs != c.np) fail(CKH_ERR_INVALID_PARAMS, "p_limbs must be 0 or the special prime count");
@@ -53,12 +58,14 @@ PolyPtr poly_upload(Context& c, const u64* host, u32 level, u32 p_limbs, bool nt
 
 void poly_download(Context& c, const Poly& p, u64* host) {
     if (!host) fail(CKH_ERR_INVALID_PARAMS, "null output");
+    use(c, p);
     hipStream_t s = c.stream();
     hip_check(hipMemcpyAsync(host, p.data(), (u64)p.limbs() * c.n * 8, hipMemcpyDeviceToHost, s), "download");
     hip_check(hipStreamSynchronize(s), "download");
 }
 
 u64 poly_digest(Context& c, const Poly& p) {
+    use(c, p);
     hipStream_t s = c.stream();
     BufPtr acc = c.alloc(1, s);
     hip_check(hipMemsetAsync(acc->ptr, 0, 8, s), "memset");
@@ -78,6 +85,7 @@ PolyPtr poly_uniform(Context& c, u32 level, u32 p_limbs, bool ntt, u64 seed, u64
 
 PolyPtr poly_transform(Context& c, const Poly& in, bool forward) {
     if (in.ntt == forward) fail(CKH_ERR_INVALID_PARAMS, forward ? "already in NTT form" : "already in coefficient form");
+    use(c, in);
     PolyPtr out = poly_alloc(c, in.level, in.p_limbs, forward);
     c.ntt_poly(in.data(), out->data(), in.limbs(), in.level, in.p_limbs, forward, c.stream());
     hip_check(hipGetLastError(), "ntt");
@@ -87,6 +95,7 @@ PolyPtr poly_transform(Context& c, const Poly& in, bool forward) {
 PolyPtr poly_automorphism(Context& c, const Poly& in, u64 g) {
     if (!in.ntt) fail(CKH_ERR_INVALID_PARAMS, "automorphism needs NTT form");
     if (g % 2 == 0 || g >= 2 * c.n) fail(CKH_ERR_INVALID_PARAMS, "Galois element must be odd and < 2N");
+    use(c, in);
     PolyPtr out = poly_alloc(c, in.level, in.p_limbs, true);
     automorphism_ntt_launch(in.data(), out->data(), in.limbs(), g, c.log_n, c.stream());
     hip_check(hipGetLastError(), "automorphism");
@@ -95,11 +104,13 @@ PolyPtr poly_automorphism(Context& c, const Poly& in, u64 g) {
 
 double time_ntt(Context& c, const std::vector<const Poly*>& polys, bool forward, u32 reps) {
     std::vector<LimbRef> refs;
-    for (const Poly* p : polys)
+    for (const Poly* p : polys) {
+        use(c, *p);
         for (u32 i = 0; i < p->limbs(); ++i) {
             u32 prime = i <= p->level ? i : c.nq + (i - p->level - 1);
             refs.push_back({p->data() + i * c.n, p->data() + i * c.n, prime, 0});
         }
+    }
     if (refs.empty() || reps == 0) fail(CKH_ERR_INVALID_PARAMS, "nothing to time");
     hipStream_t s = c.stream();
     BufPtr d = c.upload_blob(refs.data(), refs.size() * sizeof(LimbRef), s);
@@ -122,6 +133,7 @@ double time_ntt(Context& c, const std::vector<const Poly*>& polys, bool forward,
 PolyPtr poly_rescale(Context& c, const Poly& in) {
     if (!in.ntt || in.p_limbs) fail(CKH_ERR_INVALID_PARAMS, "rescale needs an NTT-form Q polynomial");
     if (in.level == 0) fail(CKH_ERR_NO_LEVELS_LEFT, "cannot rescale at level 0");
+    use(c, in);
     PolyPtr out = poly_alloc(c, in.level - 1, 0, true);
     BufPtr scratch = c.alloc((u64)(1 + in.level) * c.n);
     c.rescale({{in.data(), in.level, out->data(), scratch->ptr, scratch->ptr + c.n}}, c.stream());
@@ -133,6 +145,7 @@ PolyPtr poly_rescale(Context& c, const Poly& in) {
 // each q_i and applied as a Shoup product), then RnsPoly::neg (poly.rs:157-161) when neg is set.
 PolyPtr poly_scalar_mul(Context& c, const Poly& a, u64 k, bool neg) {
     if (a.p_limbs) fail(CKH_ERR_INVALID_PARAMS, "scalar products take Q polynomials");
+    use(c, a);
     PolyPtr out = poly_alloc(c, a.level, 0, a.ntt);
     std::vector<u64> ws(2 * (a.level + 1));
     for (u32 i = 0; i <= a.level; ++i) {
@@ -151,6 +164,7 @@ PolyPtr poly_scalar_mul(Context& c, const Poly& a, u64 k, bool neg) {
 PolyPtr poly_raise(Context& c, const Poly& a, u32 top) {
     if (a.level != 0 || a.p_limbs) fail(CKH_ERR_INVALID_PARAMS, "raise_from_level0 needs a level-0 Q polynomial");
     if (top >= c.nq) fail(CKH_ERR_INVALID_PARAMS, "level out of range");
+    use(c, a);
     PolyPtr out = poly_alloc(c, top, 0, a.ntt);
     BufPtr x = c.alloc(c.n);
     c.raise_level0({{a.data(), top, out->data(), x->ptr, a.ntt}}, c.stream());
@@ -162,6 +176,8 @@ PolyPtr poly_pointwise(Context& c, PwOp op, const Poly& a, const Poly* b) {
     if (b && (b->level != a.level || b->p_limbs != a.p_limbs || b->ntt != a.ntt))
         fail(CKH_ERR_LEVEL_MISMATCH, "operands at different levels or forms");
     if (a.p_limbs) fail(CKH_ERR_INVALID_PARAMS, "pointwise ops take Q polynomials");
+    use(c, a);
+    if (b) use(c, *b);
     PolyPtr out = poly_alloc(c, a.level, 0, a.ntt);
     pointwise_launch(op, a.data(), b ? b->data() : nullptr, out->data(), a.level + 1, 0, c.d_pc, c.log_n, c.stream());
     hip_check(hipGetLastError(), "pointwise");
@@ -170,6 +186,7 @@ PolyPtr poly_pointwise(Context& c, PwOp op, const Poly& a, const Poly* b) {
 
 PolyPtr poly_mod_down(Context& c, const Poly& qp) {
     if (!qp.ntt || qp.p_limbs != c.np) fail(CKH_ERR_INVALID_PARAMS, "mod_down needs an NTT-form QpPoly");
+    use(c, qp);
     const u64 L = qp.level + 1;
     PolyPtr out = poly_alloc(c, qp.level, 0, true);
     BufPtr scratch = c.alloc((c.np + L) * c.n);
@@ -238,6 +255,8 @@ void check_switch(const Context& c, const Poly& d, const Ksk& key) {
     if (!d.ntt || d.p_limbs) fail(CKH_ERR_INVALID_PARAMS, "key switching needs an NTT-form Q polynomial");
     if (key.ctx != &c) fail(CKH_ERR_INVALID_PARAMS, "key is not over this context");
     if (d.level > key.max_level) fail(CKH_ERR_INVALID_PARAMS, "key does not serve this level");
+    use(c, d);
+    use(c, key);
 }
 
 // decompose + apply for a batch of requests, in one launch per stage.
@@ -278,6 +297,7 @@ void run_ks(Context& c, const std::vector<KsReq>& reqs) { run_ks_on(c, reqs, c.s
 
 std::vector<PolyPtr> ks_decompose(Context& c, const Poly& d) {
     if (!d.ntt || d.p_limbs) fail(CKH_ERR_INVALID_PARAMS, "key switching needs an NTT-form Q polynomial");
+    use(c, d);
     const u32 nd = c.ndigits(d.level);
     const u64 dw = c.digit_words(d.level);
     BufPtr digs = c.alloc(nd * dw);
@@ -303,7 +323,9 @@ std::pair<PolyPtr, PolyPtr> ks_apply(Context& c, const std::vector<PolyPtr>& dig
         if (d.level != level || d.p_limbs != c.np || !d.ntt)
             fail(CKH_ERR_INVALID_PARAMS, "digits must share one level and be NTT-form QpPolys");
         if (d.data() != digits[0]->data() + j * dw) contiguous = false;
+        use(c, d);
     }
+    use(c, key);
     hipStream_t s = c.stream();
     BufPtr packed;
     const u64* dig = digits[0]->data();
@@ -456,6 +478,7 @@ bool Evaluator::owns(const Ct& ct) const {
 
 void Evaluator::check_basis(const Ct& ct) const {
     if (!owns(ct)) fail(CKH_ERR_INVALID_PARAMS, "operand is not over this evaluator's data basis");
+    for (auto& p : ct.comps) use(ctx, *p);
 }
 
 static void check_level(u32 a, u32 b) {
@@ -489,6 +512,7 @@ u64 Evaluator::rotation_elt(int32_t steps) const {
 const Ksk& Evaluator::relin_key_checked(u32 level) const {
     if (!relin) fail(CKH_ERR_INVALID_PARAMS, "evaluator has no relinearization key");
     if (level > relin->max_level) fail(CKH_ERR_INVALID_PARAMS, "relinearization key does not serve this level");
+    use(ctx, *relin);
     return *relin;
 }
 
@@ -497,6 +521,7 @@ const Ksk& Evaluator::galois_key(u64 elt, u32 level) const {
     auto it = galois.find(elt);
     if (it == galois.end()) fail(CKH_ERR_INVALID_PARAMS, "missing galois key");
     if (level > it->second->max_level) fail(CKH_ERR_INVALID_PARAMS, "galois key does not serve this level");
+    use(ctx, *it->second);
     return *it->second;
 }
 
@@ -528,6 +553,7 @@ CtPtr Evaluator::negate(const Ct& a) {
 static void check_plain(const Context& c, const Poly& pt) {
     if (pt.buf->ctx != &c || pt.p_limbs || !pt.ntt)
         fail(CKH_ERR_INVALID_PARAMS, "operand is not over this evaluator's data basis");
+    use(c, pt);
 }
 
 // Evaluator::add_plain (evaluator.rs:287-296): c0 + p, scales within tolerance.

@@ -73,6 +73,7 @@ Context::Context(int ordinal, const ckh_params& p, u32 n_streams) {
         hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
         streams_.push_back(s);
     }
+    hip_check(hipStreamCreateWithFlags(&capture_stream_, hipStreamNonBlocking), "hipStreamCreate");
     build_tables();
 }
 
@@ -98,6 +99,7 @@ Context::~Context() {
     for (hipEvent_t e : cache_events_) (void)hipEventDestroy(e);
     for (void* a : perm_allocs_) (void)hipFree(a);
     for (auto s : streams_) (void)hipStreamDestroy(s);
+    if (capture_stream_) (void)hipStreamDestroy(capture_stream_);
 }
 
 // A thread's choice is kept per context address; an entry left by a destroyed context at the
@@ -278,20 +280,21 @@ BufPtr Context::alloc(size_t words, hipStream_t s) {
 void Context::free_buffer(Buffer* b) {
     if (!b->ptr) return;
     (void)hipSetDevice(device);
-    const hipStream_t cur = stream();
-    std::unique_lock<std::mutex> lk(plan_mu_, std::defer_lock);
-    if (cur != b->stream) {
-        // The releasing thread's stream may still read the buffer: the free waits for it. The
-        // allocation stream belongs to another thread; holding the plan lock keeps this off a
-        // graph capture in progress there (captures run under that lock).
-        lk.lock();
-        hipEvent_t ev = nullptr;
-        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
-            if (hipEventRecord(ev, cur) == hipSuccess) (void)hipStreamWaitEvent(b->stream, ev, 0);
-            (void)hipEventDestroy(ev);
-        } else {
-            (void)hipGetLastError();
-            (void)hipStreamSynchronize(cur);
+    // Streams whose queued work may still use the buffer: the releasing thread's and every other
+    // user recorded at use; the free (on the allocation stream) is ordered after each of them.
+    std::vector<hipStream_t> order = free_order(b->stream, stream(), b->users);
+    if (!order.empty()) {
+        // Holding the plan lock keeps these records off a graph capture in progress.
+        std::lock_guard<std::mutex> lk(plan_mu_);
+        for (hipStream_t r : order) {
+            hipEvent_t ev = nullptr;
+            if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
+                if (hipEventRecord(ev, r) == hipSuccess) (void)hipStreamWaitEvent(b->stream, ev, 0);
+                (void)hipEventDestroy(ev);
+            } else {
+                (void)hipGetLastError();
+                (void)hipStreamSynchronize(r);
+            }
         }
     }
     if (b->cls)
@@ -852,10 +855,13 @@ bool Context::execute_graph(const Program& p, hipStream_t s) {
     // after that replay in stream order; the host does not wait.
     h2d(pl->desc, p.bytes.data(), p.bytes.size(), s);
     if (!pl->exec) {
+        // Captured on the context's private stream (under plan_mu_, so no other thread touches
+        // it): work that other threads queue on s meanwhile cannot leak into the graph, and a
+        // synchronization of s elsewhere cannot end the capture. The graph is launched on s.
         hipGraph_t g = nullptr;
-        hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
-        run(p, (const uint8_t*)pl->desc, s);
-        const hipError_t e_end = hipStreamEndCapture(s, &g);
+        hip_check(hipStreamBeginCapture(capture_stream_, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+        run(p, (const uint8_t*)pl->desc, capture_stream_);
+        const hipError_t e_end = hipStreamEndCapture(capture_stream_, &g);
         hip_check(e_end, "hipStreamEndCapture");
         const hipError_t e_inst = hipGraphInstantiate(&pl->exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);

@@ -15,6 +15,7 @@
 #include "../../include/ckh.h"
 #include "basis.h"
 #include "common.h"
+#include "lifetime.h"
 #include "ntt.h"
 #include "ops.h"
 
@@ -41,6 +42,17 @@ struct Buffer {
     size_t words;
     hipStream_t stream;
     size_t cls = 0;  // size class of a block from the context's cache (0: from the default pool)
+    // The other streams that have had work on this buffer queued (a caller's input read on the
+    // serving loop's stream, a key read by every thread's stream): the free waits for each.
+    std::mutex use_mu;
+    std::vector<hipStream_t> users;
+    void note_use(hipStream_t s) {
+        if (s == stream) return;
+        std::lock_guard<std::mutex> lk(use_mu);
+        for (hipStream_t u : users)
+            if (u == s) return;
+        users.push_back(s);
+    }
     ~Buffer();
 };
 using BufPtr = std::shared_ptr<Buffer>;
@@ -201,9 +213,9 @@ class Context : public std::enable_shared_from_this<Context> {
     void synchronize_all();
     hipStream_t stream_at(u32 i) const { return streams_[i]; }
 
-    // Device memory is stream-ordered: a buffer is freed on the stream it was allocated on. A
-    // release on another thread's stream first makes the allocation stream wait for the work
-    // enqueued so far on the releasing thread's stream (the buffer's last readers there).
+    // Device memory is stream-ordered: a buffer is freed on the stream it was allocated on, after
+    // the allocation stream has been made to wait for the work enqueued so far on the releasing
+    // thread's stream and on every other stream that used the buffer (Buffer::users).
     // Tuning key "pool": 1 (default) = the context's block cache: device blocks in size classes
     // (four per power of two), kept for reuse instead of returned to the driver (whose pools
     // released memory at synchronizations and stalled the host for the whole device queue when
@@ -303,6 +315,7 @@ class Context : public std::enable_shared_from_this<Context> {
         u64 last_use = 0;
     };
     std::mutex plan_mu_;
+    hipStream_t capture_stream_ = nullptr;  // graphs are captured here (under plan_mu_), launched on the caller's
     std::vector<std::unique_ptr<Plan>> plans_;
     std::vector<std::unique_ptr<Plan>> retired_;  // evicted plans whose last replay may still run
     void sweep_retired(bool wait);

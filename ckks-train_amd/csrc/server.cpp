@@ -70,7 +70,7 @@ void Server::fail_op(const OpPtr& op, ckh_status st, const std::string& msg) {
 void Server::launch(std::vector<OpPtr>& ops) {
     std::vector<Evaluator*> evs;
     for (auto& op : ops)
-        if (std::find(evs.begin(), evs.end(), op->ev) == evs.end()) evs.push_back(op->ev);
+        if (std::find(evs.begin(), evs.end(), op->ev.get()) == evs.end()) evs.push_back(op->ev.get());
     for (Evaluator* ev : evs) {
         Context& c = ev->ctx;
         hip_check(hipSetDevice(c.device), "hipSetDevice");
@@ -78,7 +78,7 @@ void Server::launch(std::vector<OpPtr>& ops) {
         hipStream_t s = c.stream();
         std::vector<OpPtr> mine;
         for (auto& op : ops)
-            if (op->ev == ev) {
+            if (op->ev.get() == ev) {
                 mine.push_back(op);
                 (void)hipStreamWaitEvent(s, op->ready, 0);
             }

@@ -15,7 +15,7 @@ namespace ckh {
 struct ServedOp {
     u64 ticket = 0;
     u32 kind = 0;
-    Evaluator* ev = nullptr;
+    std::shared_ptr<Evaluator> ev;  // kept alive (with its keys) until the request completes
     CtPtr a, b;
     std::vector<int32_t> steps;
     bool inject_failure = false;

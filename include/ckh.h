@@ -102,9 +102,10 @@ ckh_status ckh_autotune_ks(ckh_context *ctx, uint32_t n, const ckh_poly *const *
  * workgroup); "fuse_ip" (1 = row pass fused into the inner product); "small_limbs" (batched
  * switches of at most this many input limbs in all, a single switch for one, use their own
  * schedule: "conv_variant_small", "conv_tg_small", "fuse_ip_small"); "pool" (1 = the context's
- * block cache, the default: freed device blocks stay cached per size class and serve their own
- * stream at once and another stream once their last use has completed on the device; 0 = the
- * device's default stream-ordered pool); "service_streams"
+ * block cache, the only mode: freed device blocks stay cached per size class and serve their own
+ * stream at once and another stream once their last use has completed on the device; 0, the
+ * device's default stream-ordered pool, is refused with CKH_ERR_INVALID_PARAMS: it gave wrong
+ * results with batches on several streams at once, for a cause not shown); "service_streams"
  * (1..4 streams the serving loop spreads its batches over, default 4); test hook "fail_alloc_after"
  * (the n-th device allocation from now fails with CKH_ERR_DEVICE, as an out-of-memory would;
  * negative = off). */

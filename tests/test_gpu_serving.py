@@ -132,18 +132,13 @@ def test_caller_keeps_working_while_serving():
     srv.close()
 
 
-def test_pool_modes_agree():
-    """The context's own pool (default) and the device's default pool give the same results."""
+def test_pool_mode_zero_refused():
+    """The device's default pool (mode 0) is switched off: selecting it is an InvalidParams error
+    and the context keeps its block cache."""
     envs = _setup(9500)
-    reqs = _requests(envs, 6, 95, 4000.0)
     ctx = envs[0][0]
     assert ctx.tuning("pool") == 1
-    first = [[o.digests() for o in _alone(envs, r)] for r in reqs]
-    for c, _, _ in envs:
-        c.set_tuning("pool", 0)
-    try:
-        second = [[o.digests() for o in _alone(envs, r)] for r in reqs]
-    finally:
-        for c, _, _ in envs:
-            c.set_tuning("pool", 1)
-    assert first == second
+    with pytest.raises(ckh.InvalidParams):
+        ctx.set_tuning("pool", 0)
+    assert ctx.tuning("pool") == 1
+    ctx.set_tuning("pool", 1)
