@@ -152,6 +152,7 @@ _SIGS = {
     "ckh_apply_key": (C.c_int, [VP, C.POINTER(VP), u32, VP, C.POINTER(VP), C.POINTER(VP)]),
     "ckh_key_switch": (C.c_int, [VP, VP, VP, C.POINTER(VP), C.POINTER(VP)]),
     "ckh_key_switch_batch": (C.c_int, [VP, u32, C.POINTER(VP), VP, C.POINTER(VP), C.POINTER(VP)]),
+    "ckh_key_switch_batch_static": (C.c_int, [VP, u32, C.POINTER(VP), VP, C.POINTER(VP), C.POINTER(VP)]),
     "ckh_ct_upload": (C.c_int, [VP, P64, u32, u32, C.c_double, C.POINTER(VP)]),
     "ckh_ct_from_polys": (C.c_int, [VP, C.POINTER(VP), u32, C.c_double, C.POINTER(VP)]),
     "ckh_ct_download": (C.c_int, [VP, VP, P64]),
@@ -389,6 +390,15 @@ class Context:
         kb, ka = (VP * n)(), (VP * n)()
         _check(lib().ckh_key_switch_batch(self.h, n, arr, key.h, kb, ka))
         return [(Poly(self, VP(kb[i])), Poly(self, VP(ka[i]))) for i in range(n)]
+
+    def key_switch_batch_static(self, ds, key, outs):
+        """The batched switch into existing outputs [(kb, ka), ...] (overwritten in place), with
+        no device allocation: intermediates in the GPU's fixed workspace."""
+        n = len(ds)
+        arr = (VP * n)(*[d.h for d in ds])
+        kb = (VP * n)(*[o[0].h for o in outs])
+        ka = (VP * n)(*[o[1].h for o in outs])
+        _check(lib().ckh_key_switch_batch_static(self.h, n, arr, key.h, kb, ka))
 
     def time_ntt(self, polys, forward=True, reps=20):
         """Mean ms per batched in-place NTT launch over every limb of `polys` (HIP events on

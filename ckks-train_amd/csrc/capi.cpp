@@ -424,6 +424,23 @@ ckh_status ckh_key_switch_batch(ckh_context* ctx, uint32_t n, const ckh_poly* co
     });
 }
 
+ckh_status ckh_key_switch_batch_static(ckh_context* ctx, uint32_t n, const ckh_poly* const* d, const ckh_ksk* key,
+                                       ckh_poly* const* kb, ckh_poly* const* ka) {
+    return guarded([&] {
+        need(d, "d");
+        need(kb, "kb");
+        need(ka, "ka");
+        std::vector<const Poly*> v;
+        std::vector<Poly*> ob, oa;
+        for (uint32_t i = 0; i < n; ++i) {
+            v.push_back(&poly_of(d[i]));
+            ob.push_back(const_cast<Poly*>(&poly_of(kb[i])));
+            oa.push_back(const_cast<Poly*>(&poly_of(ka[i])));
+        }
+        ks_switch_batch_static(ctx_of(ctx), v, key_of(key), ob, oa);
+    });
+}
+
 ckh_status ckh_ct_upload(ckh_context* ctx, const uint64_t* polys, uint32_t n_comp, uint32_t level, double scale,
                          ckh_ct** out) {
     return guarded([&] {

@@ -259,19 +259,17 @@ void check_switch(const Context& c, const Poly& d, const Ksk& key) {
     use(c, key);
 }
 
-// decompose + apply for a batch of requests, in one launch per stage.
-static void run_ks_on(Context& c, const std::vector<KsReq>& reqs, hipStream_t s,
-                      const std::vector<AutJob>* pre = nullptr) {
-    u64 total = 0;
-    for (auto& r : reqs) total += ks_scratch_words(c, r.d->level, true);
-    BufPtr scratch = c.alloc(total, s);
+// The program of a batch of requests (decompose + apply, one launch per stage), its scratch laid
+// out from base (ks_scratch_words per request, in request order).
+static Program ks_program(Context& c, const std::vector<KsReq>& reqs, u64* base,
+                          const std::vector<AutJob>* pre = nullptr) {
     u64 limbs = 0;
     for (auto& r : reqs) limbs += r.d->level + 1;
     const bool small = limbs <= c.small_limbs;
     const bool fuse = (small ? c.fuse_ip_s : c.fuse_ip) && c.fuse_ip_possible();
     std::vector<DecTask> dec;
     std::vector<ApplyTask> app;
-    u64* p = scratch->ptr;
+    u64* p = base;
     for (auto& r : reqs) {
         const u64 L = r.d->level + 1, k = c.np;
         u64* x = p;
@@ -288,7 +286,16 @@ static void run_ks_on(Context& c, const std::vector<KsReq>& reqs, hipStream_t s,
     c.prog_decompose(prog, dec);
     c.prog_apply(prog, app);
     if (small) prog.set_conv_schedule(c.conv_variant_s, c.conv_tg_s);
-    c.execute(prog, s);
+    return prog;
+}
+
+// decompose + apply for a batch of requests, in one launch per stage.
+static void run_ks_on(Context& c, const std::vector<KsReq>& reqs, hipStream_t s,
+                      const std::vector<AutJob>* pre = nullptr) {
+    u64 total = 0;
+    for (auto& r : reqs) total += ks_scratch_words(c, r.d->level, true);
+    BufPtr scratch = c.alloc(total, s);
+    c.execute(ks_program(c, reqs, scratch->ptr, pre), s);
     hip_check(hipGetLastError(), "key switch");
 }
 
@@ -375,6 +382,49 @@ std::vector<std::pair<PolyPtr, PolyPtr>> ks_switch_batch(Context& c, const std::
     }
     run_ks(c, reqs);
     return res;
+}
+
+void ks_switch_batch_static(Context& c, const std::vector<const Poly*>& ds, const Ksk& key,
+                            const std::vector<Poly*>& kb, const std::vector<Poly*>& ka) {
+    if (kb.size() != ds.size() || ka.size() != ds.size()) fail(CKH_ERR_INVALID_PARAMS, "one output pair per input");
+    // Every word range the call reads or writes: inputs, then outputs; outputs may overlap nothing.
+    std::vector<std::pair<const u64*, const u64*>> in_r, out_r;
+    u64 words = 0;
+    for (size_t i = 0; i < ds.size(); ++i) {
+        const Poly& d = *ds[i];
+        check_switch(c, d, key);
+        for (const Poly* o : {kb[i], ka[i]}) {
+            if (o->buf->ctx != &c || o->p_limbs || !o->ntt || o->level != d.level)
+                fail(CKH_ERR_INVALID_PARAMS, "outputs must be NTT-form Q polynomials of this context at the input's level");
+            out_r.push_back({o->data(), o->data() + (u64)(o->level + 1) * c.n});
+        }
+        in_r.push_back({d.data(), d.data() + (u64)(d.level + 1) * c.n});
+        words += ks_scratch_words(c, d.level, true);
+    }
+    if (ds.empty()) return;
+    std::sort(out_r.begin(), out_r.end());
+    for (size_t i = 1; i < out_r.size(); ++i)
+        if (out_r[i].first < out_r[i - 1].second) fail(CKH_ERR_INVALID_PARAMS, "outputs overlap each other");
+    for (auto& o : out_r)
+        for (auto& r : in_r)
+            if (o.first < r.second && r.first < o.second) fail(CKH_ERR_INVALID_PARAMS, "an output overlaps an input");
+    const u64 scratch_bytes = words * 8;
+    if (scratch_bytes > CKH_STATIC_SCRATCH_BYTES) fail(CKH_ERR_INVALID_PARAMS, "batch needs more than the static workspace");
+    std::vector<KsReq> reqs;
+    for (size_t i = 0; i < ds.size(); ++i) {
+        use(c, *kb[i]);
+        use(c, *ka[i]);
+        reqs.push_back({ds[i], &key, nullptr, nullptr, kb[i]->data(), ka[i]->data()});
+    }
+    // Workspace: the switches' scratch from the start, the descriptor bytes after it.
+    Program prog = ks_program(c, reqs, (u64*)c.static_scratch);
+    const u64 desc_off = (scratch_bytes + 255) & ~(u64)255;
+    if (desc_off + prog.bytes.size() > CKH_STATIC_SCRATCH_BYTES)
+        fail(CKH_ERR_INVALID_PARAMS, "batch needs more than the static workspace");
+    hipStream_t s = c.stream();
+    c.h2d(c.static_scratch + desc_off, prog.bytes.data(), prog.bytes.size(), s);
+    c.run(prog, c.static_scratch + desc_off, s);
+    hip_check(hipGetLastError(), "key switch");
 }
 
 u64 galois_elt(u64 n, int32_t step) {

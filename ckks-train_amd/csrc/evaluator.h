@@ -32,6 +32,10 @@ std::pair<PolyPtr, PolyPtr> ks_switch(Context& c, const Poly& d, const Ksk& key)
 // n switches under one key in batched launches.
 std::vector<std::pair<PolyPtr, PolyPtr>> ks_switch_batch(Context& c, const std::vector<const Poly*>& d,
                                                          const Ksk& key);
+// The same switches into caller-given outputs, with no device allocation: intermediates and
+// descriptors in the GPU's fixed workspace (Context::static_scratch), launched without a graph.
+void ks_switch_batch_static(Context& c, const std::vector<const Poly*>& d, const Ksk& key,
+                            const std::vector<Poly*>& kb, const std::vector<Poly*>& ka);
 KskPtr ksk_upload(Context& c, const u64* host, u32 max_level, u64 galois);
 KskPtr ksk_uniform(Context& c, u32 max_level, u64 galois, u64 seed);
 

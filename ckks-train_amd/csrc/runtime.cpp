@@ -74,7 +74,21 @@ Context::Context(int ordinal, const ckh_params& p, u32 n_streams) {
         streams_.push_back(s);
     }
     hip_check(hipStreamCreateWithFlags(&capture_stream_, hipStreamNonBlocking), "hipStreamCreate");
+    // The pinned staging ring of h2d and a stock of its events, made here so that uploads
+    // (descriptors of the no-allocation key switch included) allocate nothing later.
+    {
+        ring_cap_ = (size_t)16 << 20;
+        void* rp = nullptr;
+        hip_check(hipHostMalloc(&rp, ring_cap_, hipHostMallocDefault), "hipHostMalloc(staging ring)");
+        ring_ = (uint8_t*)rp;
+        for (int i = 0; i < 32; ++i) {
+            hipEvent_t ev;
+            hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
+            ring_events_.push_back(ev);
+        }
+    }
     build_tables();
+    static_scratch = static_scratch_address(device);
 }
 
 Context::~Context() {
@@ -334,12 +348,6 @@ size_t Context::ring_reserve(size_t len) {
 void Context::h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
     if (bytes == 0) return;
     std::lock_guard<std::mutex> lk(ring_mu_);
-    if (!ring_) {
-        ring_cap_ = (size_t)16 << 20;
-        void* p = nullptr;
-        hip_check(hipHostMalloc(&p, ring_cap_, hipHostMallocDefault), "hipHostMalloc(staging ring)");
-        ring_ = (uint8_t*)p;
-    }
     const size_t chunk_max = ring_cap_ / 4;
     for (size_t done = 0; done < bytes;) {
         const size_t len = std::min(chunk_max, bytes - done);

@@ -275,6 +275,8 @@ class Context : public std::enable_shared_from_this<Context> {
     std::string autotune_small(const Poly& d, const Ksk& key);
     void set_tuning(const std::string& key, int64_t value);
     int64_t get_tuning(const std::string& key) const;
+    // This GPU's fixed workspace (ckh_key_switch_batch_static), looked up at context creation.
+    uint8_t* static_scratch = nullptr;
     // Copies a host blob to a fresh device buffer on stream s.
     BufPtr upload_blob(const void* host, size_t bytes, hipStream_t s);
     // Host-to-device copy ordered on stream s: the bytes are staged in a pinned ring (so the copy
@@ -360,5 +362,9 @@ class Context : public std::enable_shared_from_this<Context> {
 
 // Launch helpers that split grids above the 65535 row limit.
 void launch_ntt(const Context& c, const LimbRef* d_refs, u32 nrefs, bool forward, hipStream_t s);
+
+// Device address of this GPU's fixed workspace (CKH_STATIC_SCRATCH_BYTES, static_scratch.hip),
+// looked up once per device.
+uint8_t* static_scratch_address(int device);
 
 }  // namespace ckh

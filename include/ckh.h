@@ -194,6 +194,24 @@ ckh_status ckh_key_switch(ckh_context *ctx, const ckh_poly *d, const ckh_ksk *ke
 ckh_status ckh_key_switch_batch(ckh_context *ctx, uint32_t n, const ckh_poly *const *d,
                                 const ckh_ksk *key, ckh_poly **kb, ckh_poly **ka);
 
+/* Device bytes of the fixed workspace of ckh_key_switch_batch_static (one per GPU, held from the
+ * first context on that GPU until the process ends). */
+#define CKH_STATIC_SCRATCH_BYTES (256u << 20)
+/* ckh_key_switch_batch without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate (code under stream capture, serving loops that forbid hipMalloc).
+ * The caller passes the outputs: kb[i] and ka[i] are existing polynomials of this context at the
+ * level of d[i] (NTT form, Q limbs only), overwritten in place (every handle sharing them sees
+ * the new words); they may not overlap any input or each other. Intermediates live in the
+ * GPU's fixed workspace of CKH_STATIC_SCRATCH_BYTES (descriptors included; a 24-limb switch at
+ * N = 2^16 with 6 special primes and dnum 4 needs about 132 MiB); a batch needing more is
+ * rejected with CKH_ERR_INVALID_PARAMS before any work. Launches go directly on the calling
+ * thread's stream, with no graph capture. Every call on one GPU shares that workspace: calls that
+ * overlap in time on one GPU (other streams, other contexts) race and must be ordered by the
+ * caller; calls on one stream are ordered by it. Results equal ckh_key_switch_batch's. */
+ckh_status ckh_key_switch_batch_static(ckh_context *ctx, uint32_t n, const ckh_poly *const *d,
+                                       const ckh_ksk *key, ckh_poly *const *kb,
+                                       ckh_poly *const *ka);
+
 /* ------------------------------------------------------------------ ciphertexts */
 
 /* n_comp polynomials of level + 1 limbs each, back to back, NTT form; scale is the f64 label. */
