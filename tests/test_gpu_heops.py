@@ -426,3 +426,91 @@ def test_matvec_plain_sum_matches_oracle(bsgs):
         ev.matvec_plain_sum([xs_d[0], other], ents_d, DELTA, baby, giant, stride, offset)
     with pytest.raises(ckh.InvalidParams):
         ev.matvec_plain_sum([], [], DELTA, baby, giant, stride, offset)
+
+
+def test_add_plain_mul_plain_match_oracle(env):
+    """add_plain / mul_plain (evaluator.rs:287-308): c0 + p with a's scale (p's scale within 2^-10),
+    every component times p with the product of the scales, one ct_pt_mults per mul_plain; the
+    checks in the reference's order (basis, level, scale), failures counting nothing."""
+    ctx, ref = env
+    L = len(ref.q) - 1
+    ev = ckh.Evaluator(ctx)
+    for level in (L, 1, 0):
+        for ncomp in (2, 3):
+            ct, h = _ct(ctx, ref, level, 900 + 10 * level + ncomp, ncomp)
+            w = host_uniform(ref, level, 950 + level)
+            pt = ctx.upload(w, level)
+            ev.reset_counter()
+            s = ev.add_plain(ct, pt, DELTA * (1 + 2.0 ** -11))
+            assert s.scale == DELTA and s.level == level and s.size() == ncomp
+            d = s.download()
+            assert (d[0] == ref.poly_add(h[0], w, level)).all()
+            for c in range(1, ncomp):
+                assert (d[c] == h[c]).all()
+            m = ev.mul_plain(ct, pt, 3.0)
+            assert m.scale == DELTA * 3.0 and m.level == level and m.size() == ncomp
+            d = m.download()
+            for c in range(ncomp):
+                assert (d[c] == ref.poly_mul(h[c], w, level)).all(), (level, ncomp, c)
+            cnt = ev.counter()
+            assert cnt.ct_pt_mults == 1 and cnt.rescales == 0 and cnt.rotations == 0
+    ct, _ = _ct(ctx, ref, L, 990)
+    pt = ctx.upload(host_uniform(ref, L, 991), L)
+    ev.reset_counter()
+    with pytest.raises(ckh.ScaleMismatch):
+        ev.add_plain(ct, pt, DELTA * (1 + 2.0 ** -9))
+    low = ctx.upload(host_uniform(ref, L - 1, 992), L - 1)
+    with pytest.raises(ckh.LevelMismatch):  # the level check comes before the scale check
+        ev.add_plain(ct, low, 5.0)
+    with pytest.raises(ckh.LevelMismatch):
+        ev.mul_plain(ct, low, DELTA)
+    with pytest.raises(ckh.InvalidParams):  # a Q*P polynomial is not over the data basis
+        ev.mul_plain(ct, ctx.uniform(L, seed=993, p_limbs=len(ref.p)), DELTA)
+    ctx2 = ckh.Context(ctx.log_n, ref.q, ref.p, ctx.dnum)
+    foreign = ctx2.upload(host_uniform(ref, L, 994), L)
+    with pytest.raises(ckh.InvalidParams):  # plaintext of another context, before the level check
+        ev.add_plain(ct, foreign, DELTA)
+    with pytest.raises(ckh.InvalidParams):
+        ev.mul_plain(ct, foreign, DELTA)
+    assert ev.counter().ct_pt_mults == 0
+    del foreign
+    ctx2.close()
+
+
+def test_mod_switch_to_level_and_relabel(env):
+    """mod_switch_to_level (evaluator.rs:513-525): the first level + 1 limbs of every component,
+    scale unchanged, a copy at the same level, InvalidParams upwards; the result outlives its
+    input and feeds later operations like an uploaded one. relabel (ops.rs:475-477): the same
+    words under a new scale, which must be positive and finite (ciphertext.rs:40-60)."""
+    ctx, ref = env
+    L = len(ref.q) - 1
+    ev = ckh.Evaluator(ctx)
+    ct, h = _ct(ctx, ref, L, 1000, 3)
+    for level in (L, L - 1, 1, 0):
+        s = ev.mod_switch_to_level(ct, level)
+        assert s.level == level and s.scale == DELTA and s.size() == 3
+        d = s.download()
+        for c in range(3):
+            assert (d[c] == h[c][: level + 1]).all()
+    with pytest.raises(ckh.InvalidParams):
+        ev.mod_switch_to_level(ev.mod_switch_to_level(ct, 1), 2)
+    two, h2 = _ct(ctx, ref, L, 1010)
+    s = ev.mod_switch_to_level(two, L - 1)
+    del two  # the switched ciphertext keeps its words
+    r = ev.rescale(s)
+    want = ref.rescale_ct(np.stack([x[:L] for x in h2]), L - 1)
+    d = r.download()
+    for c in range(2):
+        assert (d[c] == want[c]).all()
+    ev.reset_counter()
+    for scale in (1.0, DELTA * 7.5, 2.0 ** -30):
+        x = ev.relabel(ct, scale)
+        assert x.scale == scale and x.level == L
+        d = x.download()
+        for c in range(3):
+            assert (d[c] == h[c]).all()
+    for bad in (0.0, -1.0, float("nan"), float("inf"), -float("inf")):
+        with pytest.raises(ckh.InvalidParams):
+            ev.relabel(ct, bad)
+    cnt = ev.counter()
+    assert cnt.ct_pt_mults == 0 and cnt.rescales == 0
