@@ -144,6 +144,7 @@ _SIGS = {
     "ckh_rescale_poly": (C.c_int, [VP, VP, C.POINTER(VP)]),
     "ckh_time_ntt": (C.c_int, [VP, C.POINTER(VP), u32, C.c_int, u32, C.POINTER(C.c_double)]),
     "ckh_mod_down": (C.c_int, [VP, VP, C.POINTER(VP)]),
+    "ckh_time_copy": (C.c_int, [VP, u64, u32, C.POINTER(C.c_double)]),
     "ckh_ksk_upload": (C.c_int, [VP, P64, u32, u64, C.POINTER(VP)]),
     "ckh_ksk_uniform": (C.c_int, [VP, u32, u64, u64, C.POINTER(VP)]),
     "ckh_ksk_info": (C.c_int, [VP, C.POINTER(u32), P64, P64]),
@@ -399,6 +400,13 @@ class Context:
         kb = (VP * n)(*[o[0].h for o in outs])
         ka = (VP * n)(*[o[1].h for o in outs])
         _check(lib().ckh_key_switch_batch_static(self.h, n, arr, key.h, kb, ka))
+
+    def time_copy(self, nbytes, reps=5):
+        """Mean ms of a streaming device copy of nbytes (16-byte words, HIP events on this
+        context's stream)."""
+        ms = C.c_double()
+        _check(lib().ckh_time_copy(self.h, nbytes, reps, C.byref(ms)))
+        return ms.value
 
     def time_ntt(self, polys, forward=True, reps=20):
         """Mean ms per batched in-place NTT launch over every limb of `polys` (HIP events on

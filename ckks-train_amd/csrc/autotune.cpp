@@ -59,6 +59,12 @@ void Context::set_tuning(const std::string& key, int64_t value) {
         if (value != 1) fail(CKH_ERR_INVALID_PARAMS, "pool mode 0 (the device's default pool) is disabled");
         pool_mode = 1;
     }
+    else if (key == "ntt_fused")
+        ntt_fused = value != 0;
+    else if (key == "fused_lag")
+        fused.lag = (u32)std::min<int64_t>(std::max<int64_t>(value, 1), 64);
+    else if (key == "fused_wg")  // workgroups per CU of the one-launch transforms
+        fused.grid = cu_count * (u32)std::min<int64_t>(std::max<int64_t>(value, 1), 8);
     else if (key == "service_streams")
         service_streams = (u32)std::min<int64_t>(std::max<int64_t>(value, 1), SERVICE_STREAMS);
     else
@@ -80,6 +86,10 @@ int64_t Context::get_tuning(const std::string& key) const {
     if (key == "conv_tg") return conv_tg;
     if (key == "pool") return pool_mode;
     if (key == "service_streams") return service_streams;
+    if (key == "ntt_fused") return fused_usable() ? 1 : 0;
+    if (key == "fused_lag") return fused.lag;
+    if (key == "fused_wg") return cu_count ? fused.grid / cu_count : 0;
+    if (key == "xcds") return fused.nx;
     if (key == "fail_alloc_after") return fail_alloc_after.load();
     fail(CKH_ERR_INVALID_PARAMS, "unknown tuning key " + key);
 }

@@ -321,6 +321,13 @@ ckh_status ckh_time_ntt(ckh_context* ctx, ckh_poly* const* polys, uint32_t n, in
     });
 }
 
+ckh_status ckh_time_copy(ckh_context* ctx, uint64_t bytes, uint32_t reps, double* ms_per_launch) {
+    return guarded([&] {
+        need(ms_per_launch, "ms_per_launch");
+        *ms_per_launch = time_copy(ctx_of(ctx), bytes, reps);
+    });
+}
+
 ckh_status ckh_rescale_poly(ckh_context* ctx, const ckh_poly* in, ckh_poly** out) {
     return guarded([&] {
         need(out, "out");

@@ -62,6 +62,7 @@ void poly_download(Context& c, const Poly& p, u64* host) {
     hipStream_t s = c.stream();
     hip_check(hipMemcpyAsync(host, p.data(), (u64)p.limbs() * c.n * 8, hipMemcpyDeviceToHost, s), "download");
     hip_check(hipStreamSynchronize(s), "download");
+    c.check_device_error(s);
 }
 
 u64 poly_digest(Context& c, const Poly& p) {
@@ -93,11 +94,13 @@ PolyPtr poly_transform(Context& c, const Poly& in, bool forward) {
 }
 
 PolyPtr poly_automorphism(Context& c, const Poly& in, u64 g) {
-    if (!in.ntt) fail(CKH_ERR_INVALID_PARAMS, "automorphism needs NTT form");
     if (g % 2 == 0 || g >= 2 * c.n) fail(CKH_ERR_INVALID_PARAMS, "Galois element must be odd and < 2N");
     use(c, in);
-    PolyPtr out = poly_alloc(c, in.level, in.p_limbs, true);
-    automorphism_ntt_launch(in.data(), out->data(), in.limbs(), g, c.log_n, c.stream());
+    PolyPtr out = poly_alloc(c, in.level, in.p_limbs, in.ntt);
+    if (in.ntt)
+        automorphism_ntt_launch(in.data(), out->data(), in.limbs(), g, c.log_n, c.stream());
+    else
+        automorphism_coeff_launch(in.data(), out->data(), in.limbs(), in.level, c.nq, g, c.d_pc, c.log_n, c.stream());
     hip_check(hipGetLastError(), "automorphism");
     return out;
 }
@@ -113,20 +116,56 @@ double time_ntt(Context& c, const std::vector<const Poly*>& polys, bool forward,
     }
     if (refs.empty() || reps == 0) fail(CKH_ERR_INVALID_PARAMS, "nothing to time");
     hipStream_t s = c.stream();
-    BufPtr d = c.upload_blob(refs.data(), refs.size() * sizeof(LimbRef), s);
+    // The descriptors, then one zeroed control block per launch (the one-launch forward transform
+    // needs a fresh one each time).
+    const u32 nr = (u32)refs.size();
+    const u64 ref_bytes = ((u64)nr * sizeof(LimbRef) + 255) & ~(u64)255;
+    const u64 ctl_bytes = ((fused_ctl_words(nr) * 4) + 255) & ~(u64)255;
+    std::vector<uint8_t> blob(ref_bytes + (reps + 1) * ctl_bytes, 0);
+    std::memcpy(blob.data(), refs.data(), nr * sizeof(LimbRef));
+    BufPtr d = c.upload_blob(blob.data(), blob.size(), s);
     const LimbRef* dr = (const LimbRef*)d->ptr;
-    launch_ntt(c, dr, (u32)refs.size(), forward, s);  // warm up
+    const bool fused = c.fused_usable();
+    auto launch = [&](u32 r) {
+        u32* ctl = (u32*)((uint8_t*)d->ptr + ref_bytes + r * ctl_bytes);
+        if (fused && forward)
+            ntt_fast_forward_fused(c.ntt, dr, nr, nullptr, ctl, c.fused, s);
+        else if (fused)
+            ntt_fast_inverse_fused(c.ntt, dr, nr, ctl, c.fused, s);
+        else
+            launch_ntt(c, dr, nr, forward, s);
+    };
+    launch(reps);  // warm up
     hipEvent_t a, b;
     hip_check(hipEventCreate(&a), "event");
     hip_check(hipEventCreate(&b), "event");
     hip_check(hipEventRecord(a, s), "event");
-    for (u32 r = 0; r < reps; ++r) launch_ntt(c, dr, (u32)refs.size(), forward, s);
+    for (u32 r = 0; r < reps; ++r) launch(r);
     hip_check(hipEventRecord(b, s), "event");
     hip_check(hipEventSynchronize(b), "event");
     float ms = 0;
     hip_check(hipEventElapsedTime(&ms, a, b), "event");
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
+    return (double)ms / reps;
+}
+
+double time_copy(Context& c, u64 bytes, u32 reps) {
+    if (bytes < 16 || bytes % 16 || reps == 0) fail(CKH_ERR_INVALID_PARAMS, "copy size must be a positive multiple of 16");
+    hipStream_t s = c.stream();
+    BufPtr a = c.alloc(bytes / 8, s), b = c.alloc(bytes / 8, s);
+    stream_copy_launch(a->ptr, b->ptr, bytes, s);  // warm up
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, s), "event");
+    for (u32 r = 0; r < reps; ++r) stream_copy_launch(a->ptr, b->ptr, bytes, s);
+    hip_check(hipEventRecord(e1, s), "event");
+    hip_check(hipEventSynchronize(e1), "event");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return (double)ms / reps;
 }
 
@@ -823,6 +862,94 @@ std::vector<CtPtr> Evaluator::hoisted_rotate(const Ct& ct, const std::vector<int
                          ct.scale);
         ++t;
     }
+    Program prog;
+    prog.step(S_AUTOMORPH, prog.add(aut), (u32)aut.size());
+    ctx.prog_apply(prog, tasks);
+    ctx.execute(prog, s);
+    hip_check(hipGetLastError(), "hoisted rotate");
+    count(&ckh_op_counter::rotations, nontrivial);
+    return res;
+}
+
+// Hoisted rotations of several ciphertexts (each with its own steps) in batched launches: every
+// check first (nothing runs if one fails), one decomposition program for all inputs, then one
+// program of every step's automorphism, inner product and mod_down. The same words and counts
+// as hoisted_rotate per item.
+std::vector<std::vector<CtPtr>> Evaluator::hoisted_rotate_batch(
+    const std::vector<std::pair<const Ct*, std::vector<int32_t>>>& items) {
+    std::vector<std::vector<CtPtr>> res(items.size());
+    std::vector<std::vector<std::pair<u64, const Ksk*>>> keys(items.size());
+    u64 dig_words = 0, scr_words = 0, out_words = 0;
+    u32 nontrivial = 0;
+    const u64 n = ctx.n, k = ctx.np;
+    for (size_t i = 0; i < items.size(); ++i) {
+        const Ct& ct = *items[i].first;
+        check_basis(ct);
+        check_galois_input(ct);
+        const u32 level = ct.level();
+        const u64 L = level + 1;
+        u32 nt = 0;
+        for (int32_t st : items[i].second) {
+            const u64 e = rotation_elt(st);
+            keys[i].push_back({e, e ? &galois_key(e, level) : nullptr});
+            nt += e != 0;
+        }
+        if (nt) dig_words += ctx.ndigits(level) * ctx.digit_words(level) + L * n;
+        scr_words += nt * (2 * (L + k) + 2 * k + 2 * L + L) * n;
+        out_words += nt * 2 * L * n;
+        nontrivial += nt;
+    }
+    if (!config.hoist) {
+        return counted([&] {
+            for (size_t i = 0; i < items.size(); ++i) res[i] = hoisted_rotate(*items[i].first, items[i].second);
+            return res;
+        });
+    }
+    if (nontrivial == 0) {
+        for (size_t i = 0; i < items.size(); ++i)
+            for (size_t j = 0; j < items[i].second.size(); ++j)
+                res[i].push_back(make_ct(items[i].first->comps, items[i].first->scale));
+        return res;
+    }
+    hipStream_t s = ctx.stream();
+    BufPtr digs = ctx.alloc(dig_words), scratch = ctx.alloc(scr_words), out = ctx.alloc(out_words);
+    std::vector<DecTask> dec;
+    std::vector<ApplyTask> tasks;
+    std::vector<AutJob> aut;
+    u64 dpos = 0, spos = 0, opos = 0;
+    for (size_t i = 0; i < items.size(); ++i) {
+        const Ct& ct = *items[i].first;
+        const u32 level = ct.level();
+        const u64 L = level + 1;
+        const u64 per = (2 * (L + k) + 2 * k + 2 * L + L) * n;
+        bool any = false;
+        for (auto& kk : keys[i]) any |= kk.first != 0;
+        u64* dig = nullptr;
+        if (any) {
+            dig = digs->ptr + dpos;
+            const u64 dlen = ctx.ndigits(level) * ctx.digit_words(level);
+            dec.push_back({ct.comps[1]->data(), level, dig, dig + dlen});
+            dpos += dlen + L * n;
+        }
+        for (size_t j = 0; j < keys[i].size(); ++j) {
+            const u64 e = keys[i][j].first;
+            if (!e) {
+                res[i].push_back(make_ct(ct.comps, ct.scale));
+                continue;
+            }
+            u64* acc = scratch->ptr + spos;
+            u64* y = acc + 2 * (L + k) * n;
+            u64* z = y + 2 * k * n;
+            u64* r0 = z + 2 * L * n;
+            spos += per;
+            for (u32 li = 0; li < L; ++li) aut.push_back({ct.comps[0]->data() + li * n, r0 + li * n, e});
+            u64* ob = out->ptr + opos;
+            tasks.push_back({dig, level, e, keys[i][j].second, ob, ob + L * n, r0, nullptr, acc, y, z});
+            res[i].push_back(make_ct({view(out, opos, level, 0, true), view(out, opos + L * n, level, 0, true)}, ct.scale));
+            opos += 2 * L * n;
+        }
+    }
+    ctx.decompose(dec, s);
     Program prog;
     prog.step(S_AUTOMORPH, prog.add(aut), (u32)aut.size());
     ctx.prog_apply(prog, tasks);

@@ -20,6 +20,8 @@ PolyPtr poly_transform(Context& c, const Poly& in, bool forward);
 PolyPtr poly_automorphism(Context& c, const Poly& in, u64 g);
 PolyPtr poly_rescale(Context& c, const Poly& in);
 double time_ntt(Context& c, const std::vector<const Poly*>& polys, bool forward, u32 reps);
+// Mean ms of a streaming device copy of bytes (two fresh buffers), reps launches on the stream.
+double time_copy(Context& c, u64 bytes, u32 reps);
 PolyPtr poly_mod_down(Context& c, const Poly& qp);
 PolyPtr poly_pointwise(Context& c, PwOp op, const Poly& a, const Poly* b);
 PolyPtr poly_scalar_mul(Context& c, const Poly& a, u64 k, bool neg);
@@ -100,6 +102,9 @@ class Evaluator {
     CtPtr rotate(const Ct& ct, int32_t steps);
     CtPtr conjugate(const Ct& ct);
     std::vector<CtPtr> hoisted_rotate(const Ct& ct, const std::vector<int32_t>& steps);
+    // hoisted_rotate of several ciphertexts, each with its own steps, in batched launches.
+    std::vector<std::vector<CtPtr>> hoisted_rotate_batch(
+        const std::vector<std::pair<const Ct*, std::vector<int32_t>>>& items);
     // Rotations of several ciphertexts (their own steps) switched in one batch.
     std::vector<CtPtr> rotate_batch(const std::vector<std::pair<const Ct*, int32_t>>& items);
     CtPtr matvec_plain(const Ct& x, const std::vector<const Poly*>& entries, double pt_scale, u32 baby, u32 giant,

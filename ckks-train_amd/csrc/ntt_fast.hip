@@ -19,6 +19,9 @@
 //    values as f64 bits; the last stage writes canonical residues.
 //  * q >= 2^50 (IntAr): Harvey butterflies with 64-bit Shoup products, values in [0, 4q)
 //    (forward) or [0, 2q) (inverse), as in ntt.hip.
+#include <algorithm>
+#include <vector>
+
 #include "basis.h"
 #include "common.h"
 #include "ntt.h"
@@ -30,6 +33,11 @@ constexpr u32 TL = 12;
 constexpr u32 TILE = 1u << TL;
 constexpr u32 NT = 256;
 constexpr u32 LDS_WORDS = TILE + TILE / 16;
+// s_getreg operand for HW_REG_XCC_ID (register 20), bits 0..3: the XCD the wave runs on.
+constexpr int XCC_ID_HWREG = 20 | (0 << 6) | ((4 - 1) << 11);
+// s_waitcnt operand: vmcnt(0), expcnt and lgkmcnt not waited for.
+constexpr int WAITCNT_VM0 = 0x0F70;
+constexpr u32 FUSED_SPIN_MAX = 1u << 22;
 
 // Padded LDS index: one spare word per 16 spreads strided accesses over the banks.
 CKH_DEV u32 pad(u32 a) { return a + (a >> 4); }
@@ -242,12 +250,12 @@ CKH_DEV void rows_round2_inv(const Ar& ar, typename Ar::V (&v)[16], u32 A, u32 r
 
 template <u32 M, class Ar>
 CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in_p, u64* __restrict__ out_p, u64* lds, u32 log_n,
-                      u32 tile) {
+                      u32 tile, u32 tid = threadIdx.x) {
     const auto in = glb(in_p);
     const auto out = glb(out_p);
     constexpr u32 R2 = M - 4, U = 4 - R2, TCL = TL - M, TC = 1u << TCL, JM = (1u << R2) - 1;
     const u32 B = log_n - M;
-    const u32 tid = threadIdx.x, cc = tid & (TC - 1), o = tid >> TCL;
+    const u32 cc = tid & (TC - 1), o = tid >> TCL;
     const u32 c0 = tile * TC;
     typename Ar::V v[16];
 #pragma unroll
@@ -288,7 +296,14 @@ CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in_p, u64* __restric
 
 // The row pass of one tile, its canonical outputs left in LDS in natural order (word i of the
 // tile at pad(i)): as u64 residues, or (F64) as the f64 bits of the residue.
-template <u32 M, bool F64, class Ar>
+// A word of a column-pass output written by another workgroup of the same launch: loaded at agent
+// scope (the sc1 bit), so the load misses this CU's vector L1 (which may hold an older copy of the
+// line) and reads the XCD's L2, where the writer's completed stores are.
+CKH_DEV u64 load_coherent(const u64* p) {
+    return __hip_atomic_load(glb(const_cast<u64*>(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <u32 M, bool F64, class Ar, bool COH = false>
 CKH_DEV void fwd_rows_to_lds(const Ar& ar, const u64* __restrict__ in_p, u64* lds, u32 log_n, u32 tile,
                              u32 tid = threadIdx.x) {
     const auto in = glb(in_p);
@@ -299,7 +314,8 @@ CKH_DEV void fwd_rows_to_lds(const Ar& ar, const u64* __restrict__ in_p, u64* ld
     const u64 base = (u64)row << M;
     typename Ar::V v[16];
 #pragma unroll
-    for (u32 k = 0; k < 16; ++k) v[k] = ar.from_mid(in[base + o + (k << R2)]);
+    for (u32 k = 0; k < 16; ++k)
+        v[k] = ar.from_mid(COH ? load_coherent(in_p + base + o + (k << R2)) : in[base + o + (k << R2)]);
     // Round 1: stage A + s uses tab[2^(A+s) + row 2^s + m], m = k >> (4 - s) < 2^s.
     rows_round1_fwd<Ar>(ar, v, A, row);
 #pragma unroll
@@ -322,10 +338,9 @@ CKH_DEV void fwd_rows_to_lds(const Ar& ar, const u64* __restrict__ in_p, u64* ld
     __syncthreads();
 }
 
-template <u32 M, class Ar>
+template <u32 M, class Ar, bool COH = false>
 CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict__ out_p, u64* lds, u32 log_n,
-                      const FinishJob* fin, u64 q, u32 tile) {
-    const u32 tid = threadIdx.x;
+                      const FinishJob* fin, u64 q, u32 tile, u32 tid = threadIdx.x) {
     const auto out = glb(out_p);
     const u64 tbase = (u64)tile << TL;
     if (fin) {
@@ -341,7 +356,7 @@ CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
             av[k] = fa[tbase + tid + k * NT];
             dv[k] = has_add ? fadd[tbase + tid + k * NT] : 0;
         }
-        fwd_rows_to_lds<M, false>(ar, in, lds, log_n, tile);
+        fwd_rows_to_lds<M, false, Ar, COH>(ar, in, lds, log_n, tile, tid);
 #pragma unroll
         for (u32 k = 0; k < 16; ++k) {
             u64 v = shoup(sub_mod(av[k], lds[pad(tid + k * NT)], q), inv, inv_s, q);
@@ -350,7 +365,7 @@ CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
         }
         return;
     }
-    fwd_rows_to_lds<M, false>(ar, in, lds, log_n, tile);
+    fwd_rows_to_lds<M, false, Ar, COH>(ar, in, lds, log_n, tile, tid);
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) {
         const u32 idx = tid + k * NT;
@@ -415,11 +430,11 @@ CKH_DEV void rows_ip(const FpAr& ar, const RowIpJob& J, u64* lds, u32 log_n, u32
 
 template <u32 M, class Ar>
 CKH_DEV void inv_rows(const Ar& ar, const u64* __restrict__ in_p, u64* __restrict__ out_p, u64* lds, u32 log_n,
-                      u32 tile) {
+                      u32 tile, u32 tid = threadIdx.x) {
     const auto in = glb(in_p);
     const auto out = glb(out_p);
     constexpr u32 R2 = M - 4, U = 4 - R2, TRL = TL - M, JM = (1u << R2) - 1;
-    const u32 tid = threadIdx.x, o = tid & JM, rr = tid >> R2;
+    const u32 o = tid & JM, rr = tid >> R2;
     const u32 row = (tile << TRL) + rr;
     const u64 tbase = (u64)tile << TL;
 #pragma unroll
@@ -454,18 +469,19 @@ CKH_DEV void inv_rows(const Ar& ar, const u64* __restrict__ in_p, u64* __restric
     for (u32 k = 0; k < 16; ++k) out[base + o + (k << R2)] = ar.to_mid(v[k]);
 }
 
-template <u32 M, class Ar>
-CKH_DEV void inv_cols(const Ar& ar, u64* __restrict__ io_p, u64* lds, u32 log_n, u32 tile) {
+template <u32 M, class Ar, bool COH = false>
+CKH_DEV void inv_cols(const Ar& ar, u64* __restrict__ io_p, u64* lds, u32 log_n, u32 tile, u32 tid = threadIdx.x) {
     const auto io = glb(io_p);
     constexpr u32 R2 = M - 4, U = 4 - R2, TCL = TL - M, TC = 1u << TCL, JM = (1u << R2) - 1;
     const u32 B = log_n - M;
-    const u32 tid = threadIdx.x, cc = tid & (TC - 1), o = tid >> TCL;
+    const u32 cc = tid & (TC - 1), o = tid >> TCL;
     const u32 c0 = tile * TC;
     typename Ar::V v[16];
 #pragma unroll
     for (u32 e = 0; e < 16; ++e) {
         const u32 r = ((((o << U) | (e >> R2)) << R2) | (e & JM));
-        v[e] = ar.from_mid(io[((u64)r << B) + c0 + cc]);
+        const u64 idx = ((u64)r << B) + c0 + cc;
+        v[e] = ar.from_mid(COH ? load_coherent(io_p + idx) : io[idx]);
     }
 #pragma unroll
     for (u32 u = 0; u < R2; ++u) {
@@ -570,6 +586,165 @@ __global__ __launch_bounds__(NT) void k_inv_cols(const LimbRef* __restrict__ ref
     CKH_AR_DISPATCH((inv_cols<M>(ar, ref.dst, lds, log_n, tile)))
 }
 
+// ---------------------------------------------------------------- one-launch forward NTT
+//
+// Both passes of the forward transform in one persistent launch, so that a limb's column-pass
+// output is read back by its row pass from the L2 of the XCD that wrote it instead of HBM. The
+// limbs are split into one contiguous chunk per XCD (nx of them, ids from the hardware register;
+// checked at context creation); the workgroups running on an XCD take tickets from that XCD's
+// counter in the order they start, and ticket t (slot s = t / tiles, tile w = t % tiles) runs
+// column tile w of the chunk's limb s, then row tile w of its limb s - lag. A row tile waits
+// (thread 0, bounded) until every column tile of its limb has been counted done: those tiles hold
+// earlier tickets of the same counter, so their workgroups are running or finished and the wait
+// ends. Column tiles count themselves done after their stores have completed (vmcnt 0) and row
+// tiles read the intermediate with agent-scope loads (load_coherent); the limb's words pass
+// through the L2 of that XCD only, which every one of its CUs reads and writes. An overlong wait
+// sets the error word (the row tile then runs anyway; the host reports the error).
+//
+// Control block (u32 words, zero before the launch): ticket counter of XCD x at x * CTL_STRIDE,
+// then one done-counter per limb from FUSED_CTL_HEAD. err: the context's device error word.
+template <u32 MA, u32 MB, bool FIN>
+__global__ __launch_bounds__(NT) void k_fwd_fused(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
+                                                  const Tw* __restrict__ tw, const FTw* __restrict__ twf,
+                                                  const FTw* __restrict__ rt, u32 log_n, u32 nrefs,
+                                                  const FinishJob* __restrict__ fins, u32* ctl, u32 nx, u32 lag,
+                                                  u32* err) {
+    __shared__ u64 lds[LDS_WORDS];
+    __shared__ u32 s_ticket;
+    const u32 xcc = __builtin_amdgcn_s_getreg(XCC_ID_HWREG) & 15;
+    if (xcc >= nx) return;
+    const u32 tiles = 1u << (log_n - TL);
+    const u32 r0 = (u32)((u64)nrefs * xcc / nx), mine = (u32)((u64)nrefs * (xcc + 1) / nx) - r0;
+    u32* const done = ctl + FUSED_CTL_HEAD + r0;
+    for (;;) {
+        __syncthreads();  // the previous ticket's LDS use and s_ticket reads are over
+        if (threadIdx.x == 0)
+            s_ticket = __hip_atomic_fetch_add(ctl + xcc * CTL_STRIDE, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const u32 t = s_ticket, s = t >> (log_n - TL), w = t & (tiles - 1);
+        if (s >= mine + lag) return;
+        // The thread index, opaque per ticket: otherwise the compiler keeps every address derived
+        // from it for both passes live across the loop (about 290 VGPRs instead of the passes' own).
+        u32 tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        if (s < mine) {
+            const u32 ref_i = r0 + s;
+            const LimbRef ref = refs[ref_i];
+            const PrimeConst* pc = pcs + ref.prime;
+            const u64 toff = (u64)ref.prime << log_n;
+            if (pc->q < FP_PRIME_BOUND) {
+                const FpAr ar{twf + toff, pc->qf, pc->qinv, pc->n_inv_f, pc->w_n_inv_f,
+                              rt + ((u64)ref.prime << (log_n - log_n / 2)) * ROW_FACTOR_SLOTS};
+                fwd_cols<MA>(ar, ref.src, ref.dst, lds, log_n, w, tid);
+            } else {
+                const IntAr ar{tw + toff, pc->q, pc->n_inv, pc->n_inv_s, pc->w_n_inv, pc->w_n_inv_s};
+                fwd_cols<MA>(ar, ref.src, ref.dst, lds, log_n, w, tid);
+            }
+            __builtin_amdgcn_s_waitcnt(WAITCNT_VM0);  // this thread's stores have reached the L2
+            __syncthreads();
+            if (threadIdx.x == 0) __hip_atomic_fetch_add(done + s, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (s >= lag && s - lag < mine) {
+            const u32 m = s - lag, ref_i = r0 + m;
+            if (threadIdx.x == 0) {
+                u32 spins = 0;
+                while (__hip_atomic_load(done + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < tiles) {
+                    __builtin_amdgcn_s_sleep(2);
+                    if (++spins > FUSED_SPIN_MAX) {
+                        __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+            }
+            __syncthreads();
+            const LimbRef ref = refs[ref_i];
+            const PrimeConst* pc = pcs + ref.prime;
+            const u64 toff = (u64)ref.prime << log_n;
+            if (pc->q < FP_PRIME_BOUND) {
+                const FpAr ar{twf + toff, pc->qf, pc->qinv, pc->n_inv_f, pc->w_n_inv_f,
+                              rt + ((u64)ref.prime << (log_n - log_n / 2)) * ROW_FACTOR_SLOTS};
+                fwd_rows<MB, FpAr, true>(ar, ref.dst, ref.dst, lds, log_n, FIN ? fins + ref_i : nullptr, pc->q, w, tid);
+            } else {
+                const IntAr ar{tw + toff, pc->q, pc->n_inv, pc->n_inv_s, pc->w_n_inv, pc->w_n_inv_s};
+                fwd_rows<MB, IntAr, true>(ar, ref.dst, ref.dst, lds, log_n, FIN ? fins + ref_i : nullptr, pc->q, w, tid);
+            }
+        }
+    }
+}
+
+// The inverse transform in one launch, with the same ticket scheme: ticket t runs row tile w of the
+// chunk's limb s (the first inverse pass, src -> dst), then column tile w of its limb s - lag
+// (in place on dst), once every row tile of that limb has been counted done.
+template <u32 MA, u32 MB>
+__global__ __launch_bounds__(NT) void k_inv_fused(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
+                                                  const Tw* __restrict__ tw, const FTw* __restrict__ twf,
+                                                  const FTw* __restrict__ rt, u32 log_n, u32 nrefs, u32* ctl, u32 nx,
+                                                  u32 lag, u32* err) {
+    __shared__ u64 lds[LDS_WORDS];
+    __shared__ u32 s_ticket;
+    const u32 xcc = __builtin_amdgcn_s_getreg(XCC_ID_HWREG) & 15;
+    if (xcc >= nx) return;
+    const u32 tiles = 1u << (log_n - TL);
+    const u32 r0 = (u32)((u64)nrefs * xcc / nx), mine = (u32)((u64)nrefs * (xcc + 1) / nx) - r0;
+    u32* const done = ctl + FUSED_CTL_HEAD + r0;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0)
+            s_ticket = __hip_atomic_fetch_add(ctl + xcc * CTL_STRIDE, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const u32 t = s_ticket, s = t >> (log_n - TL), w = t & (tiles - 1);
+        if (s >= mine + lag) return;
+        u32 tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        if (s < mine) {
+            const LimbRef ref = refs[r0 + s];
+            const PrimeConst* pc = pcs + ref.prime;
+            const u64 toff = (u64)ref.prime << log_n;
+            if (pc->q < FP_PRIME_BOUND) {
+                const FpAr ar{twf + toff, pc->qf, pc->qinv, pc->n_inv_f, pc->w_n_inv_f,
+                              rt + ((u64)ref.prime << (log_n - log_n / 2)) * ROW_FACTOR_SLOTS};
+                inv_rows<MB>(ar, ref.src, ref.dst, lds, log_n, w, tid);
+            } else {
+                const IntAr ar{tw + toff, pc->q, pc->n_inv, pc->n_inv_s, pc->w_n_inv, pc->w_n_inv_s};
+                inv_rows<MB>(ar, ref.src, ref.dst, lds, log_n, w, tid);
+            }
+            __builtin_amdgcn_s_waitcnt(WAITCNT_VM0);
+            __syncthreads();
+            if (threadIdx.x == 0) __hip_atomic_fetch_add(done + s, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (s >= lag && s - lag < mine) {
+            const u32 m = s - lag;
+            if (threadIdx.x == 0) {
+                u32 spins = 0;
+                while (__hip_atomic_load(done + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < tiles) {
+                    __builtin_amdgcn_s_sleep(2);
+                    if (++spins > FUSED_SPIN_MAX) {
+                        __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+            }
+            __syncthreads();
+            const LimbRef ref = refs[r0 + m];
+            const PrimeConst* pc = pcs + ref.prime;
+            const u64 toff = (u64)ref.prime << log_n;
+            if (pc->q < FP_PRIME_BOUND) {
+                const FpAr ar{twf + toff, pc->qf, pc->qinv, pc->n_inv_f, pc->w_n_inv_f,
+                              rt + ((u64)ref.prime << (log_n - log_n / 2)) * ROW_FACTOR_SLOTS};
+                inv_cols<MA, FpAr, true>(ar, ref.dst, lds, log_n, w, tid);
+            } else {
+                const IntAr ar{tw + toff, pc->q, pc->n_inv, pc->n_inv_s, pc->w_n_inv, pc->w_n_inv_s};
+                inv_cols<MA, IntAr, true>(ar, ref.dst, lds, log_n, w, tid);
+            }
+        }
+    }
+}
+
+// XCD ids of a probe grid (one word per workgroup), for the check at context creation.
+__global__ __launch_bounds__(64) void k_xcc_ids(u32* out) {
+    if (threadIdx.x == 0) glb(out)[blockIdx.x] = __builtin_amdgcn_s_getreg(XCC_ID_HWREG) & 15;
+}
+
 template <u32 M>
 __global__ __launch_bounds__(NT) void k_rows_ip(const RowIpJob* __restrict__ jobs, const PrimeConst* __restrict__ pcs,
                                                 const FTw* __restrict__ twf, const FTw* __restrict__ rt, u32 log_n,
@@ -607,6 +782,85 @@ static void launch_one(bool fwd, bool cols, const NttTables& t, const LimbRef* r
     else
         hipLaunchKernelGGL(fast::k_inv_rows<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.ri, t.log_n,
                            nrefs);
+}
+
+template <u32 MA, u32 MB>
+static void launch_fused_ab(const NttTables& t, const LimbRef* refs, u32 nrefs, const FinishJob* fin, u32* ctl,
+                            const FusedCfg& f, hipStream_t s) {
+    if (fin)
+        hipLaunchKernelGGL((fast::k_fwd_fused<MA, MB, true>), dim3(f.grid), dim3(fast::NT), 0, s, refs, t.pc, t.fwd,
+                           t.fwd_f, t.rf, t.log_n, nrefs, fin, ctl, f.nx, f.lag, f.err);
+    else
+        hipLaunchKernelGGL((fast::k_fwd_fused<MA, MB, false>), dim3(f.grid), dim3(fast::NT), 0, s, refs, t.pc, t.fwd,
+                           t.fwd_f, t.rf, t.log_n, nrefs, fin, ctl, f.nx, f.lag, f.err);
+}
+
+void ntt_fast_forward_fused(const NttTables& t, const LimbRef* refs, u32 nrefs, const FinishJob* fin, u32* ctl,
+                            const FusedCfg& f, hipStream_t s) {
+    if (nrefs == 0) return;
+    const u32 B = t.log_n / 2, A = t.log_n - B;
+    if (A == 6 && B == 6) launch_fused_ab<6, 6>(t, refs, nrefs, fin, ctl, f, s);
+    else if (A == 7 && B == 6) launch_fused_ab<7, 6>(t, refs, nrefs, fin, ctl, f, s);
+    else if (A == 7 && B == 7) launch_fused_ab<7, 7>(t, refs, nrefs, fin, ctl, f, s);
+    else if (A == 8 && B == 7) launch_fused_ab<8, 7>(t, refs, nrefs, fin, ctl, f, s);
+    else launch_fused_ab<8, 8>(t, refs, nrefs, fin, ctl, f, s);
+}
+
+template <u32 MA, u32 MB>
+static void launch_inv_fused_ab(const NttTables& t, const LimbRef* refs, u32 nrefs, u32* ctl, const FusedCfg& f,
+                                hipStream_t s) {
+    hipLaunchKernelGGL((fast::k_inv_fused<MA, MB>), dim3(f.grid), dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.ri,
+                       t.log_n, nrefs, ctl, f.nx, f.lag, f.err);
+}
+
+void ntt_fast_inverse_fused(const NttTables& t, const LimbRef* refs, u32 nrefs, u32* ctl, const FusedCfg& f,
+                            hipStream_t s) {
+    if (nrefs == 0) return;
+    const u32 B = t.log_n / 2, A = t.log_n - B;
+    if (A == 6 && B == 6) launch_inv_fused_ab<6, 6>(t, refs, nrefs, ctl, f, s);
+    else if (A == 7 && B == 6) launch_inv_fused_ab<7, 6>(t, refs, nrefs, ctl, f, s);
+    else if (A == 7 && B == 7) launch_inv_fused_ab<7, 7>(t, refs, nrefs, ctl, f, s);
+    else if (A == 8 && B == 7) launch_inv_fused_ab<8, 7>(t, refs, nrefs, ctl, f, s);
+    else launch_inv_fused_ab<8, 8>(t, refs, nrefs, ctl, f, s);
+}
+
+u32 fused_occupancy() {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fast::k_fwd_fused<8, 8, false>, fast::NT, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return (u32)blocks;
+}
+
+u32 xcc_layout_check(hipStream_t s) {
+    constexpr u32 G = 4096;
+    u32* d = nullptr;
+    if (hipMalloc(&d, G * sizeof(u32)) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    std::vector<u32> h(G, 0xffffffffu);
+    hipLaunchKernelGGL(fast::k_xcc_ids, dim3(G), dim3(64), 0, s, d);
+    const bool ok = hipMemcpyAsync(h.data(), d, G * sizeof(u32), hipMemcpyDeviceToHost, s) == hipSuccess &&
+                    hipStreamSynchronize(s) == hipSuccess;
+    (void)hipFree(d);
+    if (!ok) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    // Ids 0..nx-1 each present (the round-robin dispatch may start at any XCD, so the id of a
+    // workgroup is not its index modulo nx; the one-launch transform reads the id itself).
+    u32 nx = 0;
+    u32 seen[16] = {0};
+    for (u32 v : h) {
+        if (v >= 8) return 0;
+        nx = std::max(nx, v + 1);
+        ++seen[v];
+    }
+    for (u32 x = 0; x < nx; ++x)
+        if (seen[x] < G / (4 * nx)) return 0;
+    return nx;
 }
 
 static void launch_m(u32 m, bool fwd, bool cols, const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s,

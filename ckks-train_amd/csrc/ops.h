@@ -42,6 +42,8 @@ void inner_product_launch(const IpJob* jobs, u32 njobs, const PrimeConst* pc, u3
                           hipStream_t stream);
 
 // ref.dst <- ref.src (one limb per descriptor).
+// Grid-stride copy of bytes (a multiple of 16) with 16-byte loads and stores.
+void stream_copy_launch(const void* in, void* out, u64 bytes, hipStream_t stream);
 void copy_launch(const LimbRef* refs, u32 nrefs, u32 log_n, hipStream_t stream);
 
 enum PwOp : u32 { PW_ADD = 0, PW_SUB = 1, PW_NEG = 2, PW_MUL = 3, PW_COPY = 4 };
@@ -57,6 +59,10 @@ void scalar_mul_launch(const u64* a, u64* out, u32 limbs, const u64* consts, boo
 
 // out limb i <- sigma_g(in limb i) in NTT form (index permutation, prime independent).
 void automorphism_ntt_launch(const u64* in, u64* out, u32 limbs, u64 g, u32 log_n, hipStream_t stream);
+// The same in coefficient form (a signed permutation: limb r of the layout reduced by its prime,
+// Q primes for r <= level, then P primes).
+void automorphism_coeff_launch(const u64* in, u64* out, u32 limbs, u32 level, u32 nq, u64 g, const PrimeConst* pc,
+                               u32 log_n, hipStream_t stream);
 // The same for many (limb, Galois element) jobs in one launch.
 struct AutJob {
     const u64* in;

@@ -1,5 +1,7 @@
 // Key-switch inner product, pointwise arithmetic, NTT-form automorphism and the seeded
 // input generator, for gfx950.
+#include <algorithm>
+
 #include "ops.h"
 
 namespace ckh {
@@ -79,6 +81,18 @@ void copy_launch(const LimbRef* refs, u32 nrefs, u32 log_n, hipStream_t stream) 
     if (nrefs == 0) return;
     const u32 n = 1u << log_n;
     hipLaunchKernelGGL(k_copy, dim3((n + 255) / 256, nrefs), dim3(256), 0, stream, refs, log_n);
+}
+
+// Streaming copy of n16 16-byte words (a measurement of the HBM copy rate a kernel reaches).
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ in, uint4* __restrict__ out, u64 n16) {
+    const u64 stride = (u64)gridDim.x * 256;
+    for (u64 i = (u64)blockIdx.x * 256 + threadIdx.x; i < n16; i += stride) out[i] = in[i];
+}
+
+void stream_copy_launch(const void* in, void* out, u64 bytes, hipStream_t stream) {
+    const u64 n16 = bytes / 16;
+    const u32 blocks = (u32)std::min<u64>((n16 + 255) / 256, 16384);
+    if (blocks) hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, stream, (const uint4*)in, (uint4*)out, n16);
 }
 
 __global__ __launch_bounds__(256) void k_pointwise(u32 op, const u64* __restrict__ a, const u64* __restrict__ b,
@@ -184,6 +198,42 @@ void automorphism_ntt_launch(const u64* in, u64* out, u32 limbs, u64 g, u32 log_
     if (limbs == 0) return;
     const u32 n = 1u << log_n;
     hipLaunchKernelGGL(k_automorphism_ntt, dim3((n + 255) / 256, limbs), dim3(256), 0, stream, in, out, g, log_n);
+}
+
+// Coefficient form (crates/ckks-train-core/src/poly.rs:255-276): coefficient i moves to
+// e = i g mod 2N, negated mod its prime when e >= N (X^N = -1). Written as a gather so every
+// output word is stored once, coalesced: output j takes input i = j g^-1 mod 2N (g is odd, so
+// invertible mod 2N), as it is when i < N and negated from i - N otherwise, since (i - N) g =
+// j - N g = j + N mod 2N for odd g. Limb r of the layout is Q prime r for r <= level, else P prime
+// nq + r - level - 1.
+__global__ __launch_bounds__(256) void k_automorphism_coeff(const u64* __restrict__ in, u64* __restrict__ out,
+                                                            u64 g_inv, u32 level, u32 nq,
+                                                            const PrimeConst* __restrict__ pc, u32 log_n) {
+    const u32 n = 1u << log_n;
+    const u32 j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const u32 r = blockIdx.y;
+    const u64 q = pc[r <= level ? r : nq + r - level - 1].q;
+    const u64 base = (u64)r * n;
+    const u32 i = (u32)((j * g_inv) & (2 * (u64)n - 1));
+    if (i < n) {
+        out[base + j] = in[base + i];
+    } else {
+        const u64 c = in[base + i - n];
+        out[base + j] = c == 0 ? 0 : q - c;
+    }
+}
+
+void automorphism_coeff_launch(const u64* in, u64* out, u32 limbs, u32 level, u32 nq, u64 g, const PrimeConst* pc,
+                               u32 log_n, hipStream_t stream) {
+    if (limbs == 0) return;
+    const u64 two_n = 2ull << log_n;
+    u64 g_inv = 1;  // g^(2N/2 - 1) = g^-1 in the unit group mod 2N (order dividing N)
+    for (u64 e = (two_n >> 1) - 1, b = g % two_n; e; e >>= 1, b = b * b % two_n)
+        if (e & 1) g_inv = g_inv * b % two_n;
+    const u32 n = 1u << log_n;
+    hipLaunchKernelGGL(k_automorphism_coeff, dim3((n + 255) / 256, limbs), dim3(256), 0, stream, in, out, g_inv, level,
+                       nq, pc, log_n);
 }
 
 __global__ __launch_bounds__(256) void k_automorphism_batch(const AutJob* __restrict__ jobs, u32 log_n) {

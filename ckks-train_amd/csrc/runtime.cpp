@@ -89,6 +89,19 @@ Context::Context(int ordinal, const ckh_params& p, u32 n_streams) {
     }
     build_tables();
     static_scratch = static_scratch_address(device);
+    {
+        void* e = nullptr;
+        hip_check(hipMalloc(&e, 256), "hipMalloc(error word)");
+        perm_allocs_.push_back(e);
+        hip_check(hipMemset(e, 0, 256), "hipMemset");
+        d_err = (u32*)e;
+        int cus = 0;
+        hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "attribute");
+        fused.nx = xcc_layout_check(streams_[0]);
+        cu_count = (u32)cus;
+        fused.grid = cu_count * std::max<u32>(1, fused_occupancy());
+        fused.err = d_err;
+    }
 }
 
 Context::~Context() {
@@ -235,8 +248,23 @@ void Context::set_pool_release_threshold(u64 bytes) {
 }
 
 void Context::synchronize_all() {
-    std::lock_guard<std::mutex> lk(plan_mu_);
-    for (auto st : streams_) hip_check(hipStreamSynchronize(st), "synchronize");
+    {
+        std::lock_guard<std::mutex> lk(plan_mu_);
+        for (auto st : streams_) hip_check(hipStreamSynchronize(st), "synchronize");
+    }
+    check_device_error(stream());
+}
+
+void Context::check_device_error(hipStream_t s) {
+    if (!d_err) return;
+    u32 v = 0;
+    hip_check(hipMemcpyAsync(&v, d_err, sizeof(u32), hipMemcpyDeviceToHost, s), "error word");
+    hip_check(hipStreamSynchronize(s), "error word");
+    if (v) {
+        hip_check(hipMemsetAsync(d_err, 0, sizeof(u32), s), "error word");
+        hip_check(hipStreamSynchronize(s), "error word");
+        fail(CKH_ERR_DEVICE, "a one-launch NTT waited past its bound; results of work since the last check are unreliable");
+    }
 }
 
 void Context::trim_pool(u64 keep_bytes) {
@@ -742,12 +770,29 @@ void Program::step(StepKind k, size_t off, u32 count, size_t off2, u32 a, u32 b)
     if (count) steps.push_back({k, count, off, off2, a, b});
 }
 
+void Program::step_ntt(StepKind k, size_t off, u32 count, size_t off2) {
+    if (!count) return;
+    const size_t ctl = add(std::vector<u32>(fused_ctl_words(count), 0u));
+    steps.push_back({k, count, off, off2, 0, 0});
+    steps.back().ctl = ctl;
+}
+
 void Context::run(const Program& p, const uint8_t* d, hipStream_t s) const {
     for (const Program::Step& st : p.steps) {
         const void* x = d + st.off;
         switch (st.kind) {
-            case S_NTT_FWD: launch_ntt(*this, (const LimbRef*)x, st.count, true, s); break;
-            case S_NTT_INV: launch_ntt(*this, (const LimbRef*)x, st.count, false, s); break;
+            case S_NTT_FWD:
+                if (st.ctl && fused_usable())
+                    ntt_fast_forward_fused(ntt, (const LimbRef*)x, st.count, nullptr, (u32*)(d + st.ctl), fused, s);
+                else
+                    launch_ntt(*this, (const LimbRef*)x, st.count, true, s);
+                break;
+            case S_NTT_INV:
+                if (st.ctl && fused_usable())
+                    ntt_fast_inverse_fused(ntt, (const LimbRef*)x, st.count, (u32*)(d + st.ctl), fused, s);
+                else
+                    launch_ntt(*this, (const LimbRef*)x, st.count, false, s);
+                break;
             case S_CONV:
                 launch_conv(conv, (const ConvJob*)x, st.count, st.a, st.b, st.sched ? (st.sched >> 1) & 127 : conv_variant,
                             st.sched ? st.sched >> 8 : conv_tg, s);
@@ -755,7 +800,11 @@ void Context::run(const Program& p, const uint8_t* d, hipStream_t s) const {
             case S_COPY: launch_copy((const LimbRef*)x, st.count, log_n, s); break;
             case S_IP: launch_ip((const IpJob*)x, st.count, d_pc, log_n, s); break;
             case S_NTT_FWD_FIN:
-                launch_ntt_finish((const LimbRef*)x, (const FinishJob*)(d + st.off2), st.count, s);
+                if (st.ctl && fused_usable())
+                    ntt_fast_forward_fused(ntt, (const LimbRef*)x, st.count, (const FinishJob*)(d + st.off2),
+                                           (u32*)(d + st.ctl), fused, s);
+                else
+                    launch_ntt_finish((const LimbRef*)x, (const FinishJob*)(d + st.off2), st.count, s);
                 break;
             case S_FINISH: launch_finish((const FinishJob*)x, st.count, d_pc, log_n, s); break;
             case S_NTT_COLS: ntt_fast_forward_cols(ntt, (const LimbRef*)x, st.count, s); break;
@@ -892,7 +941,7 @@ void Context::ntt_poly(const u64* in, u64* out, u32 limbs, u32 level, u32 p_limb
     }
     (void)p_limbs;
     Program p;
-    p.step(forward ? S_NTT_FWD : S_NTT_INV, p.add(refs), (u32)refs.size());
+    p.step_ntt(forward ? S_NTT_FWD : S_NTT_INV, p.add(refs), (u32)refs.size());
     execute(p, s);
 }
 
@@ -914,7 +963,7 @@ static void group_by_prime(std::vector<LimbRef>& r) {
 }
 
 void Context::prog_decompose(Program& p, const std::vector<DecTask>& tasks) const {
-    std::vector<LimbRef> intt, copies, nttr, cols, rows;
+    std::vector<LimbRef> intt, copies, nttr, cols;
     std::vector<ConvJob> jobs;
     u32 max_tgt = 0;
     for (const DecTask& t : tasks) {
@@ -929,26 +978,17 @@ void Context::prog_decompose(Program& p, const std::vector<DecTask>& tasks) cons
                 for (u32 i = dr[j].first; i < dr[j].second; ++i) copies.push_back({t.d + i * n, dj + i * n, i, 0});
             for (u32 i = 0; i < L; ++i)
                 if (i < dr[j].first || i >= dr[j].second) (t.fuse ? cols : nttr).push_back({dj + i * n, dj + i * n, i, 0});
-            for (u32 k = 0; k < np; ++k) {
-                const LimbRef r{dj + (L + k) * n, dj + (L + k) * n, nq + k, 0};
-                if (t.fuse) {
-                    cols.push_back(r);
-                    rows.push_back(r);
-                } else {
-                    nttr.push_back(r);
-                }
-            }
+            // P targets are transformed whole (their inner product is not fused with a row pass)
+            for (u32 k = 0; k < np; ++k) nttr.push_back({dj + (L + k) * n, dj + (L + k) * n, nq + k, 0});
         }
         max_tgt = std::max(max_tgt, modup_maxtgt_[t.level]);
     }
     group_by_prime(nttr);
-    group_by_prime(rows);
-    p.step(S_NTT_INV, p.add(intt), (u32)intt.size());
+    p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size());
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_tgt, alpha);
     p.step(S_COPY, p.add(copies), (u32)copies.size());
-    p.step(S_NTT_FWD, p.add(nttr), (u32)nttr.size());
+    p.step_ntt(S_NTT_FWD, p.add(nttr), (u32)nttr.size());
     p.step(S_NTT_COLS, p.add(cols), (u32)cols.size());
-    p.step(S_NTT_ROWS, p.add(rows), (u32)rows.size());
 }
 
 // keyswitch::apply_key (keyswitch.rs:230-309) for a batch: the inner product over Q_l * P, then
@@ -1014,10 +1054,10 @@ void Context::prog_mod_down(Program& p, const std::vector<MdTask>& tasks) const 
         }
     }
     group_by_prime(intt);
-    p.step(S_NTT_INV, p.add(intt), (u32)intt.size());
+    p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size());
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_l, np);
     const size_t o_ntt = p.add(nttr), o_fin = p.add(fin);
-    p.step(S_NTT_FWD_FIN, o_ntt, (u32)nttr.size(), o_fin);
+    p.step_ntt(S_NTT_FWD_FIN, o_ntt, (u32)nttr.size(), o_fin);
 }
 
 // RnsPoly::rescale_by_last in NTT form (baseconv.rs:258-291) for a batch.
@@ -1036,10 +1076,10 @@ void Context::prog_rescale(Program& p, const std::vector<RescaleTask>& tasks) co
         }
         max_t = std::max(max_t, l);
     }
-    p.step(S_NTT_INV, p.add(intt), (u32)intt.size());
+    p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size());
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_t, 1);
     const size_t o_ntt = p.add(nttr), o_fin = p.add(fin);
-    p.step(S_NTT_FWD_FIN, o_ntt, (u32)nttr.size(), o_fin);
+    p.step_ntt(S_NTT_FWD_FIN, o_ntt, (u32)nttr.size(), o_fin);
 }
 
 // RnsPoly::raise_from_level0 (crates/ckks-train-core/src/poly.rs:298-345; the device form of
@@ -1064,9 +1104,9 @@ void Context::prog_raise(Program& p, const std::vector<RaiseTask>& tasks) const 
         max_t = std::max(max_t, t.top);
     }
     p.step(S_COPY, p.add(copy), (u32)copy.size());
-    p.step(S_NTT_INV, p.add(intt), (u32)intt.size());
+    p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size());
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_t, 1);
-    p.step(S_NTT_FWD, p.add(nttr), (u32)nttr.size());
+    p.step_ntt(S_NTT_FWD, p.add(nttr), (u32)nttr.size());
 }
 
 void Context::raise_level0(const std::vector<RaiseTask>& tasks, hipStream_t s) {

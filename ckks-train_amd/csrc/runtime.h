@@ -160,9 +160,10 @@ struct Program {
         size_t off, off2;
         u32 a, b;
         u32 sched = 0;  // S_CONV: 0 = the context's schedule, else 1 | variant << 1 | tg << 8
+        size_t ctl = 0;  // NTT steps: offset of the one-launch transform's zeroed control block
         bool operator==(const Step& o) const {
             return kind == o.kind && count == o.count && off == o.off && off2 == o.off2 && a == o.a && b == o.b &&
-                   sched == o.sched;
+                   sched == o.sched && ctl == o.ctl;
         }
     };
     std::vector<uint8_t> bytes;
@@ -176,6 +177,10 @@ struct Program {
         return off;
     }
     void step(StepKind k, size_t off, u32 count, size_t off2 = 0, u32 a = 0, u32 b = 0);
+    // An NTT step (S_NTT_FWD, S_NTT_INV, or S_NTT_FWD_FIN with its finish jobs at off2) with a
+    // zeroed control block in the descriptor bytes, for the one-launch transforms: every execution
+    // uploads the bytes again, so each launch finds its block zero.
+    void step_ntt(StepKind k, size_t off, u32 count, size_t off2 = 0);
     // Every base conversion of the program with this kernel variant and targets per workgroup.
     void set_conv_schedule(u32 variant, u32 tg) {
         for (Step& st : steps)
@@ -289,6 +294,15 @@ class Context : public std::enable_shared_from_this<Context> {
     // descriptor bytes copied into the plan's own device buffer before each replay. Tuning key
     // "graphs" (1 = on) and "graph_cache" (plans kept per context, least recently used evicted).
     u32 use_graphs = 1, graph_cache_max = 64;
+    // The one-launch NTTs (tuning key "ntt_fused", 1 = on where the device's XCD layout was
+    // verified at creation; "fused_lag": slots between a limb's first- and second-pass tiles).
+    FusedCfg fused{};
+    u32 ntt_fused = 1;
+    u32 cu_count = 0;
+    bool fused_usable() const { return ntt_fused && fused.nx && ntt_fast_supported(log_n); }
+    u32* d_err = nullptr;  // device error word (a one-launch transform's wait overran its bound)
+    // Raises a Device error (and clears the word) if a kernel has set the device error word.
+    void check_device_error(hipStream_t s);
     // Row pass fused into the inner product of batched key switches (tuning key "fuse_ip");
     // possible when the register-radix NTT applies and every Q prime is below 2^50.
     u32 fuse_ip = 1;

@@ -83,7 +83,7 @@ void Server::launch(std::vector<OpPtr>& ops) {
                 (void)hipStreamWaitEvent(s, op->ready, 0);
             }
         // Injected failures and invalid requests fail alone; the rest run batched.
-        std::vector<OpPtr> relin, mrr;
+        std::vector<OpPtr> relin, mrr, hoisted;
         for (auto& op : mine) {
             if (op->inject_failure) {
                 fail_op(op, CKH_ERR_DEVICE, "injected device failure");
@@ -91,11 +91,23 @@ void Server::launch(std::vector<OpPtr>& ops) {
             }
             if (op->kind == CKH_OP_RELINEARIZE) relin.push_back(op);
             else if (op->kind == CKH_OP_MUL_RELIN_RESCALE) mrr.push_back(op);
-            else {
-                try {
-                    op->outs = ev->hoisted_rotate(*op->a, op->steps);
-                } catch (const Error& e) {
-                    fail_op(op, e.status, e.msg);
+            else hoisted.push_back(op);
+        }
+        // Hoisted rotations of every request in one batch (one decomposition launch sequence and
+        // one switch program for all their steps); a failing member falls back to one call each.
+        if (!hoisted.empty()) {
+            std::vector<std::pair<const Ct*, std::vector<int32_t>>> items;
+            for (auto& op : hoisted) items.push_back({op->a.get(), op->steps});
+            try {
+                auto outs = ev->hoisted_rotate_batch(items);
+                for (size_t i = 0; i < hoisted.size(); ++i) hoisted[i]->outs = std::move(outs[i]);
+            } catch (const Error&) {
+                for (auto& op : hoisted) {
+                    try {
+                        op->outs = ev->hoisted_rotate(*op->a, op->steps);
+                    } catch (const Error& e) {
+                        fail_op(op, e.status, e.msg);
+                    }
                 }
             }
         }

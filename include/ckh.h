@@ -148,7 +148,8 @@ ckh_status ckh_poly_uniform(ckh_context *ctx, uint32_t level, uint32_t p_limbs, 
 /* RnsPoly::to_ntt / to_coeff (crates/ckks-train-core/src/poly.rs:192-211). */
 ckh_status ckh_ntt(ckh_context *ctx, const ckh_poly *in, ckh_poly **out);
 ckh_status ckh_intt(ckh_context *ctx, const ckh_poly *in, ckh_poly **out);
-/* RnsPoly::automorphism in NTT form (poly.rs:242-283). */
+/* RnsPoly::automorphism (poly.rs:242-283): the NTT-form permutation or, for coefficient-form
+ * input, the signed permutation X -> X^g. */
 ckh_status ckh_automorphism(ckh_context *ctx, const ckh_poly *in, uint64_t galois_elt,
                             ckh_poly **out);
 /* Measurement hook (no counterpart in the reference): transforms every limb of the n polys in
@@ -157,6 +158,10 @@ ckh_status ckh_automorphism(ckh_context *ctx, const ckh_poly *in, uint64_t galoi
  * polynomials' contents (residues stay reduced). */
 ckh_status ckh_time_ntt(ckh_context *ctx, ckh_poly *const *polys, uint32_t n, int forward,
                         uint32_t reps, double *ms_per_launch);
+/* Measurement hook (no counterpart in the reference): a streaming device copy of bytes (a
+ * multiple of 16) between two fresh buffers, 16-byte loads and stores, reps launches between two
+ * HIP events on the calling thread's stream; writes the mean milliseconds per launch. */
+ckh_status ckh_time_copy(ckh_context *ctx, uint64_t bytes, uint32_t reps, double *ms_per_launch);
 /* RnsPoly::rescale_by_last (crates/ckks-train-core/src/baseconv.rs:258-291). */
 ckh_status ckh_rescale_poly(ckh_context *ctx, const ckh_poly *in, ckh_poly **out);
 /* RnsPoly::mod_down of a QpPoly (baseconv.rs:459-511): Q part level l -> level l. */

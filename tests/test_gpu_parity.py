@@ -55,6 +55,48 @@ def test_ntt_matches_oracle(log_n, bits, limbs):
     assert (z.download() == 0).all()
 
 
+@pytest.mark.parametrize("log_n", [12, 13, 14, 15, 16])
+def test_one_launch_forward_ntt(log_n):
+    """The one-launch transforms (the first pass handed to the second through one XCD's L2),
+    forward and inverse, against the oracle and against the two-launch forms, for mixed 50/60-bit limbs in limb counts
+    that leave XCDs with no limb, one, or uneven shares, in place and out of place, at several
+    lags between a limb's column and row tiles; the device error word stays clear."""
+    n = 1 << log_n
+    q = O.ntt_primes(50, n, 5)
+    p = O.ntt_primes(60, n, 3)
+    ctx = ckh.Context(log_n, q, p, 1)
+    assert ctx.tuning("xcds") in (0, 8)
+    if ctx.tuning("xcds") == 0:
+        pytest.skip("XCD layout not verified on this device: the one-launch transform is off")
+    for lag in (1, 4, 16):
+        ctx.set_tuning("fused_lag", lag)
+        for level, pl in ((0, 0), (2, 3), (4, 3)):
+            say("one-launch ntt", log_n, "lag", lag, "limbs", level + 1 + pl)
+            primes = q[: level + 1] + (p if pl else [])
+            a = O.uniform_poly(primes, n, 31 + lag + level)
+            want = O.ntt_forward(n, primes, a.ravel()).reshape(len(primes), n)
+            d = ctx.upload(a, level, p_limbs=pl, ntt=False)
+            f = ctx.upload(want, level, p_limbs=pl, ntt=True)
+            ctx.set_tuning("ntt_fused", 1)
+            got, inv = d.ntt().download(), f.intt().download()
+            ctx.set_tuning("ntt_fused", 0)
+            two, inv2 = d.ntt().download(), f.intt().download()
+            ctx.set_tuning("ntt_fused", 1)
+            assert (got == want).all() and (two == want).all(), (lag, level, pl)
+            assert (inv == a).all() and (inv2 == a).all(), (lag, level, pl)
+    # many limbs in one launch (every XCD busy with several), in place through the timing hook
+    polys = [ctx.uniform(4, seed=70 + i, p_limbs=3, ntt=False) for i in range(13)]
+    hosts = [x.download() for x in polys]
+    ctx.time_ntt(polys, forward=True, reps=1)  # two in-place transforms of every limb
+    primes = q + p
+    for h, x in zip(hosts, polys):
+        once = O.ntt_forward(n, primes, h.ravel())
+        # the transform applied twice to the same words (the warm-up launch and the timed one)
+        twice = O.ntt_forward(n, primes, once).reshape(len(primes), n)
+        assert (x.download() == twice).all()
+    ctx.synchronize()  # raises if a wait overran its bound
+
+
 def test_c1_plain_ntt_round_trip():
     """N = 2^13, the 4 largest 50-bit primes = 1 mod 2N, forward then inverse, bit-exact."""
     log_n, n = 13, 1 << 13
@@ -86,6 +128,31 @@ def test_automorphism(env):
     for g in [ckh.galois_elt(ref.log_n, 1), ckh.galois_elt(ref.log_n, -3), ckh.conjugation_elt(ref.log_n)]:
         got = d.automorphism(g).download()
         assert (got == O.automorphism_ntt(ref.n, g, a.ravel()).reshape(a.shape)).all()
+
+
+def test_automorphism_coefficient_form(env):
+    """The signed permutation of coefficient-form input (poly.rs:255-276) against the oracle, on a
+    Q polynomial and a Q*P one (each limb negated mod its own prime), zero words included; it
+    commutes with the NTT-form permutation through the transform."""
+    ctx, ref = env
+    level = len(ref.q) - 1
+    a = host_uniform(ref, level, 5)
+    a[0, :7] = 0
+    qp = host_uniform(ref, level, 6, p_limbs=1)
+    steps = [1, -3, 7]
+    for g in [ckh.galois_elt(ref.log_n, s) for s in steps] + [ckh.conjugation_elt(ref.log_n), 1]:
+        got = ctx.upload(a, level, ntt=False).automorphism(g).download()
+        assert (got == O.automorphism_coeff(ref.n, g, ref.q[: level + 1], a.ravel()).reshape(a.shape)).all(), g
+        got = ctx.upload(qp, level, p_limbs=len(ref.p), ntt=False).automorphism(g).download()
+        want = O.automorphism_coeff(ref.n, g, ref.q[: level + 1] + ref.p, qp.ravel()).reshape(qp.shape)
+        assert (got == want).all(), g
+    g = ckh.galois_elt(ref.log_n, 5)
+    c = ctx.upload(a, level, ntt=False)
+    via_coeff = c.automorphism(g).ntt().download()
+    via_ntt = c.ntt().automorphism(g).download()
+    assert (via_coeff == via_ntt).all()
+    with pytest.raises(ckh.InvalidParams):
+        c.automorphism(2)
 
 
 def test_rescale_poly(env):
@@ -322,25 +389,33 @@ def test_c2_full_size_key_switch():
     assert (kb.download() == wb).all() and (ka.download() == wa).all()
 
 
-def test_c3_batch_slice_full_size():
-    """48 polynomials of the benchmark workload (N = 2^16, its seeded levels 3..23) under one key:
-    the batched switch, first run and graph replay, equals one switch per call word for word
-    (by digest), and one member equals the oracle."""
-    say("c3 slice full size")
+def test_c3_full_batch():
+    """The benchmark workload at its stated size: 256 polynomials at N = 2^16 (24 + 6 primes,
+    dnum 4), limbs U{4..24} from its seed, one shared key. The batched switch, on its first run
+    and on graph replays, equals one switch per call for every member (by digest), and eight
+    members whose levels span the workload's range equal the oracle word for word."""
+    say("c3 full batch")
     ctx, ref = make(C2)
     rng = np.random.default_rng(20261202)
-    levels = [int(x) - 1 for x in rng.integers(4, 25, size=256)][:48]
+    levels = [int(x) - 1 for x in rng.integers(4, 25, size=256)]
     key = ctx.ksk_uniform(23, seed=515151)
     polys = [ctx.uniform(lv, seed=5150 + i) for i, lv in enumerate(levels)]
     single = [(kb.digest(), ka.digest()) for kb, ka in (ctx.key_switch(p, key) for p in polys)]
     for rep in range(3):  # direct, captured, replayed
         got = [(kb.digest(), ka.digest()) for kb, ka in ctx.key_switch_batch(polys, key)]
         assert got == single, rep
-    i = int(np.argmax(levels))
-    kb, ka = ctx.key_switch_batch(polys, key)[i]
+    batch = ctx.key_switch_batch(polys, key)
+    distinct = sorted(set(levels))
+    picks = [distinct[round(j * (len(distinct) - 1) / 7)] for j in range(8)]
     O.set_threads(16)
-    wb, wa = ref.key_switch(host_uniform(ref, levels[i], 5150 + i).ravel(), levels[i], ref.uniform_key(23, 515151), 23)
-    assert (kb.download() == wb).all() and (ka.download() == wa).all()
+    wkey = ref.uniform_key(23, 515151)
+    for lv in picks:
+        i = levels.index(lv)
+        say("oracle member", i, "level", lv)
+        wb, wa = ref.key_switch(host_uniform(ref, lv, 5150 + i).ravel(), lv, wkey, 23)
+        kb, ka = batch[i]
+        assert (kb.download() == wb).all() and (ka.download() == wa).all(), (i, lv)
+    assert min(picks) == min(levels) and max(picks) == max(levels)
 
 
 # ---------------------------------------------------------------- BSGS (hoisted rotations)

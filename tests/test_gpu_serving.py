@@ -142,3 +142,87 @@ def test_pool_mode_zero_refused():
         ctx.set_tuning("pool", 0)
     assert ctx.tuning("pool") == 1
     ctx.set_tuning("pool", 1)
+
+
+FULL_SETS = [("c2", 16, [50] * 24, [60] * 6, 4), ("c4", 15, [50] * 20, [60] * 4, 5)]
+N_ROT_KEYS = 64
+
+
+def test_served_mix_full_size_64_resident_keys():
+    """The served mixed stream at its stated size: N = 2^16 (24 + 6 primes, dnum 4) and N = 2^15
+    (20 + 4, dnum 5), 64 rotation keys and a relinearization key resident per set, 48 Poisson
+    arrivals of the 50/30/20 mix at levels 3..top. Every result equals the same operation run
+    alone (by digest); three members (one of each kind, both sets) equal the oracle's composition
+    word for word."""
+    import oracle as O
+
+    O.set_threads(16)
+    envs, refs = [], []
+    for i, case in enumerate(FULL_SETS):
+        print("set", case[0], flush=True)
+        ctx, ref = make(case, n_streams=2)
+        L = len(ref.q) - 1
+        relin = ctx.ksk_uniform(L, seed=60000 + 100 * i)
+        gk = [ctx.ksk_uniform(L, seed=60000 + 100 * i + s, galois=ckh.galois_elt(case[1], s))
+              for s in range(1, N_ROT_KEYS + 1)]
+        envs.append((ctx, ref, ckh.Evaluator(ctx, relin, gk)))
+        refs.append((relin, gk))
+    rng = np.random.default_rng(61)
+    reqs, t = [], 0.0
+    for i in range(48):
+        t += rng.exponential(1.0 / 4000.0)
+        e = int(rng.integers(0, 2))
+        ctx, ref, ev = envs[e]
+        L = len(ref.q) - 1
+        level = int(rng.integers(3, L + 1))
+        u = rng.random()
+        comps = lambda k, s: ctx.ct_from_polys([ctx.uniform(level, seed=s + j) for j in range(k)], 2.0 ** 40)
+        if u < 0.5:
+            reqs.append((ckh.OP_RELINEARIZE, e, comps(3, 70000 + 10 * i), None, [], t, level))
+        elif u < 0.8:
+            reqs.append((ckh.OP_HOISTED_ROTATE, e, comps(2, 70000 + 10 * i), None, list(range(1, 17)), t, level))
+        else:
+            reqs.append((ckh.OP_MUL_RELIN_RESCALE, e, comps(2, 70000 + 10 * i), comps(2, 70005 + 10 * i), [], t,
+                         level))
+    srv = ckh.Server(hbm_budget_bytes=200 * 10 ** 9, max_batch=16)
+    t0 = srv.now() + 0.05
+    tickets = [srv.submit(k, envs[e][2], a, b, steps, arrival_s=t0 + ta) for (k, e, a, b, steps, ta, _) in reqs]
+    results = [srv.poll(tk, wait=True) for tk in tickets]
+    st = srv.stats()
+    assert st.completed == 48 and st.failed == 0 and st.rejected == 0
+    for i, ((status, msg, outs, _), req) in enumerate(zip(results, reqs)):
+        assert status == 0, msg
+        alone = _alone(envs, req[:6])
+        assert [o.digests() for o in outs] == [w.digests() for w in alone], (i, req[0])
+    srv.close()
+    # one member of each kind against the oracle composition, words downloaded from the served result
+    checked = set()
+    for i, (req, (_, _, outs, _)) in enumerate(zip(reqs, results)):
+        kind, e, a, b, steps, _, level = req
+        if kind in checked or (kind == ckh.OP_HOISTED_ROTATE and e != 1):
+            continue
+        checked.add(kind)
+        ctx, ref, ev = envs[e]
+        L = len(ref.q) - 1
+        relin, gk = refs[e]
+        print("oracle check", i, kind, "set", e, "level", level, flush=True)
+        h = a.download()
+        if kind == ckh.OP_RELINEARIZE:
+            w0, w1 = ref.relinearize(h[0], h[1], h[2], level, ref.uniform_key(L, 60000 + 100 * e), L)
+            d = outs[0].download()
+            assert (d[0] == w0).all() and (d[1] == w1).all()
+        elif kind == ckh.OP_HOISTED_ROTATE:
+            elts = np.array([ckh.galois_elt(ctx.log_n, s) for s in steps], dtype=np.uint64)
+            keys = [ref.uniform_key(L, 60000 + 100 * e + s) for s in steps]
+            want = ref.hoisted_galois(h[0], h[1], level, elts, keys, L)
+            for j, o in enumerate(outs):
+                d = o.download()
+                assert (d[0] == want[j][0]).all() and (d[1] == want[j][1]).all(), j
+        else:
+            hb = b.download()
+            d3 = ref.mul_no_relin(h[0], h[1], hb[0], hb[1], level)
+            r0, r1 = ref.relinearize(d3[0], d3[1], d3[2], level, ref.uniform_key(L, 60000 + 100 * e), L)
+            want = ref.rescale_ct(np.stack([r0, r1]), level)
+            d = outs[0].download()
+            assert (d[0] == want[0]).all() and (d[1] == want[1]).all()
+    assert len(checked) == 3, checked
