@@ -72,45 +72,64 @@ class HipEvents:
         return ms.value
 
 
-def cpu_baseline(q, p, levels, budget_s=12.0, budget_1t_s=8.0):
+def cpu_threads():
+    """(threads used, nproc, CPUs this process may run on): the box's CPU share is given by
+    OMP_NUM_THREADS where set (16 per GPU), else every CPU of the affinity mask."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = nproc
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else affinity
+    return min(threads, affinity), nproc, affinity
+
+
+def cpu_baseline(q, p, levels, budget_s=8.0):
     """The CPU oracle (C restatement of the ckks-train CPU path, OpenMP over digits and limbs as
-    the reference's rayon feature) on the first ciphertexts of the same workload until the time
-    budget is spent (at least two switches), at the host's thread count (at most 16, the box's
-    CPU share) and again at one thread."""
+    the reference's rayon feature) on the first ciphertexts of the same workload: the sample is
+    fixed by a time budget at the host's thread share, then the same sample is timed again at one
+    thread."""
     import oracle as O
 
-    threads = min(16, os.cpu_count() or 1)
+    threads, nproc, affinity = cpu_threads()
     ref = O.Params(LOG_N, q=q, p=p, dnum=DNUM)
     key = ref.uniform_key(23, SEED + 1)
+    # inputs generated outside the timed loops (more than the budget can use)
+    inputs = [O.uniform_poly(q[: lv + 1], N, SEED + 100 + i).ravel() for i, lv in enumerate(levels[:64])]
 
-    def run(nthreads, budget):
+    def run(nthreads, count=None):
         O.set_threads(nthreads)
         done, t0 = 0, time.perf_counter()
-        for i, lv in enumerate(levels):
-            d = O.uniform_poly(q[: lv + 1], N, SEED + 100 + i)
-            ref.key_switch(d.ravel(), lv, key, 23)
+        for i, lv in enumerate(levels[: len(inputs)]):
+            if count is not None and done >= count:
+                break
+            ref.key_switch(inputs[i], lv, key, 23)
             done += 1
-            if done >= 2 and time.perf_counter() - t0 > budget:
+            if count is None and done >= 2 and time.perf_counter() - t0 > budget_s:
                 break
         return done, time.perf_counter() - t0
 
-    done, el = run(threads, budget_s)
-    done1, el1 = run(1, budget_1t_s)
+    done, el = run(threads)
+    _, el1 = run(1, done)
     O.set_threads(threads)
     return {"value": done / el, "unit": "key_switches/s", "cores": threads, "kind": "port",
             "sample": f"first {done} ciphertexts of the workload (levels {levels[:done]}), oracle "
                       f"key_switch with {threads} OpenMP threads, {el:.1f} s",
-            "value_1_thread": done1 / el1,
-            "sample_1_thread": f"first {done1} ciphertexts, 1 thread, {el1:.1f} s"}
+            "value_1_thread": done / el1,
+            "sample_1_thread": f"the same {done} ciphertexts at 1 thread, {el1:.1f} s",
+            "host_nproc": nproc, "host_affinity_cpus": affinity,
+            "threads_note": "threads = the box's CPU share for one GPU (OMP_NUM_THREADS) within the affinity mask"}
 
 
-def served_stream(ckh, n_ops=240, rates=(1500.0, 4000.0), seed=SEED + 5):
-    """The served mixed stream: Poisson arrivals (each of `rates` ops/s) over N = 2^16 (24 + 6 primes,
-    dnum 4) and N = 2^15 (20 + 4 primes, dnum 5); 50% relinearize, 30% hoisted 16-rotation
-    batch, 20% mul + relinearize + rescale; limbs U{4..24} (at most 20 at 2^15); 64 rotation
-    keys and a relinearization key resident per parameter set; 200 GB admission budget.
-    The same arrival trace runs through the plain path (one op at a time, one stream) and
-    every served result is compared with the plain one by device digest."""
+def served_stream(ckh, n_ops=2000, rates=(1000.0, 2000.0, 4000.0, 6000.0, 8000.0), seed=SEED + 5):
+    """The served mixed stream: Poisson arrivals (each of `rates` ops/s, n_ops per rate) over
+    N = 2^16 (24 + 6 primes, dnum 4) and N = 2^15 (20 + 4 primes, dnum 5); 50% relinearize, 30%
+    hoisted 16-rotation batch, 20% mul + relinearize + rescale; limbs U{4..24} (at most 20 at
+    2^15); 64 rotation keys and a relinearization key resident per parameter set; 200 GB
+    admission budget. One warm-up trace first (its figures are the cold start). Per rate the same
+    arrival trace runs through the plain path (one op at a time, one stream), and every served
+    result is compared with the plain one by device digest."""
     sets = [(16, [50] * 24, [60] * 6, 4), (15, [50] * 20, [60] * 4, 5)]
     envs = []
     for i, (log_n, qb, pb, dnum) in enumerate(sets):
@@ -120,9 +139,20 @@ def served_stream(ckh, n_ops=240, rates=(1500.0, 4000.0), seed=SEED + 5):
         relin = ctx.ksk_uniform(L, seed=seed + 1000 * i)
         rot = [ctx.ksk_uniform(L, seed=seed + 1000 * i + s, galois=ckh.galois_elt(log_n, s)) for s in range(1, 65)]
         envs.append((ctx, L, ckh.Evaluator(ctx, relin, rot)))
-    out = {}
+    warm = _trace(ckh, envs, 400, max(rates), seed + 999)
+    for op in warm[:6]:  # both contexts' kernels loaded
+        _run_plain(ckh, envs, op)
+    cold = _serve(ckh, envs, warm)[1]
+    del warm
+    out = {"cold_start": {"ops": 400, "arrival_rate_per_s": max(rates), "ops_per_s": cold.ops_per_s,
+                          "p50_ms": cold.p50_ms, "p99_ms": cold.p99_ms,
+                          "note": "first trace on a fresh server, pools and plans"}}
     for k, rate in enumerate(rates):
         out[f"at_{int(rate)}_per_s"] = _served_at(ckh, envs, n_ops, rate, seed + 7 * k)
+        r = out[f"at_{int(rate)}_per_s"]
+        print(f"# served at {rate:.0f}/s: {r['served']['ops_per_s']:.0f} ops/s p50 {r['served']['p50_ms']:.2f} "
+              f"p99 {r['served']['p99_ms']:.2f} ms; plain {r['plain']['ops_per_s']:.0f} ops/s; "
+              f"identical {r['identical_alone_and_in_mix']}", file=sys.stderr, flush=True)
     return out
 
 
@@ -155,88 +185,65 @@ def _trace(ckh, envs, n_ops, rate, seed):
     return ops
 
 
+def _run_plain(ckh, envs, op):
+    kind, e, a, b, steps, _ = op
+    ev = envs[e][2]
+    if kind == ckh.OP_RELINEARIZE:
+        return [ev.relinearize(a)]
+    if kind == ckh.OP_HOISTED_ROTATE:
+        return ev.hoisted_rotate(a, steps)
+    return [ev.rescale(ev.relinearize(ev.mul_no_relin(a, b)))]
+
+
 def _serve(ckh, envs, ops):
-    """Runs a trace through a fresh server at its arrival times; returns (results, stats)."""
+    """Runs a trace through a fresh server at its arrival times; results are polled in order and
+    reduced to (status, digests) at once, so outputs do not pile up. Returns (results, stats)."""
     srv = ckh.Server(hbm_budget_bytes=200 * 10 ** 9, max_batch=64)
     t0 = srv.now() + 0.02
     tickets = [srv.submit(k, envs[e][2], a, b, steps, arrival_s=t0 + ta) for (k, e, a, b, steps, ta) in ops]
-    served = [srv.poll(tk) for tk in tickets]
+    served = []
+    for tk in tickets:
+        status, _, outs, _ = srv.poll(tk)
+        served.append((status, [o.digests() for o in outs]))
+        del outs
     st = srv.stats()
     srv.close()
     return served, st
 
 
 def _served_at(ckh, envs, n_ops, rate, seed):
-    """One arrival trace at `rate` ops/s through the server and through the plain path, after a
-    warm-up trace of the same mix (another seed) at 1.5x the rate through the server: the timed
-    trace meets a memory pool already grown past its peak (growing it mid-trace stalls the
-    scheduler) and the launch plans of shapes seen before, as a running server does; the
-    warm-up trace's own figures are reported as the cold start."""
-    warm_rate = 1.5 * rate
-    warm_ops = _trace(ckh, envs, n_ops, warm_rate, seed + 1)
+    """One arrival trace at `rate` ops/s through the server and through the plain path."""
     ops = _trace(ckh, envs, n_ops, rate, seed)
-
-    def run_plain(op):
-        kind, e, a, b, steps, _ = op
-        ev = envs[e][2]
-        if kind == ckh.OP_RELINEARIZE:
-            return [ev.relinearize(a)]
-        if kind == ckh.OP_HOISTED_ROTATE:
-            return ev.hoisted_rotate(a, steps)
-        return [ev.rescale(ev.relinearize(ev.mul_no_relin(a, b)))]
-
-    for op in warm_ops[:6]:  # warm both contexts' kernels
-        run_plain(op)
-    for ctx, _, _ in envs:
-        ctx.synchronize()
-    cold = _serve(ckh, envs, warm_ops)[1]
-    del warm_ops
-
     served, st = _serve(ckh, envs, ops)
 
     # plain, same arrivals, one op at a time
-    lat, plain_out = [], []
+    lat, plain = [], []
     start = time.perf_counter() + 0.02
     for op in ops:
         arrival = start + op[5]
         while time.perf_counter() < arrival:
             pass
-        out = run_plain(op)
+        out = _run_plain(ckh, envs, op)
         envs[op[1]][0].synchronize()
         lat.append((time.perf_counter() - arrival) * 1e3)
-        plain_out.append(out)
+        plain.append([o.digests() for o in out])
+        del out
     makespan = time.perf_counter() - start
-    same = [s[0] == 0 and len(s[2]) == len(p) and all(a.digests() == b.digests() for a, b in zip(s[2], p))
-            for s, p in zip(served, plain_out)]
-    identical = all(same)
-    # a mismatch is recomputed once more with direct launches, to say which side differs
-    mismatches = []
-    for i, ok in enumerate(same):
-        if ok:
-            continue
-        op = ops[i]
-        ctx = envs[op[1]][0]
-        ctx.set_tuning("graphs", 0)
-        again = run_plain(op)
-        ctx.set_tuning("graphs", 1)
-        d_again = [a.digests() for a in again]
-        mismatches.append({"op": i, "kind": op[0], "level": op[2].level, "steps": len(op[4]),
-                           "status": served[i][0], "served_ok": [a.digests() for a in served[i][2]] == d_again,
-                           "plain_ok": [a.digests() for a in plain_out[i]] == d_again})
+    same = [s[0] == 0 and s[1] == p for s, p in zip(served, plain)]
     lat = np.array(lat)
     mix = {k: sum(1 for o in ops if o[0] == k) for k in (0, 1, 2)}
+    del ops
     return {
         "ops": n_ops, "arrival_rate_per_s": rate,
         "mix": {"relinearize": mix[0], "hoisted_16": mix[1], "mul_relin_rescale": mix[2]},
         "served": {"ops_per_s": st.ops_per_s, "p50_ms": st.p50_ms, "p99_ms": st.p99_ms, "mean_ms": st.mean_ms,
-                   "batches": st.batches, "peak_admitted_GB": st.peak_bytes / 1e9, "failed": st.failed},
-        "served_cold_start": {"ops_per_s": cold.ops_per_s, "p50_ms": cold.p50_ms, "p99_ms": cold.p99_ms,
-                              "arrival_rate_per_s": warm_rate,
-                              "note": "the warm-up trace (seed + 1): first server on fresh pools and plans"},
+                   "max_ms": st.max_ms, "batches": st.batches, "peak_admitted_GB": st.peak_bytes / 1e9,
+                   "failed": st.failed, "scheduler_launch_ms_total": st.launch_ms,
+                   "scheduler_launch_ms_max": st.launch_max_ms},
         "plain": {"ops_per_s": n_ops / makespan, "p50_ms": float(np.percentile(lat, 50)),
                   "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean())},
-        "identical_alone_and_in_mix": bool(identical),
-        "mismatches": mismatches[:8],
+        "identical_alone_and_in_mix": bool(all(same)),
+        "mismatched_ops": [i for i, ok in enumerate(same) if not ok][:8],
     }
 
 
@@ -373,28 +380,11 @@ def measured_traffic():
     return None
 
 
-def copy_peak(ev, stream, nbytes=2 << 30, reps=5):
-    """A device-to-device copy of 2 GiB (eight times the 256 MiB last-level cache) on the
-    library's stream, timed with HIP events: the measured HBM rate (bytes read + written)."""
-    h = ev.h
-    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-    h.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
-    h.hipFree.argtypes = [C.c_void_p]
-    h.hipStreamSynchronize.argtypes = [C.c_void_p]
-    a, b = C.c_void_p(), C.c_void_p()
-    assert h.hipMalloc(C.byref(a), nbytes) == 0 and h.hipMalloc(C.byref(b), nbytes) == 0
-    try:
-        assert h.hipMemcpyAsync(b, a, nbytes, 3, stream) == 0  # warm (hipMemcpyDeviceToDevice)
-        e0, e1 = ev.event(), ev.event()
-        ev.record(e0, stream)
-        for _ in range(reps):
-            assert h.hipMemcpyAsync(b, a, nbytes, 3, stream) == 0
-        ev.record(e1, stream)
-        ms = ev.elapsed_ms(e0, e1) / reps
-    finally:
-        assert h.hipStreamSynchronize(stream) == 0
-        h.hipFree(a)
-        h.hipFree(b)
+def copy_peak(ctx, nbytes=2 << 30, reps=5):
+    """A streaming copy kernel over 2 GiB (eight times the 256 MiB last-level cache; 16-byte loads
+    and stores, grid-stride) on the library's stream, timed with HIP events: the HBM rate a
+    kernel reaches (bytes read + written)."""
+    ms = ctx.time_copy(nbytes, reps)
     return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
@@ -567,7 +557,7 @@ def main():
     achieved = ntt_bytes / (ntt_ms * 1e-3) / 1e9
     del tpolys
     ctx.synchronize()
-    copy_gbs = copy_peak(ev, stream)
+    copy_gbs = copy_peak(ctx)
     # the same launch over data primes only (240 x 50-bit limbs, the f64 path)
     qpolys = [ctx.uniform(23, seed=SEED + 19 + i, ntt=False) for i in range(10)]
     ntt_q_ms = ctx.time_ntt(qpolys, forward=True, reps=20)
@@ -582,7 +572,11 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(q, p, levels)
 
-    mean_bytes = float(np.mean([ks_bytes(lv) for lv in levels]))
+    # algorithmic bytes of one batched step: every input read and both outputs written once, the
+    # shared key's limbs the batch uses read once per launch (its top level's digits)
+    top_l = max(levels) + 1
+    d_top = -(-top_l // 6)
+    step_bytes = 8 * N * (sum(3 * (lv + 1) for lv in levels) + 2 * d_top * (top_l + 6))
     line = {
         "metric": "bit-exact key switches/sec at N=2^16 (batched vs plain) + p50/p99 op latency",
         "value": batched,
@@ -608,7 +602,9 @@ def main():
         "latency_ms_24_limbs": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                                 "mean": float(lat.mean())},
         "bit_exact": exact,
-        "effective_ks_GBps": batched * mean_bytes / 1e9,
+        "effective_ks_GBps": step_bytes / (ms_step * 1e-3) / 1e9,
+        "step_bytes_alg": step_bytes,
+        "batched_over_plain_target": 5.0,
         "roofline": {
             "kernel": "forward NTT (column pass + row pass), one launch over 240 limbs at N=2^16 "
                       "(192 x 50-bit, 48 x 60-bit)",

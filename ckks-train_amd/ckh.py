@@ -66,7 +66,7 @@ class OpCounter(C.Structure):
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("hbm_budget_bytes", u64), ("max_batch", u32)]
+    _fields_ = [("hbm_budget_bytes", u64), ("max_batch", u32), ("max_inflight", u32)]
 
 
 class Request(C.Structure):
@@ -756,8 +756,8 @@ class Evaluator:
 class Server:
     """The serving loop (include/ckh.h, "serving")."""
 
-    def __init__(self, hbm_budget_bytes, max_batch=64):
-        cfg = ServerConfig(hbm_budget_bytes, max_batch)
+    def __init__(self, hbm_budget_bytes, max_batch=64, max_inflight=0):
+        cfg = ServerConfig(hbm_budget_bytes, max_batch, max_inflight)
         h = VP()
         _check(lib().ckh_server_create(C.byref(cfg), C.byref(h)))
         self.h = h

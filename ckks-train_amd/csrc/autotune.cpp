@@ -63,6 +63,8 @@ void Context::set_tuning(const std::string& key, int64_t value) {
         ntt_fused = value != 0;
     else if (key == "fused_lag")
         fused.lag = (u32)std::min<int64_t>(std::max<int64_t>(value, 1), 64);
+    else if (key == "hoisted_batch")
+        hoisted_batch = (u32)std::min<int64_t>(std::max<int64_t>(value, 1), 64);
     else if (key == "fused_wg")  // workgroups per CU of the one-launch transforms
         fused.grid = cu_count * (u32)std::min<int64_t>(std::max<int64_t>(value, 1), 8);
     else if (key == "service_streams")
@@ -88,6 +90,7 @@ int64_t Context::get_tuning(const std::string& key) const {
     if (key == "service_streams") return service_streams;
     if (key == "ntt_fused") return fused_usable() ? 1 : 0;
     if (key == "fused_lag") return fused.lag;
+    if (key == "hoisted_batch") return hoisted_batch;
     if (key == "fused_wg") return cu_count ? fused.grid / cu_count : 0;
     if (key == "xcds") return fused.nx;
     if (key == "fail_alloc_after") return fail_alloc_after.load();

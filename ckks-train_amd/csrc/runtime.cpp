@@ -770,9 +770,9 @@ void Program::step(StepKind k, size_t off, u32 count, size_t off2, u32 a, u32 b)
     if (count) steps.push_back({k, count, off, off2, a, b});
 }
 
-void Program::step_ntt(StepKind k, size_t off, u32 count, size_t off2) {
+void Program::step_ntt(StepKind k, size_t off, u32 count, size_t off2, bool one_launch) {
     if (!count) return;
-    const size_t ctl = add(std::vector<u32>(fused_ctl_words(count), 0u));
+    const size_t ctl = one_launch ? add(std::vector<u32>(fused_ctl_words(count), 0u)) : 0;
     steps.push_back({k, count, off, off2, 0, 0});
     steps.back().ctl = ctl;
 }
@@ -941,7 +941,7 @@ void Context::ntt_poly(const u64* in, u64* out, u32 limbs, u32 level, u32 p_limb
     }
     (void)p_limbs;
     Program p;
-    p.step_ntt(forward ? S_NTT_FWD : S_NTT_INV, p.add(refs), (u32)refs.size());
+    p.step_ntt(forward ? S_NTT_FWD : S_NTT_INV, p.add(refs), (u32)refs.size(), 0, fused_usable());
     execute(p, s);
 }
 
@@ -984,10 +984,10 @@ void Context::prog_decompose(Program& p, const std::vector<DecTask>& tasks) cons
         max_tgt = std::max(max_tgt, modup_maxtgt_[t.level]);
     }
     group_by_prime(nttr);
-    p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size());
+    p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size(), 0, fused_usable());
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_tgt, alpha);
     p.step(S_COPY, p.add(copies), (u32)copies.size());
-    p.step_ntt(S_NTT_FWD, p.add(nttr), (u32)nttr.size());
+    p.step_ntt(S_NTT_FWD, p.add(nttr), (u32)nttr.size(), 0, fused_usable());
     p.step(S_NTT_COLS, p.add(cols), (u32)cols.size());
 }
 
@@ -1054,10 +1054,10 @@ void Context::prog_mod_down(Program& p, const std::vector<MdTask>& tasks) const 
         }
     }
     group_by_prime(intt);
-    p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size());
+    p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size(), 0, fused_usable());
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_l, np);
     const size_t o_ntt = p.add(nttr), o_fin = p.add(fin);
-    p.step_ntt(S_NTT_FWD_FIN, o_ntt, (u32)nttr.size(), o_fin);
+    p.step_ntt(S_NTT_FWD_FIN, o_ntt, (u32)nttr.size(), o_fin, fused_usable());
 }
 
 // RnsPoly::rescale_by_last in NTT form (baseconv.rs:258-291) for a batch.
@@ -1076,10 +1076,10 @@ void Context::prog_rescale(Program& p, const std::vector<RescaleTask>& tasks) co
         }
         max_t = std::max(max_t, l);
     }
-    p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size());
+    p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size(), 0, fused_usable());
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_t, 1);
     const size_t o_ntt = p.add(nttr), o_fin = p.add(fin);
-    p.step_ntt(S_NTT_FWD_FIN, o_ntt, (u32)nttr.size(), o_fin);
+    p.step_ntt(S_NTT_FWD_FIN, o_ntt, (u32)nttr.size(), o_fin, fused_usable());
 }
 
 // RnsPoly::raise_from_level0 (crates/ckks-train-core/src/poly.rs:298-345; the device form of
@@ -1104,9 +1104,9 @@ void Context::prog_raise(Program& p, const std::vector<RaiseTask>& tasks) const 
         max_t = std::max(max_t, t.top);
     }
     p.step(S_COPY, p.add(copy), (u32)copy.size());
-    p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size());
+    p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size(), 0, fused_usable());
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_t, 1);
-    p.step_ntt(S_NTT_FWD, p.add(nttr), (u32)nttr.size());
+    p.step_ntt(S_NTT_FWD, p.add(nttr), (u32)nttr.size(), 0, fused_usable());
 }
 
 void Context::raise_level0(const std::vector<RaiseTask>& tasks, hipStream_t s) {

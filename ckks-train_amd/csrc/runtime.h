@@ -177,10 +177,10 @@ struct Program {
         return off;
     }
     void step(StepKind k, size_t off, u32 count, size_t off2 = 0, u32 a = 0, u32 b = 0);
-    // An NTT step (S_NTT_FWD, S_NTT_INV, or S_NTT_FWD_FIN with its finish jobs at off2) with a
-    // zeroed control block in the descriptor bytes, for the one-launch transforms: every execution
-    // uploads the bytes again, so each launch finds its block zero.
-    void step_ntt(StepKind k, size_t off, u32 count, size_t off2 = 0);
+    // An NTT step (S_NTT_FWD, S_NTT_INV, or S_NTT_FWD_FIN with its finish jobs at off2); with
+    // one_launch, a zeroed control block in the descriptor bytes for the one-launch transforms
+    // (every execution uploads the bytes again, so each launch finds its block zero).
+    void step_ntt(StepKind k, size_t off, u32 count, size_t off2, bool one_launch);
     // Every base conversion of the program with this kernel variant and targets per workgroup.
     void set_conv_schedule(u32 variant, u32 tg) {
         for (Step& st : steps)
@@ -295,10 +295,15 @@ class Context : public std::enable_shared_from_this<Context> {
     // "graphs" (1 = on) and "graph_cache" (plans kept per context, least recently used evicted).
     u32 use_graphs = 1, graph_cache_max = 64;
     // The one-launch NTTs (tuning key "ntt_fused", 1 = on where the device's XCD layout was
-    // verified at creation; "fused_lag": slots between a limb's first- and second-pass tiles).
+    // verified at creation, off by default: measured no faster than the two launches on the key
+    // switch, slower on small launches; "fused_lag": slots between a limb's first- and
+    // second-pass tiles; "fused_wg": workgroups per CU).
     FusedCfg fused{};
-    u32 ntt_fused = 1;
+    u32 ntt_fused = 0;
     u32 cu_count = 0;
+    // Hoisted-rotation requests the serving loop switches in one program (tuning key
+    // "hoisted_batch"; 1 = one request per program).
+    u32 hoisted_batch = 4;
     bool fused_usable() const { return ntt_fused && fused.nx && ntt_fast_supported(log_n); }
     u32* d_err = nullptr;  // device error word (a one-launch transform's wait overran its bound)
     // Raises a Device error (and clears the word) if a kernel has set the device error word.

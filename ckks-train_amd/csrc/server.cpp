@@ -8,6 +8,7 @@ namespace ckh {
 
 Server::Server(const ckh_server_config& cfg) : cfg_(cfg), t0_(std::chrono::steady_clock::now()) {
     if (cfg_.max_batch == 0) cfg_.max_batch = 64;
+    if (cfg_.max_inflight == 0) cfg_.max_inflight = 4;
     th_ = std::thread([this] { loop(); });
 }
 
@@ -95,18 +96,22 @@ void Server::launch(std::vector<OpPtr>& ops) {
         }
         // Hoisted rotations of every request in one batch (one decomposition launch sequence and
         // one switch program for all their steps); a failing member falls back to one call each.
-        if (!hoisted.empty()) {
+        // Groups of at most c.hoisted_batch requests (tuning key "hoisted_batch"), so one
+        // program's scratch (about 70 MB per step at N = 2^16 and 24 limbs) stays modest.
+        const size_t per = std::max<u32>(c.hoisted_batch, 1);
+        for (size_t g0 = 0; g0 < hoisted.size(); g0 += per) {
+            const size_t g1 = std::min(hoisted.size(), g0 + per);
             std::vector<std::pair<const Ct*, std::vector<int32_t>>> items;
-            for (auto& op : hoisted) items.push_back({op->a.get(), op->steps});
+            for (size_t i = g0; i < g1; ++i) items.push_back({hoisted[i]->a.get(), hoisted[i]->steps});
             try {
                 auto outs = ev->hoisted_rotate_batch(items);
-                for (size_t i = 0; i < hoisted.size(); ++i) hoisted[i]->outs = std::move(outs[i]);
+                for (size_t i = g0; i < g1; ++i) hoisted[i]->outs = std::move(outs[i - g0]);
             } catch (const Error&) {
-                for (auto& op : hoisted) {
+                for (size_t i = g0; i < g1; ++i) {
                     try {
-                        op->outs = ev->hoisted_rotate(*op->a, op->steps);
+                        hoisted[i]->outs = ev->hoisted_rotate(*hoisted[i]->a, hoisted[i]->steps);
                     } catch (const Error& e) {
-                        fail_op(op, e.status, e.msg);
+                        fail_op(hoisted[i], e.status, e.msg);
                     }
                 }
             }
@@ -207,8 +212,10 @@ void Server::loop() {
                 }
             }
             if (stop_ && pending_.empty() && inflight_.empty()) return;
-            // admission in arrival order within the budget
-            while (!pending_.empty() && admitted.size() < cfg_.max_batch) {
+            // Admission in arrival order within the budget, while fewer than max_inflight batches
+            // run: beyond that the device queue only grows (memory held, latency added, nothing
+            // finished sooner), so arrivals wait here and join the next, larger batch.
+            while (inflight_.size() < cfg_.max_inflight && !pending_.empty() && admitted.size() < cfg_.max_batch) {
                 OpPtr op = pending_.front();
                 if (op->t_arrival > now()) break;  // not arrived yet (pre-scheduled arrivals)
                 u64 fp = 0;

@@ -392,6 +392,8 @@ typedef enum ckh_op_kind {
 typedef struct ckh_server_config {
     uint64_t hbm_budget_bytes; /* bound on the device bytes of admitted, unfinished requests */
     uint32_t max_batch;        /* requests admitted per scheduler turn */
+    uint32_t max_inflight;     /* batches launched and unfinished at once (0: 4); while that many
+                                * run, arrivals wait unadmitted (no device memory held) */
 } ckh_server_config;
 
 typedef struct ckh_request {
