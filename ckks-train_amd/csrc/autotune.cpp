@@ -53,11 +53,11 @@ void Context::set_tuning(const std::string& key, int64_t value) {
     else if (key == "graph_cache")
         graph_cache_max = (u32)std::max<int64_t>(value, 1);
     else if (key == "pool") {
-        // The device's default stream-ordered pool (mode 0) gave wrong results in the served
-        // stream with batches on several streams at once, for a cause not yet shown; only the
-        // context's block cache may be selected.
-        if (value != 1) fail(CKH_ERR_INVALID_PARAMS, "pool mode 0 (the device's default pool) is disabled");
-        pool_mode = 1;
+        // 1: the context's block cache; 0: the device's default stream-ordered pool (its frees are
+        // ordered after every stream that used the buffer, as the cache's are).
+        if (value > 1) fail(CKH_ERR_INVALID_PARAMS, "pool mode is 0 or 1");
+        synchronize_all();
+        pool_mode = (u32)value;
     }
     else if (key == "ntt_fused")
         ntt_fused = value != 0;

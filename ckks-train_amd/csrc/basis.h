@@ -51,15 +51,6 @@ struct ConvJob {
     u32 plan, pad;
 };
 
-// out <- (a - t) * inv mod q (+ add when add != null), one limb: finishes rescale and mod_down.
-struct FinishJob {
-    const u64* a;
-    const u64* t;
-    u64* out;
-    const u64* add;
-    u64 inv, inv_s;
-    u32 prime, pad;
-};
 
 // Matrix-core (int8 MFMA) form of a conversion plan. Sum_j y_j c_jt is computed as
 // sum_p 2^(8p) sum_(j,b) y_j[b] w_p(2^(8b) c_jt mod r_t): y_j[b] are the bytes of y_j (offset by

@@ -1,10 +1,6 @@
-// Shared definitions of the gfx950 RNS-CKKS library: device-side types, limb descriptors,
-// per-prime constants and the modular arithmetic used by every kernel.
-//
-// Residues are u64 in HBM, limb-major: limb i of a polynomial is N contiguous words. All
-// arithmetic is exact modular integer arithmetic, so results are unique residues whatever the
-// evaluation order; the one floating-point step (the overflow estimate of the exact base
-// conversion) lives in basis.hip and is compiled without FMA contraction.
+// Device prelude of the gfx950 RNS-CKKS library: the HIP runtime, the device forms of the few
+// primitives the portable arithmetic (arith.h: descriptors, per-prime constants, modular
+// arithmetic) leaves to its includer, and device-only helpers (address-space casts, XCD tiling).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,100 +10,18 @@ typedef uint64_t u64;
 typedef uint32_t u32;
 
 namespace ckh {
-
-// One limb of work: where to read, where to write, and which prime (index into the context's
-// prime list: Q primes first, then P primes).
-struct LimbRef {
-    const u64* src;
-    u64* dst;
-    u32 prime;
-    u32 aux;
-};
-
-// A twiddle and its Shoup companion, loaded as one 16-byte word.
-struct Tw {
-    u64 w, ws;
-};
-
-// A twiddle for the f64 path (primes below 2^50): w exactly, and w / q rounded.
-struct FTw {
-    double w, wq;
-};
-
-// Primes below this bound use the f64 arithmetic (every intermediate stays an exact integer
-// below 2^53); larger primes use 64-bit integer Shoup arithmetic.
-constexpr u64 FP_PRIME_BOUND = 1ull << 50;
-
-// Per-prime constants (device).
-struct PrimeConst {
-    u64 q;
-    u64 n_inv, n_inv_s;     // N^-1 and its Shoup companion
-    u64 w_n_inv, w_n_inv_s; // psi^-1 * N^-1 (fused last inverse stage)
-    u64 one_s;              // floor(2^64 / q): Shoup companion of 1
-    u64 two64, two64_s;     // 2^64 mod q and companion
-    double qf, qinv;        // q and 1/q as f64 (f64 path)
-    FTw n_inv_f, w_n_inv_f; // the last inverse stage constants, f64 path
-};
-
 #define CKH_DEV __device__ __forceinline__
-
 // High word of a 64x64 product (the compiler builds it from 32-bit multiplies on gfx950).
 CKH_DEV u64 mulhi(u64 a, u64 b) { return __umul64hi(a, b); }
+// The bits of a u64 as an f64 and back (signed pass intermediates are kept as raw f64 bits).
+CKH_DEV double bits_f64(u64 x) { return __longlong_as_double((long long)x); }
+CKH_DEV u64 f64_bits(double d) { return (u64)__double_as_longlong(d); }
+}  // namespace ckh
 
-// Shoup product in [0, 2q) for any u64 a (w < q fixed, ws = floor(w 2^64 / q)).
-CKH_DEV u64 shoup_lazy(u64 a, u64 w, u64 ws, u64 q) { return a * w - mulhi(a, ws) * q; }
+#include "arith.h"
 
-CKH_DEV u64 csub(u64 a, u64 q) { return a >= q ? a - q : a; }
+namespace ckh {
 
-CKH_DEV u64 shoup(u64 a, u64 w, u64 ws, u64 q) { return csub(shoup_lazy(a, w, ws, q), q); }
-
-CKH_DEV u64 add_mod(u64 a, u64 b, u64 q) { return csub(a + b, q); }
-
-CKH_DEV u64 sub_mod(u64 a, u64 b, u64 q) { return a >= b ? a - b : a + q - b; }
-
-// Any u64 reduced mod q.
-CKH_DEV u64 reduce64(u64 a, const PrimeConst& c) { return csub(a - mulhi(a, c.one_s) * c.q, c.q); }
-
-// (hi 2^64 + lo) mod q for any hi, lo.
-CKH_DEV u64 reduce128(u64 hi, u64 lo, const PrimeConst& c) {
-    return add_mod(shoup(hi, c.two64, c.two64_s, c.q), reduce64(lo, c), c.q);
-}
-
-// a * b mod q for reduced a, b.
-CKH_DEV u64 mul_mod(u64 a, u64 b, const PrimeConst& c) { return reduce128(mulhi(a, b), a * b, c); }
-
-// ---- f64 arithmetic for primes below 2^50 (FP_PRIME_BOUND): residues as integer-valued f64.
-
-// u64 < 2^53 -> f64, exact.
-CKH_DEV double u2d(u64 v) { return __builtin_fma((double)(u32)(v >> 32), 4294967296.0, (double)(u32)v); }
-
-// Non-negative integer-valued f64 below 2^53 -> u64, exact.
-CKH_DEV u64 d2u(double d) {
-    const u32 hi = (u32)(d * (1.0 / 4294967296.0));
-    const double lo = __builtin_fma(-(double)hi, 4294967296.0, d);
-    return ((u64)hi << 32) | (u32)lo;
-}
-
-// y w mod q as a signed integer-valued f64, for integer-valued y, w with |w| < q < 2^50 and
-// wq = w / q rounded once or twice: h = RN(y w) and l = fma(y, w, -h) split the product
-// exactly; k = rint(y wq) misses y w / q by at most 1/2 + 3 |y| / (16 q) (the rounding errors
-// of wq and of y wq), so t = y w - k q has |t| <= (1/2 + 3 |y| / (16 q)) q, e.g. 0.69 q for
-// |y| <= q; while that stays far below 2^53, fma(-k, q, h) + l computes t exactly.
-CKH_DEV double fmulmod(double y, double w, double wq, double q) {
-    const double h = y * w;
-    const double l = __builtin_fma(y, w, -h);
-    const double k = __builtin_rint(y * wq);
-    return __builtin_fma(-k, q, h) + l;
-}
-
-// x - rint(x / q) q: |result| <= q / 2 (plus rounding slack) for |x| < 2^53.
-CKH_DEV double freduce(double x, double q, double qinv) { return __builtin_fma(-__builtin_rint(x * qinv), q, x); }
-
-// The canonical residue in [0, q) as f64.
-CKH_DEV double fcanon(double x, double q, double qinv) {
-    const double r = freduce(x, q, qinv);
-    return r < 0.0 ? r + q : r;
-}
 
 // A pointer to memory that no thread writes during the kernel (tables, plans), as a constant
 // address-space pointer: loads through it may use the scalar cache even after the kernel's own
@@ -135,30 +49,6 @@ template <class T>
 CKH_DEV __attribute__((address_space(1))) T* glb(T* p) {
     return (__attribute__((address_space(1))) T*)p;
 }
-
-// Exact sum of products of values below 2^60 (every residue here), from 32-bit halves:
-// y c = y0 c0 + (y0 c1 + y1 c0) 2^32 + y1 c1 2^64 with y1, c1 < 2^28, one 32 x 32 -> 64 multiply
-// per partial product instead of full 64 x 64 products; the low and middle sums carry into
-// counters, the high one (below 2^56 per term) cannot overflow for fewer than 256 terms.
-struct Dot60 {
-    u64 l = 0, m = 0, h = 0;
-    u32 cl = 0, cm = 0;
-    CKH_DEV void add(u64 y, u64 c) {
-        const u32 y0 = (u32)y, y1 = (u32)(y >> 32), c0 = (u32)c, c1 = (u32)(c >> 32);
-        const u64 p = (u64)y0 * c0;
-        l += p;
-        cl += l < p ? 1u : 0u;
-        const u64 t = (u64)y0 * c1 + (u64)y1 * c0;
-        m += t;
-        cm += m < t ? 1u : 0u;
-        h += (u64)y1 * c1;
-    }
-    // The sum as hi 2^64 + lo.
-    CKH_DEV void get(u64& hi, u64& lo) const {
-        lo = l + (m << 32);
-        hi = h + cl + (m >> 32) + ((u64)cm << 32) + (lo < l ? 1 : 0);
-    }
-};
 
 // blockIdx.x remapped so that consecutive logical tiles of one launch land on one XCD
 // (workgroups are dealt round-robin over the 8 XCDs, each with its own L2).

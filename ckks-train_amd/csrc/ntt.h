@@ -20,7 +20,6 @@ struct NttTables {
     const FTw* ri;
 };
 
-constexpr u32 ROW_FACTOR_SLOTS = 8;
 
 // Register-radix passes (ntt_fast.hip) for 2^12 <= N <= 2^16.
 bool ntt_fast_supported(u32 log_n);

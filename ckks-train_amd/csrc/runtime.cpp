@@ -8,6 +8,7 @@
 #include <unordered_map>
 
 #include "hostmath.h"
+#include "ntt_tables.h"
 
 namespace ckh {
 
@@ -420,58 +421,17 @@ u32 Context::ndigits(u32 level) const { return (u32)digit_ranges(level).size(); 
 // Device tables: per-prime constants and twiddles (NttTable::new, ntt.rs:89-122), and the
 // conversion plans of ModUp (per level and digit), ModDown (per level) and rescale (per level).
 void Context::build_tables() {
-    const u32 np_all = nq + np;
-    std::vector<PrimeConst> pc(np_all);
-    std::vector<Tw> fwd((size_t)np_all * n), inv((size_t)np_all * n);
-    std::vector<FTw> fwd_f((size_t)np_all * n), inv_f((size_t)np_all * n);
-    for (u32 k = 0; k < np_all; ++k) {
-        const u64 q = primes[k];
-        const u64 psi = hm::min_root(n, q), psi_inv = hm::inv(psi, q), n_inv = hm::inv(n % q, q);
-        u64 p1 = 1, p2 = 1;
-        for (u64 i = 0; i < n; ++i) {
-            const u32 j = hm::bitrev((u32)i, log_n);
-            fwd[(size_t)k * n + j] = {p1, hm::shoup(p1, q)};
-            inv[(size_t)k * n + j] = {p2, hm::shoup(p2, q)};
-            fwd_f[(size_t)k * n + j] = {(double)p1, (double)p1 / (double)q};
-            inv_f[(size_t)k * n + j] = {(double)p2, (double)p2 / (double)q};
-            p1 = hm::mul(p1, psi, q);
-            p2 = hm::mul(p2, psi_inv, q);
-        }
-        const u64 iroot1 = n >= 2 ? inv[(size_t)k * n + 1].w : 1;
-        PrimeConst& c = pc[k];
-        c.q = q;
-        c.n_inv = n_inv;
-        c.n_inv_s = hm::shoup(n_inv, q);
-        c.w_n_inv = hm::mul(iroot1, n_inv, q);
-        c.w_n_inv_s = hm::shoup(c.w_n_inv, q);
-        c.one_s = hm::shoup(1, q);
-        c.two64 = (u64)(((hm::u128)1 << 64) % q);
-        c.two64_s = hm::shoup(c.two64, q);
-        c.qf = (double)q;
-        c.qinv = 1.0 / (double)q;
-        c.n_inv_f = {(double)n_inv, (double)n_inv / (double)q};
-        c.w_n_inv_f = {(double)c.w_n_inv, (double)c.w_n_inv / (double)q};
-    }
-    d_pc = dev_copy(pc);
+    HostNttTables t = build_ntt_tables(primes, log_n, ntt_fast_supported(log_n));
+    d_pc = dev_copy(t.pc);
     ntt.log_n = log_n;
     ntt.pc = d_pc;
-    ntt.fwd = dev_copy(fwd);
-    ntt.inv = dev_copy(inv);
-    ntt.fwd_f = dev_copy(fwd_f);
-    ntt.inv_f = dev_copy(inv_f);
-    if (ntt_fast_supported(log_n)) {
-        const u32 M = log_n / 2, A = log_n - M, rows = 1u << A;
-        std::vector<FTw> rf((size_t)np_all * rows * ROW_FACTOR_SLOTS), ri(rf.size());
-        for (u32 k = 0; k < np_all; ++k)
-            for (u32 row = 0; row < rows; ++row)
-                for (u32 s = 0; s < M; ++s) {
-                    const size_t at = ((size_t)k * rows + row) * ROW_FACTOR_SLOTS + s;
-                    const size_t src = (size_t)k * n + (1u << (A + s)) + ((size_t)row << s);
-                    rf[at] = fwd_f[src];
-                    ri[at] = inv_f[src];
-                }
-        ntt.rf = dev_copy(rf);
-        ntt.ri = dev_copy(ri);
+    ntt.fwd = dev_copy(t.fwd);
+    ntt.inv = dev_copy(t.inv);
+    ntt.fwd_f = dev_copy(t.fwd_f);
+    ntt.inv_f = dev_copy(t.inv_f);
+    if (!t.rf.empty()) {
+        ntt.rf = dev_copy(t.rf);
+        ntt.ri = dev_copy(t.ri);
     }
 
     std::vector<ConvSrc> srcs;

@@ -1,0 +1,206 @@
+// CPU replay of the register-radix NTT kernels (ckks-train_amd/csrc/ntt_passes.h, the code the
+// gfx950 kernels run), one simulated workgroup of 256 threads at a time: the tile passes of a
+// forward and an inverse transform of whole limbs, at N = 2^12 .. 2^16 (M = 6, 7, 8 for both
+// passes), for 50-bit primes (the f64 arithmetic) and 60-bit primes (the integer arithmetic).
+//
+// Checked (any failure aborts): every pass reads and writes every word of the limb exactly once;
+// every LDS exchange writes and reads each slot once per phase, after it was written; every f64
+// modular product and reduction meets its preconditions and equals the integer result; every
+// Shoup product is below 2q and congruent; the words between the passes stay within the bounds
+// the arithmetic relies on (f64 forward |x| < 3.7 q, inverse |x| <= q; integer forward < 4q,
+// inverse < 2q); the outputs equal the CPU oracle's transform (ckks_oracle.c, linked).
+// Test infrastructure only (built and run by tests/test_emu.py).
+#include <algorithm>
+#include <random>
+#include <string>
+#include <thread>
+
+#include "emu_prelude.h"
+#include "ntt_passes.h"
+#include "ntt_tables.h"
+
+extern "C" {
+int or_ntt_forward_limbs(u64 n, const u64* q, u64 limbs, u64* a);
+int or_ntt_inverse_limbs(u64 n, const u64* q, u64 limbs, u64* a);
+int or_ntt_primes_skip(unsigned bits, u64 n, u64 skip, u64 count, u64* out);
+}
+
+using namespace ckh;
+using namespace ckh::fast;
+
+namespace {
+
+// One workgroup: 256 threads running f, a barrier between them whose completion step checks the
+// LDS phase that ends; the last phase is checked after the threads have finished.
+template <class F>
+void run_group(emu::Lds& lds, F f) {
+    auto done = [&]() noexcept { lds.end_phase(); };
+    std::barrier bar(256, done);
+    emu::Group g{&lds, [&bar] { bar.arrive_and_wait(); }};
+    emu::current() = &g;
+    std::vector<std::thread> th;
+    for (u32 t = 0; t < NT; ++t)
+        th.emplace_back([&f, t] {
+            threadIdx.x = t;
+            f();
+        });
+    for (auto& x : th) x.join();
+    lds.end_phase();
+    emu::current() = nullptr;
+}
+
+void expect_once(emu::Tracked& t, bool reads, bool writes, const char* pass) {
+    for (size_t i = 0; i < t.len; ++i) {
+        const u32 r = t.reads[i], w = t.writes[i];
+        if ((reads && r != 1) || (!reads && r != 0) || (writes && w != 1) || (!writes && w != 0)) {
+            std::fprintf(stderr, "%s: %s word %zu read %u times, written %u times\n", pass, t.name, i, r, w);
+            emu::die("a pass does not touch every word exactly once");
+        }
+    }
+}
+
+struct Limb {
+    u32 log_n;
+    u64 q;
+    HostNttTables tab;
+};
+
+// Forward (or inverse) transform of one limb through the two passes, every tile replayed.
+void transform(const Limb& L, const std::vector<u64>& in, std::vector<u64>& out, bool forward, double* mid_max) {
+    const u32 log_n = L.log_n, B = log_n / 2, A = log_n - B, tiles = 1u << (log_n - TL);
+    const u64 n = 1ull << log_n;
+    std::vector<u64> src(in), dst(n, 0xdeadbeefdeadbeefull);
+    emu::Tracked ts(src.data(), n, "src"), td(dst.data(), n, "dst");
+    emu::registry() = {&ts, &td};
+    emu::Lds lds(LDS_WORDS);
+    const PrimeConst& pc = L.tab.pc[0];
+    const bool fp = L.q < FP_PRIME_BOUND;
+    const FpAr far{(forward ? L.tab.fwd_f : L.tab.inv_f).data(), pc.qf, pc.qinv, pc.n_inv_f, pc.w_n_inv_f,
+                   (forward ? L.tab.rf : L.tab.ri).data()};
+    const IntAr iar{(forward ? L.tab.fwd : L.tab.inv).data(), pc.q, pc.n_inv, pc.n_inv_s, pc.w_n_inv, pc.w_n_inv_s};
+    auto pass = [&](u32 m, auto&& body, const char* name, bool rs, bool ws, bool rd, bool wd) {
+        ts.reset();
+        td.reset();
+        for (u32 tile = 0; tile < tiles; ++tile) run_group(lds, [&] { body(m, tile); });
+        expect_once(ts, rs, ws, name);
+        expect_once(td, rd, wd, name);
+    };
+    emu::LdsView lv{&lds};
+    auto dispatch = [&](u32 m, auto&& f) {
+        switch (m) {
+            case 6: f(std::integral_constant<u32, 6>{}); break;
+            case 7: f(std::integral_constant<u32, 7>{}); break;
+            default: f(std::integral_constant<u32, 8>{}); break;
+        }
+    };
+    if (forward) {
+        pass(A, [&](u32 m, u32 tile) {
+            dispatch(m, [&](auto M) {
+                if (fp) fwd_cols<decltype(M)::value>(far, src.data(), dst.data(), lv, log_n, tile);
+                else fwd_cols<decltype(M)::value>(iar, src.data(), dst.data(), lv, log_n, tile);
+            });
+        }, "forward column pass", true, false, false, true);
+    } else {
+        pass(B, [&](u32 m, u32 tile) {
+            dispatch(m, [&](auto M) {
+                if (fp) inv_rows<decltype(M)::value>(far, src.data(), dst.data(), lv, log_n, tile);
+                else inv_rows<decltype(M)::value>(iar, src.data(), dst.data(), lv, log_n, tile);
+            });
+        }, "inverse row pass", true, false, false, true);
+    }
+    // the words between the passes
+    double mx = 0;
+    for (u64 i = 0; i < n; ++i) {
+        const double v = fp ? std::fabs(bits_f64(dst[i])) : (double)dst[i];
+        if (fp && !emu::is_int(v)) emu::die("pass intermediate not an integer", v);
+        mx = std::max(mx, v / (double)L.q);
+    }
+    const double bound = fp ? (forward ? 3.7 : 1.0) : (forward ? 4.0 : 2.0);
+    if (fp ? mx > bound : mx >= bound) emu::die("pass intermediate above its bound", mx, bound);
+    *mid_max = std::max(*mid_max, mx);
+    if (forward) {
+        pass(B, [&](u32 m, u32 tile) {
+            dispatch(m, [&](auto M) {
+                if (fp) fwd_rows<decltype(M)::value>(far, dst.data(), dst.data(), lv, log_n, nullptr, L.q, tile);
+                else fwd_rows<decltype(M)::value>(iar, dst.data(), dst.data(), lv, log_n, nullptr, L.q, tile);
+            });
+        }, "forward row pass", false, false, true, true);
+    } else {
+        pass(A, [&](u32 m, u32 tile) {
+            dispatch(m, [&](auto M) {
+                if (fp) inv_cols<decltype(M)::value>(far, dst.data(), lv, log_n, tile);
+                else inv_cols<decltype(M)::value>(iar, dst.data(), lv, log_n, tile);
+            });
+        }, "inverse column pass", false, false, true, true);
+    }
+    emu::registry().clear();
+    out = dst;
+}
+
+std::vector<u64> oracle(const Limb& L, std::vector<u64> a, bool forward) {
+    const u64 n = 1ull << L.log_n;
+    if ((forward ? or_ntt_forward_limbs : or_ntt_inverse_limbs)(n, &L.q, 1, a.data()) != 0)
+        emu::die("oracle transform failed");
+    return a;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    // argv[1]: random inputs per (N, prime); argv[2]: random +-(q-1)/2 sign patterns at N = 2^12;
+    // argv[3]: a planted mistake (0 none, 1 colliding LDS slots, 2 colliding global words, 3 one
+    // wrong row factor, 4 a row factor whose w and w/q disagree), run at N = 2^12, 50-bit primes
+    const int reps = argc > 1 ? std::atoi(argv[1]) : 1;
+    const int patterns = argc > 2 ? std::atoi(argv[2]) : 8;
+    const int plant = argc > 3 ? std::atoi(argv[3]) : 0;
+    if (plant == 1 || plant == 2) emu::mutation() = plant;
+    std::mt19937_64 rng(20261203);
+    for (u32 log_n = 12; log_n <= (plant ? 12u : 16u); ++log_n) {
+        const u64 n = 1ull << log_n;
+        for (unsigned bits : {50u, 60u}) {
+            if (plant && bits != 50) continue;
+            u64 q = 0;
+            if (or_ntt_primes_skip(bits, n, 0, 1, &q) != 0) emu::die("no prime");
+            Limb L{log_n, q, build_ntt_tables({q}, log_n, true)};
+            if (plant == 3) L.tab.rf[5 * ROW_FACTOR_SLOTS + 3] = L.tab.rf[6 * ROW_FACTOR_SLOTS + 3];
+            if (plant == 4) L.tab.rf[5 * ROW_FACTOR_SLOTS + 3].w = L.tab.rf[6 * ROW_FACTOR_SLOTS + 3].w;
+            std::vector<std::vector<u64>> inputs;
+            for (int r = 0; r < reps; ++r) {
+                std::vector<u64> a(n);
+                for (auto& x : a) x = rng() % q;
+                inputs.push_back(a);
+            }
+            inputs.push_back(std::vector<u64>(n, q - 1));
+            inputs.push_back(std::vector<u64>(n, 0));
+            {
+                std::vector<u64> a(n);
+                for (u64 i = 0; i < n; ++i) a[i] = (i & 1) ? (q - 1) / 2 : (q + 1) / 2;
+                inputs.push_back(a);
+            }
+            if (log_n == 12)
+                for (int p = 0; p < patterns; ++p) {
+                    std::vector<u64> a(n);
+                    for (auto& x : a) x = (rng() & 1) ? (q - 1) / 2 : (q + 1) / 2;
+                    inputs.push_back(a);
+                }
+            double fmax = 0, imax = 0;
+            for (const auto& a : inputs) {
+                std::vector<u64> f, b;
+                transform(L, a, f, true, &fmax);
+                if (f != oracle(L, a, true)) emu::die("forward output differs from the oracle", log_n, bits);
+                transform(L, f, b, false, &imax);
+                if (b != a) emu::die("inverse does not return the input", log_n, bits);
+            }
+            std::printf("N=2^%u q=%u-bit: %zu limbs forward and inverse exact; between passes max |x|/q "
+                        "forward %.3f inverse %.3f\n",
+                        log_n, bits, inputs.size(), fmax, imax);
+            std::fflush(stdout);
+        }
+    }
+    auto& s = emu::stats();
+    std::printf("f64 products %llu (max |y|/q %.3f, max |t|/q %.3f), reductions %llu (max |x|/q %.3f), "
+                "Shoup products %llu\nEMU OK\n",
+                (unsigned long long)s.fmul_calls, s.fmul_in, s.fmul_out, (unsigned long long)s.fred_calls, s.fred_in,
+                (unsigned long long)s.shoup_calls);
+    return 0;
+}

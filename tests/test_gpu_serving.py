@@ -66,11 +66,15 @@ def _alone(envs, req):
     return [ev.rescale(ev.relinearize(ev.mul_no_relin(a, b)))]
 
 
-@pytest.mark.parametrize("rate", [4000.0, 800.0])
-def test_served_mix_equals_alone_and_failure_isolated(rate):
+@pytest.mark.parametrize("rate,pool", [(4000.0, 1), (800.0, 1), (4000.0, 0), (800.0, 0)])
+def test_served_mix_equals_alone_and_failure_isolated(rate, pool):
     """At a high arrival rate (large batches) and a low one (many one-request batches spread
-    over the streams, repeated shapes replayed from captured graphs)."""
+    over the streams, repeated shapes replayed from captured graphs), with the context's block
+    cache and with the device's default stream-ordered pool (buffers freed while batches run on
+    the four service streams and the caller's)."""
     envs = _setup(7000)
+    for ctx, _, _ in envs:
+        ctx.set_tuning("pool", pool)
     reqs = _requests(envs, 40, 7, rate)
     srv = ckh.Server(hbm_budget_bytes=64 << 30, max_batch=16)
     t0 = srv.now() + 0.05
@@ -132,16 +136,18 @@ def test_caller_keeps_working_while_serving():
     srv.close()
 
 
-def test_pool_mode_zero_refused():
-    """The device's default pool (mode 0) is switched off: selecting it is an InvalidParams error
-    and the context keeps its block cache."""
+def test_pool_mode_values():
+    """Pool mode 1 (block cache) is the default; 0 (the device's pool) may be selected; other
+    values are refused."""
     envs = _setup(9500)
     ctx = envs[0][0]
     assert ctx.tuning("pool") == 1
     with pytest.raises(ckh.InvalidParams):
-        ctx.set_tuning("pool", 0)
-    assert ctx.tuning("pool") == 1
+        ctx.set_tuning("pool", 2)
+    ctx.set_tuning("pool", 0)
+    assert ctx.tuning("pool") == 0
     ctx.set_tuning("pool", 1)
+    assert ctx.tuning("pool") == 1
 
 
 FULL_SETS = [("c2", 16, [50] * 24, [60] * 6, 4), ("c4", 15, [50] * 20, [60] * 4, 5)]
