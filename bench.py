@@ -389,27 +389,17 @@ def copy_peak(ctx, nbytes=2 << 30, reps=5):
 
 
 def roofline_only(ctx, ev, reps=3):
-    """The roofline launch alone (forward NTT over 8 polynomials x 30 limbs), then a
-    device-to-device copy of the same byte count: the copy calibrates the HBM byte counters
-    (its algorithmic traffic is exactly bytes read + bytes written)."""
+    """The roofline launch alone (forward NTT over 8 polynomials x 30 limbs), then the streaming
+    copy kernel over the same byte count: the copy calibrates the HBM byte counters (its
+    algorithmic traffic is exactly bytes read + bytes written)."""
     tpolys = [ctx.uniform(23, seed=SEED + 9 + i, p_limbs=6, ntt=False) for i in range(8)]
     ctx.synchronize()
     ms = ctx.time_ntt(tpolys, forward=True, reps=reps)
     nbytes = 8 * N * 8 * 30
-    h = ev.h
-    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-    h.hipFree.argtypes = [C.c_void_p]
-    h.hipDeviceSynchronize.argtypes = []
-    a, b = C.c_void_p(), C.c_void_p()
-    assert h.hipMalloc(C.byref(a), nbytes) == 0 and h.hipMalloc(C.byref(b), nbytes) == 0
-    for _ in range(reps):
-        assert h.hipMemcpy(b, a, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
-    assert h.hipDeviceSynchronize() == 0
-    h.hipFree(a)
-    h.hipFree(b)
+    copy_ms = ctx.time_copy(nbytes, reps)
+    ctx.synchronize()
     print(json.dumps({"ntt_forward_limbs": 240, "ntt_ms_per_launch": ms, "ntt_bytes_alg": 16 * N * 240,
-                      "copy_bytes": nbytes, "copy_reps": reps}), flush=True)
+                      "copy_bytes": nbytes, "copy_reps": reps + 1, "copy_ms": copy_ms}), flush=True)
 
 
 def main():
@@ -567,6 +557,9 @@ def main():
 
     del polys, key, top
     ctx.synchronize()
+    graphs = dict(zip(("captured", "replays"), ctx.graph_stats()))
+    # the served stream runs on contexts of its own: this one's cached blocks go back to the driver
+    ctx.trim_pool(0)
     serve = None if args.no_serve else served_stream(ckh)
     print(f"# served stream: {json.dumps(serve)}", file=sys.stderr, flush=True)
 
@@ -616,7 +609,7 @@ def main():
         },
         "ntt_forward_q_primes_only": ntt_q,
         "tuning": tuning,
-        "graphs": dict(zip(("captured", "replays"), ctx.graph_stats())),
+        "graphs": graphs,
         "cpu_baseline": cpu,
         "served_stream": serve,
     }

@@ -289,14 +289,26 @@ BufPtr Context::alloc(size_t words, hipStream_t s) {
         e = hipSuccess;
         if (!p) {
             e = hipMalloc(&p, cls);
-            if (e != hipSuccess) {  // the cache back to the driver, then once more
+            if (e != hipSuccess) {  // cached blocks back to the driver, then once more
                 (void)hipGetLastError();
                 {
                     std::lock_guard<std::mutex> lk(plan_mu_);  // as in synchronize_all
                     for (auto st : streams_) (void)hipStreamSynchronize(st);
                 }
-                cache_release(0);
+                // enough for this block and as much again (largest classes first), so that a
+                // device near full does not empty and regrow the whole cache on every miss
+                u64 cached;
+                {
+                    std::lock_guard<std::mutex> lk(cache_mu_);
+                    cached = cache_bytes_;
+                }
+                cache_release(cached > 2 * cls ? cached - 2 * cls : 0);
                 e = hipMalloc(&p, cls);
+                if (e != hipSuccess) {
+                    (void)hipGetLastError();
+                    cache_release(0);
+                    e = hipMalloc(&p, cls);
+                }
             }
         }
         bytes = cls;
