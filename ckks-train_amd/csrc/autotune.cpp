@@ -91,6 +91,13 @@ int64_t Context::get_tuning(const std::string& key) const {
     if (key == "ntt_fused") return fused_usable() ? 1 : 0;
     if (key == "fused_lag") return fused.lag;
     if (key == "hoisted_batch") return hoisted_batch;
+    if (key == "stat_mallocs") return (int64_t)stat_mallocs.load();
+    if (key == "stat_malloc_us") return (int64_t)stat_malloc_us.load();
+    if (key == "stat_oom") return (int64_t)stat_oom.load();
+    if (key == "stat_cached_bytes") {
+        std::lock_guard<std::mutex> lk(const_cast<std::mutex&>(cache_mu_));
+        return (int64_t)cache_bytes_;
+    }
     if (key == "fused_wg") return cu_count ? fused.grid / cu_count : 0;
     if (key == "xcds") return fused.nx;
     if (key == "fail_alloc_after") return fail_alloc_after.load();

@@ -3,6 +3,7 @@
 #include "runtime.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <sstream>
 #include <unordered_map>
@@ -158,23 +159,28 @@ static size_t size_class(size_t b) {
 
 // A cached block of class cls for stream s: one last used on s (stream order makes it free for
 // the work queued after), else one whose event has completed; null if none.
-void* Context::cache_take(size_t cls, hipStream_t s) {
+void* Context::cache_take(size_t& cls, hipStream_t s) {
     std::lock_guard<std::mutex> lk(cache_mu_);
-    auto it = cache_.find(cls);
-    if (it == cache_.end() || it->second.empty()) return nullptr;
-    std::vector<Cached>& v = it->second;
-    size_t pick = v.size();
-    for (size_t i = 0; i < v.size() && pick == v.size(); ++i)
-        if (v[i].s == s) pick = i;
-    for (size_t i = 0; i < v.size() && pick == v.size(); ++i)
-        if (hipEventQuery(v[i].ready) == hipSuccess) pick = i;
-    if (pick == v.size()) return nullptr;
-    void* p = v[pick].p;
-    cache_events_.push_back(v[pick].ready);
-    v[pick] = v.back();
-    v.pop_back();
-    cache_bytes_ -= cls;
-    return p;
+    // The class itself first, then larger classes up to twice its size (a fitting free block is
+    // reused instead of growing the cache with another class); within a class, a block last used
+    // on s, else one whose event has completed.
+    for (auto it = cache_.lower_bound(cls); it != cache_.end() && it->first <= 2 * cls; ++it) {
+        std::vector<Cached>& v = it->second;
+        size_t pick = v.size();
+        for (size_t i = 0; i < v.size() && pick == v.size(); ++i)
+            if (v[i].s == s) pick = i;
+        for (size_t i = 0; i < v.size() && pick == v.size(); ++i)
+            if (hipEventQuery(v[i].ready) == hipSuccess) pick = i;
+        if (pick == v.size()) continue;
+        void* p = v[pick].p;
+        cache_events_.push_back(v[pick].ready);
+        v[pick] = v.back();
+        v.pop_back();
+        cls = it->first;
+        cache_bytes_ -= cls;
+        return p;
+    }
+    return nullptr;
 }
 
 // Caches a block whose last use is queued on s (over the release threshold it goes back to the
@@ -288,9 +294,13 @@ BufPtr Context::alloc(size_t words, hipStream_t s) {
         p = cache_take(cls, s);
         e = hipSuccess;
         if (!p) {
+            const auto t0 = std::chrono::steady_clock::now();
             e = hipMalloc(&p, cls);
+            ++stat_mallocs;
+            stat_malloc_us += (u64)std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t0).count();
             if (e != hipSuccess) {  // cached blocks back to the driver, then once more
                 (void)hipGetLastError();
+                ++stat_oom;
                 {
                     std::lock_guard<std::mutex> lk(plan_mu_);  // as in synchronize_all
                     for (auto st : streams_) (void)hipStreamSynchronize(st);

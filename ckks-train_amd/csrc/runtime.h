@@ -242,6 +242,9 @@ class Context : public std::enable_shared_from_this<Context> {
     // Test hook (tuning key "fail_alloc_after"): the n-th allocation from now fails with a device
     // error, as an out-of-memory would; negative = off.
     std::atomic<int64_t> fail_alloc_after{-1};
+    // Block-cache statistics (read-only tuning keys "stat_mallocs", "stat_malloc_us", "stat_oom"):
+    // blocks obtained from the driver, the host time spent doing so, and allocation failures.
+    std::atomic<u64> stat_mallocs{0}, stat_malloc_us{0}, stat_oom{0};
     std::atomic<u64> live_bytes{0};
 
     // Digit ranges of a level-l polynomial (keyswitch.rs:68-74).
@@ -360,7 +363,8 @@ class Context : public std::enable_shared_from_this<Context> {
     std::map<size_t, std::vector<Cached>> cache_;
     std::vector<hipEvent_t> cache_events_;
     u64 cache_bytes_ = 0, cache_limit_ = ~0ull;
-    void* cache_take(size_t cls, hipStream_t s);
+    // A free block of class cls or of a larger class up to 2 cls (cls is set to the block's).
+    void* cache_take(size_t& cls, hipStream_t s);
     void cache_put(void* p, size_t cls, hipStream_t s);
     void cache_release(u64 keep_bytes);  // after every stream has finished
     u32 n_user_streams_ = 0;
