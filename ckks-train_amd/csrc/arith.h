@@ -1,13 +1,13 @@
 // Portable part of the library's shared definitions: limb descriptors, per-prime constants and
 // the modular arithmetic of the kernels, written so that a host program can include it too (the
-// CPU replay of the NTT passes in tests/emu). The includer defines CKH_DEV (the device prelude,
+// CPU replay of the kernels in tests/emu). The includer defines CKH_DEV (the device prelude,
 // common.h: __device__ __forceinline__), mulhi(a, b) (the high word of a 64 x 64 product) and
 // the f64 bit casts bits_f64 / f64_bits before including this header.
 //
 // Residues are u64 in HBM, limb-major: limb i of a polynomial is N contiguous words. All
 // arithmetic is exact modular integer arithmetic, so results are unique residues whatever the
 // evaluation order; the one floating-point step (the overflow estimate of the exact base
-// conversion) lives in basis.hip and is compiled without FMA contraction.
+// conversion) lives in conv_arith.h, and everything is compiled without FMA contraction.
 #pragma once
 
 #include <stdint.h>

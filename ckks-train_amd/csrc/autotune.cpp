@@ -29,6 +29,14 @@ void Context::set_tuning(const std::string& key, int64_t value) {
         fail_alloc_after = value;
         return;
     }
+    if (key == "fail_launch_after") {  // test hook, as fail_alloc_after
+        fail_launch_after = value;
+        return;
+    }
+    if (key == "fail_upload") {  // test hook: 1 = every upload fails
+        fail_upload = value != 0;
+        return;
+    }
     if (value < 0) fail(CKH_ERR_INVALID_PARAMS, "tuning values are non-negative");
     if (key == "ntt_chunk_fwd")
         ntt_chunk_fwd = (u32)value;
@@ -101,6 +109,8 @@ int64_t Context::get_tuning(const std::string& key) const {
     if (key == "fused_wg") return cu_count ? fused.grid / cu_count : 0;
     if (key == "xcds") return fused.nx;
     if (key == "fail_alloc_after") return fail_alloc_after.load();
+    if (key == "fail_launch_after") return fail_launch_after.load();
+    if (key == "fail_upload") return fail_upload.load() ? 1 : 0;
     fail(CKH_ERR_INVALID_PARAMS, "unknown tuning key " + key);
 }
 

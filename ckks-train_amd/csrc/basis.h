@@ -1,56 +1,9 @@
 // Centered exact RNS base conversion (ModUp, ModDown, rescale) and the division finish.
 #pragma once
 #include "common.h"
+#include "conv_arith.h"
 
 namespace ckh {
-
-constexpr u32 MAX_SRC = 16;
-
-// One source prime q_j of a conversion from Q = prod q_j.
-struct ConvSrc {
-    u64 q;
-    u64 h;               // q_j >> 1: [x + (Q-1)/2]_Q has residue x + h mod q_j
-    u64 qhinv, qhinv_s;  // (Q / q_j)^-1 mod q_j
-    double inv;          // 1.0 / f64(q_j), round to nearest (host computed)
-    double qf;           // q_j as f64 (f64 path)
-    double qhinv_f, qhinv_q;  // (Q / q_j)^-1 and its quotient by q_j, f64 path
-    u32 fp, pad;              // q_j < 2^50
-};
-
-// One target prime r of a conversion.
-struct ConvTgt {
-    u32 prime;  // index into the context's prime constants
-    u32 limb;   // output limb index within the job's output polynomial
-    u64 qmod, qmod_s;  // Q mod r
-    u64 hmod;          // (Q - 1)/2 mod r
-    u64 qhat[MAX_SRC];                   // (Q / q_j) mod r
-    double qhat_f[MAX_SRC], qhat_q[MAX_SRC];      // f64 path: (Q / q_j) mod r and its quotient by r
-    double qhat30_f[MAX_SRC], qhat30_q[MAX_SRC];  // split path: 2^30 (Q / q_j) mod r, quotient by r
-    double half[2][MAX_SRC];  // piece path: (Q / q_j) mod r in 25-bit halves (low, high)
-    FTw p20, p40, p65;        // piece path: 2^20, 2^40, 2^65 mod r (and their quotients by r)
-    double qmod_f, qmod_q, hmod_f, rf, rinv;
-    // TGT_INT: r at least 2^50, 128-bit integer sums; TGT_FP: r and every source below 2^50, f64
-    // products of the sources; TGT_SPLIT: r below 2^50 with larger sources, each source split
-    // into 30-bit halves (or, variant 1, 20-bit pieces summed exactly).
-    u32 mode, pad;
-};
-
-enum : u32 { TGT_INT = 0, TGT_FP = 1, TGT_SPLIT = 2 };
-
-// A conversion: sources [src_first, src_first + nsrc) of the ConvSrc table (their input limbs
-// are src_limb0 + j of the job's input), targets [tgt_first, tgt_first + ntgt) of the ConvTgt
-// table. exact = use the f64 overflow estimate (nsrc > 1).
-struct ConvPlan {
-    u32 src_first, nsrc, tgt_first, ntgt, src_limb0, exact;
-};
-
-// One launch row: plan `plan`, input polynomial (its limb 0) at `in`, output at `out`.
-struct ConvJob {
-    const u64* in;
-    u64* out;
-    u32 plan, pad;
-};
-
 
 // Matrix-core (int8 MFMA) form of a conversion plan. Sum_j y_j c_jt is computed as
 // sum_p 2^(8p) sum_(j,b) y_j[b] w_p(2^(8b) c_jt mod r_t): y_j[b] are the bytes of y_j (offset by

@@ -1,31 +1,14 @@
-// Centered exact fast base conversion and the division finish, for gfx950.
-//
-// For sources q_0..q_{k-1} (product Q) and coefficient-form residues x_j, every target r gets
-//   y_j = [x_j + (q_j >> 1)]_{q_j} * (Q/q_j)^-1 mod q_j
-//   e   = floor(sum_j f64(y_j) * f64(1/q_j))             (k > 1 only)
-//   out = sum_j y_j (Q/q_j mod r) - e (Q mod r) - ((Q-1)/2 mod r)   mod r
-// which is BaseConverter::convert_exact([x + h]_Q) - h
-// (crates/ckks-train-core/src/baseconv.rs:171-233; keyswitch.rs:140-210; baseconv.rs:258-291,
-// 459-511). The modular sums give unique residues in any order. The f64 estimate is the one
-// non-modular step: it uses the CPU's exact sequence (round-to-nearest u64->f64 conversion,
-// round-to-nearest product with the host-computed reciprocal, round-to-nearest sum in source
-// order from +0.0, then floor); FMA contraction is disabled for this file.
+// Centered exact fast base conversion and the division finish, for gfx950: the vector kernel
+// (its per-coefficient arithmetic in conv_arith.h, shared with the CPU replay) and the
+// matrix-core kernel. FMA contraction is disabled for this file (the f64 overflow estimate must
+// round like the CPU path).
 #include "basis.h"
 
 #pragma clang fp contract(off)
 
 namespace ckh {
 
-// Round-to-nearest u64 -> f64: both halves convert exactly and the sum rounds once.
-CKH_DEV double u64_to_f64_rn(u64 v) {
-    const double hi = (double)(u32)(v >> 32) * 4294967296.0;
-    const double lo = (double)(u32)v;
-    return hi + lo;
-}
-
-// One block: 256 coefficients of one job, targets [t0, t0 + tg) of its plan; per-target
-// constants are read through the scalar cache. 60-bit targets sum exact 128-bit integer
-// products and reduce once.
+// One block: 256 coefficients of one job, targets [t0, t0 + tg) of its plan.
 template <u32 MAXS, bool PIECES>
 __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __restrict__ jobs, u32 tg) {
     const u32 n = 1u << T.log_n;
@@ -34,137 +17,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
     const ConvPlan P = T.plans[job.plan];
     const u32 t0 = blockIdx.z * tg;
     if (t0 >= P.ntgt || c >= n) return;
-    const u32 t1 = min(t0 + tg, P.ntgt);
-    const auto inp = glb(job.in);
-    u64 y[MAXS];
-    // f64 source: yd = y, yh = 0; integer source: yd / yh = its low / high 30 bits, or (PIECES)
-    // yd / yh / yx = its 20-bit pieces from the low end
-    double yd[MAXS], yh[MAXS], yx[MAXS];
-    double f = 0.0;
-    // the piece path needs every source integer (at most 7: the bracketed sums below 2^53)
-    bool srcs_int = PIECES && MAXS <= 7;
-    for (u32 j = 0; j < P.nsrc && srcs_int; ++j) srcs_int = !cst(T.src + P.src_first + j)->fp;
-#pragma unroll
-    for (u32 j = 0; j < MAXS; ++j) {
-        y[j] = 0;
-        yd[j] = 0.0;
-        yh[j] = 0.0;
-        yx[j] = 0.0;
-        if (j < P.nsrc) {
-            const ConvSrc* s = T.src + P.src_first + j;
-            const u64 x = inp[(u64)(P.src_limb0 + j) * n + c];
-            if (s->fp) {
-                double sv = u2d(x) + (double)s->h;
-                sv = sv >= s->qf ? sv - s->qf : sv;
-                double v = fmulmod(sv, s->qhinv_f, s->qhinv_q, s->qf);
-                v = v < 0.0 ? v + s->qf : v;
-                yd[j] = v;
-                y[j] = d2u(v);
-            } else {
-                y[j] = shoup(add_mod(x, s->h, s->q), s->qhinv, s->qhinv_s, s->q);
-                if (srcs_int) {
-                    yd[j] = (double)(u32)(y[j] & ((1u << 20) - 1));
-                    yh[j] = (double)(u32)((y[j] >> 20) & ((1u << 20) - 1));
-                    yx[j] = (double)(u32)(y[j] >> 40);
-                } else {
-                    yd[j] = (double)(u32)(y[j] & ((1u << 30) - 1));  // low 30 bits, exact
-                    yh[j] = (double)(u32)(y[j] >> 30);                // high 30 bits, exact
-                }
-            }
-            if (P.exact) {
-                const double prod = u64_to_f64_rn(y[j]) * s->inv;
-                f = f + prod;
-            }
-        }
-    }
-    const u64 e = P.exact ? (u64)f : 0;  // f >= 0: truncation is floor
-    const auto outp = glb(job.out);
-    for (u32 t = t0; t < t1; ++t) {
-        const auto* g = cst(T.tgt + P.tgt_first + t);
-        u64 o;
-        if (PIECES && g->mode == TGT_SPLIT && srcs_int) {
-            // y_j = y0 + y1 2^20 + y2 2^40 (20-bit pieces), c_j = c0 + c1 2^25 (25-bit halves):
-            // S_ab = sum_j y_ja c_jb is exact (every product < 2^45, at most 7 terms), and
-            // sum_j y_j c_j = S00 + (32 S01 + S10) 2^20 + (32 S11 + S20) 2^40 + S21 2^65, the two
-            // bracketed sums staying below 2^53.
-            const double r = g->rf, rinv = g->rinv;
-            double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0, s20 = 0.0, s21 = 0.0;
-#pragma unroll
-            for (u32 j = 0; j < MAXS; ++j) {
-                const double c0 = g->half[0][j], c1 = g->half[1][j];
-                s00 = __builtin_fma(yd[j], c0, s00);
-                s01 = __builtin_fma(yd[j], c1, s01);
-                s10 = __builtin_fma(yh[j], c0, s10);
-                s11 = __builtin_fma(yh[j], c1, s11);
-                s20 = __builtin_fma(yx[j], c0, s20);
-                s21 = __builtin_fma(yx[j], c1, s21);
-            }
-            const double t1 = __builtin_fma(s01, 32.0, s10), t2 = __builtin_fma(s11, 32.0, s20);
-            double acc = s00 + fmulmod(t1, g->p20.w, g->p20.wq, r);
-            acc = acc + fmulmod(t2, g->p40.w, g->p40.wq, r);
-            acc = acc + fmulmod(s21, g->p65.w, g->p65.wq, r);
-            if (P.exact) acc = acc - fmulmod((double)e, g->qmod_f, g->qmod_q, r);
-            acc = acc - g->hmod_f;
-            o = d2u(fcanon(acc, r, rinv));
-        } else if (g->mode == TGT_SPLIT) {
-            // y_j = yh 2^30 + yl: y_j c = yh (2^30 c) + yl c (mod r), four terms between reductions.
-            const double r = g->rf, rinv = g->rinv;
-            double acc = 0.0;
-#pragma unroll
-            for (u32 j = 0; j < MAXS; ++j) {
-                acc = acc + fmulmod(yd[j], g->qhat_f[j], g->qhat_q[j], r);
-                acc = acc + fmulmod(yh[j], g->qhat30_f[j], g->qhat30_q[j], r);
-                if (j & 1) acc = freduce(acc, r, rinv);
-            }
-            if (P.exact) acc = acc - fmulmod((double)e, g->qmod_f, g->qmod_q, r);
-            acc = acc - g->hmod_f;
-            o = d2u(fcanon(acc, r, rinv));
-        } else if (g->mode == TGT_FP) {
-            const double r = g->rf, rinv = g->rinv;
-            double acc = 0.0;
-#pragma unroll
-            for (u32 j = 0; j < MAXS; ++j) {
-                acc = acc + fmulmod(yd[j], g->qhat_f[j], g->qhat_q[j], r);
-                if ((j & 3) == 3) acc = freduce(acc, r, rinv);
-            }
-            if (P.exact) acc = acc - fmulmod((double)e, g->qmod_f, g->qmod_q, r);
-            acc = acc - g->hmod_f;
-            o = d2u(fcanon(acc, r, rinv));
-        } else {
-            const PrimeConst C = pc_int(T.pc + g->prime);
-            if constexpr (MAXS <= 7) {
-                // sum_j y_j c_j from 32-bit halves (y_j, c_j < 2^60): y c = y0 c0 + (y0 c1 +
-                // y1 c0) 2^32 + y1 c1 2^64 with y1, c1 < 2^28, so the middle sum stays below
-                // 7 * 2^61 < 2^64 and the high one below 2^59; one 32 x 32 -> 64 multiply per
-                // partial product instead of full 64 x 64 products. The same exact sum.
-                u64 l = 0, m = 0, h = 0;
-                u32 carry = 0;
-#pragma unroll
-                for (u32 j = 0; j < MAXS; ++j) {
-                    const u64 cj = g->qhat[j];
-                    const u32 y0 = (u32)y[j], y1 = (u32)(y[j] >> 32), c0 = (u32)cj, c1 = (u32)(cj >> 32);
-                    const u64 p = (u64)y0 * c0;
-                    l += p;
-                    carry += l < p ? 1u : 0u;
-                    m += (u64)y0 * c1 + (u64)y1 * c0;
-                    h += (u64)y1 * c1;
-                }
-                const u64 lo = l + (m << 32);
-                const u64 hi = h + carry + (m >> 32) + (lo < l ? 1 : 0);
-                o = reduce128(hi, lo, C);
-            } else {
-                Dot60 acc;
-#pragma unroll
-                for (u32 j = 0; j < MAXS; ++j) acc.add(y[j], g->qhat[j]);
-                u64 hi, lo;
-                acc.get(hi, lo);
-                o = reduce128(hi, lo, C);
-            }
-            if (P.exact) o = sub_mod(o, shoup(e, g->qmod, g->qmod_s, C.q), C.q);
-            o = sub_mod(o, g->hmod, C.q);
-        }
-        outp[(u64)g->limb * n + c] = o;
-    }
+    conv_coeff<MAXS, PIECES>(T.src, T.tgt, T.pc, P, job.in, job.out, n, c, t0, min(t0 + tg, P.ntgt));
 }
 
 // ---------------------------------------------------------------- matrix-core conversion

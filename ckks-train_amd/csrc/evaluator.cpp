@@ -44,6 +44,7 @@ static u64 prime_of_limb(const Context& c, const Poly& p, u32 i) {
 
 PolyPtr poly_upload(Context& c, const u64* host, u32 level, u32 p_limbs, bool ntt) {
     if (!host) fail(CKH_ERR_INVALID_PARAMS, "null limbs");
+    c.tick_upload();
     PolyPtr p = poly_alloc(c, level, p_limbs, ntt);
     for (u32 i = 0; i < p->limbs(); ++i) {
         const u64 q = prime_of_limb(c, *p, i);
@@ -97,6 +98,7 @@ PolyPtr poly_automorphism(Context& c, const Poly& in, u64 g) {
     if (g % 2 == 0 || g >= 2 * c.n) fail(CKH_ERR_INVALID_PARAMS, "Galois element must be odd and < 2N");
     use(c, in);
     PolyPtr out = poly_alloc(c, in.level, in.p_limbs, in.ntt);
+    c.tick_launch();
     if (in.ntt)
         automorphism_ntt_launch(in.data(), out->data(), in.limbs(), g, c.log_n, c.stream());
     else
@@ -194,6 +196,7 @@ PolyPtr poly_scalar_mul(Context& c, const Poly& a, u64 k, bool neg) {
     }
     hipStream_t s = c.stream();
     BufPtr consts = c.upload_blob(ws.data(), ws.size() * 8, s);
+    c.tick_launch();
     scalar_mul_launch(a.data(), out->data(), a.level + 1, consts->ptr, neg, c.d_pc, c.log_n, s);
     hip_check(hipGetLastError(), "scalar product");
     return out;
@@ -218,6 +221,7 @@ PolyPtr poly_pointwise(Context& c, PwOp op, const Poly& a, const Poly* b) {
     use(c, a);
     if (b) use(c, *b);
     PolyPtr out = poly_alloc(c, a.level, 0, a.ntt);
+    c.tick_launch();
     pointwise_launch(op, a.data(), b ? b->data() : nullptr, out->data(), a.level + 1, 0, c.d_pc, c.log_n, c.stream());
     hip_check(hipGetLastError(), "pointwise");
     return out;
@@ -248,6 +252,7 @@ KskPtr ksk_upload(Context& c, const u64* host, u32 max_level, u64 galois) {
     const u64 words = (u64)k->parts * 2 * k->poly_words();
     k->buf = c.alloc(words);
     if (host) {
+        c.tick_upload();
         const u32 lp = max_level + 1 + c.np;
         for (u32 pj = 0; pj < 2 * k->parts; ++pj)
             for (u32 i = 0; i < lp; ++i) {
@@ -461,6 +466,7 @@ void ks_switch_batch_static(Context& c, const std::vector<const Poly*>& ds, cons
     if (desc_off + prog.bytes.size() > CKH_STATIC_SCRATCH_BYTES)
         fail(CKH_ERR_INVALID_PARAMS, "batch needs more than the static workspace");
     hipStream_t s = c.stream();
+    c.tick_launch();
     c.h2d(c.static_scratch + desc_off, prog.bytes.data(), prog.bytes.size(), s);
     c.run(prog, c.static_scratch + desc_off, s);
     hip_check(hipGetLastError(), "key switch");

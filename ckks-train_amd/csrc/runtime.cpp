@@ -9,6 +9,7 @@
 #include <unordered_map>
 
 #include "hostmath.h"
+#include "conv_tables.h"
 #include "ntt_tables.h"
 
 namespace ckh {
@@ -456,80 +457,11 @@ void Context::build_tables() {
         ntt.ri = dev_copy(t.ri);
     }
 
-    std::vector<ConvSrc> srcs;
-    std::vector<ConvTgt> tgts;
-    std::vector<ConvPlan> plans;
-    // sources s (prime indices) -> targets (prime index, output limb)
+    HostConvTables ct;
+    auto& plans = ct.plans;
     auto add_plan = [&](const std::vector<u32>& src, const std::vector<std::pair<u32, u32>>& tg, u32 limb0,
                         bool share_tgt_first, u32 shared_first) {
-        ConvPlan pl{};
-        pl.src_first = (u32)srcs.size();
-        pl.nsrc = (u32)src.size();
-        pl.src_limb0 = limb0;
-        pl.exact = src.size() > 1;
-        std::vector<u64> sq;
-        for (u32 s : src) sq.push_back(primes[s]);
-        for (size_t j = 0; j < src.size(); ++j) {
-            const u64 q = sq[j];
-            u64 qh = 1 % q;
-            for (size_t i = 0; i < src.size(); ++i)
-                if (i != j) qh = hm::mul(qh, sq[i] % q, q);
-            ConvSrc cs{};
-            cs.q = q;
-            cs.h = q >> 1;
-            cs.qhinv = hm::inv(qh, q);
-            cs.qhinv_s = hm::shoup(cs.qhinv, q);
-            cs.inv = 1.0 / (double)q;
-            cs.fp = q < FP_PRIME_BOUND;
-            cs.qf = (double)q;
-            cs.qhinv_f = (double)cs.qhinv;
-            cs.qhinv_q = (double)cs.qhinv / (double)q;
-            srcs.push_back(cs);
-        }
-        if (share_tgt_first) {
-            pl.tgt_first = shared_first;
-        } else {
-            pl.tgt_first = (u32)tgts.size();
-            for (auto [prime, limb] : tg) {
-                const u64 r = primes[prime];
-                ConvTgt t{};
-                t.prime = prime;
-                t.limb = limb;
-                t.qmod = hm::prod_mod(sq.data(), sq.size(), r);
-                t.qmod_s = hm::shoup(t.qmod, r);
-                t.hmod = hm::half_mod(sq.data(), sq.size(), r);
-                bool srcs_fp = true;
-                for (u64 sqj : sq) srcs_fp = srcs_fp && sqj < FP_PRIME_BOUND;
-                t.mode = r >= FP_PRIME_BOUND ? TGT_INT : (srcs_fp ? TGT_FP : TGT_SPLIT);
-                t.rf = (double)r;
-                t.rinv = 1.0 / (double)r;
-                t.qmod_f = (double)t.qmod;
-                t.qmod_q = (double)t.qmod / (double)r;
-                t.hmod_f = (double)t.hmod;
-                if (r < FP_PRIME_BOUND) {
-                    for (auto [sh, dst] : {std::pair<u32, FTw*>{20, &t.p20}, {40, &t.p40}, {65, &t.p65}}) {
-                        const u64 v = hm::pow(2, sh, r);
-                        *dst = {(double)v, (double)v / (double)r};
-                    }
-                }
-                for (size_t j = 0; j < src.size(); ++j) {
-                    u64 qh = 1 % r;
-                    for (size_t i = 0; i < src.size(); ++i)
-                        if (i != j) qh = hm::mul(qh, sq[i] % r, r);
-                    t.qhat[j] = qh;
-                    t.qhat_f[j] = (double)qh;
-                    t.qhat_q[j] = (double)qh / (double)r;
-                    const u64 qh30 = hm::mul(qh, (1ull << 30) % r, r);
-                    t.qhat30_f[j] = (double)qh30;
-                    t.qhat30_q[j] = (double)qh30 / (double)r;
-                    for (u32 b = 0; b < 2; ++b) t.half[b][j] = (double)((qh >> (25 * b)) & ((1u << 25) - 1));
-                }
-                tgts.push_back(t);
-            }
-        }
-        pl.ntgt = (u32)tg.size();
-        plans.push_back(pl);
-        return (u32)plans.size() - 1;
+        return ct.add_plan(primes, src, tg, limb0, share_tgt_first, shared_first);
     };
 
     modup_plan_.assign(nq, {});
@@ -594,12 +526,12 @@ void Context::build_tables() {
             raise_plan_[l] = add_plan({0}, tg, 0, true, first);
         }
     }
-    conv.src = dev_copy(srcs);
-    conv.tgt = dev_copy(tgts);
+    conv.src = dev_copy(ct.srcs);
+    conv.tgt = dev_copy(ct.tgts);
     conv.plans = dev_copy(plans);
     conv.pc = d_pc;
     conv.log_n = log_n;
-    build_mf_tables(srcs, tgts, plans);
+    build_mf_tables(ct.srcs, ct.tgts, plans);
     hip_check(hipDeviceSynchronize(), "table upload");
 }
 
@@ -812,6 +744,7 @@ void Context::run(const Program& p, const uint8_t* d, hipStream_t s) const {
 
 void Context::execute(const Program& p, hipStream_t s) {
     if (p.steps.empty()) return;
+    tick_launch();
     if (use_graphs && execute_graph(p, s)) return;
     BufPtr d = upload_blob(p.bytes.data(), p.bytes.size(), s);
     run(p, (const uint8_t*)d->ptr, s);

@@ -242,6 +242,20 @@ class Context : public std::enable_shared_from_this<Context> {
     // Test hook (tuning key "fail_alloc_after"): the n-th allocation from now fails with a device
     // error, as an out-of-memory would; negative = off.
     std::atomic<int64_t> fail_alloc_after{-1};
+    // Test hooks as the reference's failing backend (crates/ckks-train-cuda/tests/cases/
+    // failing.rs:34-67): "fail_launch_after" makes the n-th primitive launch from now (a
+    // program's launches, or one direct kernel launch) fail with a device error before anything
+    // of it is queued, and disarms itself; "fail_upload" (1) makes every upload of host words
+    // fail so.
+    std::atomic<int64_t> fail_launch_after{-1};
+    std::atomic<bool> fail_upload{false};
+    void tick_launch() {
+        if (fail_launch_after.load() >= 0 && fail_launch_after.fetch_sub(1) == 0)
+            fail(CKH_ERR_DEVICE, "injected launch failure");
+    }
+    void tick_upload() const {
+        if (fail_upload.load()) fail(CKH_ERR_DEVICE, "injected upload failure");
+    }
     // Block-cache statistics (read-only tuning keys "stat_mallocs", "stat_malloc_us", "stat_oom"):
     // blocks obtained from the driver, the host time spent doing so, and allocation failures.
     std::atomic<u64> stat_mallocs{0}, stat_malloc_us{0}, stat_oom{0};

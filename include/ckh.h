@@ -102,13 +102,21 @@ ckh_status ckh_autotune_ks(ckh_context *ctx, uint32_t n, const ckh_poly *const *
  * workgroup); "fuse_ip" (1 = row pass fused into the inner product); "small_limbs" (batched
  * switches of at most this many input limbs in all, a single switch for one, use their own
  * schedule: "conv_variant_small", "conv_tg_small", "fuse_ip_small"); "pool" (1 = the context's
- * block cache, the only mode: freed device blocks stay cached per size class and serve their own
- * stream at once and another stream once their last use has completed on the device; 0, the
- * device's default stream-ordered pool, is refused with CKH_ERR_INVALID_PARAMS: it gave wrong
- * results with batches on several streams at once, for a cause not shown); "service_streams"
- * (1..4 streams the serving loop spreads its batches over, default 4); test hook "fail_alloc_after"
- * (the n-th device allocation from now fails with CKH_ERR_DEVICE, as an out-of-memory would;
- * negative = off). */
+ * block cache, the default: freed device blocks stay cached per size class and serve their own
+ * stream at once and another stream once their last use has completed on the device; 0 = the
+ * device's default stream-ordered pool, its frees ordered after every stream that used the
+ * buffer; the switch waits for every stream first); "service_streams" (1..4 streams the serving
+ * loop spreads its batches over, default 4); "hoisted_batch" (hoisted-rotation requests the
+ * serving loop runs as one batch, default 4); "ntt_fused" (1 = whole transforms of key-switch
+ * programs as one persistent launch per direction, default 0), "fused_lag", "fused_wg" (its
+ * wait depth and workgroups per CU). Read only (ckh_get_tuning): "xcds", "stat_mallocs",
+ * "stat_malloc_us", "stat_oom", "stat_cached_bytes". Test hooks, as the reference's failing
+ * backend (crates/ckks-train-cuda/tests/cases/failing.rs:34-67), negative = off:
+ * "fail_alloc_after" (the n-th device allocation from now fails with CKH_ERR_DEVICE, as an
+ * out-of-memory would); "fail_launch_after" (the n-th launch from now, counted per program,
+ * elementwise kernel or automorphism, fails with CKH_ERR_DEVICE before anything of it is queued;
+ * the hook disarms itself when it fires); "fail_upload" (1 = every upload of polynomial,
+ * ciphertext or key limbs fails with CKH_ERR_DEVICE). Setting a hook keeps captured plans. */
 ckh_status ckh_set_tuning(ckh_context *ctx, const char *key, int64_t value);
 /* CudaDevice::pool_used_memory / set_pool_release_threshold / trim_pool
  * (crates/ckks-train-cuda/src/device.rs:478-527) for the context's block cache (and the

@@ -1,0 +1,274 @@
+// CPU replay of the vector base-conversion kernel (ckks-train_amd/csrc/conv_arith.h, the code the
+// gfx950 kernel k_conv runs per coefficient), over the conversion plans a context builds
+// (conv_tables.h): ModUp digits, ModDown and rescale, for the prime shapes of the key-switch
+// configurations (24 x 50-bit q + 6 x 60-bit p with 6-prime digits, 20 x 50-bit + 4 x 60-bit
+// with 4-prime digits), a shape with a 60-bit q_0 among 40-bit primes (the split and piece
+// paths of 50-bit targets from 60-bit sources), and wide digits (8 and 16 sources).
+//
+// Every coefficient of a plan is converted in target groups as the launch splits them, at the
+// kernel's source bound and at larger ones, with both 50-bit-target variants. Checked (any
+// failure aborts): each f64 modular product and reduction meets its preconditions and equals the
+// integer result (the prelude's hooks); each output word of each target is written exactly
+// once, no other word is, and each source word is read once per target group; the outputs equal
+// the oracle's BaseConverter::convert_exact of [x + h]_Q minus h (ckks_oracle.c, linked).
+// Inputs: random residues, all q - 1, all zero, every y_j = q_j - 1 (the largest products), and
+// the near ties of the f64 overflow estimate: [x + h]_Q within 2^20 of 0 and of Q - 1, and
+// exactly 0 and Q - 1, where the estimate's sum sits at an integer and only the exact IEEE
+// sequence gives the oracle's floor.
+// Test infrastructure only (built and run by tests/test_emu.py).
+#include <random>
+#include <string>
+#include <utility>
+
+#include "emu_prelude.h"
+#include "conv_arith.h"
+#include "conv_tables.h"
+
+extern "C" {
+int or_ntt_primes_skip(unsigned bits, u64 n, u64 skip, u64 count, u64* out);
+void or_convert(const u64* from, u64 k, const u64* to, u64 t, u64 n, const u64* src, u64* out, int exact);
+}
+
+using namespace ckh;
+
+namespace {
+
+typedef unsigned __int128 u128;
+
+u64 mulm(u64 a, u64 b, u64 q) { return (u64)((u128)a * b % q); }
+
+constexpr u64 N = 4096;  // coefficients per plan: eight patterns of 512
+
+struct Shape {
+    const char* name;
+    std::vector<u64> q, p;
+    u32 alpha;
+};
+
+std::vector<u64> primes_of(unsigned bits, u64 count, u64 skip = 0) {
+    std::vector<u64> v(count);
+    if (count && or_ntt_primes_skip(bits, 1ull << 16, skip, count, v.data()) != 0) emu::die("no primes");
+    return v;
+}
+
+// Known [x + h]_Q of the near-tie coefficients: delta (low = true) or Q - 1 - delta.
+struct Tie {
+    bool known = false, low = true;
+    u64 delta = 0;
+};
+
+// Source words of a plan: x_j for coefficient c, by pattern.
+std::vector<u64> make_inputs(const std::vector<u64>& sq, std::mt19937_64& rng, std::vector<Tie>& ties) {
+    ties.assign(N, Tie{});
+    const size_t k = sq.size();
+    std::vector<u64> x(k * N);
+    auto set_from_shifted = [&](u64 c, const std::vector<u64>& shifted) {  // [x + h]_Q residues
+        for (size_t j = 0; j < k; ++j) x[j * N + c] = (shifted[j] + sq[j] - (sq[j] >> 1)) % sq[j];
+    };
+    std::vector<u64> qhat(k);  // (Q / q_j) mod q_j
+    for (size_t j = 0; j < k; ++j) {
+        qhat[j] = 1;
+        for (size_t i = 0; i < k; ++i)
+            if (i != j) qhat[j] = mulm(qhat[j], sq[i] % sq[j], sq[j]);
+    }
+    for (u64 c = 0; c < N; ++c) {
+        const u32 pat = (u32)(c / 512);
+        std::vector<u64> s(k);
+        const u64 delta = rng() & ((1u << 20) - 1);
+        if (pat == 4 || pat == 5) ties[c] = Tie{true, pat == 4, delta};
+        if (pat == 6) ties[c] = Tie{true, (c & 1) == 0, 0};
+        for (size_t j = 0; j < k; ++j) {
+            const u64 q = sq[j];
+            switch (pat) {
+                case 0: s[j] = rng() % q; break;                                        // random
+                case 1: s[j] = (q - 1 + (q >> 1)) % q; break;                            // x = q - 1
+                case 2: s[j] = q >> 1; break;                                            // x = 0
+                case 3: s[j] = mulm(q - 1, qhat[j], q); break;                           // y_j = q_j - 1
+                case 4: s[j] = delta % q; break;                                         // near 0
+                case 5: s[j] = (q - 1 - delta % q + q) % q; break;                       // near Q - 1
+                case 6: s[j] = (c & 1) ? q - 1 : 0; break;                               // exactly 0, Q - 1
+                default: s[j] = (rng() & 1) ? mulm((q - 1) / 2, qhat[j], q) : rng() % q; break;
+            }
+        }
+        set_from_shifted(c, s);
+    }
+    return x;
+}
+
+struct Run {
+    u64 plans = 0, words = 0, off_by_q = 0;
+};
+
+template <u32 MAXS, bool PIECES>
+void run_plan(const HostConvTables& T, const std::vector<PrimeConst>& pcs, u32 pi, const std::vector<u64>& primes,
+              std::mt19937_64& rng, u32 tg, Run& stats) {
+    const ConvPlan& P = T.plans[pi];
+    if (P.nsrc > MAXS) return;
+    std::vector<u64> sq(P.nsrc), tq(P.ntgt);
+    for (u32 j = 0; j < P.nsrc; ++j) sq[j] = T.srcs[P.src_first + j].q;
+    u32 out_limbs = 0;
+    for (u32 t = 0; t < P.ntgt; ++t) {
+        const ConvTgt& g = T.tgts[P.tgt_first + t];
+        tq[t] = primes[g.prime];
+        out_limbs = std::max(out_limbs, g.limb + 1);
+    }
+    std::vector<Tie> ties;
+    const std::vector<u64> x = make_inputs(sq, rng, ties);
+    std::vector<u64> in((size_t)(P.src_limb0 + P.nsrc) * N, 0);
+    std::copy(x.begin(), x.end(), in.begin() + (size_t)P.src_limb0 * N);
+    const u64 sentinel = 0x5a5a5a5a5a5a5a5aull;
+    std::vector<u64> out((size_t)out_limbs * N, sentinel);
+    emu::Tracked ti(in.data(), in.size(), "input"), to(out.data(), out.size(), "output");
+    emu::registry() = {&ti, &to};
+    const u32 groups = (P.ntgt + tg - 1) / tg;
+    for (u32 grp = 0; grp < groups; ++grp)
+        for (u32 c = 0; c < N; ++c)
+            conv_coeff<MAXS, PIECES>(T.srcs.data(), T.tgts.data(), pcs.data(), P, in.data(), out.data(), (u32)N, c,
+                                     grp * tg, std::min((grp + 1) * tg, P.ntgt));
+    emu::registry().clear();
+    // coverage: sources read once per group, target words written once, nothing else touched
+    for (size_t i = 0; i < in.size(); ++i) {
+        const bool src = i >= (size_t)P.src_limb0 * N;
+        if (ti.writes[i] != 0) emu::die("an input word was written", (double)i);
+        if (ti.reads[i] != (src ? groups : 0)) emu::die("a source word not read once per target group", (double)i);
+    }
+    std::vector<char> is_tgt(out_limbs, 0);
+    for (u32 t = 0; t < P.ntgt; ++t) is_tgt[T.tgts[P.tgt_first + t].limb] = 1;
+    for (size_t i = 0; i < out.size(); ++i) {
+        if (to.reads[i] != 0) emu::die("an output word was read", (double)i);
+        if (to.writes[i] != (is_tgt[i / N] ? 1u : 0u)) emu::die("an output word not written exactly once", (double)i);
+    }
+    // the oracle: convert_exact of the shifted residues, minus (Q - 1)/2 mod r
+    std::vector<u64> shifted(x.size()), want((size_t)P.ntgt * N);
+    for (u32 j = 0; j < P.nsrc; ++j)
+        for (u64 c = 0; c < N; ++c) shifted[j * N + c] = (x[j * N + c] + (sq[j] >> 1)) % sq[j];
+    or_convert(sq.data(), P.nsrc, tq.data(), P.ntgt, N, shifted.data(), want.data(), P.exact ? 1 : 0);
+    for (u32 t = 0; t < P.ntgt; ++t) {
+        const u64 r = tq[t];
+        u64 qm = 1 % r;
+        for (u64 s : sq) qm = mulm(qm, s % r, r);
+        const u64 half = mulm((qm + r - 1) % r, (r + 1) / 2, r);
+        const u32 limb = T.tgts[P.tgt_first + t].limb;
+        for (u64 c = 0; c < N; ++c) {
+            const u64 w = (want[t * N + c] + r - half) % r;
+            if (ties[c].known && P.exact) {  // the centered CRT value, for comparison only
+                const u64 d = ties[c].delta % r;
+                const u64 xr = ties[c].low ? d : (qm + 2 * r - 1 - d) % r;
+                if ((xr + r - half) % r != w) ++stats.off_by_q;
+            }
+            if (out[(size_t)limb * N + c] != w) {
+                std::fprintf(stderr, "plan %u (%u sources) target %u coefficient %llu (pattern %llu): %llu, want %llu\n",
+                             pi, P.nsrc, t, (unsigned long long)c, (unsigned long long)(c / 512),
+                             (unsigned long long)out[(size_t)limb * N + c], (unsigned long long)w);
+                emu::die("conversion output differs from the oracle");
+            }
+        }
+    }
+    stats.plans += 1;
+    stats.words += (u64)P.ntgt * N;
+}
+
+template <u32 MAXS>
+void run_both(const HostConvTables& T, const std::vector<PrimeConst>& pcs, u32 pi, const std::vector<u64>& primes,
+              std::mt19937_64& rng, u32 tg, Run& stats) {
+    run_plan<MAXS, false>(T, pcs, pi, primes, rng, tg, stats);
+    run_plan<MAXS, true>(T, pcs, pi, primes, rng, tg, stats);
+}
+
+// The launch's source bound for a plan of k sources (conv_launch: 1..8, else 16), and a larger one.
+void run_bounds(const HostConvTables& T, const std::vector<PrimeConst>& pcs, u32 pi, const std::vector<u64>& primes,
+                std::mt19937_64& rng, Run& stats) {
+    const u32 k = T.plans[pi].nsrc;
+    const u32 tg = 1 + (u32)(rng() % 8);
+    switch (k) {
+        case 1: run_both<1>(T, pcs, pi, primes, rng, tg, stats); run_both<3>(T, pcs, pi, primes, rng, 8, stats); break;
+        case 2: run_both<2>(T, pcs, pi, primes, rng, tg, stats); run_both<6>(T, pcs, pi, primes, rng, 8, stats); break;
+        case 4: run_both<4>(T, pcs, pi, primes, rng, tg, stats); run_both<8>(T, pcs, pi, primes, rng, 8, stats); break;
+        case 6: run_both<6>(T, pcs, pi, primes, rng, tg, stats); run_both<16>(T, pcs, pi, primes, rng, 8, stats); break;
+        case 8: run_both<8>(T, pcs, pi, primes, rng, tg, stats); run_both<16>(T, pcs, pi, primes, rng, 8, stats); break;
+        default: run_both<16>(T, pcs, pi, primes, rng, tg, stats); break;
+    }
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    // argv[1]: a planted mistake (0 none; 5 the reciprocal of one source off by 2^-40 relative,
+    // which moves the overflow estimate; 6 the quotient constant of f64 products set to zero)
+    const int plant = argc > 1 ? std::atoi(argv[1]) : 0;
+    std::mt19937_64 rng(20261204);
+    const std::vector<u64> q50 = primes_of(50, 24), p60 = primes_of(60, 6), q40 = primes_of(40, 5);
+    std::vector<Shape> shapes = {
+        {"24 x 50-bit q + 6 x 60-bit p, 6-prime digits", q50, p60, 6},
+        {"20 x 50-bit q + 4 x 60-bit p, 4-prime digits", std::vector<u64>(q50.begin(), q50.begin() + 20),
+         std::vector<u64>(p60.begin(), p60.begin() + 4), 4},
+        {"60-bit q_0 + 5 x 40-bit q + 2 x 60-bit p, 2-prime digits", {}, {p60[4], p60[5]}, 2},
+        {"8 x 50-bit q + 2 x 60-bit p, one 8-prime digit", std::vector<u64>(q50.begin(), q50.begin() + 8),
+         {p60[0], p60[1]}, 8},
+        {"16 x 50-bit q + 2 x 60-bit p, one 16-prime digit", std::vector<u64>(q50.begin(), q50.begin() + 16),
+         {p60[0], p60[1]}, 16},
+    };
+    shapes[2].q.push_back(p60[3]);
+    for (u64 v : q40) shapes[2].q.push_back(v);
+    if (plant) shapes.resize(1);
+    for (const Shape& S : shapes) {
+        std::vector<u64> primes(S.q);
+        primes.insert(primes.end(), S.p.begin(), S.p.end());
+        const u32 nq = (u32)S.q.size(), np = (u32)S.p.size();
+        std::vector<PrimeConst> pcs(primes.size());
+        for (size_t i = 0; i < primes.size(); ++i) {
+            const u64 q = primes[i];
+            pcs[i].q = q;
+            pcs[i].one_s = (u64)(((u128)1 << 64) / q);
+            pcs[i].two64 = (u64)(((u128)1 << 64) % q);
+            pcs[i].two64_s = (u64)(((u128)pcs[i].two64 << 64) / q);
+        }
+        HostConvTables T;
+        std::vector<u32> plan_ids;
+        for (u32 l : {nq - 1, nq / 2, 1u}) {  // ModUp: every digit of three levels
+            for (u32 s = 0; s <= l; s += S.alpha) {
+                const u32 e = std::min(s + S.alpha, l + 1);
+                std::vector<u32> src;
+                for (u32 i = s; i < e; ++i) src.push_back(i);
+                std::vector<std::pair<u32, u32>> tg;
+                for (u32 i = 0; i <= l; ++i)
+                    if (i < s || i >= e) tg.push_back({i, i});
+                for (u32 t = 0; t < np; ++t) tg.push_back({nq + t, l + 1 + t});
+                plan_ids.push_back(T.add_plan(primes, src, tg, s, false, 0));
+            }
+        }
+        {  // ModDown P -> Q_l at the top and one lower level (a shared target list)
+            std::vector<u32> src;
+            for (u32 t = 0; t < np; ++t) src.push_back(nq + t);
+            std::vector<std::pair<u32, u32>> all;
+            for (u32 i = 0; i < nq; ++i) all.push_back({i, i});
+            const u32 top = T.add_plan(primes, src, all, 0, false, 0);
+            plan_ids.push_back(top);
+            std::vector<std::pair<u32, u32>> low(all.begin(), all.begin() + nq / 3 + 1);
+            plan_ids.push_back(T.add_plan(primes, src, low, 0, true, T.plans[top].tgt_first));
+        }
+        for (u32 l : {nq - 1, 1u}) {  // rescale q_l -> q_0..q_(l-1)
+            std::vector<std::pair<u32, u32>> tg;
+            for (u32 i = 0; i < l; ++i) tg.push_back({i, i});
+            plan_ids.push_back(T.add_plan(primes, {l}, tg, 0, false, 0));
+        }
+        if (plant == 5)
+            for (auto& s : T.srcs) s.inv *= 1.0 + 0x1p-40;
+        if (plant == 6)
+            for (auto& t : T.tgts)
+                for (u32 j = 0; j < MAX_SRC; ++j) t.qhat_q[j] = 0.0;
+        Run stats;
+        for (u32 pi : plan_ids) run_bounds(T, pcs, pi, primes, rng, stats);
+        std::printf("%s: %llu plan runs, %llu output words equal the oracle (near ties where the f64 "
+                    "estimate is off by one, as in the oracle: %llu words)\n",
+                    S.name, (unsigned long long)stats.plans, (unsigned long long)stats.words,
+                    (unsigned long long)stats.off_by_q);
+        std::fflush(stdout);
+    }
+    auto& s = emu::stats();
+    std::printf("f64 products %llu (max |y|/q %.3f, max |t|/q %.3f), reductions %llu (max |x|/q %.3f), "
+                "Shoup products %llu\nEMU OK\n",
+                (unsigned long long)s.fmul_calls, s.fmul_in, s.fmul_out, (unsigned long long)s.fred_calls, s.fred_in,
+                (unsigned long long)s.shoup_calls);
+    return 0;
+}

@@ -1,7 +1,8 @@
-// Host prelude for the CPU replay of the register-radix NTT kernels (test infrastructure).
+// Host prelude for the CPU replay of the device kernels: the register-radix NTT passes and the
+// vector base conversion (test infrastructure).
 //
-// It gives the kernel sources (ckks-train_amd/csrc/arith.h and ntt_passes.h, included unchanged)
-// what the device prelude gives them on the GPU, with checks added:
+// It gives the kernel sources (ckks-train_amd/csrc/arith.h, ntt_passes.h and conv_arith.h,
+// included unchanged) what the device prelude gives them on the GPU, with checks added:
 //  * a workgroup is 256 host threads; __syncthreads() is a barrier across them, at which the
 //    LDS accesses of the phase that ends are checked (every slot written at most once, read at
 //    most once, and read only if written in the phase before);
@@ -61,7 +62,9 @@ inline __int128 i128(double x) { return (__int128)(long long)x; }
 
 inline void check_fmulmod(double y, double w, double wq, double q, double t) {
     if (!is_int(y) || !is_int(w) || !is_int(t)) die("fmulmod operand or result not an integer", y, w, t);
-    if (std::fabs(y) > 0x1p51) die("fmulmod |y| above 2^51", y, q);
+    // |y| < 2^53 keeps y an exact integer; the exactness of t is checked below (an inexact
+    // step would leave t incongruent or outside the bound)
+    if (std::fabs(y) >= 0x1p53) die("fmulmod |y| not below 2^53", y, q);
     if (std::fabs(w) >= q) die("fmulmod |w| not below q", w, q);
     // t = y w - k q exactly, with |t| <= (1/2 + 3 |y| / (16 q)) q (+1 for the rounding of k)
     const __int128 diff = i128(y) * i128(w) - i128(t);
@@ -248,4 +251,10 @@ emu::View<T> glb(T* p) {
 namespace fast {
 inline u64 load_coherent(const u64* p) { return glb(p)[0]; }
 }  // namespace fast
+// Read-only tables: plain host pointers (the scalar cache on the device).
+template <class T>
+const T* cst(const T* p) {
+    return p;
+}
+inline PrimeConst pc_int(const PrimeConst* p) { return *p; }
 }  // namespace ckh

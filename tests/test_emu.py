@@ -1,31 +1,51 @@
-"""CPU replay of the register-radix NTT kernels (tests/emu/ntt_emu.cpp): the pass code the gfx950
-kernels run (ckks-train_amd/csrc/ntt_passes.h and arith.h, compiled for the host), replayed one
-simulated 256-thread workgroup at a time with every barrier, LDS exchange and global access
-checked, every f64 and Shoup modular product verified against 128-bit integers, the words
-between the passes held to the bounds the arithmetic relies on, and the outputs compared with
-the oracle. Mirrors the reference's kernel replay (crates/ckks-train-cuda/src/emu.rs:1-8,72-87).
+"""CPU replay of the device kernels' code, compiled for the host with a checking prelude
+(tests/emu/emu_prelude.h). Mirrors the reference's kernel replay
+(crates/ckks-train-cuda/src/emu.rs:1-8,72-87).
+
+* The register-radix NTT (tests/emu/ntt_emu.cpp): the pass code the gfx950 kernels run
+  (ckks-train_amd/csrc/ntt_passes.h and arith.h), replayed one simulated 256-thread workgroup at a
+  time with every barrier, LDS exchange and global access checked, every f64 and Shoup modular
+  product verified against 128-bit integers, the words between the passes held to the bounds the
+  arithmetic relies on, and the outputs compared with the oracle.
+* The vector base conversion (tests/emu/conv_emu.cpp): the per-coefficient code of k_conv
+  (ckks-train_amd/csrc/conv_arith.h) over the plans a context builds (conv_tables.h) for the
+  key-switch prime shapes and wide digits, with exactly-once coverage of every output word,
+  every f64 product checked, and the outputs compared with the oracle's convert_exact on random,
+  extreme and near-tie inputs of the f64 overflow estimate.
+
 The planted mistakes (an index map that is not one-to-one in LDS or in memory, a wrong twiddle, a
-twiddle whose quotient disagrees) must each be caught."""
+twiddle whose quotient disagrees, a reciprocal that moves the overflow estimate, a product
+constant whose quotient is wrong) must each be caught."""
 import os
 import subprocess
 
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SRC = os.path.join(ROOT, "tests", "emu", "ntt_emu.cpp")
-OUT = os.path.join(ROOT, "tests", "emu", "_build", "ntt_emu")
+EMU = os.path.join(ROOT, "tests", "emu")
 ORACLE_DIR = os.path.join(ROOT, "oracle", "_build")
+
+
+def _build(name):
+    if not os.path.exists(os.path.join(ORACLE_DIR, "libckks_oracle.so")):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True)
+    out = os.path.join(EMU, "_build", name)
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    # no FMA contraction: the f64 steps must round as on the device, where it is off too
+    subprocess.run(["g++", "-std=c++20", "-O2", "-pthread", "-Wall", "-Wno-unknown-pragmas", "-ffp-contract=off",
+                    "-I", os.path.join(ROOT, "ckks-train_amd", "csrc"), os.path.join(EMU, name + ".cpp"), "-o", out,
+                    "-L", ORACLE_DIR, "-lckks_oracle", "-Wl,-rpath," + ORACLE_DIR], check=True)
+    return out
 
 
 @pytest.fixture(scope="module")
 def emu_bin():
-    if not os.path.exists(os.path.join(ORACLE_DIR, "libckks_oracle.so")):
-        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True)
-    os.makedirs(os.path.dirname(OUT), exist_ok=True)
-    subprocess.run(["g++", "-std=c++20", "-O2", "-pthread", "-Wall", "-Wno-unknown-pragmas",
-                    "-I", os.path.join(ROOT, "ckks-train_amd", "csrc"), SRC, "-o", OUT,
-                    "-L", ORACLE_DIR, "-lckks_oracle", "-Wl,-rpath," + ORACLE_DIR], check=True)
-    return OUT
+    return _build("ntt_emu")
+
+
+@pytest.fixture(scope="module")
+def conv_bin():
+    return _build("conv_emu")
 
 
 def test_replay_of_the_ntt_passes_is_exact_and_race_free(emu_bin):
@@ -44,5 +64,25 @@ def test_replay_of_the_ntt_passes_is_exact_and_race_free(emu_bin):
 ])
 def test_replay_catches_planted_mistakes(emu_bin, plant, expect):
     r = subprocess.run([emu_bin, "0", "0", str(plant)], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert expect in r.stderr
+
+
+def test_replay_of_the_conversion_is_exact_and_covers_every_word(conv_bin):
+    r = subprocess.run([conv_bin], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "EMU OK" in r.stdout
+    lines = [l for l in r.stdout.splitlines() if "output words equal the oracle" in l]
+    assert len(lines) == 5  # the five prime shapes
+    # the near-tie inputs reach the boundary where the f64 estimate's floor is off by one
+    assert all(int(l.split("off by one, as in the oracle: ")[1].split()[0]) > 0 for l in lines)
+
+
+@pytest.mark.parametrize("plant,expect", [
+    (5, "conversion output differs from the oracle"),
+    (6, "fmulmod result"),
+])
+def test_conversion_replay_catches_planted_mistakes(conv_bin, plant, expect):
+    r = subprocess.run([conv_bin, str(plant)], capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
     assert expect in r.stderr

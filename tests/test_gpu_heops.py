@@ -514,3 +514,99 @@ def test_mod_switch_to_level_and_relabel(env):
             ev.relabel(ct, bad)
     cnt = ev.counter()
     assert cnt.ct_pt_mults == 0 and cnt.rescales == 0
+
+
+def _fail_each_launch(ctx, ev, op):
+    """Runs op with its k-th launch failing, k = 0, 1, ..., until it completes (the reference's
+    failing backend, crates/ckks-train-cuda/tests/cases/failing.rs:34-59, fails the k-th
+    primitive). Each failure must be a Device error that disarms the hook, leaves the operation
+    counter unchanged and leaks no device memory. Returns the completed result and how many
+    launches failed before it."""
+    ctx.synchronize()
+    live = ctx.pool_used_memory()
+    k = 0
+    while True:
+        ev.reset_counter()
+        ctx.set_tuning("fail_launch_after", k)
+        try:
+            got = op()
+        except ckh.Device as e:
+            assert "injected launch failure" in str(e)
+            assert ctx.tuning("fail_launch_after") == -1
+            assert all(v == 0 for v in ev.counter().as_dict().values()), k
+            ctx.synchronize()
+            assert ctx.pool_used_memory() == live, k
+            k += 1
+            assert k < 400
+            continue
+        ctx.set_tuning("fail_launch_after", -1)
+        return got, k
+
+
+def test_failure_at_every_launch_of_composite_operations():
+    """A device failure at any launch of a composite operation (BSGS product with encrypted
+    diagonals, hoisted rotations, multiply, rescale, rotate, a batch of key switches) is a Device
+    error with nothing counted and nothing leaked; the operation then completes with the words
+    it gave before (composite counts are success-only, src/matvec.rs `counted`)."""
+    ctx, ref = make(C4_SHAPE_SMALL_N)
+    L = len(ref.q) - 1
+    baby, giant, stride, offset = 4, 2, 1, -1
+    steps = [k2 * stride for k2 in range(1, baby)] + [(offset + baby * k1) * stride for k1 in range(giant)]
+    keys, _ = _keys_for(ctx, ref, L, steps, 1800)
+    relin = ctx.ksk_uniform(L, seed=1850)
+    ev = ckh.Evaluator(ctx, relin, keys)
+    x, ents = _enc_job(ctx, ref, L - 1, baby, giant, 1900, missing=(2,))
+    a, _ = _ct(ctx, ref, L, 2000)
+    b, _ = _ct(ctx, ref, L, 2010)
+    ab = ev.mul(a, b)
+    ops = {
+        "matvec_enc": lambda: [ev.matvec_enc(x, ents, baby, giant, stride, offset)],
+        "hoisted_rotate": lambda: ev.hoisted_rotate(a, steps[:3]),
+        "mul": lambda: [ev.mul(a, b)],
+        "rescale": lambda: [ev.rescale(ab)],
+        "rotate": lambda: [ev.rotate(a, steps[0])],
+    }
+    for name, op in ops.items():
+        print("op", name, flush=True)
+        want = [o.download() for o in op()]
+        ev.reset_counter()
+        op()
+        counts = ev.counter().as_dict()
+        got, failed = _fail_each_launch(ctx, ev, op)
+        assert failed >= 1, name
+        assert ev.counter().as_dict() == counts, name
+        assert all((g.download() == w).all() for g, w in zip(got, want)), name
+    # the batched key switch: one program, one launch
+    ds = [ctx.uniform(L - i, seed=2300 + i) for i in range(3)]
+    pairs = ctx.key_switch_batch(ds, relin)
+    want = [(p0.download(), p1.download()) for p0, p1 in pairs]
+    ctx.set_tuning("fail_launch_after", 0)
+    with pytest.raises(ckh.Device):
+        ctx.key_switch_batch(ds, relin)
+    assert ctx.tuning("fail_launch_after") == -1
+    again = ctx.key_switch_batch(ds, relin)
+    assert all((p0.download() == w0).all() and (p1.download() == w1).all() for (p0, p1), (w0, w1) in zip(again, want))
+
+
+def test_upload_failure_is_a_device_error_and_leaks_nothing():
+    """fail_upload makes every upload fail (failing.rs:46-49,61-67): polynomial, ciphertext and
+    switching-key uploads raise Device and hold no device memory; uploads work once it is off."""
+    ctx, ref = make(SMALL)
+    L = len(ref.q) - 1
+    h = host_uniform(ref, L, 5)
+    key = ref.uniform_key(L, 7)
+    ctx.synchronize()
+    live = ctx.pool_used_memory()
+    ctx.set_tuning("fail_upload", 1)
+    assert ctx.tuning("fail_upload") == 1
+    for up in (lambda: ctx.upload(h, L), lambda: ctx.ct_upload(np.stack([h, h]), L, DELTA),
+               lambda: ctx.ksk_upload(key, L)):
+        with pytest.raises(ckh.Device, match="injected upload failure"):
+            up()
+    ctx.synchronize()
+    assert ctx.pool_used_memory() == live
+    ctx.set_tuning("fail_upload", 0)
+    assert (ctx.upload(h, L).download() == h).all()
+    assert (ctx.ct_upload(np.stack([h, h]), L, DELTA).download()[1] == h).all()
+    ev = ckh.Evaluator(ctx, ctx.ksk_upload(key, L))
+    assert ev.has_relin_key()
