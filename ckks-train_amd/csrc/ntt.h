@@ -78,6 +78,22 @@ struct RowIpJob {
 };
 void rows_ip_launch(const NttTables& t, const RowIpJob* jobs, u32 njobs, hipStream_t stream);
 
+// The fused row pass + inner product of one limb followed by mod_down's finish of both sums:
+// the inner-product sums stay in registers, the limb's converted P part (zb / za, column pass
+// done) is row-transformed, and out = (sum - NTT(z)) * P^-1 (+ add) is written
+// (keyswitch.rs:254-309, baseconv.rs:459-511); ip.acc_b / ip.acc_a are not used.
+struct RowIpFinJob {
+    RowIpJob ip;
+    const u64* zb;
+    const u64* za;
+    u64* out_b;
+    u64* out_a;
+    const u64* add_b;  // optional addends (nullptr: none)
+    const u64* add_a;
+    u64 inv, inv_s;  // P^-1 mod q_i and its Shoup companion
+};
+void rows_ip_fin_launch(const NttTables& t, const RowIpFinJob* jobs, u32 njobs, hipStream_t stream);
+
 // ref.dst <- NTT(ref.src) for each of the nrefs descriptors (device array). src may equal
 // dst (in place). Inputs reduced, outputs reduced.
 void ntt_forward(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream);

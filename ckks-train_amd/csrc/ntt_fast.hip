@@ -34,13 +34,14 @@ constexpr u32 FUSED_SPIN_MAX = 1u << 22;
 
 // Row pass fused with the key inner product (f64 primes): for one limb and tile, every digit
 // not owning the limb is row-transformed from its column-pass output, the owning digit's limb
-// is read as it is (NTT form), and acc_b / acc_a = sum_j digit_j * key part j (b / a) are
-// written; the transformed digits never reach memory (keyswitch.rs:254-303).
+// is read as it is (NTT form), and sb / sa = sum_j digit_j * key part j (b / a) are formed as
+// signed f64 values (|.| <= q / 2 + 4 q); the transformed digits never reach memory
+// (keyswitch.rs:254-303).
 template <u32 M>
-CKH_DEV void rows_ip(const FpAr& ar, const RowIpJob& J, u64* lds, u32 log_n, u32 tile) {
+CKH_DEV void rows_ip_sums(const FpAr& ar, const RowIpJob& J, u64* lds, u32 log_n, u32 tile, double (&sb)[16],
+                          double (&sa)[16]) {
     const u64 tbase = (u64)tile << TL;
     const double q = ar.q, qinv = ar.qinv;
-    double sb[16], sa[16];
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) sb[k] = sa[k] = 0.0;
     for (u32 j = 0; j < J.ndig; ++j) {
@@ -79,12 +80,41 @@ CKH_DEV void rows_ip(const FpAr& ar, const RowIpJob& J, u64* lds, u32 log_n, u32
             }
         }
     }
+}
+
+// The sums written as canonical residues to J.acc_b / J.acc_a.
+template <u32 M>
+CKH_DEV void rows_ip(const FpAr& ar, const RowIpJob& J, u64* lds, u32 log_n, u32 tile) {
+    const u64 tbase = (u64)tile << TL;
+    double sb[16], sa[16];
+    rows_ip_sums<M>(ar, J, lds, log_n, tile, sb, sa);
     const u32 tid = threadIdx.x;
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) {
-        glb(J.acc_b)[tbase + tid + k * NT] = d2u(fcanon(sb[k], q, qinv));
-        glb(J.acc_a)[tbase + tid + k * NT] = d2u(fcanon(sa[k], q, qinv));
+        glb(J.acc_b)[tbase + tid + k * NT] = d2u(fcanon(sb[k], ar.q, ar.qinv));
+        glb(J.acc_a)[tbase + tid + k * NT] = d2u(fcanon(sa[k], ar.q, ar.qinv));
     }
+}
+
+// mod_down's finish of one sum for one tile (baseconv.rs:459-511): the row pass of the converted
+// limb z (column pass done) into LDS, then out = (sum - NTT(z)) * P^-1 (+ add). The LDS is free
+// again on return.
+template <u32 M>
+CKH_DEV void finish_sum(const FpAr& ar, const double (&s)[16], const u64* z, u64* out_p, const u64* add_p, u64 q,
+                        u64 inv, u64 inv_s, u64* lds, u32 log_n, u32 tile) {
+    u32 tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const u64 tbase = (u64)tile << TL;
+    fwd_rows_to_lds<M, false>(ar, z, lds, log_n, tile, tid);
+    const auto out = glb(out_p);
+#pragma unroll
+    for (u32 k = 0; k < 16; ++k) {
+        const u64 a = d2u(fcanon(s[k], ar.q, ar.qinv));
+        u64 v = shoup(sub_mod(a, lds[pad(tid + k * NT)], q), inv, inv_s, q);
+        if (add_p) v = add_mod(v, glb(add_p)[tbase + tid + k * NT], q);
+        out[tbase + tid + k * NT] = v;
+    }
+    __syncthreads();  // the next pass reuses the LDS
 }
 
 // ---------------------------------------------------------------- kernels
@@ -314,6 +344,26 @@ __global__ __launch_bounds__(64) void k_xcc_ids(u32* out) {
     if (threadIdx.x == 0) glb(out)[blockIdx.x] = __builtin_amdgcn_s_getreg(XCC_ID_HWREG) & 15;
 }
 
+// The fused row pass + inner product with mod_down's finish of both sums (RowIpFinJob); the same
+// block order as k_rows_ip.
+template <u32 M>
+__global__ __launch_bounds__(NT) void k_rows_ip_fin(const RowIpFinJob* __restrict__ jobs,
+                                                    const PrimeConst* __restrict__ pcs, const FTw* __restrict__ twf,
+                                                    const FTw* __restrict__ rt, u32 log_n, u32 njobs) {
+    __shared__ u64 lds[LDS_WORDS];
+    const u32 tiles = 1u << (log_n - TL), ji = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+    (void)njobs;
+    const RowIpFinJob J = jobs[ji];
+    const PrimeConst* pc = pcs + J.ip.prime;
+    const FpAr ar{twf + ((u64)J.ip.prime << log_n), pc->qf, pc->qinv, pc->n_inv_f, pc->w_n_inv_f,
+                  rt + ((u64)J.ip.prime << (log_n - log_n / 2)) * ROW_FACTOR_SLOTS};
+    const u64 q = pc->q;
+    double sb[16], sa[16];
+    rows_ip_sums<M>(ar, J.ip, lds, log_n, tile, sb, sa);
+    finish_sum<M>(ar, sb, J.zb, J.out_b, J.add_b, q, J.inv, J.inv_s, lds, log_n, tile);
+    finish_sum<M>(ar, sa, J.za, J.out_a, J.add_a, q, J.inv, J.inv_s, lds, log_n, tile);
+}
+
 template <u32 M>
 __global__ __launch_bounds__(NT) void k_rows_ip(const RowIpJob* __restrict__ jobs, const PrimeConst* __restrict__ pcs,
                                                 const FTw* __restrict__ twf, const FTw* __restrict__ rt, u32 log_n,
@@ -461,6 +511,16 @@ void ntt_fast_forward_cols(const NttTables& t, const LimbRef* refs, u32 nrefs, h
 
 void ntt_fast_forward_rows(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s) {
     launch_m(t.log_n / 2, true, false, t, refs, nrefs, s);
+}
+
+void rows_ip_fin_launch(const NttTables& t, const RowIpFinJob* jobs, u32 njobs, hipStream_t s) {
+    if (njobs == 0) return;
+    dim3 grid((1u << (t.log_n - fast::TL)) * njobs);
+    switch (t.log_n / 2) {
+        case 6: hipLaunchKernelGGL(fast::k_rows_ip_fin<6>, grid, dim3(fast::NT), 0, s, jobs, t.pc, t.fwd_f, t.rf, t.log_n, njobs); break;
+        case 7: hipLaunchKernelGGL(fast::k_rows_ip_fin<7>, grid, dim3(fast::NT), 0, s, jobs, t.pc, t.fwd_f, t.rf, t.log_n, njobs); break;
+        default: hipLaunchKernelGGL(fast::k_rows_ip_fin<8>, grid, dim3(fast::NT), 0, s, jobs, t.pc, t.fwd_f, t.rf, t.log_n, njobs); break;
+    }
 }
 
 void rows_ip_launch(const NttTables& t, const RowIpJob* jobs, u32 njobs, hipStream_t s) {

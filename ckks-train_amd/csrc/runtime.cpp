@@ -724,6 +724,7 @@ void Context::run(const Program& p, const uint8_t* d, hipStream_t s) const {
             case S_NTT_COLS: ntt_fast_forward_cols(ntt, (const LimbRef*)x, st.count, s); break;
             case S_NTT_ROWS: ntt_fast_forward_rows(ntt, (const LimbRef*)x, st.count, s); break;
             case S_ROWS_IP: rows_ip_launch(ntt, (const RowIpJob*)x, st.count, s); break;
+            case S_ROWS_IP_FIN: rows_ip_fin_launch(ntt, (const RowIpFinJob*)x, st.count, s); break;
             case S_ADD_SUM:
                 for (u32 i = 0; i < st.count; i += MAX_ROWS)
                     add_sum_launch((const MulSumJob*)x + i, (const u64* const*)(d + st.off2), std::min(MAX_ROWS, st.count - i),
@@ -922,6 +923,51 @@ void Context::prog_apply(Program& p, const std::vector<ApplyTask>& tasks) const 
                           t.key->poly_words(), t.acc + li * n, t.acc + aw + li * n, t.galois, nullptr,
                           ndigits(t.level), nq + k, 0, 0});
         }
+    bool all_fused = !tasks.empty();
+    for (const ApplyTask& t : tasks) all_fused = all_fused && t.fused;
+    if (all_fused && fuse_fin) {
+        // The P limbs' inner product, mod_down of the P parts up to the column pass of the
+        // converted limbs, then per Q limb one kernel: row pass + inner product + finish.
+        p.step(S_IP, p.add(ip), (u32)ip.size());
+        std::vector<LimbRef> intt, cols;
+        std::vector<ConvJob> jobs;
+        for (const ApplyTask& t : tasks) {
+            const u32 L = t.level + 1;
+            const u64 aw = (u64)(L + np) * n;
+            for (u32 part = 0; part < 2; ++part) {
+                const u64* acc = t.acc + part * aw;
+                u64* y = t.y + (u64)part * np * n;
+                u64* z = t.z + (u64)part * L * n;
+                for (u32 k = 0; k < np; ++k) intt.push_back({acc + (L + k) * n, y + k * n, nq + k, 0});
+                jobs.push_back({y, z, moddown_plan_[t.level], 0});
+                for (u32 i = 0; i < L; ++i) cols.push_back({z + i * n, z + i * n, i, 0});
+            }
+        }
+        std::vector<RowIpFinJob> rf;
+        for (u32 i = 0; i < max_l; ++i)
+            for (const ApplyTask& t : tasks) {
+                const u32 L = t.level + 1;
+                if (i >= L) continue;
+                RowIpFinJob j{};
+                j.ip = {t.dig + i * n, digit_words(t.level), t.own + i * n, t.key->buf->ptr + i * n, t.key->poly_words(),
+                        nullptr, nullptr, ndigits(t.level), i, i / alpha, 0};
+                j.zb = t.z + i * n;
+                j.za = t.z + ((u64)L + i) * n;
+                j.out_b = t.out_b + i * n;
+                j.out_a = t.out_a + i * n;
+                j.add_b = t.add_b ? t.add_b + i * n : nullptr;
+                j.add_a = t.add_a ? t.add_a + i * n : nullptr;
+                j.inv = pinv_[i];
+                j.inv_s = pinv_s_[i];
+                rf.push_back(j);
+            }
+        group_by_prime(intt);
+        p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size(), 0, fused_usable());
+        p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_l, np);
+        p.step(S_NTT_COLS, p.add(cols), (u32)cols.size());
+        p.step(S_ROWS_IP_FIN, p.add(rf), (u32)rf.size());
+        return;
+    }
     std::vector<RowIpJob> rip;
     for (u32 i = 0; i < max_l; ++i)
         for (const ApplyTask& t : tasks) {

@@ -151,7 +151,7 @@ struct RaiseTask {
 // Descriptor bytes plus the launch steps that read them (see runtime.cpp, "engine").
 enum StepKind : uint8_t {
     S_NTT_FWD, S_NTT_INV, S_CONV, S_COPY, S_IP, S_NTT_FWD_FIN, S_FINISH, S_NTT_COLS, S_NTT_ROWS, S_ROWS_IP,
-    S_MUL_SUM, S_AUTOMORPH, S_ADD_SUM
+    S_MUL_SUM, S_AUTOMORPH, S_ADD_SUM, S_ROWS_IP_FIN
 };
 struct Program {
     struct Step {
@@ -326,8 +326,10 @@ class Context : public std::enable_shared_from_this<Context> {
     // Raises a Device error (and clears the word) if a kernel has set the device error word.
     void check_device_error(hipStream_t s);
     // Row pass fused into the inner product of batched key switches (tuning key "fuse_ip");
-    // possible when the register-radix NTT applies and every Q prime is below 2^50.
-    u32 fuse_ip = 1;
+    // possible when the register-radix NTT applies and every Q prime is below 2^50. With it,
+    // mod_down's finish of the Q limbs in the same kernel (tuning key "fuse_fin"): the sums never
+    // reach memory.
+    u32 fuse_ip = 1, fuse_fin = 1;
     bool fuse_ip_possible() const;
     std::atomic<u64> graph_captures{0}, graph_launches{0};
 
