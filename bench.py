@@ -445,8 +445,8 @@ def main():
         return
     levels = workload_levels()
     # --batched-only runs without the autotuner, at the choices it makes on this workload
-    tuning = ({"ntt_chunk_fwd": 0, "ntt_chunk_inv": 0, "conv_variant": 0, "conv_tg": 24, "fuse_ip": 1}
-              if args.batched_only else ctx.autotune())
+    tuning = ({"ntt_chunk_fwd": 0, "ntt_chunk_inv": 0, "conv_variant": 0, "conv_tg": 24, "fuse_ip": 1,
+               "conv_variant_md": 1} if args.batched_only else ctx.autotune())
     if args.batched_only:
         for kv in args.tune:
             k, v = kv.split("=")
@@ -460,7 +460,7 @@ def main():
     if not args.batched_only:
         # the key-switch schedule chosen on the workload itself (untimed)
         tuning["ks_on_workload_ms"] = ctx.autotune_ks(polys, key)
-        for k in ("conv_variant", "conv_tg", "fuse_ip", "ntt_chunk_inv"):
+        for k in ("conv_variant", "conv_tg", "fuse_ip", "conv_variant_md", "ntt_chunk_inv"):
             tuning[k] = ctx.tuning(k)
         print(f"# autotune on the workload: {json.dumps(tuning['ks_on_workload_ms'])}", file=sys.stderr, flush=True)
     print(f"# workload: {N_CT} polys, limbs 4..24 (mean {np.mean(levels) + 1:.1f}), key {key.info()[2] / 2**20:.0f} MiB",

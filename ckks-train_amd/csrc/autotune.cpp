@@ -46,6 +46,8 @@ void Context::set_tuning(const std::string& key, int64_t value) {
         use_graphs = value != 0;
     else if (key == "conv_variant")
         conv_variant = (u32)std::min<int64_t>(value, 2);
+    else if (key == "conv_variant_md")
+        conv_variant_md = (u32)std::min<int64_t>(value, 3);
     else if (key == "fuse_ip")
         fuse_ip = value != 0;
     else if (key == "fuse_fin")
@@ -90,6 +92,7 @@ int64_t Context::get_tuning(const std::string& key) const {
     if (key == "graphs") return use_graphs;
     if (key == "graph_cache") return graph_cache_max;
     if (key == "conv_variant") return conv_variant;
+    if (key == "conv_variant_md") return conv_variant_md;
     if (key == "fuse_ip") return fuse_ip;
     if (key == "fuse_fin") return fuse_fin;
     if (key == "small_limbs") return small_limbs;
@@ -159,7 +162,7 @@ std::string Context::autotune() {
     os << "}, \"ks_candidates_ms\": " << autotune_ks(draw, *key);
     os << ", \"small_candidates_ms\": " << autotune_small(*ds[0], *key);
     os << ", \"ntt_chunk_fwd\": " << ntt_chunk_fwd << ", \"ntt_chunk_inv\": " << ntt_chunk_inv << ", \"conv_variant\": " << conv_variant
-       << ", \"fuse_ip\": " << fuse_ip << ", \"small\": [" << conv_variant_s << ", " << conv_tg_s << ", " << fuse_ip_s
+       << ", \"conv_variant_md\": " << conv_variant_md << ", \"fuse_ip\": " << fuse_ip << ", \"small\": [" << conv_variant_s << ", " << conv_tg_s << ", " << fuse_ip_s
        << "]}";
     return os.str();
 }
@@ -236,6 +239,17 @@ std::string Context::autotune_ks(const std::vector<const Poly*>& ds, const Ksk& 
     conv_variant = best_var;
     conv_tg = best_tg;
     fuse_ip = best_fuse;
+    // mod_down's conversions (60-bit sources) on a variant of their own, against following
+    // conv_variant (3); `best` was timed with the current choice.
+    u32 best_md = conv_variant_md;
+    for (u32 md : {3u, 0u, 1u, 2u}) {
+        if (md == best_md || (md == 2 && !conv.mf_ok) || md == conv_variant) continue;
+        conv_variant_md = md;
+        const double t = time_ks();
+        os << ", \"md" << md << "\": " << t;
+        if (t < best) best = t, best_md = md;
+    }
+    conv_variant_md = best_md;
     // The inverse-NTT chunk picked on plain transforms, against no chunking, on the switch
     // itself (its mod_down INTT runs over P limbs grouped by prime, which changes what pays).
     u32 best_chunk = ntt_chunk_inv;

@@ -964,6 +964,7 @@ void Context::prog_apply(Program& p, const std::vector<ApplyTask>& tasks) const 
         group_by_prime(intt);
         p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size(), 0, fused_usable());
         p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_l, np);
+        if (conv_variant_md < 3 && !jobs.empty()) p.steps.back().sched = 1u | (conv_variant_md << 1) | (conv_tg << 8);
         p.step(S_NTT_COLS, p.add(cols), (u32)cols.size());
         p.step(S_ROWS_IP_FIN, p.add(rf), (u32)rf.size());
         return;
@@ -1017,6 +1018,7 @@ void Context::prog_mod_down(Program& p, const std::vector<MdTask>& tasks) const 
     group_by_prime(intt);
     p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size(), 0, fused_usable());
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_l, np);
+    if (conv_variant_md < 3 && !jobs.empty()) p.steps.back().sched = 1u | (conv_variant_md << 1) | (conv_tg << 8);
     const size_t o_ntt = p.add(nttr), o_fin = p.add(fin);
     p.step_ntt(S_NTT_FWD_FIN, o_ntt, (u32)nttr.size(), o_fin, fused_usable());
 }

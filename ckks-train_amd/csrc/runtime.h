@@ -287,6 +287,11 @@ class Context : public std::enable_shared_from_this<Context> {
     // chunk small enough keeps the pass intermediate in the on-die caches between the passes.
     u32 ntt_chunk_fwd = 0, ntt_chunk_inv = 0;
     u32 conv_variant = 0, conv_tg = 8;  // base-conversion kernel variant and targets per workgroup
+    // The variant of mod_down's conversions (P -> Q) in batched programs (tuning key
+    // "conv_variant_md"; 3 = conv_variant). Default 1: with 60-bit sources the exact f64 piece
+    // sums beat two modular products per source (step 19.98 -> 19.54 ms on the 256-switch
+    // workload), while the ModUp conversions (50-bit sources) keep conv_variant.
+    u32 conv_variant_md = 1;
     // Batched key switches of at most small_limbs input limbs in all (a single switch, a few
     // small ones) have too few workgroups per launch to fill the device; they run their own
     // schedule (tuning keys "small_limbs", "conv_variant_small", "conv_tg_small", "fuse_ip_small").

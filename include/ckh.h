@@ -98,7 +98,8 @@ ckh_status ckh_autotune_ks(ckh_context *ctx, uint32_t n, const ckh_poly *const *
  * "graphs" (1 = replay repeated launch sequences from captured hipGraphs, the default; 0 = launch
  * directly); "graph_cache" (captured sequences kept per context); "conv_variant" (base conversion:
  * 0 = vector kernel, 1 = vector kernel with exact f64-piece sums for 60-bit sources, 2 = int8
- * matrix-core kernel); "conv_tg" (targets per base-conversion
+ * matrix-core kernel); "conv_variant_md" (the variant of mod_down's conversions in batched
+ * switches, default 1; 3 = conv_variant); "conv_tg" (targets per base-conversion
  * workgroup); "fuse_ip" (1 = row pass fused into the inner product); "fuse_fin" (1, the default:
  * with fuse_ip, mod_down's finish of the Q limbs in the same kernel, so the inner-product sums
  * never reach memory); "small_limbs" (batched
