@@ -214,4 +214,167 @@ CKH_DEV void conv_coeff(const ConvSrc* srcs, const ConvTgt* tgts, const PrimeCon
     }
 }
 
+// ---------------------------------------------------------------- matrix-core form
+//
+// Sum_j y_j c_jt computed as sum_p 2^(8p) sum_(j,b) y_j[b] w_p(2^(8b) c_jt mod r_t): y_j[b] are
+// the bytes of y_j (offset by -128 to signed), w_p the balanced signed base-256 digits of the
+// constant; one 32 x 32 x 32 int8 matrix product multiplies 32 (target, digit) rows by 32
+// coefficients over 4 sources. Targets are grouped four at a time, f64-path and integer-path
+// targets in separate groups.
+constexpr u32 MF_MAX_GROUPS = 12;
+struct MfPlan {
+    u32 ngroups;  // 0: no matrix form (the vector kernel runs)
+    u32 kb;       // K blocks of 32 (sources / 4, rounded up)
+    u32 a_off;    // first A tile (64 lanes x 16 bytes) of the plan
+    u32 t_off;    // first MfTgt of the plan (4 per group)
+};
+struct MfTgt {
+    double rf, rinv, c32_f, c32_q, bias_f, qmod_f, qmod_q;  // f64 path
+    u64 q, one_s, two64, two64_s, qmod, qmod_s, corr;       // integer path
+    u32 limb;  // output limb; ~0u = padding
+    u32 fp;    // 1: f64 recombination (r < 2^50)
+};
+
+// One block of the matrix-core conversion: 256 coefficients (block bx) of one job (plan P with
+// matrix form F), all targets of the plan; thread tid of 256 (four waves of 64). W provides the
+// wave operations: the operand types v4i (16 bytes per lane) and v16i (16 int32 per lane),
+// mfma(a, b, d) (the wave's 32 x 32 x 32 int8 product: lane l holds A row l % 32 and B column
+// l % 32 at k = 16 (l / 32) + byte, and receives D column l % 32, rows 8 (r / 4) + 4 (l / 32) +
+// r % 4 in its element r) and shfl_xor32(v) (lane l ^ 32's value). a_lds / t_lds are the
+// block's shared copies of the plan's A tiles and target constants.
+// Each wave takes 2 tiles of
+// 32 coefficients; lane l works on coefficient l % 32 of a tile and on sources 2h, 2h + 1,
+// 4 + 2h, 5 + 2h (h = l / 32), which is exactly its B operand (k = 16 h + i: source 2h + i / 8,
+// byte i % 8; second K block: sources + 4). The overflow estimate gathers the partner lane's
+// products (lane l ^ 32) and sums them in source order.
+template <u32 KB, class W>
+CKH_DEV void conv_mf_block(const ConvSrc* srcs, const ConvPlan& P, const MfPlan& F, const typename W::v4i* a_tiles,
+                           const MfTgt* tgts, const u64* in_p, u64* out_p, u32 n, u32 bx, u32 tid,
+                           typename W::v4i* a_lds, MfTgt* t_lds) {
+    typedef typename W::v4i v4i;
+    typedef typename W::v16i v16i;
+    {
+        const u32 na = F.ngroups * F.kb * 64;  // the plan's own K blocks (F.kb <= KB)
+        for (u32 i = tid; i < na; i += 256) a_lds[i] = a_tiles[(u64)F.a_off * 64 + i];
+        const u32 words = F.ngroups * 4 * (u32)(sizeof(MfTgt) / 8);
+        const u64* src = (const u64*)(tgts + F.t_off);
+        u64* dst = (u64*)t_lds;
+        for (u32 i = tid; i < words; i += 256) dst[i] = src[i];
+    }
+    __syncthreads();
+    const u32 lane = tid & 63, w = tid >> 6, h = lane >> 5, nl = lane & 31;
+    const auto inp = glb(in_p);
+    const auto outp = glb(out_p);
+    constexpr u64 FLIP = 0x8080808080808080ull;  // byte - 128: the signed view of each byte
+    // The wave's two tiles of 32 coefficients run side by side (two independent products per
+    // group, one read of each target's constants for both).
+    u32 c[2];
+    v4i b[2][KB];
+    u64 e[2] = {0, 0};
+#pragma unroll
+    for (u32 nt = 0; nt < 2; ++nt) {
+        c[nt] = bx * 256 + w * 64 + nt * 32 + nl;
+        u64 y[2 * KB];
+        double pr[2 * KB];
+#pragma unroll
+        for (u32 q = 0; q < 2 * KB; ++q) {
+            const u32 j = 4 * (q / 2) + 2 * h + (q % 2);
+            y[q] = 0;
+            pr[q] = 0.0;
+            if (j < P.nsrc) {
+                const auto* sp = cst(srcs + P.src_first + j);
+                const u64 x = inp[(u64)(P.src_limb0 + j) * n + c[nt]];
+                if (sp->fp) {
+                    double sv = u2d(x) + (double)sp->h;
+                    sv = sv >= sp->qf ? sv - sp->qf : sv;
+                    double v = fmulmod(sv, sp->qhinv_f, sp->qhinv_q, sp->qf);
+                    v = v < 0.0 ? v + sp->qf : v;
+                    y[q] = d2u(v);
+                } else {
+                    y[q] = shoup(add_mod(x, sp->h, sp->q), sp->qhinv, sp->qhinv_s, sp->q);
+                }
+                pr[q] = u64_to_f64_rn(y[q]) * sp->inv;
+            }
+        }
+        // e = floor(sum_j f64(y_j) / q_j) in source order j = 0, 1, ... (as the vector kernel)
+        if (P.exact) {
+            double other[2 * KB];
+#pragma unroll
+            for (u32 q = 0; q < 2 * KB; ++q) other[q] = W::shfl_xor32(pr[q]);
+            double f = 0.0;
+#pragma unroll
+            for (u32 j = 0; j < 4 * KB; ++j) {
+                if (j < P.nsrc) {
+                    const u32 hj = (j / 2) % 2, q = 2 * (j / 4) + (j % 2);
+                    f = f + (hj == h ? pr[q] : other[q]);
+                }
+            }
+            e[nt] = (u64)f;
+        }
+#pragma unroll
+        for (u32 kb = 0; kb < KB; ++kb) {
+            const u64 y0 = y[2 * kb] ^ FLIP, y1 = y[2 * kb + 1] ^ FLIP;
+            b[nt][kb] = v4i{(int)(u32)y0, (int)(u32)(y0 >> 32), (int)(u32)y1, (int)(u32)(y1 >> 32)};
+        }
+    }
+    auto product = [&](u32 g, u32 nt) {
+        v16i d = {};
+#pragma unroll
+        for (u32 kb = 0; kb < KB; ++kb)
+            if (kb < F.kb)
+                d = W::mfma(a_lds[(g * F.kb + kb) * 64 + lane], b[nt][kb], d);
+        return d;
+    };
+    v16i dn0 = product(0, 0), dn1 = product(0, 1);
+#pragma unroll 1
+    for (u32 g = 0; g < F.ngroups; ++g) {
+        const v16i d0 = dn0, d1 = dn1;
+        if (g + 1 < F.ngroups) {  // the next group's products overlap this group's recombination
+            dn0 = product(g + 1, 0);
+            dn1 = product(g + 1, 1);
+        }
+#pragma unroll
+        for (u32 uu = 0; uu < 2; ++uu) {
+            const MfTgt& tg = t_lds[4 * g + 2 * h + uu];
+            const u32 limb = tg.limb;
+            if (limb == ~0u) continue;
+            const bool fp = tg.fp;
+            const double r = tg.rf, rinv = tg.rinv, c32_f = tg.c32_f, c32_q = tg.c32_q, bias_f = tg.bias_f;
+            const double qmod_f = tg.qmod_f, qmod_q = tg.qmod_q;
+#pragma unroll
+            for (u32 nt = 0; nt < 2; ++nt) {
+                const v16i& d = nt ? d1 : d0;
+                // X = lo + hi 2^32, lo / hi = sum of four digit planes each (|.| < 2^45, exact)
+                double lo = (double)d[8 * uu + 3];
+                lo = __builtin_fma(lo, 256.0, (double)d[8 * uu + 2]);
+                lo = __builtin_fma(lo, 256.0, (double)d[8 * uu + 1]);
+                lo = __builtin_fma(lo, 256.0, (double)d[8 * uu + 0]);
+                double hi = (double)d[8 * uu + 7];
+                hi = __builtin_fma(hi, 256.0, (double)d[8 * uu + 6]);
+                hi = __builtin_fma(hi, 256.0, (double)d[8 * uu + 5]);
+                hi = __builtin_fma(hi, 256.0, (double)d[8 * uu + 4]);
+                u64 o;
+                if (fp) {
+                    double acc = lo + fmulmod(hi, c32_f, c32_q, r) + bias_f;
+                    if (P.exact) acc = acc - fmulmod((double)e[nt], qmod_f, qmod_q, r);
+                    o = d2u(fcanon(acc, r, rinv));
+                } else {
+                    // X + 2^45 + 2^77 = (lo + 2^45) + (hi + 2^45) 2^32 >= 0, below 2^79
+                    const u64 lu = d2u(lo + 0x1p45), hu = d2u(hi + 0x1p45);
+                    const u64 xl = lu + (hu << 32);
+                    const u64 xh = (hu >> 32) + (xl < lu ? 1 : 0);
+                    PrimeConst C{};
+                    C.q = tg.q;
+                    C.one_s = tg.one_s;
+                    C.two64 = tg.two64;
+                    C.two64_s = tg.two64_s;
+                    o = add_mod(reduce128(xh, xl, C), tg.corr, C.q);
+                    if (P.exact) o = sub_mod(o, shoup(e[nt], tg.qmod, tg.qmod_s, C.q), C.q);
+                }
+                outp[(u64)limb * n + c[nt]] = o;
+            }
+        }
+    }
+}
+
 }  // namespace ckh

@@ -16,6 +16,10 @@ struct HostConvTables {
     std::vector<ConvSrc> srcs;
     std::vector<ConvTgt> tgts;
     std::vector<ConvPlan> plans;
+    // the matrix-core form (build_mf): per plan, the A tiles (1024 bytes each), per target
+    std::vector<MfPlan> mf_plans;
+    std::vector<int8_t> mf_a;
+    std::vector<MfTgt> mf_tgt;
 
     // A plan from the primes `src` (indices into primes; input limbs limb0 + j) to the targets
     // `tg` (prime index, output limb). With share_tgt_first the plan's targets are the first
@@ -90,6 +94,109 @@ struct HostConvTables {
         pl.ntgt = (u32)tg.size();
         plans.push_back(pl);
         return (u32)plans.size() - 1;
+    }
+
+    // Balanced base-256 digits of v < 2^63: v = sum_p d_p 256^p, d_p in [-128, 127], 8 digits.
+    static void balanced_digits(u64 v, int8_t d[8]) {
+        for (u32 p = 0; p < 8; ++p) {
+            int b = (int)(v & 255);
+            if (b >= 128) b -= 256;
+            d[p] = (int8_t)b;
+            v = (v - (u64)(int64_t)b) >> 8;  // exact: v - b is a multiple of 256
+        }
+    }
+
+    // The matrix-core form of every conversion plan (MfPlan): A tiles in the lane order of
+    // v_mfma_i32_32x32x32_i8 (lane l holds row l % 32, k = 16 (l / 32) + i for byte i; row
+    // m = 16 (u % 2) + 8 (p / 4) + 4 (u / 2) + p % 4 for target u of the group and digit p, so
+    // that after the product lane l holds all eight digits of targets 2 (l / 32) and 2 (l / 32) + 1
+    // for coefficient l % 32; k = 8 j + b for source j, byte b), and the per-target recombination
+    // constants.
+    // Returns whether every plan has one (the matrix-core kernel may run).
+    bool build_mf(const std::vector<u64>& primes, u64 n) {
+        std::vector<MfPlan>& mp = mf_plans;
+        std::vector<int8_t>& a = mf_a;
+        std::vector<MfTgt>& mt = mf_tgt;
+        mp.assign(plans.size(), MfPlan{});
+        a.clear();
+        mt.clear();
+        bool all = n >= 256;
+        for (size_t pi = 0; pi < plans.size(); ++pi) {
+            const ConvPlan& P = plans[pi];
+            MfPlan& F = mp[pi];
+            F = {0, 0, 0, 0};
+            if (P.nsrc == 0 || P.nsrc > 8 || n < 256) {
+                all = false;
+                continue;
+            }
+            // targets by recombination path, each list padded to a multiple of 4
+            std::vector<int> order;
+            for (int fp = 1; fp >= 0; --fp) {
+                u32 cnt = 0;
+                for (u32 t = 0; t < P.ntgt; ++t)
+                    if ((primes[tgts[P.tgt_first + t].prime] < FP_PRIME_BOUND) == (fp == 1)) order.push_back((int)t), ++cnt;
+                while (cnt % 4) order.push_back(-1), ++cnt;
+            }
+            const u32 ng = (u32)order.size() / 4;
+            if (ng > MF_MAX_GROUPS) {
+                all = false;
+                continue;
+            }
+            F.ngroups = ng;
+            F.kb = (P.nsrc + 3) / 4;
+            F.a_off = (u32)(a.size() / 1024);
+            F.t_off = (u32)mt.size();
+            a.resize(a.size() + (size_t)ng * F.kb * 1024, 0);
+            int8_t* A = a.data() + (size_t)F.a_off * 1024;
+            for (u32 g = 0; g < ng; ++g)
+                for (u32 u = 0; u < 4; ++u) {
+                    const int ti = order[4 * g + u];
+                    MfTgt T{};
+                    T.limb = ~0u;
+                    if (ti >= 0) {
+                        const ConvTgt& ct = tgts[P.tgt_first + ti];
+                        const u64 r = primes[ct.prime];
+                        // W'_(j,b) = 2^(8b) c_j mod r and its digits into the A tiles; the byte offset
+                        // (+128 per byte) contributes 128 sum W'.
+                        u64 bsum = 0;
+                        for (u32 j = 0; j < P.nsrc; ++j)
+                            for (u32 b = 0; b < 8; ++b) {
+                                const u64 w = hm::mul(ct.qhat[j], hm::pow(256 % r, b, r), r);
+                                bsum = (bsum + w) % r;
+                                int8_t d[8];
+                                balanced_digits(w, d);
+                                const u32 k = 8 * j + b, kb = k / 32, kk = k % 32;
+                                for (u32 p = 0; p < 8; ++p) {
+                                    const u32 m = 16 * (u % 2) + 8 * (p / 4) + 4 * (u / 2) + p % 4;
+                                    const u32 lane = m + 32 * (kk / 16), byte = kk % 16;
+                                    A[((size_t)(g * F.kb + kb) * 64 + lane) * 16 + byte] = d[p];
+                                }
+                            }
+                        const u64 bias = hm::mul(128 % r, bsum, r);
+                        T.limb = ct.limb;
+                        T.fp = r < FP_PRIME_BOUND;
+                        T.rf = (double)r;
+                        T.rinv = 1.0 / (double)r;
+                        const u64 c32 = hm::pow(2, 32, r);
+                        T.c32_f = (double)c32;
+                        T.c32_q = (double)c32 / (double)r;
+                        T.bias_f = (double)((bias + r - ct.hmod % r) % r);
+                        T.qmod_f = ct.qmod_f;
+                        T.qmod_q = ct.qmod_q;
+                        T.q = r;
+                        T.one_s = hm::shoup(1, r);
+                        T.two64 = (u64)(((hm::u128)1 << 64) % r);
+                        T.two64_s = hm::shoup(T.two64, r);
+                        T.qmod = ct.qmod;
+                        T.qmod_s = ct.qmod_s;
+                        // integer path adds 2^45 + 2^77 to make the product non-negative
+                        const u64 off = (u64)((((hm::u128)1 << 45) + ((hm::u128)1 << 77)) % r);
+                        T.corr = (u64)(((hm::u128)bias + 2 * (hm::u128)r - ct.hmod % r - off) % r);
+                    }
+                    mt.push_back(T);
+                }
+        }
+        return all;
     }
 };
 

@@ -188,6 +188,8 @@ struct Program {
     }
 };
 
+struct HostConvTables;
+
 class Context : public std::enable_shared_from_this<Context> {
    public:
     Context(int ordinal, const ckh_params& p, u32 n_streams);
@@ -398,8 +400,7 @@ class Context : public std::enable_shared_from_this<Context> {
     std::vector<u64> pinv_, pinv_s_;           // P^-1 mod q_i
     std::vector<std::vector<u64>> qlinv_, qlinv_s_;  // [l][i]: q_l^-1 mod q_i
     void build_tables();
-    void build_mf_tables(const std::vector<ConvSrc>& srcs, const std::vector<ConvTgt>& tgts,
-                         const std::vector<ConvPlan>& plans);
+    void build_mf_tables(HostConvTables& ct);
     template <class T>
     T* dev_copy(const std::vector<T>& v);
 };

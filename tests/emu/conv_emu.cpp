@@ -16,8 +16,10 @@
 // exactly 0 and Q - 1, where the estimate's sum sits at an integer and only the exact IEEE
 // sequence gives the oracle's floor.
 // Test infrastructure only (built and run by tests/test_emu.py).
+#include <barrier>
 #include <random>
 #include <string>
+#include <thread>
 #include <utility>
 
 #include "emu_prelude.h"
@@ -96,14 +98,104 @@ std::vector<u64> make_inputs(const std::vector<u64>& sq, std::mt19937_64& rng, s
 }
 
 struct Run {
-    u64 plans = 0, words = 0, off_by_q = 0;
+    u64 plans = 0, words = 0, off_by_q = 0, mf_plans = 0;
 };
 
-template <u32 MAXS, bool PIECES>
+// The wave operations of the matrix-core conversion, replayed: the 64 lanes of a wave are 64 host
+// threads that exchange their operands through the wave's slots between two barriers. The
+// product follows the documented layout of v_mfma_i32_32x32x32_i8 (lane l: A row l % 32 and B
+// column l % 32 at k = 16 (l / 32) + byte; D column l % 32, rows 8 (r / 4) + 4 (l / 32) + r % 4
+// in element r), in 64-bit arithmetic with a check that every sum fits the int32 accumulator.
+struct EmuWave {
+    struct v4i {
+        int x[4];
+        int& operator[](int i) { return x[i]; }
+        int operator[](int i) const { return x[i]; }
+    };
+    struct v16i {
+        int x[16];
+        int& operator[](int i) { return x[i]; }
+        int operator[](int i) const { return x[i]; }
+    };
+    // A barrier of the wave's 64 lanes (spinning with yields: lanes meet hundreds of times).
+    struct Bar {
+        std::atomic<u32> count{0}, gen{0};
+        void arrive_and_wait() {
+            const u32 g = gen.load();
+            if (count.fetch_add(1) + 1 == 64) {
+                count.store(0);
+                gen.fetch_add(1);
+            } else {
+                while (gen.load() == g) std::this_thread::yield();
+            }
+        }
+    };
+    struct Slots {
+        v4i a[64], b[64];
+        double v[64];
+        Bar bar;
+    };
+    static Slots*& wave() {
+        static thread_local Slots* w = nullptr;
+        return w;
+    }
+    static int byte(const v4i& v, u32 k) { return (int8_t)(((u32)v.x[k / 4] >> (8 * (k % 4))) & 255); }
+    static v16i mfma(v4i a, v4i b, v16i d) {
+        Slots& w = *wave();
+        const u32 l = threadIdx.x & 63;
+        w.a[l] = a;
+        w.b[l] = b;
+        w.bar.arrive_and_wait();
+        v16i out;
+        for (u32 r = 0; r < 16; ++r) {
+            const u32 i = 8 * (r / 4) + 4 * (l / 32) + r % 4, j = l % 32;
+            int64_t acc = d[r];
+            for (u32 k = 0; k < 32; ++k) acc += (int64_t)byte(w.a[i + 32 * (k / 16)], k % 16) * byte(w.b[j + 32 * (k / 16)], k % 16);
+            if (acc < INT32_MIN || acc > INT32_MAX) emu::die("int32 accumulator of the matrix product overflows", (double)acc);
+            out[r] = (int)acc;
+        }
+        w.bar.arrive_and_wait();
+        return out;
+    }
+    static double shfl_xor32(double v) {
+        Slots& w = *wave();
+        const u32 l = threadIdx.x & 63;
+        w.v[l] = v;
+        w.bar.arrive_and_wait();
+        const double r = w.v[l ^ 32];
+        w.bar.arrive_and_wait();
+        return r;
+    }
+};
+
+// One block of the matrix-core kernel: 256 host threads (four waves) with a block barrier.
+template <u32 KB>
+void run_mf_block(const HostConvTables& T, const ConvPlan& P, const MfPlan& F, const u64* in, u64* out, u32 bx) {
+    EmuWave::Slots waves[4];
+    std::vector<EmuWave::v4i> a_lds(MF_MAX_GROUPS * 2 * 64);
+    std::vector<MfTgt> t_lds(MF_MAX_GROUPS * 4);
+    std::barrier<> bar(256);
+    emu::Group g{nullptr, [&bar] { bar.arrive_and_wait(); }};
+    emu::current() = &g;
+    std::vector<std::thread> th;
+    for (u32 t = 0; t < 256; ++t)
+        th.emplace_back([&, t] {
+            threadIdx.x = t;
+            EmuWave::wave() = &waves[t / 64];
+            conv_mf_block<KB, EmuWave>(T.srcs.data(), P, F, (const EmuWave::v4i*)T.mf_a.data(), T.mf_tgt.data(), in,
+                                       out, (u32)N, bx, t, a_lds.data(), t_lds.data());
+        });
+    for (auto& x : th) x.join();
+    emu::current() = nullptr;
+}
+
+// How a plan is converted: the vector kernel (MAXS, PIECES, targets per group) or the matrix-core
+// kernel (KB > 0).
+template <u32 MAXS, bool PIECES, u32 KB = 0>
 void run_plan(const HostConvTables& T, const std::vector<PrimeConst>& pcs, u32 pi, const std::vector<u64>& primes,
               std::mt19937_64& rng, u32 tg, Run& stats) {
     const ConvPlan& P = T.plans[pi];
-    if (P.nsrc > MAXS) return;
+    if (KB ? (T.mf_plans[pi].ngroups == 0 || T.mf_plans[pi].kb > KB) : P.nsrc > MAXS) return;
     std::vector<u64> sq(P.nsrc), tq(P.ntgt);
     for (u32 j = 0; j < P.nsrc; ++j) sq[j] = T.srcs[P.src_first + j].q;
     u32 out_limbs = 0;
@@ -120,11 +212,15 @@ void run_plan(const HostConvTables& T, const std::vector<PrimeConst>& pcs, u32 p
     std::vector<u64> out((size_t)out_limbs * N, sentinel);
     emu::Tracked ti(in.data(), in.size(), "input"), to(out.data(), out.size(), "output");
     emu::registry() = {&ti, &to};
-    const u32 groups = (P.ntgt + tg - 1) / tg;
-    for (u32 grp = 0; grp < groups; ++grp)
-        for (u32 c = 0; c < N; ++c)
-            conv_coeff<MAXS, PIECES>(T.srcs.data(), T.tgts.data(), pcs.data(), P, in.data(), out.data(), (u32)N, c,
-                                     grp * tg, std::min((grp + 1) * tg, P.ntgt));
+    const u32 groups = KB ? 1 : (P.ntgt + tg - 1) / tg;
+    if constexpr (KB > 0) {
+        for (u32 bx = 0; bx < N / 256; ++bx) run_mf_block<KB>(T, P, T.mf_plans[pi], in.data(), out.data(), bx);
+    } else {
+        for (u32 grp = 0; grp < groups; ++grp)
+            for (u32 c = 0; c < N; ++c)
+                conv_coeff<MAXS, PIECES>(T.srcs.data(), T.tgts.data(), pcs.data(), P, in.data(), out.data(), (u32)N, c,
+                                         grp * tg, std::min((grp + 1) * tg, P.ntgt));
+    }
     emu::registry().clear();
     // coverage: sources read once per group, target words written once, nothing else touched
     for (size_t i = 0; i < in.size(); ++i) {
@@ -165,6 +261,7 @@ void run_plan(const HostConvTables& T, const std::vector<PrimeConst>& pcs, u32 p
         }
     }
     stats.plans += 1;
+    stats.mf_plans += KB ? 1 : 0;
     stats.words += (u64)P.ntgt * N;
 }
 
@@ -188,13 +285,20 @@ void run_bounds(const HostConvTables& T, const std::vector<PrimeConst>& pcs, u32
         case 8: run_both<8>(T, pcs, pi, primes, rng, tg, stats); run_both<16>(T, pcs, pi, primes, rng, 8, stats); break;
         default: run_both<16>(T, pcs, pi, primes, rng, tg, stats); break;
     }
+    // the matrix-core kernel at its launch bound (one K block up to 4 sources, else two), and
+    // every fourth plan of at most 4 sources also at two
+    if (T.mf_plans[pi].ngroups) {
+        if (k <= 4) run_plan<0, false, 1>(T, pcs, pi, primes, rng, 0, stats);
+        if (k > 4 || pi % 4 == 0) run_plan<0, false, 2>(T, pcs, pi, primes, rng, 0, stats);
+    }
 }
 
 }  // namespace
 
 int main(int argc, char** argv) {
     // argv[1]: a planted mistake (0 none; 5 the reciprocal of one source off by 2^-40 relative,
-    // which moves the overflow estimate; 6 the quotient constant of f64 products set to zero)
+    // which moves the overflow estimate; 6 the quotient constant of f64 products set to zero; 7
+    // two rows of a matrix-core A tile exchanged)
     const int plant = argc > 1 ? std::atoi(argv[1]) : 0;
     std::mt19937_64 rng(20261204);
     const std::vector<u64> q50 = primes_of(50, 24), p60 = primes_of(60, 6), q40 = primes_of(40, 5);
@@ -252,6 +356,9 @@ int main(int argc, char** argv) {
             for (u32 i = 0; i < l; ++i) tg.push_back({i, i});
             plan_ids.push_back(T.add_plan(primes, {l}, tg, 0, false, 0));
         }
+        T.build_mf(primes, N);
+        if (plant == 7)  // two rows of the first A tile exchanged: a lane mapping that is wrong
+            for (u32 b = 0; b < 16; ++b) std::swap(T.mf_a[b], T.mf_a[16 + b]);
         if (plant == 5)
             for (auto& s : T.srcs) s.inv *= 1.0 + 0x1p-40;
         if (plant == 6)
@@ -259,10 +366,10 @@ int main(int argc, char** argv) {
                 for (u32 j = 0; j < MAX_SRC; ++j) t.qhat_q[j] = 0.0;
         Run stats;
         for (u32 pi : plan_ids) run_bounds(T, pcs, pi, primes, rng, stats);
-        std::printf("%s: %llu plan runs, %llu output words equal the oracle (near ties where the f64 "
-                    "estimate is off by one, as in the oracle: %llu words)\n",
-                    S.name, (unsigned long long)stats.plans, (unsigned long long)stats.words,
-                    (unsigned long long)stats.off_by_q);
+        std::printf("%s: %llu plan runs (%llu on the matrix-core kernel), %llu output words equal the oracle "
+                    "(near ties where the f64 estimate is off by one, as in the oracle: %llu words)\n",
+                    S.name, (unsigned long long)stats.plans, (unsigned long long)stats.mf_plans,
+                    (unsigned long long)stats.words, (unsigned long long)stats.off_by_q);
         std::fflush(stdout);
     }
     auto& s = emu::stats();

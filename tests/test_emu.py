@@ -7,15 +7,18 @@
   time with every barrier, LDS exchange and global access checked, every f64 and Shoup modular
   product verified against 128-bit integers, the words between the passes held to the bounds the
   arithmetic relies on, and the outputs compared with the oracle.
-* The vector base conversion (tests/emu/conv_emu.cpp): the per-coefficient code of k_conv
-  (ckks-train_amd/csrc/conv_arith.h) over the plans a context builds (conv_tables.h) for the
-  key-switch prime shapes and wide digits, with exactly-once coverage of every output word,
-  every f64 product checked, and the outputs compared with the oracle's convert_exact on random,
+* The base conversion (tests/emu/conv_emu.cpp): the per-coefficient code of k_conv and the block
+  code of the matrix-core k_conv_mf (ckks-train_amd/csrc/conv_arith.h; its wave-wide int8 matrix
+  product and lane exchange modelled on 64 host threads per wave) over the plans a context builds
+  (conv_tables.h, A tiles included) for the key-switch prime shapes and wide digits, with
+  exactly-once coverage of every output word, every f64 product checked, the int32 accumulators
+  checked for overflow, and the outputs compared with the oracle's convert_exact on random,
   extreme and near-tie inputs of the f64 overflow estimate.
 
 The planted mistakes (an index map that is not one-to-one in LDS or in memory, a wrong twiddle, a
 twiddle whose quotient disagrees, a reciprocal that moves the overflow estimate, a product
-constant whose quotient is wrong) must each be caught."""
+constant whose quotient is wrong, two rows of a matrix-core operand tile exchanged) must each be
+caught."""
 import os
 import subprocess
 
@@ -74,6 +77,8 @@ def test_replay_of_the_conversion_is_exact_and_covers_every_word(conv_bin):
     assert "EMU OK" in r.stdout
     lines = [l for l in r.stdout.splitlines() if "output words equal the oracle" in l]
     assert len(lines) == 5  # the five prime shapes
+    # every shape has plans with a matrix form (at most 8 sources), replayed on the matrix-core kernel
+    assert all(int(l.split("(")[1].split()[0]) > 0 for l in lines)
     # the near-tie inputs reach the boundary where the f64 estimate's floor is off by one
     assert all(int(l.split("off by one, as in the oracle: ")[1].split()[0]) > 0 for l in lines)
 
@@ -81,6 +86,7 @@ def test_replay_of_the_conversion_is_exact_and_covers_every_word(conv_bin):
 @pytest.mark.parametrize("plant,expect", [
     (5, "conversion output differs from the oracle"),
     (6, "fmulmod result"),
+    (7, "conversion output differs from the oracle"),
 ])
 def test_conversion_replay_catches_planted_mistakes(conv_bin, plant, expect):
     r = subprocess.run([conv_bin, str(plant)], capture_output=True, text=True, timeout=120)
