@@ -58,6 +58,36 @@ struct FinishJob {
     u32 prime, pad;
 };
 
+// One limb of a key-switch inner product fused with the forward row pass (primes below 2^50):
+// digit j's limb is the column-pass output at mid + j * mid_stride, except digit own_j whose
+// limb is the NTT-form own (not transformed); key part j's b limb at key + 2 j key_poly, its a
+// limb key_poly further; acc_b / acc_a receive the sums.
+struct RowIpJob {
+    const u64* mid;
+    u64 mid_stride;
+    const u64* own;
+    const u64* key;
+    u64 key_poly;
+    u64* acc_b;
+    u64* acc_a;
+    u32 ndig, prime, own_j, pad;
+};
+
+// The fused row pass + inner product of one limb followed by mod_down's finish of both sums:
+// the inner-product sums stay in registers, the limb's converted P part (zb / za, column pass
+// done) is row-transformed, and out = (sum - NTT(z)) * P^-1 (+ add) is written
+// (keyswitch.rs:254-309, baseconv.rs:459-511); ip.acc_b / ip.acc_a are not used.
+struct RowIpFinJob {
+    RowIpJob ip;
+    const u64* zb;
+    const u64* za;
+    u64* out_b;
+    u64* out_a;
+    const u64* add_b;  // optional addends (nullptr: none)
+    const u64* add_a;
+    u64 inv, inv_s;  // P^-1 mod q_i and its Shoup companion
+};
+
 // Row factors per row of the register-radix row pass (one 128-byte line of (w, w/q) pairs).
 constexpr u32 ROW_FACTOR_SLOTS = 8;
 

@@ -16,7 +16,11 @@ CKH_DEV u64 mulhi(u64 a, u64 b) { return __umul64hi(a, b); }
 // The bits of a u64 as an f64 and back (signed pass intermediates are kept as raw f64 bits).
 CKH_DEV double bits_f64(u64 x) { return __longlong_as_double((long long)x); }
 CKH_DEV u64 f64_bits(double d) { return (u64)__double_as_longlong(d); }
+// 32-bit bit reversal.
+CKH_DEV u32 brev32(u32 x) { return __brev(x); }
 }  // namespace ckh
+// A value the optimizer must treat as unknown (it cannot hoist what is derived from it).
+#define CKH_OPAQUE(x) asm volatile("" : "+v"(x))
 
 #include "arith.h"
 

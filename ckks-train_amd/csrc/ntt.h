@@ -62,36 +62,8 @@ void ntt_fast_forward_finish(const NttTables& t, const LimbRef* refs, const Fini
 void ntt_fast_forward_cols(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream);
 void ntt_fast_forward_rows(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream);
 
-// One limb of a key-switch inner product fused with the forward row pass (primes below 2^50):
-// digit j's limb is the column-pass output at mid + j * mid_stride, except digit own_j whose
-// limb is the NTT-form own (not transformed); key part j's b limb at key + 2 j key_poly, its a
-// limb key_poly further; acc_b / acc_a receive the sums.
-struct RowIpJob {
-    const u64* mid;
-    u64 mid_stride;
-    const u64* own;
-    const u64* key;
-    u64 key_poly;
-    u64* acc_b;
-    u64* acc_a;
-    u32 ndig, prime, own_j, pad;
-};
 void rows_ip_launch(const NttTables& t, const RowIpJob* jobs, u32 njobs, hipStream_t stream);
 
-// The fused row pass + inner product of one limb followed by mod_down's finish of both sums:
-// the inner-product sums stay in registers, the limb's converted P part (zb / za, column pass
-// done) is row-transformed, and out = (sum - NTT(z)) * P^-1 (+ add) is written
-// (keyswitch.rs:254-309, baseconv.rs:459-511); ip.acc_b / ip.acc_a are not used.
-struct RowIpFinJob {
-    RowIpJob ip;
-    const u64* zb;
-    const u64* za;
-    u64* out_b;
-    u64* out_a;
-    const u64* add_b;  // optional addends (nullptr: none)
-    const u64* add_a;
-    u64 inv, inv_s;  // P^-1 mod q_i and its Shoup companion
-};
 void rows_ip_fin_launch(const NttTables& t, const RowIpFinJob* jobs, u32 njobs, hipStream_t stream);
 
 // ref.dst <- NTT(ref.src) for each of the nrefs descriptors (device array). src may equal

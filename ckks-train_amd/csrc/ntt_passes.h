@@ -22,8 +22,9 @@
 //    (forward) or [0, 2q) (inverse), as in ntt.hip.
 //
 // The includer provides, besides arith.h: glb(p) (a global-memory view of a pointer),
-// load_coherent(p) (a load that sees other workgroups' completed stores), __syncthreads() and
-// threadIdx.x. The LDS tile is a template parameter (a __shared__ array on the device).
+// load_coherent(p) (a load that sees other workgroups' completed stores), __syncthreads(),
+// threadIdx.x and CKH_OPAQUE(x) (hides a value's origin from the optimizer). The LDS tile is a
+// template parameter (a __shared__ array on the device).
 #pragma once
 
 #include "arith.h"
@@ -451,6 +452,93 @@ CKH_DEV void inv_cols(const Ar& ar, u64* __restrict__ io_p, Lds lds, u32 log_n, 
     }
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) io[((u64)(o + (k << R2)) << B) + c0 + cc] = ar.canon_out(v[k]);
+}
+
+// ---------------------------------------------------------------- row pass + inner product
+
+// Row pass fused with the key inner product (f64 primes): for one limb and tile, every digit
+// not owning the limb is row-transformed from its column-pass output, the owning digit's limb
+// is read as it is (NTT form), and sb / sa = sum_j digit_j * key part j (b / a) are formed as
+// signed f64 values (|.| <= q / 2 + 4 q); the transformed digits never reach memory
+// (keyswitch.rs:254-303).
+template <u32 M, class Lds>
+CKH_DEV void rows_ip_sums(const FpAr& ar, const RowIpJob& J, Lds lds, u32 log_n, u32 tile, double (&sb)[16],
+                          double (&sa)[16]) {
+    const u64 tbase = (u64)tile << TL;
+    const double q = ar.q, qinv = ar.qinv;
+#pragma unroll
+    for (u32 k = 0; k < 16; ++k) sb[k] = sa[k] = 0.0;
+    for (u32 j = 0; j < J.ndig; ++j) {
+        // The thread index, opaque to the compiler per digit: otherwise it hoists every address
+        // derived from it (16 words x the row pass's and the products' operands) out of the
+        // digit loop and keeps them live throughout, 244 VGPRs (2 waves per SIMD) instead of 166
+        // (3 waves); recomputing them costs a few integer ops per digit.
+        u32 tid = threadIdx.x;
+        CKH_OPAQUE(tid);
+        const auto kb = glb(J.key + (u64)(2 * j) * J.key_poly + tbase);
+        const auto ka = glb(J.key + (u64)(2 * j + 1) * J.key_poly + tbase);
+        if (j == J.own_j) {
+#pragma unroll
+            for (u32 k = 0; k < 16; ++k) {
+                const double x = u2d(glb(J.own)[tbase + tid + k * NT]);
+                const double b = u2d(kb[tid + k * NT]), a = u2d(ka[tid + k * NT]);
+                sb[k] += fmulmod(x, b, b * qinv, q);
+                sa[k] += fmulmod(x, a, a * qinv, q);
+            }
+        } else {
+            fwd_rows_to_lds<M, true>(ar, J.mid + j * J.mid_stride, lds, log_n, tile, tid);
+#pragma unroll
+            for (u32 k = 0; k < 16; ++k) {
+                const double x = bits_f64(lds[pad(tid + k * NT)]);
+                const double b = u2d(kb[tid + k * NT]), a = u2d(ka[tid + k * NT]);
+                sb[k] += fmulmod(x, b, b * qinv, q);
+                sa[k] += fmulmod(x, a, a * qinv, q);
+            }
+            __syncthreads();  // the next digit's pass reuses the LDS
+        }
+        if ((j & 3) == 3) {
+#pragma unroll
+            for (u32 k = 0; k < 16; ++k) {
+                sb[k] = freduce(sb[k], q, qinv);
+                sa[k] = freduce(sa[k], q, qinv);
+            }
+        }
+    }
+}
+
+// The sums written as canonical residues to J.acc_b / J.acc_a.
+template <u32 M, class Lds>
+CKH_DEV void rows_ip(const FpAr& ar, const RowIpJob& J, Lds lds, u32 log_n, u32 tile) {
+    const u64 tbase = (u64)tile << TL;
+    double sb[16], sa[16];
+    rows_ip_sums<M>(ar, J, lds, log_n, tile, sb, sa);
+    const u32 tid = threadIdx.x;
+#pragma unroll
+    for (u32 k = 0; k < 16; ++k) {
+        glb(J.acc_b)[tbase + tid + k * NT] = d2u(fcanon(sb[k], ar.q, ar.qinv));
+        glb(J.acc_a)[tbase + tid + k * NT] = d2u(fcanon(sa[k], ar.q, ar.qinv));
+    }
+}
+
+// mod_down's finish of one sum for one tile (baseconv.rs:459-511): the row pass of the converted
+// limb z (column pass done) into LDS, then out = (sum - NTT(z)) * P^-1 (+ add). The LDS is free
+// again on return.
+template <u32 M, class Lds>
+CKH_DEV void finish_sum(const FpAr& ar, const double (&s)[16], const u64* z, u64* out_p, const u64* add_p, u64 q,
+                        u64 inv, u64 inv_s, Lds lds, u32 log_n, u32 tile) {
+    u32 tid = threadIdx.x;
+    CKH_OPAQUE(tid);
+    const u64 tbase = (u64)tile << TL;
+    fwd_rows_to_lds<M, false>(ar, z, lds, log_n, tile, tid);
+    const auto out = glb(out_p);
+#pragma unroll
+    for (u32 k = 0; k < 16; ++k) {
+        const u64 a = d2u(fcanon(s[k], ar.q, ar.qinv));
+        u64 v = shoup(sub_mod(a, lds[pad(tid + k * NT)], q), inv, inv_s, q);
+        if (add_p) v = add_mod(v, glb(add_p)[tbase + tid + k * NT], q);
+        out[tbase + tid + k * NT] = v;
+    }
+    __syncthreads();  // the next pass reuses the LDS
 }
 
 }  // namespace fast

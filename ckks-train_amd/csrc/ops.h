@@ -1,6 +1,7 @@
 // Inner product, pointwise, automorphism and seeded-fill launchers.
 #pragma once
 #include "common.h"
+#include "ip_arith.h"
 
 namespace ckh {
 
@@ -26,17 +27,6 @@ void mul_sum_launch(const MulSumJob* jobs, const MulSumTerm* terms, u32 njobs, c
 void add_sum_launch(const MulSumJob* jobs, const u64* const* xs, u32 njobs, const PrimeConst* pc, u32 log_n,
                     hipStream_t stream);
 
-struct IpJob {
-    const u64* dig;
-    u64 dig_stride;
-    const u64* key;
-    u64 key_poly;
-    u64* acc_b;
-    u64* acc_a;
-    u64 galois;
-    const u64* own;
-    u32 ndig, prime, own_j, pad;
-};
 
 void inner_product_launch(const IpJob* jobs, u32 njobs, const PrimeConst* pc, u32 log_n,
                           hipStream_t stream);

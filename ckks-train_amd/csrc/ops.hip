@@ -6,61 +6,14 @@
 
 namespace ckh {
 
-// NTT-form Galois permutation: sigma_g(a)[i] = a[perm(i)],
-// perm(i) = bitrev(((g * (2 bitrev(i) + 1)) mod 2N - 1) / 2)
-// (crates/ckks-train-core/src/rns.rs:5-17,147-157), computed inline instead of read from a table.
-CKH_DEV u32 galois_index(u32 i, u64 g, u32 log_n) {
-    const u32 rev = (__brev(i) >> (32 - log_n)) * 2 + 1;
-    const u32 raw = (u32)(((g * rev) >> 1) & ((1u << log_n) - 1));
-    return __brev(raw) >> (32 - log_n);
-}
-
-// acc_b = sum_j d_j * b_j and acc_a = sum_j d_j * a_j per limb of Q_l * P
-// (crates/ckks-train-ckks/src/keyswitch.rs:254-303; residues are unique, so any exact summation
-// order gives the reference's words). Primes below 2^50 sum f64 modular products (reduced every
-// four digits); larger primes sum exact 128-bit products and reduce once. Digit limbs may be read
-// through the Galois permutation (hoisted rotations, evaluator.rs:596-604).
+// The key inner product of one coefficient (ip_arith.h: ip_coeff) per thread.
 __global__ __launch_bounds__(256) void k_inner_product(const IpJob* __restrict__ jobs,
                                                        const PrimeConst* __restrict__ pc, u32 log_n) {
     const u32 n = 1u << log_n;
     const u32 c = blockIdx.x * 256 + threadIdx.x;
     if (c >= n) return;
     const IpJob J = jobs[blockIdx.y];
-    const PrimeConst C = pc[J.prime];
-    const u32 cs = J.galois ? galois_index(c, J.galois, log_n) : c;
-    if (C.q < FP_PRIME_BOUND) {
-        const double q = C.qf, qinv = C.qinv;
-        double sb = 0.0, sa = 0.0;
-        for (u32 j = 0; j < J.ndig; ++j) {
-            const auto src = glb((J.own && j == J.own_j) ? J.own : J.dig + j * J.dig_stride);
-            const auto key = glb(J.key);
-            const double d = u2d(src[cs]);
-            const double b = u2d(key[(2 * j) * J.key_poly + c]);
-            const double a = u2d(key[(2 * j + 1) * J.key_poly + c]);
-            sb += fmulmod(d, b, b * qinv, q);
-            sa += fmulmod(d, a, a * qinv, q);
-            if ((j & 3) == 3) {
-                sb = freduce(sb, q, qinv);
-                sa = freduce(sa, q, qinv);
-            }
-        }
-        glb(J.acc_b)[c] = d2u(fcanon(sb, q, qinv));
-        glb(J.acc_a)[c] = d2u(fcanon(sa, q, qinv));
-        return;
-    }
-    Dot60 sb, sa;
-    for (u32 j = 0; j < J.ndig; ++j) {
-        const auto src = glb((J.own && j == J.own_j) ? J.own : J.dig + j * J.dig_stride);
-        const auto key = glb(J.key);
-        const u64 d = src[cs];
-        sb.add(d, key[(2 * j) * J.key_poly + c]);
-        sa.add(d, key[(2 * j + 1) * J.key_poly + c]);
-    }
-    u64 hi, lo;
-    sb.get(hi, lo);
-    glb(J.acc_b)[c] = reduce128(hi, lo, C);
-    sa.get(hi, lo);
-    glb(J.acc_a)[c] = reduce128(hi, lo, C);
+    ip_coeff(J, pc[J.prime], log_n, c);
 }
 
 void inner_product_launch(const IpJob* jobs, u32 njobs, const PrimeConst* pc, u32 log_n, hipStream_t stream) {

@@ -1,5 +1,6 @@
-// CPU replay of the vector base-conversion kernel (ckks-train_amd/csrc/conv_arith.h, the code the
-// gfx950 kernel k_conv runs per coefficient), over the conversion plans a context builds
+// CPU replay of the base-conversion kernels (ckks-train_amd/csrc/conv_arith.h: the code the gfx950
+// kernel k_conv runs per coefficient and the block code of the matrix-core k_conv_mf) and of the
+// key inner product (ip_arith.h, k_inner_product), over the conversion plans a context builds
 // (conv_tables.h): ModUp digits, ModDown and rescale, for the prime shapes of the key-switch
 // configurations (24 x 50-bit q + 6 x 60-bit p with 6-prime digits, 20 x 50-bit + 4 x 60-bit
 // with 4-prime digits), a shape with a 60-bit q_0 among 40-bit primes (the split and piece
@@ -25,10 +26,13 @@
 #include "emu_prelude.h"
 #include "conv_arith.h"
 #include "conv_tables.h"
+#include "ip_arith.h"
 
 extern "C" {
 int or_ntt_primes_skip(unsigned bits, u64 n, u64 skip, u64 count, u64* out);
 void or_convert(const u64* from, u64 k, const u64* to, u64 t, u64 n, const u64* src, u64* out, int exact);
+void or_galois_perm_ntt(u64 n, u64 g, uint32_t* perm);
+u64 or_galois_elt(u64 n, int step);
 }
 
 using namespace ckh;
@@ -293,6 +297,55 @@ void run_bounds(const HostConvTables& T, const std::vector<PrimeConst>& pcs, u32
     }
 }
 
+// The key inner product of every coefficient of one limb (ip_coeff, the body of
+// k_inner_product): ndig digits (one given as `own` when own is set), read through the NTT-form
+// Galois permutation when galois != 0, against 2 ndig key limbs. Every digit, key and own word is
+// read exactly once and every output written once; the sums equal the 128-bit ones with the
+// oracle's permutation (or_galois_perm_ntt).
+u64 replay_ip(u64 q, u32 log_n, u32 ndig, u64 galois, bool own, bool extreme, std::mt19937_64& rng) {
+    const u64 n = 1ull << log_n;
+    PrimeConst C{};
+    C.q = q;
+    C.qf = (double)q;
+    C.qinv = 1.0 / (double)q;
+    C.one_s = (u64)(((u128)1 << 64) / q);
+    C.two64 = (u64)(((u128)1 << 64) % q);
+    C.two64_s = (u64)(((u128)C.two64 << 64) / q);
+    std::vector<u64> dig((size_t)ndig * n), ownv(n), key((size_t)2 * ndig * n), ab(n, 7), aa(n, 7);
+    for (auto* v : {&dig, &ownv, &key})
+        for (auto& x : *v) x = extreme ? q - 1 : rng() % q;
+    const u32 own_j = ndig / 2;
+    IpJob J{dig.data(), n, key.data(), n, ab.data(), aa.data(), galois, own ? ownv.data() : nullptr, ndig, 0, own_j, 0};
+    emu::Tracked td(dig.data(), dig.size(), "digits"), to(ownv.data(), n, "own"), tk(key.data(), key.size(), "key"),
+        tb(ab.data(), n, "acc_b"), ta(aa.data(), n, "acc_a");
+    emu::registry() = {&td, &to, &tk, &tb, &ta};
+    for (u32 c = 0; c < n; ++c) ip_coeff(J, C, log_n, c);
+    emu::registry().clear();
+    for (u32 j = 0; j < ndig; ++j)
+        for (u64 i = 0; i < n; ++i)
+            if (td.reads[(size_t)j * n + i] != ((own && j == own_j) ? 0u : 1u))
+                emu::die("a digit word not read exactly once", j, (double)i);
+    for (size_t i = 0; i < n; ++i)
+        if (to.reads[i] != (own ? 1u : 0u)) emu::die("an own-digit word not read exactly once", (double)i);
+    for (size_t i = 0; i < key.size(); ++i)
+        if (tk.reads[i] != 1) emu::die("a key word not read exactly once", (double)i);
+    for (size_t i = 0; i < n; ++i)
+        if (tb.writes[i] != 1 || ta.writes[i] != 1) emu::die("an output word not written exactly once", (double)i);
+    std::vector<uint32_t> perm(n);
+    for (u64 i = 0; i < n; ++i) perm[i] = (uint32_t)i;
+    if (galois) or_galois_perm_ntt(n, galois, perm.data());
+    for (u64 c = 0; c < n; ++c) {
+        u64 wb = 0, wa = 0;
+        for (u32 j = 0; j < ndig; ++j) {
+            const u64 d = (own && j == own_j) ? ownv[perm[c]] : dig[(size_t)j * n + perm[c]];
+            wb = (wb + mulm(d, key[(size_t)2 * j * n + c], q)) % q;
+            wa = (wa + mulm(d, key[(size_t)(2 * j + 1) * n + c], q)) % q;
+        }
+        if (ab[c] != wb || aa[c] != wa) emu::die("inner product differs", (double)c, (double)ndig);
+    }
+    return 2 * n;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -371,6 +424,19 @@ int main(int argc, char** argv) {
                     S.name, (unsigned long long)stats.plans, (unsigned long long)stats.mf_plans,
                     (unsigned long long)stats.words, (unsigned long long)stats.off_by_q);
         std::fflush(stdout);
+    }
+    if (!plant) {
+        // the key inner product: 50-bit (f64 sums) and 60-bit (128-bit sums) primes, 1..16 digits,
+        // with and without the Galois permutation and an own digit
+        const u32 log_n = 12;
+        const u64 g = or_galois_elt(1ull << log_n, 3);
+        u64 words = 0;
+        for (u64 q : {q50[0], p60[0]})
+            for (u32 ndig : {1u, 4u, 5u, 16u})
+                for (u64 gal : {(u64)0, g})
+                    for (bool own : {false, true})
+                        for (bool extreme : {false, true}) words += replay_ip(q, log_n, ndig, gal, own, extreme, rng);
+        std::printf("inner product: %llu output words equal the 128-bit sums\n", (unsigned long long)words);
     }
     auto& s = emu::stats();
     std::printf("f64 products %llu (max |y|/q %.3f, max |t|/q %.3f), reductions %llu (max |x|/q %.3f), "

@@ -105,6 +105,7 @@ inline void check_shoup(u64 a, u64 w, u64 ws, u64 q, u64 r) {
 #define CKH_CHECK_FMULMOD(y, w, wq, q, t) emu::check_fmulmod(y, w, wq, q, t)
 #define CKH_CHECK_FREDUCE(x, q, r) emu::check_freduce(x, q, r)
 #define CKH_CHECK_SHOUP(a, w, ws, q, r) emu::check_shoup(a, w, ws, q, r)
+#define CKH_OPAQUE(x) ((void)(x))
 
 namespace ckh {
 #define CKH_DEV inline
@@ -118,6 +119,11 @@ inline u64 f64_bits(double d) {
     u64 x;
     std::memcpy(&x, &d, 8);
     return x;
+}
+inline u32 brev32(u32 x) {
+    u32 r = 0;
+    for (u32 i = 0; i < 32; ++i) r |= ((x >> i) & 1u) << (31 - i);
+    return r;
 }
 }  // namespace ckh
 

@@ -1,7 +1,9 @@
 // CPU replay of the register-radix NTT kernels (ckks-train_amd/csrc/ntt_passes.h, the code the
 // gfx950 kernels run), one simulated workgroup of 256 threads at a time: the tile passes of a
 // forward and an inverse transform of whole limbs, at N = 2^12 .. 2^16 (M = 6, 7, 8 for both
-// passes), for 50-bit primes (the f64 arithmetic) and 60-bit primes (the integer arithmetic).
+// passes), for 50-bit primes (the f64 arithmetic) and 60-bit primes (the integer arithmetic);
+// then the fused row pass + key inner product + mod_down finish of the batched key switch
+// (k_rows_ip_fin's body) over 1, 3 and 4 digits, its sums and finish against the oracle.
 //
 // Checked (any failure aborts): every pass reads and writes every word of the limb exactly once;
 // every LDS exchange writes and reads each slot once per phase, after it was written; every f64
@@ -137,6 +139,126 @@ void transform(const Limb& L, const std::vector<u64>& in, std::vector<u64>& out,
     out = dst;
 }
 
+// The column pass alone (forward): the row pass's input, as the device leaves it in memory.
+std::vector<u64> cols_only(const Limb& L, const std::vector<u64>& in) {
+    const u32 log_n = L.log_n, A = log_n - log_n / 2, tiles = 1u << (log_n - TL);
+    const u64 n = 1ull << log_n;
+    std::vector<u64> src(in), dst(n, 0);
+    emu::Tracked ts(src.data(), n, "src"), td(dst.data(), n, "dst");
+    emu::registry() = {&ts, &td};
+    emu::Lds lds(LDS_WORDS);
+    emu::LdsView lv{&lds};
+    const PrimeConst& pc = L.tab.pc[0];
+    const FpAr far{L.tab.fwd_f.data(), pc.qf, pc.qinv, pc.n_inv_f, pc.w_n_inv_f, L.tab.rf.data()};
+    for (u32 tile = 0; tile < tiles; ++tile)
+        run_group(lds, [&] {
+            switch (A) {
+                case 6: fwd_cols<6>(far, src.data(), dst.data(), lv, log_n, tile); break;
+                case 7: fwd_cols<7>(far, src.data(), dst.data(), lv, log_n, tile); break;
+                default: fwd_cols<8>(far, src.data(), dst.data(), lv, log_n, tile); break;
+            }
+        });
+    emu::registry().clear();
+    return dst;
+}
+
+std::vector<u64> oracle(const Limb& L, std::vector<u64> a, bool forward);
+
+// The fused row pass + inner product + mod_down finish of one limb (k_rows_ip_fin's body): ndig
+// digits (digit own_j given in NTT form, the others as column-pass outputs), key parts, the
+// converted limbs z_b / z_a as column-pass outputs, an addend for b only. Every word of every
+// input is read exactly once and every output word written once; the outputs equal
+// ((sum_j X_j k_j) - NTT(z)) P^-1 (+ add) computed with the oracle's transforms.
+void replay_rows_ip_fin(const Limb& L, u32 ndig, u32 own_j, std::mt19937_64& rng, int pattern) {
+    const u32 log_n = L.log_n, M = log_n / 2, tiles = 1u << (log_n - TL);
+    const u64 n = 1ull << log_n, q = L.q;
+    auto rnd = [&](std::vector<u64>& v) {
+        for (auto& x : v) x = pattern == 1 ? q - 1 : rng() % q;
+    };
+    std::vector<u64> mid((size_t)ndig * n), own(n), key((size_t)2 * ndig * n), zb_mid, za_mid, add(n), outb(n, 7),
+        outa(n, 7);
+    std::vector<std::vector<u64>> X(ndig);
+    for (u32 j = 0; j < ndig; ++j) {
+        std::vector<u64> x(n);
+        rnd(x);
+        if (j == own_j) {
+            std::copy(x.begin(), x.end(), own.begin());
+            X[j] = x;
+        } else {
+            const std::vector<u64> m = cols_only(L, x);
+            std::copy(m.begin(), m.end(), mid.begin() + (size_t)j * n);
+            X[j] = oracle(L, x, true);
+        }
+    }
+    rnd(key);
+    std::vector<u64> zb(n), za(n);
+    rnd(zb);
+    rnd(za);
+    zb_mid = cols_only(L, zb);
+    za_mid = cols_only(L, za);
+    rnd(add);
+    const u64 inv = 1 + rng() % (q - 1), inv_s = (u64)(((unsigned __int128)inv << 64) / q);
+    RowIpFinJob J{};
+    J.ip = {mid.data(), n, own.data(), key.data(), n, nullptr, nullptr, ndig, 0, own_j, 0};
+    J.zb = zb_mid.data();
+    J.za = za_mid.data();
+    J.out_b = outb.data();
+    J.out_a = outa.data();
+    J.add_b = add.data();
+    J.add_a = nullptr;
+    J.inv = inv;
+    J.inv_s = inv_s;
+    emu::Tracked tm(mid.data(), mid.size(), "digits"), to(own.data(), n, "own digit"), tk(key.data(), key.size(), "key"),
+        tzb(zb_mid.data(), n, "z_b"), tza(za_mid.data(), n, "z_a"), tad(add.data(), n, "addend"),
+        tob(outb.data(), n, "out_b"), toa(outa.data(), n, "out_a");
+    emu::registry() = {&tm, &to, &tk, &tzb, &tza, &tad, &tob, &toa};
+    emu::Lds lds(LDS_WORDS);
+    emu::LdsView lv{&lds};
+    const PrimeConst& pc = L.tab.pc[0];
+    const FpAr far{L.tab.fwd_f.data(), pc.qf, pc.qinv, pc.n_inv_f, pc.w_n_inv_f, L.tab.rf.data()};
+    for (u32 tile = 0; tile < tiles; ++tile)
+        run_group(lds, [&] {
+            auto body = [&](auto Mc) {
+                constexpr u32 MM = decltype(Mc)::value;
+                double sb[16], sa[16];
+                rows_ip_sums<MM>(far, J.ip, lv, log_n, tile, sb, sa);
+                finish_sum<MM>(far, sb, J.zb, J.out_b, J.add_b, q, J.inv, J.inv_s, lv, log_n, tile);
+                finish_sum<MM>(far, sa, J.za, J.out_a, J.add_a, q, J.inv, J.inv_s, lv, log_n, tile);
+            };
+            switch (M) {
+                case 6: body(std::integral_constant<u32, 6>{}); break;
+                case 7: body(std::integral_constant<u32, 7>{}); break;
+                default: body(std::integral_constant<u32, 8>{}); break;
+            }
+        });
+    emu::registry().clear();
+    // coverage: the own digit's and the other digits' words read once (digit own_j's slot of mid
+    // is not read), the key, z and the addend read once, the outputs written once
+    for (u32 j = 0; j < ndig; ++j)
+        for (u64 i = 0; i < n; ++i) {
+            const u32 want = j == own_j ? 0 : 1;
+            if (tm.reads[(size_t)j * n + i] != want || tm.writes[(size_t)j * n + i] != 0)
+                emu::die("a digit word not read exactly once", j, (double)i);
+        }
+    for (emu::Tracked* t : {&to, &tk, &tzb, &tza, &tad})
+        for (size_t i = 0; i < t->len; ++i)
+            if (t->reads[i] != 1 || t->writes[i] != 0) emu::die("an input word not read exactly once", (double)i);
+    for (emu::Tracked* t : {&tob, &toa})
+        for (size_t i = 0; i < t->len; ++i)
+            if (t->writes[i] != 1 || t->reads[i] != 0) emu::die("an output word not written exactly once", (double)i);
+    const std::vector<u64> Zb = oracle(L, zb, true), Za = oracle(L, za, true);
+    auto mulm = [&](u64 a, u64 b) { return (u64)((unsigned __int128)a * b % q); };
+    for (u64 i = 0; i < n; ++i) {
+        u64 sbv = 0, sav = 0;
+        for (u32 j = 0; j < ndig; ++j) {
+            sbv = (sbv + mulm(X[j][i], key[(size_t)2 * j * n + i])) % q;
+            sav = (sav + mulm(X[j][i], key[(size_t)(2 * j + 1) * n + i])) % q;
+        }
+        const u64 wb = (mulm((sbv + q - Zb[i]) % q, inv) + add[i]) % q, wa = mulm((sav + q - Za[i]) % q, inv);
+        if (outb[i] != wb || outa[i] != wa) emu::die("fused row pass + inner product + finish differs", (double)i);
+    }
+}
+
 std::vector<u64> oracle(const Limb& L, std::vector<u64> a, bool forward) {
     const u64 n = 1ull << L.log_n;
     if ((forward ? or_ntt_forward_limbs : or_ntt_inverse_limbs)(n, &L.q, 1, a.data()) != 0)
@@ -194,6 +316,20 @@ int main(int argc, char** argv) {
             std::printf("N=2^%u q=%u-bit: %zu limbs forward and inverse exact; between passes max |x|/q "
                         "forward %.3f inverse %.3f\n",
                         log_n, bits, inputs.size(), fmax, imax);
+            std::fflush(stdout);
+        }
+    }
+    if (!plant) {
+        // the fused row pass + inner product + finish, 50-bit primes, N = 2^12 .. 2^16
+        for (u32 log_n = 12; log_n <= 16; ++log_n) {
+            const u64 n = 1ull << log_n;
+            u64 q = 0;
+            if (or_ntt_primes_skip(50, n, 0, 1, &q) != 0) emu::die("no prime");
+            Limb L{log_n, q, build_ntt_tables({q}, log_n, true)};
+            for (auto [ndig, pattern] : {std::pair<u32, int>{1, 0}, {3, 0}, {4, 0}, {4, 1}})
+                replay_rows_ip_fin(L, ndig, ndig / 2, rng, pattern);
+            std::printf("N=2^%u: fused row pass + inner product + finish exact (1, 3, 4 digits; all q - 1)\n",
+                        log_n);
             std::fflush(stdout);
         }
     }

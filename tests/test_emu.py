@@ -6,14 +6,18 @@
   (ckks-train_amd/csrc/ntt_passes.h and arith.h), replayed one simulated 256-thread workgroup at a
   time with every barrier, LDS exchange and global access checked, every f64 and Shoup modular
   product verified against 128-bit integers, the words between the passes held to the bounds the
-  arithmetic relies on, and the outputs compared with the oracle.
+  arithmetic relies on, and the outputs compared with the oracle; then the fused row pass + key
+  inner product + mod_down finish of the batched key switch (k_rows_ip_fin's body), every input
+  word read once and every output written once, against the oracle's transforms and products.
 * The base conversion (tests/emu/conv_emu.cpp): the per-coefficient code of k_conv and the block
   code of the matrix-core k_conv_mf (ckks-train_amd/csrc/conv_arith.h; its wave-wide int8 matrix
   product and lane exchange modelled on 64 host threads per wave) over the plans a context builds
   (conv_tables.h, A tiles included) for the key-switch prime shapes and wide digits, with
   exactly-once coverage of every output word, every f64 product checked, the int32 accumulators
   checked for overflow, and the outputs compared with the oracle's convert_exact on random,
-  extreme and near-tie inputs of the f64 overflow estimate.
+  extreme and near-tie inputs of the f64 overflow estimate; and the key inner product
+  (ckks-train_amd/csrc/ip_arith.h, k_inner_product) over 1..16 digits on 50- and 60-bit primes,
+  with and without the Galois permutation, against 128-bit sums.
 
 The planted mistakes (an index map that is not one-to-one in LDS or in memory, a wrong twiddle, a
 twiddle whose quotient disagrees, a reciprocal that moves the overflow estimate, a product
@@ -55,8 +59,10 @@ def test_replay_of_the_ntt_passes_is_exact_and_race_free(emu_bin):
     r = subprocess.run([emu_bin, "1", "8"], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-2000:]
     assert "EMU OK" in r.stdout
-    lines = [l for l in r.stdout.splitlines() if l.startswith("N=2^")]
+    lines = [l for l in r.stdout.splitlines() if "limbs forward and inverse exact" in l]
     assert len(lines) == 10  # N = 2^12..2^16, 50- and 60-bit primes (M = 6, 7, 8 in both passes)
+    fused = [l for l in r.stdout.splitlines() if "fused row pass + inner product + finish exact" in l]
+    assert len(fused) == 5  # N = 2^12..2^16
 
 
 @pytest.mark.parametrize("plant,expect", [
@@ -79,6 +85,7 @@ def test_replay_of_the_conversion_is_exact_and_covers_every_word(conv_bin):
     assert len(lines) == 5  # the five prime shapes
     # every shape has plans with a matrix form (at most 8 sources), replayed on the matrix-core kernel
     assert all(int(l.split("(")[1].split()[0]) > 0 for l in lines)
+    assert "inner product: 524288 output words equal the 128-bit sums" in r.stdout
     # the near-tie inputs reach the boundary where the f64 estimate's floor is off by one
     assert all(int(l.split("off by one, as in the oracle: ")[1].split()[0]) > 0 for l in lines)
 
