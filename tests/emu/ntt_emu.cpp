@@ -9,8 +9,10 @@
 // every LDS exchange writes and reads each slot once per phase, after it was written; every f64
 // modular product and reduction meets its preconditions and equals the integer result; every
 // Shoup product is below 2q and congruent; the words between the passes stay within the bounds
-// the arithmetic relies on (f64 forward |x| < 3.7 q, inverse |x| <= q; integer forward < 4q,
-// inverse < 2q); the outputs equal the CPU oracle's transform (ckks_oracle.c, linked).
+// the arithmetic relies on (f64 forward |x| <= 1.7 q after an even number of column stages, 2.6 q
+// after an odd one, inverse |x| <= q; integer forward < 4q, inverse < 2q); the outputs equal the
+// CPU oracle's transform (ckks_oracle.c, linked). An adversarial search on the column pass's
+// inputs and a row pass fed words at 2.6 q (checked against the integer pass) probe those bounds.
 // Test infrastructure only (built and run by tests/test_emu.py).
 #include <algorithm>
 #include <random>
@@ -117,7 +119,9 @@ void transform(const Limb& L, const std::vector<u64>& in, std::vector<u64>& out,
         if (fp && !emu::is_int(v)) emu::die("pass intermediate not an integer", v);
         mx = std::max(mx, v / (double)L.q);
     }
-    const double bound = fp ? (forward ? 3.7 : 1.0) : (forward ? 4.0 : 2.0);
+    // f64 forward: the column pass ends with a reduced stage (1.7 q) when it has an even number of
+    // stages, with a lazy one (2.6 q) when odd
+    const double bound = fp ? (forward ? (A % 2 ? 2.6 : 1.7) : 1.0) : (forward ? 4.0 : 2.0);
     if (fp ? mx > bound : mx >= bound) emu::die("pass intermediate above its bound", mx, bound);
     *mid_max = std::max(*mid_max, mx);
     if (forward) {
@@ -160,6 +164,99 @@ std::vector<u64> cols_only(const Limb& L, const std::vector<u64>& in) {
         });
     emu::registry().clear();
     return dst;
+}
+
+// The forward row pass alone, in place, with the f64 or the integer arithmetic.
+std::vector<u64> rows_only(const Limb& L, std::vector<u64> mid, bool fp) {
+    const u32 log_n = L.log_n, M = log_n / 2, tiles = 1u << (log_n - TL);
+    const u64 n = 1ull << log_n;
+    emu::Tracked tm(mid.data(), n, "mid");
+    emu::registry() = {&tm};
+    emu::Lds lds(LDS_WORDS);
+    emu::LdsView lv{&lds};
+    const PrimeConst& pc = L.tab.pc[0];
+    const FpAr far{L.tab.fwd_f.data(), pc.qf, pc.qinv, pc.n_inv_f, pc.w_n_inv_f, L.tab.rf.data()};
+    const IntAr iar{L.tab.fwd.data(), pc.q, pc.n_inv, pc.n_inv_s, pc.w_n_inv, pc.w_n_inv_s};
+    for (u32 tile = 0; tile < tiles; ++tile)
+        run_group(lds, [&] {
+            auto body = [&](auto Mc) {
+                constexpr u32 MM = decltype(Mc)::value;
+                if (fp) fwd_rows<MM>(far, mid.data(), mid.data(), lv, log_n, nullptr, L.q, tile);
+                else fwd_rows<MM>(iar, mid.data(), mid.data(), lv, log_n, nullptr, L.q, tile);
+            };
+            switch (M) {
+                case 6: body(std::integral_constant<u32, 6>{}); break;
+                case 7: body(std::integral_constant<u32, 7>{}); break;
+                default: body(std::integral_constant<u32, 8>{}); break;
+            }
+        });
+    emu::registry().clear();
+    return mid;
+}
+
+// Adversarial inputs for the f64 forward transform of a 50-bit prime at N = 2^log_n.
+// Column pass: coordinate ascent on the inputs of every column at once (columns are independent
+// in the column pass), a change kept where it raises that column's largest word between the
+// passes; the largest found must stay within the bound the arithmetic relies on (1.7 q after an
+// even number of column stages, 2.6 q after an odd one). Row pass: words at and near 2.6 q with
+// random signs; every product and reduction is
+// checked exact by the prelude's hooks, and the outputs must equal the integer-arithmetic pass
+// (IntAr, the 60-bit primes' Harvey butterflies) run on the same words reduced mod q.
+void adversarial_f64(std::mt19937_64& rng, u32 log_n, int iters) {
+    const u32 B = log_n / 2, A = log_n - B;
+    const u64 n = 1ull << log_n, cols = 1ull << B, rows = n / cols;
+    u64 q = 0;
+    if (or_ntt_primes_skip(50, n, 0, 1, &q) != 0) emu::die("no prime");
+    Limb L{log_n, q, build_ntt_tables({q}, log_n, true)};
+    auto col_max = [&](const std::vector<u64>& mid, std::vector<double>& m) {
+        m.assign(cols, 0.0);
+        for (u64 r = 0; r < rows; ++r)
+            for (u64 c = 0; c < cols; ++c) m[c] = std::max(m[c], std::fabs(bits_f64(mid[r * cols + c])) / (double)q);
+    };
+    const u64 pool[] = {0, q - 1, (q - 1) / 2, (q + 1) / 2, 1, q - 2};
+    std::vector<u64> x(n);
+    for (auto& v : x) v = pool[rng() % 6];
+    std::vector<double> best, cur;
+    col_max(cols_only(L, x), best);
+    for (int it = 0; it < iters; ++it) {
+        std::vector<u64> y(x);
+        for (u64 c = 0; c < cols; ++c)
+            for (int k = 0; k < 2; ++k) {
+                const u64 r = rng() % rows;
+                y[r * cols + c] = (rng() & 3) ? pool[rng() % 6] : rng() % q;
+            }
+        col_max(cols_only(L, y), cur);
+        for (u64 c = 0; c < cols; ++c)
+            if (cur[c] > best[c]) {
+                best[c] = cur[c];
+                for (u64 r = 0; r < rows; ++r) x[r * cols + c] = y[r * cols + c];
+            }
+    }
+    const double found = *std::max_element(best.begin(), best.end()), bound = A % 2 ? 2.6 : 1.7;
+    if (found > bound) emu::die("column pass above its bound on adversarial input", found, bound);
+    // the row pass at the larger of the two between-pass bounds (its first stage is lazy: up to
+    // 3.7 q inside it)
+    const double lim = std::floor(2.6 * (double)q);
+    double row_in_max = 0;
+    for (int rep = 0; rep < 4; ++rep) {
+        std::vector<u64> mid(n), red(n);
+        for (u64 i = 0; i < n; ++i) {
+            double v = (rng() & 1) ? lim : lim - (double)(rng() % 1024);
+            if (rep == 1) v = (double)(rng() % (u64)lim);
+            if (rng() & 1) v = -v;
+            mid[i] = f64_bits(v);
+            const long long r = (long long)v % (long long)q;
+            red[i] = (u64)(r < 0 ? r + (long long)q : r);
+            row_in_max = std::max(row_in_max, std::fabs(v) / (double)q);
+        }
+        const std::vector<u64> f = rows_only(L, mid, true), g = rows_only(L, red, false);
+        for (u64 i = 0; i < n; ++i)
+            if (f[i] >= q || f[i] != g[i]) emu::die("f64 row pass at the bound differs from the integer pass", (double)i);
+    }
+    std::printf("adversarial f64 inputs at N=2^%u: column pass max |x|/q %.3f after %d rounds of ascent (bound "
+                "%.1f); row pass on words up to %.3f q equals the integer pass\n",
+                log_n, found, iters, bound, row_in_max);
+    std::fflush(stdout);
 }
 
 std::vector<u64> oracle(const Limb& L, std::vector<u64> a, bool forward);
@@ -319,6 +416,8 @@ int main(int argc, char** argv) {
             std::fflush(stdout);
         }
     }
+    if (!plant)
+        for (u32 log_n : {12u, 13u}) adversarial_f64(rng, log_n, 200);
     if (!plant) {
         // the fused row pass + inner product + finish, 50-bit primes, N = 2^12 .. 2^16
         for (u32 log_n = 12; log_n <= 16; ++log_n) {
