@@ -3,7 +3,7 @@
 // forward and an inverse transform of whole limbs, at N = 2^12 .. 2^16 (M = 6, 7, 8 for both
 // passes), for 50-bit primes (the f64 arithmetic) and 60-bit primes (the integer arithmetic);
 // then the fused row pass + key inner product + mod_down finish of the batched key switch
-// (k_rows_ip_fin's body) over 1, 3 and 4 digits, its sums and finish against the oracle.
+// (k_rows_ip_fin's body) over 1, 3, 4 and 7 digits, its sums and finish against the oracle.
 //
 // Checked (any failure aborts): every pass reads and writes every word of the limb exactly once;
 // every LDS exchange writes and reads each slot once per phase, after it was written; every f64
@@ -425,9 +425,9 @@ int main(int argc, char** argv) {
             u64 q = 0;
             if (or_ntt_primes_skip(50, n, 0, 1, &q) != 0) emu::die("no prime");
             Limb L{log_n, q, build_ntt_tables({q}, log_n, true)};
-            for (auto [ndig, pattern] : {std::pair<u32, int>{1, 0}, {3, 0}, {4, 0}, {4, 1}})
+            for (auto [ndig, pattern] : {std::pair<u32, int>{1, 0}, {3, 0}, {4, 0}, {4, 1}, {7, 1}})
                 replay_rows_ip_fin(L, ndig, ndig / 2, rng, pattern);
-            std::printf("N=2^%u: fused row pass + inner product + finish exact (1, 3, 4 digits; all q - 1)\n",
+            std::printf("N=2^%u: fused row pass + inner product + finish exact (1, 3, 4, 7 digits; all q - 1)\n",
                         log_n);
             std::fflush(stdout);
         }
