@@ -533,7 +533,8 @@ def test_graph_replays_equal_direct_launches():
 @pytest.mark.parametrize("case", [C2_SHAPE_N4096, SMALL_N8192, C4])
 def test_schedules_give_the_same_switch(case):
     """Every conversion kernel variant with and without the row pass fused into the inner product
-    gives the same batched switch, equal to the oracle."""
+    (and with it, the mod_down finish fused or not; mod_down's conversions on each variant of
+    their own) gives the same batched switch, equal to the oracle."""
     ctx, ref = make(case)
     L = len(ref.q) - 1
     key = ctx.ksk_uniform(L, seed=61)
@@ -550,6 +551,19 @@ def test_schedules_give_the_same_switch(case):
                 want = got
             for (wb, wa), (gb, ga) in zip(want, got):
                 assert (wb == gb).all() and (wa == ga).all(), (var, fuse)
+    ctx.set_tuning("conv_tg", 8)
+    ctx.set_tuning("conv_variant", 0)
+    ctx.set_tuning("fuse_ip", 1)
+    for fin in (0, 1):
+        for md in (0, 1, 2, 3):
+            ctx.set_tuning("fuse_fin", fin)
+            ctx.set_tuning("conv_variant_md", md)
+            assert ctx.tuning("fuse_fin") == fin and ctx.tuning("conv_variant_md") == md
+            got = [(kb.download(), ka.download()) for kb, ka in ctx.key_switch_batch(polys, key)]
+            for (wb, wa), (gb, ga) in zip(want, got):
+                assert (wb == gb).all() and (wa == ga).all(), ("fin", fin, "md", md)
+    ctx.set_tuning("fuse_fin", 1)
+    ctx.set_tuning("conv_variant_md", 1)
     ob, oa = ref.key_switch(polys[-1].download().ravel(), lv[-1], ref.uniform_key(L, 61), L)
     assert (want[-1][0] == ob).all() and (want[-1][1] == oa).all()
     # the small-batch schedule: the whole batch (and each single switch) under every choice
