@@ -418,6 +418,8 @@ def main():
                          "BSGS product; GPU and CPU oracle) as one JSON line and exit")
     ap.add_argument("--bsgs-only", action="store_true",
                     help="run only the hoisted BSGS product at N=2^15 (3 timed products, for traces)")
+    ap.add_argument("--single-only", action="store_true",
+                    help="run only single key switches at 24 limbs (N=2^16; 5 untimed + 40 timed calls, for traces)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the dominant-kernel launch and a device copy of the same size (for counter runs)")
     args = ap.parse_args()
@@ -438,6 +440,18 @@ def main():
         return
     q, p = ckh.params_primes(LOG_N, Q_BITS, P_BITS)
     ctx = ckh.Context(LOG_N, q, p, DNUM)
+    if args.single_only:
+        key = ctx.ksk_uniform(23, seed=SEED + 1)
+        d = ctx.uniform(23, seed=SEED + 2)
+        for _ in range(5):
+            ctx.key_switch(d, key)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(40):
+            ctx.key_switch(d, key)
+        ctx.synchronize()
+        print(json.dumps({"single_switch_ms_per_call": (time.perf_counter() - t0) / 40 * 1e3}), flush=True)
+        return
     stream = ctx.stream_handle()
     ev = HipEvents()
     if args.roofline_only:
