@@ -698,6 +698,53 @@ CtPtr Evaluator::mul_no_relin(const Ct& a, const Ct& b) {
     return r;
 }
 
+// mul_no_relin of several pairs in one launch: per output limb the product (two products summed
+// for the middle component) as a multiply-sum job; the same residues as mul_no_relin.
+std::vector<CtPtr> Evaluator::mul_no_relin_batch(const std::vector<std::pair<const Ct*, const Ct*>>& pairs) {
+    u64 total = 0;
+    for (const auto& pr : pairs) {
+        check_mul_operands(*pr.first, *pr.second);
+        total += 3 * (u64)(pr.first->level() + 1) * ctx.n;
+    }
+    std::vector<CtPtr> res;
+    if (pairs.empty()) return res;
+    BufPtr out = ctx.alloc(total);
+    std::vector<MulSumJob> jobs;
+    std::vector<MulSumTerm> terms;
+    u64 off = 0;
+    for (const auto& pr : pairs) {
+        const u32 level = pr.first->level();
+        const u64 L = level + 1, n = ctx.n;
+        const auto& x = pr.first->comps;
+        const auto& y = pr.second->comps;
+        for (const Poly* q : {x[0].get(), x[1].get(), y[0].get(), y[1].get()}) use(ctx, *q);
+        u64* d0 = out->ptr + off;
+        u64* d1 = d0 + L * n;
+        u64* d2 = d1 + L * n;
+        for (u32 i = 0; i < L; ++i) {
+            const u64 o = (u64)i * n;
+            jobs.push_back({d0 + o, (u32)terms.size(), 1, i, 0});
+            terms.push_back({x[0]->data() + o, y[0]->data() + o});
+            jobs.push_back({d1 + o, (u32)terms.size(), 2, i, 0});
+            terms.push_back({x[0]->data() + o, y[1]->data() + o});
+            terms.push_back({x[1]->data() + o, y[0]->data() + o});
+            jobs.push_back({d2 + o, (u32)terms.size(), 1, i, 0});
+            terms.push_back({x[1]->data() + o, y[1]->data() + o});
+        }
+        res.push_back(make_ct({view(out, off, level, 0, true), view(out, off + L * n, level, 0, true),
+                               view(out, off + 2 * L * n, level, 0, true)},
+                              pr.first->scale * pr.second->scale));
+        off += 3 * L * n;
+    }
+    Program prog;
+    const size_t oj = prog.add(jobs), ot = prog.add(terms);
+    prog.step(S_MUL_SUM, oj, (u32)jobs.size(), ot);
+    ctx.execute(prog, ctx.stream());
+    hip_check(hipGetLastError(), "tensor products");
+    count(&ckh_op_counter::ct_ct_tensor, pairs.size());
+    return res;
+}
+
 // Evaluator::relinearize (evaluator.rs:441-458): (c0 + kb, c1 + ka), the additions fused into
 // the mod_down finish.
 CtPtr Evaluator::relinearize(const Ct& ct) {
@@ -765,6 +812,41 @@ CtPtr Evaluator::rescale(const Ct& ct) {
     CtPtr r = make_ct(std::move(comps), ct.scale / (double)ctx.primes[l]);
     count(&ckh_op_counter::rescales);
     return r;
+}
+
+std::vector<CtPtr> Evaluator::rescale_batch(const std::vector<const Ct*>& cts) {
+    u64 out_words = 0, scratch_words = 0;
+    for (const Ct* ct : cts) {
+        check_basis(*ct);
+        const u32 l = ct->level();
+        if (l == 0) fail(CKH_ERR_NO_LEVELS_LEFT, "no levels left");
+        out_words += (u64)ct->comps.size() * l * ctx.n;
+        scratch_words += (u64)ct->comps.size() * (1 + l) * ctx.n;
+    }
+    std::vector<CtPtr> res;
+    if (cts.empty()) return res;
+    BufPtr out = ctx.alloc(out_words);
+    BufPtr scratch = ctx.alloc(scratch_words);
+    std::vector<RescaleTask> tasks;
+    u64 oo = 0, so = 0;
+    for (const Ct* ct : cts) {
+        const u32 l = ct->level();
+        const u64 L = l;  // output limbs
+        std::vector<PolyPtr> comps;
+        for (const auto& comp : ct->comps) {
+            use(ctx, *comp);
+            u64* x = scratch->ptr + so;
+            tasks.push_back({comp->data(), l, out->ptr + oo, x, x + ctx.n});
+            comps.push_back(view(out, oo, l - 1, 0, true));
+            oo += L * ctx.n;
+            so += (1 + L) * ctx.n;
+        }
+        res.push_back(make_ct(std::move(comps), ct->scale / (double)ctx.primes[l]));
+    }
+    ctx.rescale(tasks, ctx.stream());
+    hip_check(hipGetLastError(), "rescale");
+    count(&ckh_op_counter::rescales, cts.size());
+    return res;
 }
 
 CtPtr Evaluator::mod_switch_to_level(const Ct& ct, u32 level) {

@@ -93,10 +93,14 @@ class Evaluator {
     CtPtr add_plain(const Ct& a, const Poly& pt, double pt_scale);
     CtPtr mul_plain(const Ct& a, const Poly& pt, double pt_scale);
     CtPtr mul_no_relin(const Ct& a, const Ct& b);
+    // mul_no_relin of several pairs in one launch (the server's turns).
+    std::vector<CtPtr> mul_no_relin_batch(const std::vector<std::pair<const Ct*, const Ct*>>& pairs);
     CtPtr relinearize(const Ct& ct);
     std::vector<CtPtr> relinearize_batch(const std::vector<const Ct*>& cts);
     CtPtr mul(const Ct& a, const Ct& b);
     CtPtr rescale(const Ct& ct);
+    // rescale of several ciphertexts, at any levels, in one program (the server's turns).
+    std::vector<CtPtr> rescale_batch(const std::vector<const Ct*>& cts);
     CtPtr mod_switch_to_level(const Ct& ct, u32 level);
     CtPtr relabel(const Ct& ct, double scale);
     CtPtr rotate(const Ct& ct, int32_t steps);

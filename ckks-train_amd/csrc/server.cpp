@@ -67,7 +67,8 @@ void Server::fail_op(const OpPtr& op, ckh_status st, const std::string& msg) {
 }
 
 // Runs one scheduler turn's admitted requests: grouped by evaluator, relinearizations (also the
-// one inside mul+relin+rescale) batched, on the next stream of the evaluator's context.
+// one inside mul+relin+rescale), hoisted rotations, the products and the rescales of
+// mul+relin+rescale batched, on the next stream of the evaluator's context.
 void Server::launch(std::vector<OpPtr>& ops) {
     std::vector<Evaluator*> evs;
     for (auto& op : ops)
@@ -116,15 +117,24 @@ void Server::launch(std::vector<OpPtr>& ops) {
                 }
             }
         }
-        // mul_no_relin first (a failing product fails its request alone).
+        // mul_no_relin first, every pair in one launch (a failing product: one call each, so it
+        // fails its request alone).
         std::vector<CtPtr> prods;
         std::vector<OpPtr> mrr_ok;
-        for (auto& op : mrr) {
-            try {
-                prods.push_back(ev->mul_no_relin(*op->a, *op->b));
-                mrr_ok.push_back(op);
-            } catch (const Error& e) {
-                fail_op(op, e.status, e.msg);
+        try {
+            std::vector<std::pair<const Ct*, const Ct*>> pairs;
+            for (auto& op : mrr) pairs.push_back({op->a.get(), op->b.get()});
+            prods = ev->mul_no_relin_batch(pairs);
+            mrr_ok = mrr;
+        } catch (const Error&) {
+            prods.clear();
+            for (auto& op : mrr) {
+                try {
+                    prods.push_back(ev->mul_no_relin(*op->a, *op->b));
+                    mrr_ok.push_back(op);
+                } catch (const Error& e) {
+                    fail_op(op, e.status, e.msg);
+                }
             }
         }
         std::vector<const Ct*> batch;
@@ -151,15 +161,27 @@ void Server::launch(std::vector<OpPtr>& ops) {
                 }
             }
         }
+        // the rescales of the products, in one program (a failing one: one call each)
+        std::vector<const Ct*> to_rescale;
+        std::vector<size_t> idx;
         for (size_t i = 0; i < owners.size(); ++i) {
             if (!outs[i]) continue;
             if (owners[i]->kind == CKH_OP_RELINEARIZE) {
                 owners[i]->outs = {outs[i]};
             } else {
+                to_rescale.push_back(outs[i].get());
+                idx.push_back(i);
+            }
+        }
+        try {
+            auto rs = ev->rescale_batch(to_rescale);
+            for (size_t k = 0; k < idx.size(); ++k) owners[idx[k]]->outs = {rs[k]};
+        } catch (const Error&) {
+            for (size_t k = 0; k < idx.size(); ++k) {
                 try {
-                    owners[i]->outs = {ev->rescale(*outs[i])};
+                    owners[idx[k]]->outs = {ev->rescale(*to_rescale[k])};
                 } catch (const Error& e) {
-                    fail_op(owners[i], e.status, e.msg);
+                    fail_op(owners[idx[k]], e.status, e.msg);
                 }
             }
         }
