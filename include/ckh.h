@@ -390,7 +390,8 @@ ckh_status ckh_relinearize_batch(ckh_evaluator *ev, uint32_t n, const ckh_ct *co
  * keeps evaluators and their keys resident, admits requests in arrival order while the bytes
  * of the admitted work stay within hbm_budget_bytes, and batches whatever is admitted at each
  * turn: relinearizations of one evaluator as one batched key switch, mul+relin+rescale with its
- * relinearization batched likewise, hoisted rotations per request. Batches run round-robin on
+ * products in one launch, its relinearization batched likewise and its rescales in one program,
+ * hoisted rotations in groups of "hoisted_batch" requests. Batches run round-robin on
  * service streams of each context that no caller thread can select (tuning "service_streams"),
  * so work admitted while other batches run overlaps with them and callers may keep working on
  * their own streams meanwhile. Results are bit-identical to the same operations called alone. */
