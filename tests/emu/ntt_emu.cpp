@@ -263,7 +263,7 @@ std::vector<u64> oracle(const Limb& L, std::vector<u64> a, bool forward);
 
 // The fused row pass + inner product + mod_down finish of one limb (k_rows_ip_fin's body): ndig
 // digits (digit own_j given in NTT form, the others as column-pass outputs), key parts, the
-// converted limbs z_b / z_a as column-pass outputs, an addend for b only. Every word of every
+// converted limbs z_b / z_a as column-pass outputs, an addend for b (and, with pattern 1, for a). Every word of every
 // input is read exactly once and every output word written once; the outputs equal
 // ((sum_j X_j k_j) - NTT(z)) P^-1 (+ add) computed with the oracle's transforms.
 void replay_rows_ip_fin(const Limb& L, u32 ndig, u32 own_j, std::mt19937_64& rng, int pattern) {
@@ -272,8 +272,8 @@ void replay_rows_ip_fin(const Limb& L, u32 ndig, u32 own_j, std::mt19937_64& rng
     auto rnd = [&](std::vector<u64>& v) {
         for (auto& x : v) x = pattern == 1 ? q - 1 : rng() % q;
     };
-    std::vector<u64> mid((size_t)ndig * n), own(n), key((size_t)2 * ndig * n), zb_mid, za_mid, add(n), outb(n, 7),
-        outa(n, 7);
+    std::vector<u64> mid((size_t)ndig * n), own(n), key((size_t)2 * ndig * n), zb_mid, za_mid, add(n), add2(n),
+        outb(n, 7), outa(n, 7);
     std::vector<std::vector<u64>> X(ndig);
     for (u32 j = 0; j < ndig; ++j) {
         std::vector<u64> x(n);
@@ -294,6 +294,8 @@ void replay_rows_ip_fin(const Limb& L, u32 ndig, u32 own_j, std::mt19937_64& rng
     zb_mid = cols_only(L, zb);
     za_mid = cols_only(L, za);
     rnd(add);
+    rnd(add2);
+    const bool both = pattern == 1;  // an addend on both sums (relinearization), else on b only (rotation)
     const u64 inv = 1 + rng() % (q - 1), inv_s = (u64)(((unsigned __int128)inv << 64) / q);
     RowIpFinJob J{};
     J.ip = {mid.data(), n, own.data(), key.data(), n, nullptr, nullptr, ndig, 0, own_j, 0};
@@ -302,13 +304,13 @@ void replay_rows_ip_fin(const Limb& L, u32 ndig, u32 own_j, std::mt19937_64& rng
     J.out_b = outb.data();
     J.out_a = outa.data();
     J.add_b = add.data();
-    J.add_a = nullptr;
+    J.add_a = both ? add2.data() : nullptr;
     J.inv = inv;
     J.inv_s = inv_s;
     emu::Tracked tm(mid.data(), mid.size(), "digits"), to(own.data(), n, "own digit"), tk(key.data(), key.size(), "key"),
         tzb(zb_mid.data(), n, "z_b"), tza(za_mid.data(), n, "z_a"), tad(add.data(), n, "addend"),
-        tob(outb.data(), n, "out_b"), toa(outa.data(), n, "out_a");
-    emu::registry() = {&tm, &to, &tk, &tzb, &tza, &tad, &tob, &toa};
+        tad2(add2.data(), n, "addend a"), tob(outb.data(), n, "out_b"), toa(outa.data(), n, "out_a");
+    emu::registry() = {&tm, &to, &tk, &tzb, &tza, &tad, &tad2, &tob, &toa};
     emu::Lds lds(LDS_WORDS);
     emu::LdsView lv{&lds};
     const PrimeConst& pc = L.tab.pc[0];
@@ -340,6 +342,8 @@ void replay_rows_ip_fin(const Limb& L, u32 ndig, u32 own_j, std::mt19937_64& rng
     for (emu::Tracked* t : {&to, &tk, &tzb, &tza, &tad})
         for (size_t i = 0; i < t->len; ++i)
             if (t->reads[i] != 1 || t->writes[i] != 0) emu::die("an input word not read exactly once", (double)i);
+    for (size_t i = 0; i < n; ++i)
+        if (tad2.reads[i] != (both ? 1u : 0u) || tad2.writes[i] != 0) emu::die("an addend word not read once", (double)i);
     for (emu::Tracked* t : {&tob, &toa})
         for (size_t i = 0; i < t->len; ++i)
             if (t->writes[i] != 1 || t->reads[i] != 0) emu::die("an output word not written exactly once", (double)i);
@@ -351,7 +355,8 @@ void replay_rows_ip_fin(const Limb& L, u32 ndig, u32 own_j, std::mt19937_64& rng
             sbv = (sbv + mulm(X[j][i], key[(size_t)2 * j * n + i])) % q;
             sav = (sav + mulm(X[j][i], key[(size_t)(2 * j + 1) * n + i])) % q;
         }
-        const u64 wb = (mulm((sbv + q - Zb[i]) % q, inv) + add[i]) % q, wa = mulm((sav + q - Za[i]) % q, inv);
+        const u64 wb = (mulm((sbv + q - Zb[i]) % q, inv) + add[i]) % q,
+                  wa = (mulm((sav + q - Za[i]) % q, inv) + (both ? add2[i] : 0)) % q;
         if (outb[i] != wb || outa[i] != wa) emu::die("fused row pass + inner product + finish differs", (double)i);
     }
 }
