@@ -45,6 +45,10 @@ struct ConvTgt {
     double half[2][MAX_SRC];  // piece path: (Q / q_j) mod r in 25-bit halves (low, high)
     FTw p20, p40, p65;        // piece path: 2^20, 2^40, 2^65 mod r (and their quotients by r)
     double qmod_f, qmod_q, hmod_f, rf, rinv;
+    // The correction terms as addends: nq = -Q mod r (e nq = -e Q mod r), nh = -(Q - 1)/2 mod r;
+    // nq_c = nq centered in (-r/2, r/2]; nq_half / nh_half their 25-bit halves (r < 2^50).
+    u64 nq, nh;
+    double nq_c, nq_half[2], nh_half[2];
     // TGT_INT: r at least 2^50, 128-bit integer sums; TGT_FP: r and every source below 2^50, f64
     // products of the sources; TGT_SPLIT: r below 2^50 with larger sources, each source split
     // into 30-bit halves (or, variant 1, 20-bit pieces summed exactly).
@@ -125,6 +129,10 @@ CKH_DEV void conv_coeff(const ConvSrc* srcs, const ConvTgt* tgts, const PrimeCon
         }
     }
     const u64 e = P.exact ? (u64)f : 0;  // f >= 0: truncation is floor
+    // With at most 8 sources, e < 8: -e (Q mod r) and -(Q - 1)/2 enter the sums as addends (e nq_c
+    // and the half products are exact) instead of a modular product and two subtractions; every
+    // f64 sum stays an integer below 2^53, which fcanon reduces exactly.
+    const double ed = (double)e;
     const auto outp = glb(out_p);
     for (u32 t = t0; t < t1; ++t) {
         const auto* g = cst(tgts + P.tgt_first + t);
@@ -134,8 +142,10 @@ CKH_DEV void conv_coeff(const ConvSrc* srcs, const ConvTgt* tgts, const PrimeCon
             // S_ab = sum_j y_ja c_jb is exact (every product < 2^45, at most 7 terms), and
             // sum_j y_j c_j = S00 + (32 S01 + S10) 2^20 + (32 S11 + S20) 2^40 + S21 2^65, the two
             // bracketed sums staying below 2^53.
+            // -(Q - 1)/2 starts the two low sums and e (-Q mod r) is one more term of them (e < 7:
+            // both stay below 7 2^45 + 2^28, the bracketed sums below 2^53)
             const double r = g->rf, rinv = g->rinv;
-            double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0, s20 = 0.0, s21 = 0.0;
+            double s00 = g->nh_half[0], s01 = g->nh_half[1], s10 = 0.0, s11 = 0.0, s20 = 0.0, s21 = 0.0;
 #pragma unroll
             for (u32 j = 0; j < MAXS; ++j) {
                 const double c0 = g->half[0][j], c1 = g->half[1][j];
@@ -146,36 +156,40 @@ CKH_DEV void conv_coeff(const ConvSrc* srcs, const ConvTgt* tgts, const PrimeCon
                 s20 = __builtin_fma(yx[j], c0, s20);
                 s21 = __builtin_fma(yx[j], c1, s21);
             }
+            s00 = __builtin_fma(ed, g->nq_half[0], s00);
+            s01 = __builtin_fma(ed, g->nq_half[1], s01);
             const double t1 = __builtin_fma(s01, 32.0, s10), t2 = __builtin_fma(s11, 32.0, s20);
             double acc = s00 + fmulmod(t1, g->p20.w, g->p20.wq, r);
             acc = acc + fmulmod(t2, g->p40.w, g->p40.wq, r);
             acc = acc + fmulmod(s21, g->p65.w, g->p65.wq, r);
-            if (P.exact) acc = acc - fmulmod((double)e, g->qmod_f, g->qmod_q, r);
-            acc = acc - g->hmod_f;
             o = d2u(fcanon(acc, r, rinv));
         } else if (g->mode == TGT_SPLIT) {
             // y_j = yh 2^30 + yl: y_j c = yh (2^30 c) + yl c (mod r), four terms between reductions.
             const double r = g->rf, rinv = g->rinv;
-            double acc = 0.0;
+            double acc = -g->hmod_f;
 #pragma unroll
             for (u32 j = 0; j < MAXS; ++j) {
                 acc = acc + fmulmod(yd[j], g->qhat_f[j], g->qhat_q[j], r);
                 acc = acc + fmulmod(yh[j], g->qhat30_f[j], g->qhat30_q[j], r);
                 if (j & 1) acc = freduce(acc, r, rinv);
             }
-            if (P.exact) acc = acc - fmulmod((double)e, g->qmod_f, g->qmod_q, r);
-            acc = acc - g->hmod_f;
+            if constexpr (MAXS <= 8)
+                acc = __builtin_fma(ed, g->nq_c, acc);
+            else if (P.exact)
+                acc = acc - fmulmod(ed, g->qmod_f, g->qmod_q, r);
             o = d2u(fcanon(acc, r, rinv));
         } else if (g->mode == TGT_FP) {
             const double r = g->rf, rinv = g->rinv;
-            double acc = 0.0;
+            double acc = -g->hmod_f;
 #pragma unroll
             for (u32 j = 0; j < MAXS; ++j) {
                 acc = acc + fmulmod(yd[j], g->qhat_f[j], g->qhat_q[j], r);
                 if ((j & 3) == 3) acc = freduce(acc, r, rinv);
             }
-            if (P.exact) acc = acc - fmulmod((double)e, g->qmod_f, g->qmod_q, r);
-            acc = acc - g->hmod_f;
+            if constexpr (MAXS <= 8)
+                acc = __builtin_fma(ed, g->nq_c, acc);
+            else if (P.exact)
+                acc = acc - fmulmod(ed, g->qmod_f, g->qmod_q, r);
             o = d2u(fcanon(acc, r, rinv));
         } else {
             const PrimeConst C = pc_int(pcs + g->prime);
@@ -183,8 +197,9 @@ CKH_DEV void conv_coeff(const ConvSrc* srcs, const ConvTgt* tgts, const PrimeCon
                 // sum_j y_j c_j from 32-bit halves (y_j, c_j < 2^60): y c = y0 c0 + (y0 c1 +
                 // y1 c0) 2^32 + y1 c1 2^64 with y1, c1 < 2^28, so the middle sum stays below
                 // 7 * 2^61 < 2^64 and the high one below 2^59; one 32 x 32 -> 64 multiply per
-                // partial product instead of full 64 x 64 products. The same exact sum.
-                u64 l = 0, m = 0, h = 0;
+                // partial product instead of full 64 x 64 products. The same exact sum, which
+                // starts at -(Q - 1)/2 mod r and takes e (-Q mod r) as one more term (e < 7).
+                u64 l = g->nh, m = 0, h = 0;
                 u32 carry = 0;
 #pragma unroll
                 for (u32 j = 0; j < MAXS; ++j) {
@@ -196,6 +211,12 @@ CKH_DEV void conv_coeff(const ConvSrc* srcs, const ConvTgt* tgts, const PrimeCon
                     m += (u64)y0 * c1 + (u64)y1 * c0;
                     h += (u64)y1 * c1;
                 }
+                {
+                    const u64 p = (u64)(u32)e * (u32)g->nq;
+                    l += p;
+                    carry += l < p ? 1u : 0u;
+                    m += (u64)(u32)e * (u32)(g->nq >> 32);
+                }
                 const u64 lo = l + (m << 32);
                 const u64 hi = h + carry + (m >> 32) + (lo < l ? 1 : 0);
                 o = reduce128(hi, lo, C);
@@ -206,9 +227,9 @@ CKH_DEV void conv_coeff(const ConvSrc* srcs, const ConvTgt* tgts, const PrimeCon
                 u64 hi, lo;
                 acc.get(hi, lo);
                 o = reduce128(hi, lo, C);
+                if (P.exact) o = sub_mod(o, shoup(e, g->qmod, g->qmod_s, C.q), C.q);
+                o = sub_mod(o, g->hmod, C.q);
             }
-            if (P.exact) o = sub_mod(o, shoup(e, g->qmod, g->qmod_s, C.q), C.q);
-            o = sub_mod(o, g->hmod, C.q);
         }
         outp[(u64)g->limb * n + c] = o;
     }
@@ -229,8 +250,8 @@ struct MfPlan {
     u32 t_off;    // first MfTgt of the plan (4 per group)
 };
 struct MfTgt {
-    double rf, rinv, c32_f, c32_q, bias_f, qmod_f, qmod_q;  // f64 path
-    u64 q, one_s, two64, two64_s, qmod, qmod_s, corr;       // integer path
+    double rf, rinv, c32_f, c32_q, bias_f, nq_c;  // f64 path (nq_c: -Q mod r, centered)
+    u64 q, one_s, two64, two64_s, nq, corr;       // integer path (nq: -Q mod r)
     u32 limb;  // output limb; ~0u = padding
     u32 fp;    // 1: f64 recombination (r < 2^50)
 };
@@ -340,36 +361,38 @@ CKH_DEV void conv_mf_block(const ConvSrc* srcs, const ConvPlan& P, const MfPlan&
             if (limb == ~0u) continue;
             const bool fp = tg.fp;
             const double r = tg.rf, rinv = tg.rinv, c32_f = tg.c32_f, c32_q = tg.c32_q, bias_f = tg.bias_f;
-            const double qmod_f = tg.qmod_f, qmod_q = tg.qmod_q;
+            const double nq_c = tg.nq_c;
 #pragma unroll
             for (u32 nt = 0; nt < 2; ++nt) {
                 const v16i& d = nt ? d1 : d0;
-                // X = lo + hi 2^32, lo / hi = sum of four digit planes each (|.| < 2^45, exact)
-                double lo = (double)d[8 * uu + 3];
-                lo = __builtin_fma(lo, 256.0, (double)d[8 * uu + 2]);
-                lo = __builtin_fma(lo, 256.0, (double)d[8 * uu + 1]);
-                lo = __builtin_fma(lo, 256.0, (double)d[8 * uu + 0]);
-                double hi = (double)d[8 * uu + 7];
-                hi = __builtin_fma(hi, 256.0, (double)d[8 * uu + 6]);
-                hi = __builtin_fma(hi, 256.0, (double)d[8 * uu + 5]);
-                hi = __builtin_fma(hi, 256.0, (double)d[8 * uu + 4]);
+                // X = lo + hi 2^32, lo / hi = sum of four digit planes each (|.| < 2^45, exact);
+                // planes p and p + 1 combine in int32 first (|d| < 2^20: |d_p + 256 d_(p+1)| < 2^29)
+                const int l01 = d[8 * uu + 0] + d[8 * uu + 1] * 256, l23 = d[8 * uu + 2] + d[8 * uu + 3] * 256;
+                const int h01 = d[8 * uu + 4] + d[8 * uu + 5] * 256, h23 = d[8 * uu + 6] + d[8 * uu + 7] * 256;
+                const double lo = __builtin_fma((double)l23, 65536.0, (double)l01);
+                const double hi = __builtin_fma((double)h23, 65536.0, (double)h01);
                 u64 o;
                 if (fp) {
+                    // e < 8: e nq_c is exact and the sum stays an integer below 2^53
                     double acc = lo + fmulmod(hi, c32_f, c32_q, r) + bias_f;
-                    if (P.exact) acc = acc - fmulmod((double)e[nt], qmod_f, qmod_q, r);
+                    acc = __builtin_fma((double)e[nt], nq_c, acc);
                     o = d2u(fcanon(acc, r, rinv));
                 } else {
-                    // X + 2^45 + 2^77 = (lo + 2^45) + (hi + 2^45) 2^32 >= 0, below 2^79
+                    // X + 2^45 + 2^77 = (lo + 2^45) + (hi + 2^45) 2^32 >= 0, below 2^79; e nq (below
+                    // 2^63) and corr (below 2^60) join the 128-bit value before its one reduction
                     const u64 lu = d2u(lo + 0x1p45), hu = d2u(hi + 0x1p45);
-                    const u64 xl = lu + (hu << 32);
-                    const u64 xh = (hu >> 32) + (xl < lu ? 1 : 0);
+                    const u32 ee = (u32)e[nt];
+                    const u64 add = (u64)ee * (u32)tg.nq + (((u64)ee * (u32)(tg.nq >> 32)) << 32) + tg.corr;
+                    const u64 xl0 = lu + (hu << 32);
+                    const u64 xh0 = (hu >> 32) + (xl0 < lu ? 1 : 0);
+                    const u64 xl = xl0 + add;
+                    const u64 xh = xh0 + (xl < add ? 1 : 0);
                     PrimeConst C{};
                     C.q = tg.q;
                     C.one_s = tg.one_s;
                     C.two64 = tg.two64;
                     C.two64_s = tg.two64_s;
-                    o = add_mod(reduce128(xh, xl, C), tg.corr, C.q);
-                    if (P.exact) o = sub_mod(o, shoup(e[nt], tg.qmod, tg.qmod_s, C.q), C.q);
+                    o = reduce128(xh, xl, C);
                 }
                 outp[(u64)limb * n + c[nt]] = o;
             }

@@ -70,6 +70,13 @@ struct HostConvTables {
                 t.qmod_f = (double)t.qmod;
                 t.qmod_q = (double)t.qmod / (double)r;
                 t.hmod_f = (double)t.hmod;
+                t.nq = (r - t.qmod % r) % r;
+                t.nh = (r - t.hmod % r) % r;
+                t.nq_c = t.nq > r / 2 ? -(double)(r - t.nq) : (double)t.nq;
+                for (u32 b = 0; b < 2; ++b) {
+                    t.nq_half[b] = (double)((t.nq >> (25 * b)) & ((1u << 25) - 1));
+                    t.nh_half[b] = (double)((t.nh >> (25 * b)) & ((1u << 25) - 1));
+                }
                 if (r < FP_PRIME_BOUND) {
                     for (auto [sh, dst] : {std::pair<u32, FTw*>{20, &t.p20}, {40, &t.p40}, {65, &t.p65}}) {
                         const u64 v = hm::pow(2, sh, r);
@@ -181,14 +188,12 @@ struct HostConvTables {
                         T.c32_f = (double)c32;
                         T.c32_q = (double)c32 / (double)r;
                         T.bias_f = (double)((bias + r - ct.hmod % r) % r);
-                        T.qmod_f = ct.qmod_f;
-                        T.qmod_q = ct.qmod_q;
+                        T.nq_c = ct.nq_c;
                         T.q = r;
                         T.one_s = hm::shoup(1, r);
                         T.two64 = (u64)(((hm::u128)1 << 64) % r);
                         T.two64_s = hm::shoup(T.two64, r);
-                        T.qmod = ct.qmod;
-                        T.qmod_s = ct.qmod_s;
+                        T.nq = ct.nq;
                         // integer path adds 2^45 + 2^77 to make the product non-negative
                         const u64 off = (u64)((((hm::u128)1 << 45) + ((hm::u128)1 << 77)) % r);
                         T.corr = (u64)(((hm::u128)bias + 2 * (hm::u128)r - ct.hmod % r - off) % r);

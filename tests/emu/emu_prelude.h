@@ -81,9 +81,11 @@ inline void check_fmulmod(double y, double w, double wq, double q, double t) {
 
 inline void check_freduce(double x, double q, double r) {
     if (!is_int(x) || !is_int(r)) die("freduce operand or result not an integer", x, r);
-    if (std::fabs(x) >= 0x1p52) die("freduce |x| not below 2^52", x, q);
+    // x an integer below 2^53: RN(x RN(1/q)) misses x / q by at most 2 |x| 2^-53 / q < 2 / q,
+    // so k = rint(.) is within 1/2 + 2/q of x / q and |x - k q| <= q/2 + 2 (computed exactly)
+    if (std::fabs(x) >= 0x1p53) die("freduce |x| not below 2^53", x, q);
     if ((i128(x) - i128(r)) % (__int128)(long long)q != 0) die("freduce result not congruent", x, r, q);
-    if (std::fabs(r) > 0.5 * q + 1.0) die("freduce result above q/2", x, r, q);
+    if (std::fabs(r) > 0.5 * q + 2.0) die("freduce result above q/2", x, r, q);
     auto& s = stats();
     s.note(s.fred_in, std::fabs(x) / q);
     std::lock_guard<std::mutex> lk(s.mu);
