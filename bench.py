@@ -412,7 +412,7 @@ def main():
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched step (warmup + steps, no autotune) and print its time (for counter runs)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
-                    help="with --batched-only: force a tuning value (repeatable)")
+                    help="with --batched-only or --single-only: force a tuning value (repeatable)")
     ap.add_argument("--configs", action="store_true",
                     help="print the per-configuration figures (NTT round trip, one key switch, the hoisted "
                          "BSGS product; GPU and CPU oracle) as one JSON line and exit")
@@ -441,6 +441,9 @@ def main():
     q, p = ckh.params_primes(LOG_N, Q_BITS, P_BITS)
     ctx = ckh.Context(LOG_N, q, p, DNUM)
     if args.single_only:
+        for kv in args.tune:
+            k, v = kv.split("=")
+            ctx.set_tuning(k, int(v))
         key = ctx.ksk_uniform(23, seed=SEED + 1)
         d = ctx.uniform(23, seed=SEED + 2)
         for _ in range(5):

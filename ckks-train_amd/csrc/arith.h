@@ -106,6 +106,17 @@ struct PrimeConst {
     FTw n_inv_f, w_n_inv_f; // the last inverse stage constants, f64 path
 };
 
+// The last stage of an inverse NTT whose output limb only feeds a base conversion, folded with
+// the conversion's first step y = [x + h]_q (Q / q)^-1 mod q (x the transform's output, h = q >> 1,
+// Q the conversion's source modulus): n^-1 (Q / q)^-1 and psi^-1 n^-1 (Q / q)^-1 mod q replace
+// the stage's constants and h (Q / q)^-1 mod q is added, so the limb holds y ("y form").
+struct YForm {
+    u64 n, n_s, wn, wn_s;  // integer path: the two constants and their Shoup companions
+    u64 h;                 // h (Q / q)^-1 mod q
+    FTw n_f, wn_f;         // f64 path: the constants and their quotients by q
+    double h_f;
+};
+
 // Shoup product in [0, 2q) for any u64 a (w < q fixed, ws = floor(w 2^64 / q)).
 CKH_DEV u64 shoup_lazy(u64 a, u64 w, u64 ws, u64 q) {
     const u64 r = a * w - mulhi(a, ws) * q;

@@ -52,6 +52,8 @@ void Context::set_tuning(const std::string& key, int64_t value) {
         fuse_ip = value != 0;
     else if (key == "fuse_fin")
         fuse_fin = value != 0;
+    else if (key == "conv_yform")
+        conv_yform = value != 0;
     else if (key == "small_limbs")
         small_limbs = (u32)std::min<int64_t>(value, 1u << 30);
     else if (key == "conv_variant_small")
@@ -95,6 +97,7 @@ int64_t Context::get_tuning(const std::string& key) const {
     if (key == "conv_variant_md") return conv_variant_md;
     if (key == "fuse_ip") return fuse_ip;
     if (key == "fuse_fin") return fuse_fin;
+    if (key == "conv_yform") return conv_yform;
     if (key == "small_limbs") return small_limbs;
     if (key == "conv_variant_small") return conv_variant_s;
     if (key == "conv_tg_small") return conv_tg_s;

@@ -15,6 +15,7 @@ struct MfTables {
 // Conversion tables (device pointers).
 struct ConvTables {
     const ConvSrc* src;
+    const YForm* yf;  // the YForm of every source (same index as src)
     const ConvTgt* tgt;
     const ConvPlan* plans;
     const PrimeConst* pc;

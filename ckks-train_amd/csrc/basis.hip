@@ -17,7 +17,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvTables T, const ConvJob* __res
     const ConvPlan P = T.plans[job.plan];
     const u32 t0 = blockIdx.z * tg;
     if (t0 >= P.ntgt || c >= n) return;
-    conv_coeff<MAXS, PIECES>(T.src, T.tgt, T.pc, P, job.in, job.out, n, c, t0, min(t0 + tg, P.ntgt));
+    conv_coeff<MAXS, PIECES>(T.src, T.tgt, T.pc, P, job.in, job.out, n, c, t0, min(t0 + tg, P.ntgt), job.yform != 0);
 }
 
 // ---------------------------------------------------------------- matrix-core conversion
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(256) void k_conv_mf(ConvTables T, const ConvJob* __
     const ConvPlan P = T.plans[job.plan];
     const MfPlan F = T.mf.plans[job.plan];
     conv_mf_block<KB, MfWave>(T.src, P, F, T.mf.a, T.mf.tgt, job.in, job.out, 1u << T.log_n, blockIdx.x, threadIdx.x,
-                              a_lds, t_lds);
+                              a_lds, t_lds, job.yform != 0);
 }
 
 template <u32 MAXS>

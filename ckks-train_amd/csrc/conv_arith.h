@@ -30,7 +30,8 @@ struct ConvSrc {
     double inv;          // 1.0 / f64(q_j), round to nearest (host computed)
     double qf;           // q_j as f64 (f64 path)
     double qhinv_f, qhinv_q;  // (Q / q_j)^-1 and its quotient by q_j, f64 path
-    u32 fp, pad;              // q_j < 2^50
+    u32 fp;                   // q_j < 2^50
+    u32 prime;                // index into the context's primes
 };
 
 // One target prime r of a conversion.
@@ -64,11 +65,13 @@ struct ConvPlan {
     u32 src_first, nsrc, tgt_first, ntgt, src_limb0, exact;
 };
 
-// One launch row: plan `plan`, input polynomial (its limb 0) at `in`, output at `out`.
+// One launch row: plan `plan`, input polynomial (its limb 0) at `in`, output at `out`. yform: the
+// input limbs already hold y_j = [x_j + h_j]_{q_j} (Q/q_j)^-1 mod q_j (YForm: an inverse NTT
+// computed it in its last stage).
 struct ConvJob {
     const u64* in;
     u64* out;
-    u32 plan, pad;
+    u32 plan, yform;
 };
 
 // Round-to-nearest u64 -> f64: both halves convert exactly and the sum rounds once.
@@ -85,7 +88,7 @@ CKH_DEV double u64_to_f64_rn(u64 v) {
 // reduce once.
 template <u32 MAXS, bool PIECES>
 CKH_DEV void conv_coeff(const ConvSrc* srcs, const ConvTgt* tgts, const PrimeConst* pcs, const ConvPlan& P,
-                        const u64* in_p, u64* out_p, u32 n, u32 c, u32 t0, u32 t1) {
+                        const u64* in_p, u64* out_p, u32 n, u32 c, u32 t0, u32 t1, bool yform) {
     const auto inp = glb(in_p);
     u64 y[MAXS];
     // f64 source: yd = y, yh = 0; integer source: yd / yh = its low / high 30 bits, or (PIECES)
@@ -105,14 +108,19 @@ CKH_DEV void conv_coeff(const ConvSrc* srcs, const ConvTgt* tgts, const PrimeCon
             const ConvSrc* s = srcs + P.src_first + j;
             const u64 x = inp[(u64)(P.src_limb0 + j) * n + c];
             if (s->fp) {
-                double sv = u2d(x) + (double)s->h;
-                sv = sv >= s->qf ? sv - s->qf : sv;
-                double v = fmulmod(sv, s->qhinv_f, s->qhinv_q, s->qf);
-                v = v < 0.0 ? v + s->qf : v;
-                yd[j] = v;
-                y[j] = d2u(v);
+                if (yform) {
+                    yd[j] = u2d(x);
+                    y[j] = x;
+                } else {
+                    double sv = u2d(x) + (double)s->h;
+                    sv = sv >= s->qf ? sv - s->qf : sv;
+                    double v = fmulmod(sv, s->qhinv_f, s->qhinv_q, s->qf);
+                    v = v < 0.0 ? v + s->qf : v;
+                    yd[j] = v;
+                    y[j] = d2u(v);
+                }
             } else {
-                y[j] = shoup(add_mod(x, s->h, s->q), s->qhinv, s->qhinv_s, s->q);
+                y[j] = yform ? x : shoup(add_mod(x, s->h, s->q), s->qhinv, s->qhinv_s, s->q);
                 if (srcs_int) {
                     yd[j] = (double)(u32)(y[j] & ((1u << 20) - 1));
                     yh[j] = (double)(u32)((y[j] >> 20) & ((1u << 20) - 1));
@@ -271,7 +279,7 @@ struct MfTgt {
 template <u32 KB, class W>
 CKH_DEV void conv_mf_block(const ConvSrc* srcs, const ConvPlan& P, const MfPlan& F, const typename W::v4i* a_tiles,
                            const MfTgt* tgts, const u64* in_p, u64* out_p, u32 n, u32 bx, u32 tid,
-                           typename W::v4i* a_lds, MfTgt* t_lds) {
+                           typename W::v4i* a_lds, MfTgt* t_lds, bool yform) {
     typedef typename W::v4i v4i;
     typedef typename W::v16i v16i;
     {
@@ -305,7 +313,9 @@ CKH_DEV void conv_mf_block(const ConvSrc* srcs, const ConvPlan& P, const MfPlan&
             if (j < P.nsrc) {
                 const auto* sp = cst(srcs + P.src_first + j);
                 const u64 x = inp[(u64)(P.src_limb0 + j) * n + c[nt]];
-                if (sp->fp) {
+                if (yform) {
+                    y[q] = x;
+                } else if (sp->fp) {
                     double sv = u2d(x) + (double)sp->h;
                     sv = sv >= sp->qf ? sv - sp->qf : sv;
                     double v = fmulmod(sv, sp->qhinv_f, sp->qhinv_q, sp->qf);

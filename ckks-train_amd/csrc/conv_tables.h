@@ -45,6 +45,7 @@ struct HostConvTables {
             cs.qhinv_s = hm::shoup(cs.qhinv, q);
             cs.inv = 1.0 / (double)q;
             cs.fp = q < FP_PRIME_BOUND;
+            cs.prime = src[j];
             cs.qf = (double)q;
             cs.qhinv_f = (double)cs.qhinv;
             cs.qhinv_q = (double)cs.qhinv / (double)q;
@@ -101,6 +102,25 @@ struct HostConvTables {
         pl.ntgt = (u32)tg.size();
         plans.push_back(pl);
         return (u32)plans.size() - 1;
+    }
+
+    // The YForm of every source (same index as srcs), from each prime's inverse-NTT constants.
+    std::vector<YForm> yform(const std::vector<PrimeConst>& pc) const {
+        std::vector<YForm> out(srcs.size());
+        for (size_t i = 0; i < srcs.size(); ++i) {
+            const ConvSrc& cs = srcs[i];
+            const u64 q = cs.q;
+            YForm& y = out[i];
+            y.n = hm::mul(pc[cs.prime].n_inv, cs.qhinv, q);
+            y.wn = hm::mul(pc[cs.prime].w_n_inv, cs.qhinv, q);
+            y.h = hm::mul(cs.h, cs.qhinv, q);
+            y.n_s = hm::shoup(y.n, q);
+            y.wn_s = hm::shoup(y.wn, q);
+            y.n_f = {(double)y.n, (double)y.n / (double)q};
+            y.wn_f = {(double)y.wn, (double)y.wn / (double)q};
+            y.h_f = (double)y.h;
+        }
+        return out;
     }
 
     // Balanced base-256 digits of v < 2^63: v = sum_p d_p 256^p, d_p in [-128, 127], 8 digits.

@@ -51,7 +51,9 @@ void ntt_fast_forward_fused(const NttTables& t, const LimbRef* refs, u32 nrefs, 
 void ntt_fast_inverse_fused(const NttTables& t, const LimbRef* refs, u32 nrefs, u32* ctl, const FusedCfg& f,
                             hipStream_t stream);
 void ntt_fast_forward(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream);
-void ntt_fast_inverse(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream);
+// yf non-null: every ref leaves in the y form of the conversion source yf[ref.aux] (YForm).
+void ntt_fast_inverse(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t stream,
+                      const YForm* yf = nullptr);
 // Forward NTT whose row pass ends with fin[r]: out = (a - NTT(src)) * inv (+ add) instead of
 // storing the transform (fin is a FinishJob array, one per ref).
 void ntt_fast_forward_finish(const NttTables& t, const LimbRef* refs, const FinishJob* fin, u32 nrefs,

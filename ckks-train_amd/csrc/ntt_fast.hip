@@ -91,13 +91,33 @@ __global__ __launch_bounds__(NT) void k_inv_rows(const LimbRef* __restrict__ ref
     CKH_AR_DISPATCH((inv_rows<M>(ar, ref.src, ref.dst, lds, log_n, tile)))
 }
 
-template <u32 M>
+// YF: every limb leaves in the y form of the conversion source yf[ref.aux] (YForm).
+template <u32 M, bool YF>
 __global__ __launch_bounds__(NT) void k_inv_cols(const LimbRef* __restrict__ refs, const PrimeConst* __restrict__ pcs,
                                                  const Tw* __restrict__ tw, const FTw* __restrict__ twf,
                                                  const FTw* __restrict__ rt, u32 log_n,
-                                                 u32 nrefs) {
+                                                 u32 nrefs, const YForm* __restrict__ yf) {
     __shared__ u64 lds[LDS_WORDS];
-    CKH_AR_DISPATCH((inv_cols<M>(ar, ref.dst, lds, log_n, tile)))
+    if constexpr (!YF) {
+        CKH_AR_DISPATCH((inv_cols<M>(ar, ref.dst, lds, log_n, tile)))
+    } else {
+        u32 ref_i, tile;
+        xcd_map(blockIdx.x, 1u << (log_n - TL), nrefs, ref_i, tile);
+        const LimbRef ref = refs[ref_i];
+        const PrimeConst* pc = pcs + ref.prime;
+        const auto* y = cst(yf + ref.aux);
+        const u64 toff = (u64)ref.prime << log_n;
+        if (pc->q < FP_PRIME_BOUND) {
+            const FTw nf{y->n_f.w, y->n_f.wq}, wnf{y->wn_f.w, y->wn_f.wq};
+            FpAr ar{twf + toff, pc->qf, pc->qinv, nf, wnf, rt + ((u64)ref.prime << (log_n - log_n / 2)) * ROW_FACTOR_SLOTS};
+            ar.add = y->h_f;
+            inv_cols<M, FpAr, false, true>(ar, ref.dst, lds, log_n, tile);
+        } else {
+            IntAr ar{tw + toff, pc->q, y->n, y->n_s, y->wn, y->wn_s};
+            ar.add = y->h;
+            inv_cols<M, IntAr, false, true>(ar, ref.dst, lds, log_n, tile);
+        }
+    }
 }
 
 // ---------------------------------------------------------------- one-launch forward NTT
@@ -301,7 +321,7 @@ bool ntt_fast_supported(u32 log_n) { return log_n >= 12 && log_n <= 16; }
 
 template <u32 M>
 static void launch_one(bool fwd, bool cols, const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s,
-                       const FinishJob* fins = nullptr) {
+                       const FinishJob* fins = nullptr, const YForm* yf = nullptr) {
     dim3 grid((1u << (t.log_n - fast::TL)) * nrefs);
     if (fwd && cols)
         hipLaunchKernelGGL(fast::k_fwd_cols<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.rf, t.log_n, nrefs);
@@ -311,8 +331,12 @@ static void launch_one(bool fwd, bool cols, const NttTables& t, const LimbRef* r
     else if (fwd)
         hipLaunchKernelGGL((fast::k_fwd_rows<M, false>), grid, dim3(fast::NT), 0, s, refs, t.pc, t.fwd, t.fwd_f, t.rf,
                            t.log_n, nrefs, fins);
+    else if (cols && yf)
+        hipLaunchKernelGGL((fast::k_inv_cols<M, true>), grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.ri,
+                           t.log_n, nrefs, yf);
     else if (cols)
-        hipLaunchKernelGGL(fast::k_inv_cols<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.ri, t.log_n, nrefs);
+        hipLaunchKernelGGL((fast::k_inv_cols<M, false>), grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.ri,
+                           t.log_n, nrefs, yf);
     else
         hipLaunchKernelGGL(fast::k_inv_rows<M>, grid, dim3(fast::NT), 0, s, refs, t.pc, t.inv, t.inv_f, t.ri, t.log_n,
                            nrefs);
@@ -398,11 +422,11 @@ u32 xcc_layout_check(hipStream_t s) {
 }
 
 static void launch_m(u32 m, bool fwd, bool cols, const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s,
-                     const FinishJob* fins = nullptr) {
+                     const FinishJob* fins = nullptr, const YForm* yf = nullptr) {
     switch (m) {
-        case 6: launch_one<6>(fwd, cols, t, refs, nrefs, s, fins); break;
-        case 7: launch_one<7>(fwd, cols, t, refs, nrefs, s, fins); break;
-        default: launch_one<8>(fwd, cols, t, refs, nrefs, s, fins); break;
+        case 6: launch_one<6>(fwd, cols, t, refs, nrefs, s, fins, yf); break;
+        case 7: launch_one<7>(fwd, cols, t, refs, nrefs, s, fins, yf); break;
+        default: launch_one<8>(fwd, cols, t, refs, nrefs, s, fins, yf); break;
     }
 }
 
@@ -448,10 +472,10 @@ void rows_ip_launch(const NttTables& t, const RowIpJob* jobs, u32 njobs, hipStre
     }
 }
 
-void ntt_fast_inverse(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s) {
+void ntt_fast_inverse(const NttTables& t, const LimbRef* refs, u32 nrefs, hipStream_t s, const YForm* yf) {
     const u32 B = t.log_n / 2, A = t.log_n - B;
     launch_m(B, false, false, t, refs, nrefs, s);
-    launch_m(A, false, true, t, refs, nrefs, s);
+    launch_m(A, false, true, t, refs, nrefs, s, nullptr, yf);
 }
 
 }  // namespace ckh

@@ -48,7 +48,8 @@ struct FpAr {
     const FTw* tab;
     double q, qinv;
     FTw ninv, wninv;
-    const FTw* rt;  // this prime's row factors, [row][ROW_FACTOR_SLOTS]
+    const FTw* rt;    // this prime's row factors, [row][ROW_FACTOR_SLOTS]
+    double add = 0.0; // added to the outputs of gs_final<true> (an integer below q)
     CKH_DEV V from_canon(u64 x) const { return u2d(x); }
     CKH_DEV V from_mid(u64 x) const { return bits_f64(x); }
     CKH_DEV u64 to_mid(V v) const { return f64_bits(v); }
@@ -97,10 +98,17 @@ struct FpAr {
         x = freduce(s, q, qinv);
         y = fmulmod(d, w.w, w.wq, q);
     }
+    // ADD: plus `add` (the products are below 0.7 q in magnitude, the sums below 1.7 q)
+    template <bool ADD = false>
     CKH_DEV void gs_final(V& x, V& y) const {
         const double s = x + y, d = x - y;
-        x = fcanon(fmulmod(s, ninv.w, ninv.wq, q), q, qinv);
-        y = fcanon(fmulmod(d, wninv.w, wninv.wq, q), q, qinv);
+        double a = fmulmod(s, ninv.w, ninv.wq, q), b = fmulmod(d, wninv.w, wninv.wq, q);
+        if (ADD) {
+            a = a + add;
+            b = b + add;
+        }
+        x = fcanon(a, q, qinv);
+        y = fcanon(b, q, qinv);
     }
     CKH_DEV u64 canon_out(V v) const { return d2u(v); }
 };
@@ -113,6 +121,7 @@ struct IntAr {
     const Tw* tab;
     u64 q;
     u64 n_inv, n_inv_s, w_n_inv, w_n_inv_s;
+    u64 add = 0;  // added to the outputs of gs_final<true> (below q)
     CKH_DEV V from_canon(u64 x) const { return x; }
     CKH_DEV V from_mid(u64 x) const { return x; }
     CKH_DEV u64 to_mid(V v) const { return v; }
@@ -141,10 +150,17 @@ struct IntAr {
         x = s >= two_q ? s - two_q : s;
         y = shoup_lazy(d, w.w, w.ws, q);
     }
+    // ADD: plus `add` (the lazy products are below 2 q, so the sums below 3 q)
+    template <bool ADD = false>
     CKH_DEV void gs_final(V& x, V& y) const {
         const u64 s = x + y, d = x + 2 * q - y;
-        x = csub(shoup_lazy(s, n_inv, n_inv_s, q), q);
-        y = csub(shoup_lazy(d, w_n_inv, w_n_inv_s, q), q);
+        if (ADD) {
+            x = csub(csub(shoup_lazy(s, n_inv, n_inv_s, q) + add, 2 * q), q);
+            y = csub(csub(shoup_lazy(d, w_n_inv, w_n_inv_s, q) + add, 2 * q), q);
+        } else {
+            x = csub(shoup_lazy(s, n_inv, n_inv_s, q), q);
+            y = csub(shoup_lazy(d, w_n_inv, w_n_inv_s, q), q);
+        }
     }
     CKH_DEV u64 canon_out(V v) const { return v; }
 };
@@ -405,7 +421,8 @@ CKH_DEV void inv_rows(const Ar& ar, const u64* __restrict__ in_p, u64* __restric
     for (u32 k = 0; k < 16; ++k) out[base + o + (k << R2)] = ar.to_mid(v[k]);
 }
 
-template <u32 M, class Ar, bool COH = false, class Lds>
+// ADD: the last stage adds ar.add to every output (the y form of a conversion's source limb).
+template <u32 M, class Ar, bool COH = false, bool ADD = false, class Lds>
 CKH_DEV void inv_cols(const Ar& ar, u64* __restrict__ io_p, Lds lds, u32 log_n, u32 tile, u32 tid = threadIdx.x) {
     const auto io = glb(io_p);
     constexpr u32 R2 = M - 4, U = 4 - R2, TCL = TL - M, TC = 1u << TCL, JM = (1u << R2) - 1;
@@ -443,7 +460,7 @@ CKH_DEV void inv_cols(const Ar& ar, u64* __restrict__ io_p, Lds lds, u32 log_n, 
         for (u32 k = 0; k < 16; ++k) {
             if (k & d) continue;
             if (u == M - 1) {
-                ar.gs_final(v[k], v[k + d]);
+                ar.template gs_final<ADD>(v[k], v[k + d]);
             } else {
                 const u32 r = o + (k << R2);
                 ar.gs(v[k], v[k + d], ar.tw((1u << (log_n - 1 - B - u)) + (r >> (u + 1))));

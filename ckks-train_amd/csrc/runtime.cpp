@@ -527,6 +527,9 @@ void Context::build_tables() {
         }
     }
     conv.src = dev_copy(ct.srcs);
+    conv.yf = dev_copy(ct.yform(t.pc));
+    conv_src_first_.clear();
+    for (const ConvPlan& pl : plans) conv_src_first_.push_back(pl.src_first);
     conv.tgt = dev_copy(ct.tgts);
     conv.plans = dev_copy(plans);
     conv.pc = d_pc;
@@ -548,6 +551,14 @@ void Context::build_mf_tables(HostConvTables& ct) {
 namespace {
 constexpr u32 MAX_ROWS = 65535;
 }  // namespace
+
+// The inverse NTT of refs whose aux names a conversion source: two-pass register-radix kernels,
+// the outputs in that source's y form.
+static void launch_ntt_y(const Context& c, const LimbRef* d_refs, u32 nrefs, hipStream_t s) {
+    u32 chunk = c.ntt_chunk_inv;
+    chunk = (chunk == 0 || chunk > MAX_ROWS) ? MAX_ROWS : chunk;
+    for (u32 i = 0; i < nrefs; i += chunk) ntt_fast_inverse(c.ntt, d_refs + i, std::min(chunk, nrefs - i), s, c.conv.yf);
+}
 
 void launch_ntt(const Context& c, const LimbRef* d_refs, u32 nrefs, bool forward, hipStream_t s) {
     u32 chunk = forward ? c.ntt_chunk_fwd : c.ntt_chunk_inv;
@@ -612,6 +623,7 @@ void Context::run(const Program& p, const uint8_t* d, hipStream_t s) const {
                 else
                     launch_ntt(*this, (const LimbRef*)x, st.count, false, s);
                 break;
+            case S_NTT_INV_Y: launch_ntt_y(*this, (const LimbRef*)x, st.count, s); break;
             case S_CONV:
                 launch_conv(conv, (const ConvJob*)x, st.count, st.a, st.b, st.sched ? (st.sched >> 1) & 127 : conv_variant,
                             st.sched ? st.sched >> 8 : conv_tg, s);
@@ -783,18 +795,24 @@ static void group_by_prime(std::vector<LimbRef>& r) {
     std::stable_sort(r.begin(), r.end(), [](const LimbRef& a, const LimbRef& b) { return a.prime < b.prime; });
 }
 
+bool Context::yform_usable() const { return conv_yform && ntt_fast_supported(log_n); }
+
 void Context::prog_decompose(Program& p, const std::vector<DecTask>& tasks) const {
     std::vector<LimbRef> intt, copies, nttr, cols;
     std::vector<ConvJob> jobs;
     u32 max_tgt = 0;
+    const bool yf = yform_usable();
     for (const DecTask& t : tasks) {
         const u32 L = t.level + 1;
         const u64 ds = digit_words(t.level);
-        for (u32 i = 0; i < L; ++i) intt.push_back({t.d + i * n, t.x + i * n, i, 0});
         auto dr = digit_ranges(t.level);
+        for (u32 j = 0; j < dr.size(); ++j)
+            for (u32 i = dr[j].first; i < dr[j].second; ++i)
+                intt.push_back({t.d + i * n, t.x + i * n, i,
+                                yf ? conv_src_first_[modup_plan_[t.level][j]] + (i - dr[j].first) : 0});
         for (u32 j = 0; j < dr.size(); ++j) {
             u64* dj = t.dig + j * ds;
-            jobs.push_back({t.x, dj, modup_plan_[t.level][j], 0});
+            jobs.push_back({t.x, dj, modup_plan_[t.level][j], yf ? 1u : 0u});
             if (t.copy_own)
                 for (u32 i = dr[j].first; i < dr[j].second; ++i) copies.push_back({t.d + i * n, dj + i * n, i, 0});
             for (u32 i = 0; i < L; ++i)
@@ -805,7 +823,10 @@ void Context::prog_decompose(Program& p, const std::vector<DecTask>& tasks) cons
         max_tgt = std::max(max_tgt, modup_maxtgt_[t.level]);
     }
     group_by_prime(nttr);
-    p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size(), 0, fused_usable());
+    if (yf)
+        p.step(S_NTT_INV_Y, p.add(intt), (u32)intt.size());
+    else
+        p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size(), 0, fused_usable());
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_tgt, alpha);
     p.step(S_COPY, p.add(copies), (u32)copies.size());
     p.step_ntt(S_NTT_FWD, p.add(nttr), (u32)nttr.size(), 0, fused_usable());
@@ -836,6 +857,7 @@ void Context::prog_apply(Program& p, const std::vector<ApplyTask>& tasks) const 
         p.step(S_IP, p.add(ip), (u32)ip.size());
         std::vector<LimbRef> intt, cols;
         std::vector<ConvJob> jobs;
+        const bool yf = yform_usable();
         for (const ApplyTask& t : tasks) {
             const u32 L = t.level + 1;
             const u64 aw = (u64)(L + np) * n;
@@ -843,8 +865,9 @@ void Context::prog_apply(Program& p, const std::vector<ApplyTask>& tasks) const 
                 const u64* acc = t.acc + part * aw;
                 u64* y = t.y + (u64)part * np * n;
                 u64* z = t.z + (u64)part * L * n;
-                for (u32 k = 0; k < np; ++k) intt.push_back({acc + (L + k) * n, y + k * n, nq + k, 0});
-                jobs.push_back({y, z, moddown_plan_[t.level], 0});
+                const u32 sf = conv_src_first_[moddown_plan_[t.level]];
+                for (u32 k = 0; k < np; ++k) intt.push_back({acc + (L + k) * n, y + k * n, nq + k, yf ? sf + k : 0});
+                jobs.push_back({y, z, moddown_plan_[t.level], yf ? 1u : 0u});
                 for (u32 i = 0; i < L; ++i) cols.push_back({z + i * n, z + i * n, i, 0});
             }
         }
@@ -867,7 +890,10 @@ void Context::prog_apply(Program& p, const std::vector<ApplyTask>& tasks) const 
                 rf.push_back(j);
             }
         group_by_prime(intt);
-        p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size(), 0, fused_usable());
+        if (yf)
+            p.step(S_NTT_INV_Y, p.add(intt), (u32)intt.size());
+        else
+            p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size(), 0, fused_usable());
         p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_l, np);
         if (conv_variant_md < 3 && !jobs.empty()) p.steps.back().sched = 1u | (conv_variant_md << 1) | (conv_tg << 8);
         p.step(S_NTT_COLS, p.add(cols), (u32)cols.size());
@@ -909,11 +935,13 @@ void Context::prog_mod_down(Program& p, const std::vector<MdTask>& tasks) const 
     std::vector<ConvJob> jobs;
     std::vector<FinishJob> fin;
     u32 max_l = 0;
+    const bool yf = yform_usable();
     for (const MdTask& t : tasks) {
         const u32 L = t.level + 1;
         max_l = std::max(max_l, L);
-        for (u32 k = 0; k < np; ++k) intt.push_back({t.acc + (L + k) * n, t.y + k * n, nq + k, 0});
-        jobs.push_back({t.y, t.z, moddown_plan_[t.level], 0});
+        const u32 sf = conv_src_first_[moddown_plan_[t.level]];
+        for (u32 k = 0; k < np; ++k) intt.push_back({t.acc + (L + k) * n, t.y + k * n, nq + k, yf ? sf + k : 0});
+        jobs.push_back({t.y, t.z, moddown_plan_[t.level], yf ? 1u : 0u});
         for (u32 i = 0; i < L; ++i) {
             u64* zi = t.z + i * n;
             nttr.push_back({zi, zi, i, 0});
@@ -921,7 +949,10 @@ void Context::prog_mod_down(Program& p, const std::vector<MdTask>& tasks) const 
         }
     }
     group_by_prime(intt);
-    p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size(), 0, fused_usable());
+    if (yf)
+        p.step(S_NTT_INV_Y, p.add(intt), (u32)intt.size());
+    else
+        p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size(), 0, fused_usable());
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_l, np);
     if (conv_variant_md < 3 && !jobs.empty()) p.steps.back().sched = 1u | (conv_variant_md << 1) | (conv_tg << 8);
     const size_t o_ntt = p.add(nttr), o_fin = p.add(fin);

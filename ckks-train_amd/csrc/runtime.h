@@ -151,7 +151,7 @@ struct RaiseTask {
 // Descriptor bytes plus the launch steps that read them (see runtime.cpp, "engine").
 enum StepKind : uint8_t {
     S_NTT_FWD, S_NTT_INV, S_CONV, S_COPY, S_IP, S_NTT_FWD_FIN, S_FINISH, S_NTT_COLS, S_NTT_ROWS, S_ROWS_IP,
-    S_MUL_SUM, S_AUTOMORPH, S_ADD_SUM, S_ROWS_IP_FIN
+    S_MUL_SUM, S_AUTOMORPH, S_ADD_SUM, S_ROWS_IP_FIN, S_NTT_INV_Y
 };
 struct Program {
     struct Step {
@@ -338,6 +338,11 @@ class Context : public std::enable_shared_from_this<Context> {
     // reach memory.
     u32 fuse_ip = 1, fuse_fin = 1;
     bool fuse_ip_possible() const;
+    // The inverse NTT before a ModUp or mod_down conversion leaves its limbs in the conversion's
+    // y form (YForm; step S_NTT_INV_Y, tuning key "conv_yform"), when the register-radix NTT
+    // applies: the conversion then skips its per-source product.
+    u32 conv_yform = 1;
+    bool yform_usable() const;
     std::atomic<u64> graph_captures{0}, graph_launches{0};
 
    private:
@@ -395,6 +400,7 @@ class Context : public std::enable_shared_from_this<Context> {
     std::vector<std::vector<u32>> modup_plan_;  // [level][digit]
     std::vector<u32> modup_maxtgt_;            // [level]
     std::vector<u32> moddown_plan_;            // [level]
+    std::vector<u32> conv_src_first_;          // [plan]: its first source in the ConvSrc table
     std::vector<u32> rescale_plan_;            // [level], level >= 1
     std::vector<u32> raise_plan_;              // [top], top >= 1: q_0 -> q_1..q_top
     std::vector<u64> pinv_, pinv_s_;           // P^-1 mod q_i

@@ -102,7 +102,7 @@ std::vector<u64> make_inputs(const std::vector<u64>& sq, std::mt19937_64& rng, s
 }
 
 struct Run {
-    u64 plans = 0, words = 0, off_by_q = 0, mf_plans = 0;
+    u64 plans = 0, words = 0, off_by_q = 0, mf_plans = 0, yform_plans = 0;
 };
 
 // The wave operations of the matrix-core conversion, replayed: the 64 lanes of a wave are 64 host
@@ -174,7 +174,8 @@ struct EmuWave {
 
 // One block of the matrix-core kernel: 256 host threads (four waves) with a block barrier.
 template <u32 KB>
-void run_mf_block(const HostConvTables& T, const ConvPlan& P, const MfPlan& F, const u64* in, u64* out, u32 bx) {
+void run_mf_block(const HostConvTables& T, const ConvPlan& P, const MfPlan& F, const u64* in, u64* out, u32 bx,
+                  bool yform) {
     EmuWave::Slots waves[4];
     std::vector<EmuWave::v4i> a_lds(MF_MAX_GROUPS * 2 * 64);
     std::vector<MfTgt> t_lds(MF_MAX_GROUPS * 4);
@@ -187,17 +188,18 @@ void run_mf_block(const HostConvTables& T, const ConvPlan& P, const MfPlan& F, c
             threadIdx.x = t;
             EmuWave::wave() = &waves[t / 64];
             conv_mf_block<KB, EmuWave>(T.srcs.data(), P, F, (const EmuWave::v4i*)T.mf_a.data(), T.mf_tgt.data(), in,
-                                       out, (u32)N, bx, t, a_lds.data(), t_lds.data());
+                                       out, (u32)N, bx, t, a_lds.data(), t_lds.data(), yform);
         });
     for (auto& x : th) x.join();
     emu::current() = nullptr;
 }
 
 // How a plan is converted: the vector kernel (MAXS, PIECES, targets per group) or the matrix-core
-// kernel (KB > 0).
+// kernel (KB > 0); yform: the input limbs hold y_j = [x_j + h_j] (Q/q_j)^-1 mod q_j (computed here
+// with 128-bit arithmetic), as an inverse NTT in y form leaves them.
 template <u32 MAXS, bool PIECES, u32 KB = 0>
 void run_plan(const HostConvTables& T, const std::vector<PrimeConst>& pcs, u32 pi, const std::vector<u64>& primes,
-              std::mt19937_64& rng, u32 tg, Run& stats) {
+              std::mt19937_64& rng, u32 tg, Run& stats, bool yform = false) {
     const ConvPlan& P = T.plans[pi];
     if (KB ? (T.mf_plans[pi].ngroups == 0 || T.mf_plans[pi].kb > KB) : P.nsrc > MAXS) return;
     std::vector<u64> sq(P.nsrc), tq(P.ntgt);
@@ -212,18 +214,26 @@ void run_plan(const HostConvTables& T, const std::vector<PrimeConst>& pcs, u32 p
     const std::vector<u64> x = make_inputs(sq, rng, ties);
     std::vector<u64> in((size_t)(P.src_limb0 + P.nsrc) * N, 0);
     std::copy(x.begin(), x.end(), in.begin() + (size_t)P.src_limb0 * N);
+    if (yform)
+        for (u32 j = 0; j < P.nsrc; ++j) {
+            const ConvSrc& cs = T.srcs[P.src_first + j];
+            for (u64 c = 0; c < N; ++c) {
+                u64& w = in[((size_t)P.src_limb0 + j) * N + c];
+                w = mulm((w + cs.h) % cs.q, cs.qhinv, cs.q);
+            }
+        }
     const u64 sentinel = 0x5a5a5a5a5a5a5a5aull;
     std::vector<u64> out((size_t)out_limbs * N, sentinel);
     emu::Tracked ti(in.data(), in.size(), "input"), to(out.data(), out.size(), "output");
     emu::registry() = {&ti, &to};
     const u32 groups = KB ? 1 : (P.ntgt + tg - 1) / tg;
     if constexpr (KB > 0) {
-        for (u32 bx = 0; bx < N / 256; ++bx) run_mf_block<KB>(T, P, T.mf_plans[pi], in.data(), out.data(), bx);
+        for (u32 bx = 0; bx < N / 256; ++bx) run_mf_block<KB>(T, P, T.mf_plans[pi], in.data(), out.data(), bx, yform);
     } else {
         for (u32 grp = 0; grp < groups; ++grp)
             for (u32 c = 0; c < N; ++c)
                 conv_coeff<MAXS, PIECES>(T.srcs.data(), T.tgts.data(), pcs.data(), P, in.data(), out.data(), (u32)N, c,
-                                         grp * tg, std::min((grp + 1) * tg, P.ntgt));
+                                         grp * tg, std::min((grp + 1) * tg, P.ntgt), yform);
     }
     emu::registry().clear();
     // coverage: sources read once per group, target words written once, nothing else touched
@@ -266,6 +276,7 @@ void run_plan(const HostConvTables& T, const std::vector<PrimeConst>& pcs, u32 p
     }
     stats.plans += 1;
     stats.mf_plans += KB ? 1 : 0;
+    stats.yform_plans += yform ? 1 : 0;
     stats.words += (u64)P.ntgt * N;
 }
 
@@ -274,6 +285,11 @@ void run_both(const HostConvTables& T, const std::vector<PrimeConst>& pcs, u32 p
               std::mt19937_64& rng, u32 tg, Run& stats) {
     run_plan<MAXS, false>(T, pcs, pi, primes, rng, tg, stats);
     run_plan<MAXS, true>(T, pcs, pi, primes, rng, tg, stats);
+    // inputs in y form: every other plan, on both arithmetic variants
+    if (pi % 2 == 0) {
+        run_plan<MAXS, false>(T, pcs, pi, primes, rng, tg, stats, true);
+        run_plan<MAXS, true>(T, pcs, pi, primes, rng, tg, stats, true);
+    }
 }
 
 // The launch's source bound for a plan of k sources (conv_launch: 1..8, else 16), and a larger one.
@@ -294,6 +310,7 @@ void run_bounds(const HostConvTables& T, const std::vector<PrimeConst>& pcs, u32
     if (T.mf_plans[pi].ngroups) {
         if (k <= 4) run_plan<0, false, 1>(T, pcs, pi, primes, rng, 0, stats);
         if (k > 4 || pi % 4 == 0) run_plan<0, false, 2>(T, pcs, pi, primes, rng, 0, stats);
+        if (pi % 2 == 1) run_plan<0, false, 2>(T, pcs, pi, primes, rng, 0, stats, true);
     }
 }
 
@@ -419,10 +436,11 @@ int main(int argc, char** argv) {
                 for (u32 j = 0; j < MAX_SRC; ++j) t.qhat_q[j] = 0.0;
         Run stats;
         for (u32 pi : plan_ids) run_bounds(T, pcs, pi, primes, rng, stats);
-        std::printf("%s: %llu plan runs (%llu on the matrix-core kernel), %llu output words equal the oracle "
-                    "(near ties where the f64 estimate is off by one, as in the oracle: %llu words)\n",
+        std::printf("%s: %llu plan runs (%llu on the matrix-core kernel, %llu on y-form inputs), %llu output words "
+                    "equal the oracle (near ties where the f64 estimate is off by one, as in the oracle: %llu words)\n",
                     S.name, (unsigned long long)stats.plans, (unsigned long long)stats.mf_plans,
-                    (unsigned long long)stats.words, (unsigned long long)stats.off_by_q);
+                    (unsigned long long)stats.yform_plans, (unsigned long long)stats.words,
+                    (unsigned long long)stats.off_by_q);
         std::fflush(stdout);
     }
     if (!plant) {

@@ -69,8 +69,10 @@ struct Limb {
     HostNttTables tab;
 };
 
-// Forward (or inverse) transform of one limb through the two passes, every tile replayed.
-void transform(const Limb& L, const std::vector<u64>& in, std::vector<u64>& out, bool forward, double* mid_max) {
+// Forward (or inverse) transform of one limb through the two passes, every tile replayed; yf
+// (inverse only): the last stage leaves the y form of a conversion source (YForm).
+void transform(const Limb& L, const std::vector<u64>& in, std::vector<u64>& out, bool forward, double* mid_max,
+               const YForm* yf = nullptr) {
     const u32 log_n = L.log_n, B = log_n / 2, A = log_n - B, tiles = 1u << (log_n - TL);
     const u64 n = 1ull << log_n;
     std::vector<u64> src(in), dst(n, 0xdeadbeefdeadbeefull);
@@ -134,8 +136,18 @@ void transform(const Limb& L, const std::vector<u64>& in, std::vector<u64>& out,
     } else {
         pass(A, [&](u32 m, u32 tile) {
             dispatch(m, [&](auto M) {
-                if (fp) inv_cols<decltype(M)::value>(far, dst.data(), lv, log_n, tile);
-                else inv_cols<decltype(M)::value>(iar, dst.data(), lv, log_n, tile);
+                constexpr u32 MM = decltype(M)::value;
+                if (yf) {  // as k_inv_cols<M, true>
+                    FpAr fy{far.tab, far.q, far.qinv, yf->n_f, yf->wn_f, far.rt};
+                    fy.add = yf->h_f;
+                    IntAr iy{iar.tab, iar.q, yf->n, yf->n_s, yf->wn, yf->wn_s};
+                    iy.add = yf->h;
+                    if (fp) inv_cols<MM, FpAr, false, true>(fy, dst.data(), lv, log_n, tile);
+                    else inv_cols<MM, IntAr, false, true>(iy, dst.data(), lv, log_n, tile);
+                } else {
+                    if (fp) inv_cols<MM>(far, dst.data(), lv, log_n, tile);
+                    else inv_cols<MM>(iar, dst.data(), lv, log_n, tile);
+                }
             });
         }, "inverse column pass", false, false, true, true);
     }
@@ -407,16 +419,36 @@ int main(int argc, char** argv) {
                     for (auto& x : a) x = (rng() & 1) ? (q - 1) / 2 : (q + 1) / 2;
                     inputs.push_back(a);
                 }
+            // the y form of a conversion source with a random (Q/q)^-1 (h = q >> 1 as in every plan)
+            YForm yf{};
+            const u64 qhinv = 1 + rng() % (q - 1), h = q >> 1;
+            {
+                const PrimeConst& pc = L.tab.pc[0];
+                auto mm = [q](u64 a, u64 b) { return (u64)((unsigned __int128)a * b % q); };
+                auto sh = [q](u64 w) { return (u64)(((unsigned __int128)w << 64) / q); };
+                yf.n = mm(pc.n_inv, qhinv);
+                yf.wn = mm(pc.w_n_inv, qhinv);
+                yf.h = mm(h, qhinv);
+                yf.n_s = sh(yf.n);
+                yf.wn_s = sh(yf.wn);
+                yf.n_f = {(double)yf.n, (double)yf.n / (double)q};
+                yf.wn_f = {(double)yf.wn, (double)yf.wn / (double)q};
+                yf.h_f = (double)yf.h;
+            }
             double fmax = 0, imax = 0;
             for (const auto& a : inputs) {
-                std::vector<u64> f, b;
+                std::vector<u64> f, b, y;
                 transform(L, a, f, true, &fmax);
                 if (f != oracle(L, a, true)) emu::die("forward output differs from the oracle", log_n, bits);
                 transform(L, f, b, false, &imax);
                 if (b != a) emu::die("inverse does not return the input", log_n, bits);
+                transform(L, f, y, false, &imax, &yf);
+                for (u64 i = 0; i < n; ++i)
+                    if (y[i] != (u64)((unsigned __int128)((a[i] + h) % q) * qhinv % q))
+                        emu::die("y-form inverse output differs from [x + h] (Q/q)^-1", log_n, bits, (double)i);
             }
-            std::printf("N=2^%u q=%u-bit: %zu limbs forward and inverse exact; between passes max |x|/q "
-                        "forward %.3f inverse %.3f\n",
+            std::printf("N=2^%u q=%u-bit: %zu limbs forward and inverse exact, inverse in y form exact; between "
+                        "passes max |x|/q forward %.3f inverse %.3f\n",
                         log_n, bits, inputs.size(), fmax, imax);
             std::fflush(stdout);
         }

@@ -61,7 +61,7 @@ def test_replay_of_the_ntt_passes_is_exact_and_race_free(emu_bin):
     r = subprocess.run([emu_bin, "1", "8"], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-2000:]
     assert "EMU OK" in r.stdout
-    lines = [l for l in r.stdout.splitlines() if "limbs forward and inverse exact" in l]
+    lines = [l for l in r.stdout.splitlines() if "limbs forward and inverse exact, inverse in y form exact" in l]
     assert len(lines) == 10  # N = 2^12..2^16, 50- and 60-bit primes (M = 6, 7, 8 in both passes)
     fused = [l for l in r.stdout.splitlines() if "fused row pass + inner product + finish exact" in l]
     assert len(fused) == 5  # N = 2^12..2^16
@@ -87,8 +87,10 @@ def test_replay_of_the_conversion_is_exact_and_covers_every_word(conv_bin):
     assert "EMU OK" in r.stdout
     lines = [l for l in r.stdout.splitlines() if "output words equal the oracle" in l]
     assert len(lines) == 5  # the five prime shapes
-    # every shape has plans with a matrix form (at most 8 sources), replayed on the matrix-core kernel
+    # every shape has plans with a matrix form (at most 8 sources), replayed on the matrix-core kernel,
+    # and plans replayed on y-form inputs (the inverse NTT's folded first step)
     assert all(int(l.split("(")[1].split()[0]) > 0 for l in lines)
+    assert all(int(l.split(" on y-form inputs")[0].split()[-1]) > 0 for l in lines)
     assert "inner product: 524288 output words equal the 128-bit sums" in r.stdout
     # the near-tie inputs reach the boundary where the f64 estimate's floor is off by one
     assert all(int(l.split("off by one, as in the oracle: ")[1].split()[0]) > 0 for l in lines)
