@@ -56,6 +56,7 @@ struct FinishJob {
     const u64* add;
     u64 inv, inv_s;
     u32 prime, pad;
+    FTw inv_f;  // inv and its quotient by q (f64 path, q < 2^50)
 };
 
 // One limb of a key-switch inner product fused with the forward row pass (primes below 2^50):
@@ -85,7 +86,7 @@ struct RowIpFinJob {
     u64* out_a;
     const u64* add_b;  // optional addends (nullptr: none)
     const u64* add_a;
-    u64 inv, inv_s;  // P^-1 mod q_i and its Shoup companion
+    FTw inv;  // P^-1 mod q_i and its quotient by q_i (q_i < 2^50)
 };
 
 // Row factors per row of the register-radix row pass (one 128-byte line of (w, w/q) pairs).

@@ -292,11 +292,10 @@ __global__ __launch_bounds__(NT) void k_rows_ip_fin(const RowIpFinJob* __restric
     const PrimeConst* pc = pcs + J.ip.prime;
     const FpAr ar{twf + ((u64)J.ip.prime << log_n), pc->qf, pc->qinv, pc->n_inv_f, pc->w_n_inv_f,
                   rt + ((u64)J.ip.prime << (log_n - log_n / 2)) * ROW_FACTOR_SLOTS};
-    const u64 q = pc->q;
     double sb[16], sa[16];
     rows_ip_sums<M>(ar, J.ip, lds, log_n, tile, sb, sa);
-    finish_sum<M>(ar, sb, J.zb, J.out_b, J.add_b, q, J.inv, J.inv_s, lds, log_n, tile);
-    finish_sum<M>(ar, sa, J.za, J.out_a, J.add_a, q, J.inv, J.inv_s, lds, log_n, tile);
+    finish_sum<M>(ar, sb, J.zb, J.out_b, J.add_b, J.inv, lds, log_n, tile);
+    finish_sum<M>(ar, sa, J.za, J.out_a, J.add_a, J.inv, lds, log_n, tile);
 }
 
 template <u32 M>

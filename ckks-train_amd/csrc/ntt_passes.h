@@ -45,6 +45,7 @@ CKH_DEV u32 pad(u32 a) { return a + (a >> 4); }
 struct FpAr {
     typedef double V;
     typedef FTw W;
+    static constexpr bool F64 = true;
     const FTw* tab;
     double q, qinv;
     FTw ninv, wninv;
@@ -118,6 +119,7 @@ struct FpAr {
 struct IntAr {
     typedef u64 V;
     typedef Tw W;
+    static constexpr bool F64 = false;
     const Tw* tab;
     u64 q;
     u64 n_inv, n_inv_s, w_n_inv, w_n_inv_s;
@@ -309,8 +311,9 @@ CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in_p, u64* __restric
 }
 
 // The row pass of one tile, its canonical outputs left in LDS in natural order (word i of the
-// tile at pad(i)): as u64 residues, or (F64) as the f64 bits of the residue.
-template <u32 M, bool F64, class Ar, bool COH = false, class Lds>
+// tile at pad(i)): as u64 residues, or (F64) as the f64 bits of the residue, or (RAW, f64 path)
+// as the f64 bits of the last stage's signed outputs (|.| <= 2.6 q), not reduced.
+template <u32 M, bool F64, class Ar, bool COH = false, class Lds, bool RAW = false>
 CKH_DEV void fwd_rows_to_lds(const Ar& ar, const u64* __restrict__ in_p, Lds lds, u32 log_n, u32 tile,
                              u32 tid = threadIdx.x) {
     const auto in = glb(in_p);
@@ -340,7 +343,7 @@ CKH_DEV void fwd_rows_to_lds(const Ar& ar, const u64* __restrict__ in_p, Lds lds
 #pragma unroll
     for (u32 e = 0; e < 16; ++e) {
         const u32 c = ((((o << U) | (e >> R2)) << R2) | (e & JM));
-        lds[pad((rr << M) + c)] = F64 ? ar.fwd_final_f(v[e]) : ar.fwd_final(v[e]);
+        lds[pad((rr << M) + c)] = RAW ? ar.to_mid(v[e]) : F64 ? ar.fwd_final_f(v[e]) : ar.fwd_final(v[e]);
     }
     __syncthreads();
 }
@@ -361,14 +364,27 @@ CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
 #pragma unroll
         for (u32 k = 0; k < 16; ++k) {
             av[k] = fa[tbase + tid + k * NT];
-            dv[k] = has_add ? fadd[tbase + tid + k * NT] : 0;
+            dv[k] = has_add ? (u64)fadd[tbase + tid + k * NT] : (u64)0;
         }
-        fwd_rows_to_lds<M, false, Ar, COH>(ar, in, lds, log_n, tile, tid);
+        if constexpr (Ar::F64) {
+            // in f64: the transform left unreduced (|.| <= 2.6 q), a - t below 3.6 q, one modular
+            // product (|.| <= 1.2 q), the addend, one reduction
+            fwd_rows_to_lds<M, false, Ar, COH, Lds, true>(ar, in, lds, log_n, tile, tid);
+            const FTw iv = fin->inv_f;
 #pragma unroll
-        for (u32 k = 0; k < 16; ++k) {
-            u64 v = shoup(sub_mod(av[k], lds[pad(tid + k * NT)], q), inv, inv_s, q);
-            if (has_add) v = add_mod(v, dv[k], q);
-            fout[tbase + tid + k * NT] = v;
+            for (u32 k = 0; k < 16; ++k) {
+                double v = fmulmod(u2d(av[k]) - bits_f64(lds[pad(tid + k * NT)]), iv.w, iv.wq, ar.q);
+                if (has_add) v = v + u2d(dv[k]);
+                fout[tbase + tid + k * NT] = d2u(fcanon(v, ar.q, ar.qinv));
+            }
+        } else {
+            fwd_rows_to_lds<M, false, Ar, COH>(ar, in, lds, log_n, tile, tid);
+#pragma unroll
+            for (u32 k = 0; k < 16; ++k) {
+                u64 v = shoup(sub_mod(av[k], lds[pad(tid + k * NT)], q), inv, inv_s, q);
+                if (has_add) v = add_mod(v, dv[k], q);
+                fout[tbase + tid + k * NT] = v;
+            }
         }
         return;
     }
@@ -540,20 +556,24 @@ CKH_DEV void rows_ip(const FpAr& ar, const RowIpJob& J, Lds lds, u32 log_n, u32 
 // mod_down's finish of one sum for one tile (baseconv.rs:459-511): the row pass of the converted
 // limb z (column pass done) into LDS, then out = (sum - NTT(z)) * P^-1 (+ add). The LDS is free
 // again on return.
+// All in f64: the row pass leaves NTT(z) unreduced (|.| <= 2.6 q), the sum is reduced to q / 2,
+// their difference (below 3.1 q) times P^-1 is one f64 modular product (|.| <= 1.1 q), the addend
+// (below q) is added and the result reduced once.
 template <u32 M, class Lds>
-CKH_DEV void finish_sum(const FpAr& ar, const double (&s)[16], const u64* z, u64* out_p, const u64* add_p, u64 q,
-                        u64 inv, u64 inv_s, Lds lds, u32 log_n, u32 tile) {
+CKH_DEV void finish_sum(const FpAr& ar, const double (&s)[16], const u64* z, u64* out_p, const u64* add_p, FTw inv,
+                        Lds lds, u32 log_n, u32 tile) {
     u32 tid = threadIdx.x;
     CKH_OPAQUE(tid);
     const u64 tbase = (u64)tile << TL;
-    fwd_rows_to_lds<M, false>(ar, z, lds, log_n, tile, tid);
+    const double q = ar.q, qinv = ar.qinv;
+    fwd_rows_to_lds<M, false, FpAr, false, Lds, true>(ar, z, lds, log_n, tile, tid);
     const auto out = glb(out_p);
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) {
-        const u64 a = d2u(fcanon(s[k], ar.q, ar.qinv));
-        u64 v = shoup(sub_mod(a, lds[pad(tid + k * NT)], q), inv, inv_s, q);
-        if (add_p) v = add_mod(v, glb(add_p)[tbase + tid + k * NT], q);
-        out[tbase + tid + k * NT] = v;
+        const double d = freduce(s[k], q, qinv) - bits_f64(lds[pad(tid + k * NT)]);
+        double v = fmulmod(d, inv.w, inv.wq, q);
+        if (add_p) v = v + u2d(glb(add_p)[tbase + tid + k * NT]);
+        out[tbase + tid + k * NT] = d2u(fcanon(v, q, qinv));
     }
     __syncthreads();  // the next pass reuses the LDS
 }

@@ -885,8 +885,7 @@ void Context::prog_apply(Program& p, const std::vector<ApplyTask>& tasks) const 
                 j.out_a = t.out_a + i * n;
                 j.add_b = t.add_b ? t.add_b + i * n : nullptr;
                 j.add_a = t.add_a ? t.add_a + i * n : nullptr;
-                j.inv = pinv_[i];
-                j.inv_s = pinv_s_[i];
+                j.inv = {(double)pinv_[i], (double)pinv_[i] / (double)primes[i]};
                 rf.push_back(j);
             }
         group_by_prime(intt);
@@ -945,7 +944,8 @@ void Context::prog_mod_down(Program& p, const std::vector<MdTask>& tasks) const 
         for (u32 i = 0; i < L; ++i) {
             u64* zi = t.z + i * n;
             nttr.push_back({zi, zi, i, 0});
-            fin.push_back({t.acc + i * n, zi, t.out + i * n, t.add ? t.add + i * n : nullptr, pinv_[i], pinv_s_[i], i, 0});
+            fin.push_back({t.acc + i * n, zi, t.out + i * n, t.add ? t.add + i * n : nullptr, pinv_[i], pinv_s_[i], i, 0,
+                           {(double)pinv_[i], (double)pinv_[i] / (double)primes[i]}});
         }
     }
     group_by_prime(intt);
@@ -971,7 +971,8 @@ void Context::prog_rescale(Program& p, const std::vector<RescaleTask>& tasks) co
         jobs.push_back({t.x, t.z, rescale_plan_[l], 0});
         for (u32 i = 0; i < l; ++i) {
             nttr.push_back({t.z + i * n, t.z + i * n, i, 0});
-            fin.push_back({t.in + i * n, t.z + i * n, t.out + i * n, nullptr, qlinv_[l][i], qlinv_s_[l][i], i, 0});
+            fin.push_back({t.in + i * n, t.z + i * n, t.out + i * n, nullptr, qlinv_[l][i], qlinv_s_[l][i], i, 0,
+                           {(double)qlinv_[l][i], (double)qlinv_[l][i] / (double)primes[i]}});
         }
         max_t = std::max(max_t, l);
     }

@@ -178,6 +178,7 @@ struct View {
     T* p;
     Tracked* t;
     Ref operator[](u64 i) const {
+        if (!t) die("access through a null pointer", (double)i);
         u64* e = const_cast<u64*>(reinterpret_cast<const u64*>(p)) + i;
         if (mutation() == 2) e = const_cast<u64*>(t->base) + ((size_t)(e - t->base) & ~(size_t)1);
         const size_t at = (size_t)(e - t->base);
@@ -254,7 +255,8 @@ inline void __syncthreads() { emu::current()->sync(); }
 namespace ckh {
 template <class T>
 emu::View<T> glb(T* p) {
-    return emu::View<T>{p, emu::find(reinterpret_cast<const u64*>(p))};
+    // a view of a null pointer (an absent optional operand) is allowed; accessing it is not
+    return emu::View<T>{p, p ? emu::find(reinterpret_cast<const u64*>(p)) : nullptr};
 }
 namespace fast {
 inline u64 load_coherent(const u64* p) { return glb(p)[0]; }

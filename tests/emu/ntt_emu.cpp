@@ -273,6 +273,51 @@ void adversarial_f64(std::mt19937_64& rng, u32 log_n, int iters) {
 
 std::vector<u64> oracle(const Limb& L, std::vector<u64> a, bool forward);
 
+// The forward row pass with the finish of mod_down and rescale (k_fwd_rows<M, true>, f64 path):
+// x's transform (from its column pass) against a and an optional addend, out = (a - NTT(x)) inv
+// (+ add). Pattern 0: random words, 1: all q - 1, 2: random without an addend. Every word of a,
+// the addend and the column-pass output is read once and every output word written once; the
+// outputs equal the same computed with the oracle's transform.
+void replay_finish(const Limb& L, std::mt19937_64& rng, int pattern) {
+    const u32 log_n = L.log_n, M = log_n / 2, tiles = 1u << (log_n - TL);
+    const u64 n = 1ull << log_n, q = L.q;
+    std::vector<u64> x(n), a(n), add(n), out(n, 0x5a5a5a5a5a5a5a5aull);
+    for (auto* v : {&x, &a, &add})
+        for (auto& w : *v) w = pattern == 1 ? q - 1 : rng() % q;
+    std::vector<u64> mid = cols_only(L, x);
+    const u64 inv = 1 + rng() % (q - 1);
+    FinishJob F{a.data(), nullptr, out.data(), pattern == 2 ? nullptr : add.data(), inv,
+                (u64)(((unsigned __int128)inv << 64) / q), 0, 0, {(double)inv, (double)inv / (double)q}};
+    emu::Tracked tm(mid.data(), n, "mid"), ta(a.data(), n, "a"), tad(add.data(), n, "addend"), to(out.data(), n, "out");
+    emu::registry() = {&tm, &ta, &tad, &to};
+    emu::Lds lds(LDS_WORDS);
+    emu::LdsView lv{&lds};
+    const PrimeConst& pc = L.tab.pc[0];
+    const FpAr far{L.tab.fwd_f.data(), pc.qf, pc.qinv, pc.n_inv_f, pc.w_n_inv_f, L.tab.rf.data()};
+    for (u32 tile = 0; tile < tiles; ++tile)
+        run_group(lds, [&] {
+            switch (M) {
+                case 6: fwd_rows<6>(far, mid.data(), mid.data(), lv, log_n, &F, q, tile); break;
+                case 7: fwd_rows<7>(far, mid.data(), mid.data(), lv, log_n, &F, q, tile); break;
+                default: fwd_rows<8>(far, mid.data(), mid.data(), lv, log_n, &F, q, tile); break;
+            }
+        });
+    emu::registry().clear();
+    for (u64 i = 0; i < n; ++i) {
+        if (tm.reads[i] != 1 || tm.writes[i] != 0) emu::die("finish: a transform word not read exactly once", (double)i);
+        if (ta.reads[i] != 1 || ta.writes[i] != 0) emu::die("finish: an a word not read exactly once", (double)i);
+        if (tad.reads[i] != (pattern == 2 ? 0u : 1u) || tad.writes[i] != 0)
+            emu::die("finish: an addend word not read exactly once", (double)i);
+        if (to.writes[i] != 1 || to.reads[i] != 0) emu::die("finish: an output word not written exactly once", (double)i);
+    }
+    const std::vector<u64> t = oracle(L, x, true);
+    for (u64 i = 0; i < n; ++i) {
+        u64 w = (u64)((unsigned __int128)((a[i] + q - t[i]) % q) * inv % q);
+        if (pattern != 2) w = (w + add[i]) % q;
+        if (out[i] != w) emu::die("finish output differs from the oracle", log_n, (double)i);
+    }
+}
+
 // The fused row pass + inner product + mod_down finish of one limb (k_rows_ip_fin's body): ndig
 // digits (digit own_j given in NTT form, the others as column-pass outputs), key parts, the
 // converted limbs z_b / z_a as column-pass outputs, an addend for b (and, with pattern 1, for a). Every word of every
@@ -308,7 +353,7 @@ void replay_rows_ip_fin(const Limb& L, u32 ndig, u32 own_j, std::mt19937_64& rng
     rnd(add);
     rnd(add2);
     const bool both = pattern == 1;  // an addend on both sums (relinearization), else on b only (rotation)
-    const u64 inv = 1 + rng() % (q - 1), inv_s = (u64)(((unsigned __int128)inv << 64) / q);
+    const u64 inv = 1 + rng() % (q - 1);
     RowIpFinJob J{};
     J.ip = {mid.data(), n, own.data(), key.data(), n, nullptr, nullptr, ndig, 0, own_j, 0};
     J.zb = zb_mid.data();
@@ -317,8 +362,7 @@ void replay_rows_ip_fin(const Limb& L, u32 ndig, u32 own_j, std::mt19937_64& rng
     J.out_a = outa.data();
     J.add_b = add.data();
     J.add_a = both ? add2.data() : nullptr;
-    J.inv = inv;
-    J.inv_s = inv_s;
+    J.inv = {(double)inv, (double)inv / (double)q};
     emu::Tracked tm(mid.data(), mid.size(), "digits"), to(own.data(), n, "own digit"), tk(key.data(), key.size(), "key"),
         tzb(zb_mid.data(), n, "z_b"), tza(za_mid.data(), n, "z_a"), tad(add.data(), n, "addend"),
         tad2(add2.data(), n, "addend a"), tob(outb.data(), n, "out_b"), toa(outa.data(), n, "out_a");
@@ -333,8 +377,8 @@ void replay_rows_ip_fin(const Limb& L, u32 ndig, u32 own_j, std::mt19937_64& rng
                 constexpr u32 MM = decltype(Mc)::value;
                 double sb[16], sa[16];
                 rows_ip_sums<MM>(far, J.ip, lv, log_n, tile, sb, sa);
-                finish_sum<MM>(far, sb, J.zb, J.out_b, J.add_b, q, J.inv, J.inv_s, lv, log_n, tile);
-                finish_sum<MM>(far, sa, J.za, J.out_a, J.add_a, q, J.inv, J.inv_s, lv, log_n, tile);
+                finish_sum<MM>(far, sb, J.zb, J.out_b, J.add_b, J.inv, lv, log_n, tile);
+                finish_sum<MM>(far, sa, J.za, J.out_a, J.add_a, J.inv, lv, log_n, tile);
             };
             switch (M) {
                 case 6: body(std::integral_constant<u32, 6>{}); break;
@@ -453,8 +497,18 @@ int main(int argc, char** argv) {
             std::fflush(stdout);
         }
     }
-    if (!plant)
+    if (!plant) {
+        for (u32 log_n = 12; log_n <= 16; ++log_n) {
+            u64 q = 0;
+            if (or_ntt_primes_skip(50, 1ull << log_n, 0, 1, &q) != 0) emu::die("no prime");
+            Limb L{log_n, q, build_ntt_tables({q}, log_n, true)};
+            for (int pattern = 0; pattern < 3; ++pattern) replay_finish(L, rng, pattern);
+            std::printf("N=2^%u: row pass + finish (a - NTT(x)) inv (+ add) exact (random, all q - 1, no addend)\n",
+                        log_n);
+            std::fflush(stdout);
+        }
         for (u32 log_n : {12u, 13u}) adversarial_f64(rng, log_n, 200);
+    }
     if (!plant) {
         // the fused row pass + inner product + finish, 50-bit primes, N = 2^12 .. 2^16
         for (u32 log_n = 12; log_n <= 16; ++log_n) {

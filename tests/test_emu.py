@@ -8,7 +8,8 @@
   product verified against 128-bit integers, the words between the passes held to the bounds the
   arithmetic relies on (also under an adversarial search of the column pass's inputs, and with the
   row pass fed words at the bound and checked against the integer-arithmetic pass), and the
-  outputs compared with the oracle; then the fused row pass + key
+  outputs compared with the oracle; the inverse transform in a conversion source's y form; the row
+  pass with mod_down's / rescale's finish (k_fwd_rows<M, true>); then the fused row pass + key
   inner product + mod_down finish of the batched key switch (k_rows_ip_fin's body), every input
   word read once and every output written once, against the oracle's transforms and products.
 * The base conversion (tests/emu/conv_emu.cpp): the per-coefficient code of k_conv and the block
@@ -65,6 +66,8 @@ def test_replay_of_the_ntt_passes_is_exact_and_race_free(emu_bin):
     assert len(lines) == 10  # N = 2^12..2^16, 50- and 60-bit primes (M = 6, 7, 8 in both passes)
     fused = [l for l in r.stdout.splitlines() if "fused row pass + inner product + finish exact" in l]
     assert len(fused) == 5  # N = 2^12..2^16
+    fin = [l for l in r.stdout.splitlines() if "row pass + finish (a - NTT(x)) inv (+ add) exact" in l]
+    assert len(fin) == 5  # N = 2^12..2^16
     adv = [l for l in r.stdout.splitlines() if l.startswith("adversarial f64 inputs")]
     assert len(adv) == 2 and all("row pass on words up to 2.600 q equals the integer pass" in l for l in adv)
 
