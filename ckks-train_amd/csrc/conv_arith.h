@@ -131,7 +131,8 @@ CKH_DEV void conv_coeff(const ConvSrc* srcs, const ConvTgt* tgts, const PrimeCon
                 }
             }
             if (P.exact) {
-                const double prod = u64_to_f64_rn(y[j]) * s->inv;
+                // f64(y_j): yd already is (y_j < 2^50 converts exactly) for an f64 source
+                const double prod = (s->fp ? yd[j] : u64_to_f64_rn(y[j])) * s->inv;
                 f = f + prod;
             }
         }

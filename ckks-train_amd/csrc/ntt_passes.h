@@ -519,7 +519,8 @@ CKH_DEV void rows_ip_sums(const FpAr& ar, const RowIpJob& J, Lds lds, u32 log_n,
                 sa[k] += fmulmod(x, a, a * qinv, q);
             }
         } else {
-            fwd_rows_to_lds<M, true>(ar, J.mid + j * J.mid_stride, lds, log_n, tile, tid);
+            // the transform left unreduced (|x| <= 2.6 q: each product below q in magnitude)
+            fwd_rows_to_lds<M, true, FpAr, false, Lds, true>(ar, J.mid + j * J.mid_stride, lds, log_n, tile, tid);
 #pragma unroll
             for (u32 k = 0; k < 16; ++k) {
                 const double x = bits_f64(lds[pad(tid + k * NT)]);
