@@ -53,7 +53,7 @@ void Context::set_tuning(const std::string& key, int64_t value) {
     else if (key == "fuse_fin")
         fuse_fin = value != 0;
     else if (key == "conv_yform")
-        conv_yform = value != 0;
+        conv_yform = (u32)std::min<int64_t>(value, 7);
     else if (key == "small_limbs")
         small_limbs = (u32)std::min<int64_t>(value, 1u << 30);
     else if (key == "conv_variant_small")

@@ -31,9 +31,10 @@ struct ConvTables {
 // sources summed as exact f64 pieces (20-bit source pieces times 25-bit constant halves) instead
 // of two modular products per source; 2 = the matrix-core kernel (when ConvTables::mf_ok; else
 // variant 0); tg:
-// targets per workgroup of the vector kernels (the sources are read once per group of tg).
+// targets per workgroup of the vector kernels (the sources are read once per group of tg);
+// yform: every job's input limbs are in y form (YForm), for kernels compiled for that input.
 void conv_launch(const ConvTables& t, const ConvJob* jobs, u32 njobs, u32 max_tgt, u32 max_src, u32 variant,
-                 u32 tg, hipStream_t stream);
+                 u32 tg, bool yform, hipStream_t stream);
 
 void finish_launch(const FinishJob* jobs, u32 njobs, const PrimeConst* pc, u32 log_n,
                    hipStream_t stream);

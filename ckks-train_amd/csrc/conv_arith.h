@@ -65,13 +65,11 @@ struct ConvPlan {
     u32 src_first, nsrc, tgt_first, ntgt, src_limb0, exact;
 };
 
-// One launch row: plan `plan`, input polynomial (its limb 0) at `in`, output at `out`. yform: the
-// input limbs already hold y_j = [x_j + h_j]_{q_j} (Q/q_j)^-1 mod q_j (YForm: an inverse NTT
-// computed it in its last stage).
+// One launch row: plan `plan`, input polynomial (its limb 0) at `in`, output at `out`.
 struct ConvJob {
     const u64* in;
     u64* out;
-    u32 plan, yform;
+    u32 plan, pad;
 };
 
 // Round-to-nearest u64 -> f64: both halves convert exactly and the sum rounds once.
@@ -86,6 +84,8 @@ CKH_DEV double u64_to_f64_rn(u64 v) {
 // f64 piece sums for 50-bit targets of 60-bit sources. Per-target constants are read through
 // cst (the scalar cache on the device); 60-bit targets sum exact 128-bit integer products and
 // reduce once.
+// yform: the input limbs already hold y_j = [x_j + h_j]_{q_j} (Q/q_j)^-1 mod q_j (YForm: an
+// inverse NTT computed it in its last stage).
 template <u32 MAXS, bool PIECES>
 CKH_DEV void conv_coeff(const ConvSrc* srcs, const ConvTgt* tgts, const PrimeConst* pcs, const ConvPlan& P,
                         const u64* in_p, u64* out_p, u32 n, u32 c, u32 t0, u32 t1, bool yform) {

@@ -327,7 +327,7 @@ static Program ks_program(Context& c, const std::vector<KsReq>& reqs, u64* base,
     }
     Program prog;  // one program (one graph when captured) per batched switch
     if (pre) prog.step(S_AUTOMORPH, prog.add(*pre), (u32)pre->size());
-    c.prog_decompose(prog, dec);
+    c.prog_decompose(prog, dec, small ? c.conv_variant_s : c.conv_variant);
     c.prog_apply(prog, app);
     if (small) prog.set_conv_schedule(c.conv_variant_s, c.conv_tg_s);
     return prog;

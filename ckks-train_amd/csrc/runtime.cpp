@@ -577,9 +577,9 @@ static void launch_copy(const LimbRef* d, u32 nrefs, u32 log_n, hipStream_t s) {
 }
 
 static void launch_conv(const ConvTables& t, const ConvJob* d, u32 njobs, u32 max_tgt, u32 max_src, u32 variant,
-                        u32 tg, hipStream_t s) {
+                        u32 tg, bool yform, hipStream_t s) {
     for (u32 i = 0; i < njobs; i += MAX_ROWS)
-        conv_launch(t, d + i, std::min(MAX_ROWS, njobs - i), max_tgt, max_src, variant, tg, s);
+        conv_launch(t, d + i, std::min(MAX_ROWS, njobs - i), max_tgt, max_src, variant, tg, yform, s);
 }
 
 static void launch_ip(const IpJob* d, u32 njobs, const PrimeConst* pc, u32 log_n, hipStream_t s) {
@@ -626,7 +626,7 @@ void Context::run(const Program& p, const uint8_t* d, hipStream_t s) const {
             case S_NTT_INV_Y: launch_ntt_y(*this, (const LimbRef*)x, st.count, s); break;
             case S_CONV:
                 launch_conv(conv, (const ConvJob*)x, st.count, st.a, st.b, st.sched ? (st.sched >> 1) & 127 : conv_variant,
-                            st.sched ? st.sched >> 8 : conv_tg, s);
+                            st.sched ? st.sched >> 8 : conv_tg, st.yform, s);
                 break;
             case S_COPY: launch_copy((const LimbRef*)x, st.count, log_n, s); break;
             case S_IP: launch_ip((const IpJob*)x, st.count, d_pc, log_n, s); break;
@@ -795,13 +795,13 @@ static void group_by_prime(std::vector<LimbRef>& r) {
     std::stable_sort(r.begin(), r.end(), [](const LimbRef& a, const LimbRef& b) { return a.prime < b.prime; });
 }
 
-bool Context::yform_usable() const { return conv_yform && ntt_fast_supported(log_n); }
+bool Context::yform_usable(u32 mask_bit) const { return (conv_yform & mask_bit) && ntt_fast_supported(log_n); }
 
-void Context::prog_decompose(Program& p, const std::vector<DecTask>& tasks) const {
+void Context::prog_decompose(Program& p, const std::vector<DecTask>& tasks, u32 variant) const {
     std::vector<LimbRef> intt, copies, nttr, cols;
     std::vector<ConvJob> jobs;
     u32 max_tgt = 0;
-    const bool yf = yform_usable();
+    const bool yf = yform_usable(variant == 2 && conv.mf_ok ? 4 : 1);
     for (const DecTask& t : tasks) {
         const u32 L = t.level + 1;
         const u64 ds = digit_words(t.level);
@@ -812,7 +812,7 @@ void Context::prog_decompose(Program& p, const std::vector<DecTask>& tasks) cons
                                 yf ? conv_src_first_[modup_plan_[t.level][j]] + (i - dr[j].first) : 0});
         for (u32 j = 0; j < dr.size(); ++j) {
             u64* dj = t.dig + j * ds;
-            jobs.push_back({t.x, dj, modup_plan_[t.level][j], yf ? 1u : 0u});
+            jobs.push_back({t.x, dj, modup_plan_[t.level][j], 0});
             if (t.copy_own)
                 for (u32 i = dr[j].first; i < dr[j].second; ++i) copies.push_back({t.d + i * n, dj + i * n, i, 0});
             for (u32 i = 0; i < L; ++i)
@@ -828,6 +828,7 @@ void Context::prog_decompose(Program& p, const std::vector<DecTask>& tasks) cons
     else
         p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size(), 0, fused_usable());
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_tgt, alpha);
+    if (!jobs.empty()) p.steps.back().yform = yf;
     p.step(S_COPY, p.add(copies), (u32)copies.size());
     p.step_ntt(S_NTT_FWD, p.add(nttr), (u32)nttr.size(), 0, fused_usable());
     p.step(S_NTT_COLS, p.add(cols), (u32)cols.size());
@@ -857,7 +858,7 @@ void Context::prog_apply(Program& p, const std::vector<ApplyTask>& tasks) const 
         p.step(S_IP, p.add(ip), (u32)ip.size());
         std::vector<LimbRef> intt, cols;
         std::vector<ConvJob> jobs;
-        const bool yf = yform_usable();
+        const bool yf = yform_usable(2);
         for (const ApplyTask& t : tasks) {
             const u32 L = t.level + 1;
             const u64 aw = (u64)(L + np) * n;
@@ -867,7 +868,7 @@ void Context::prog_apply(Program& p, const std::vector<ApplyTask>& tasks) const 
                 u64* z = t.z + (u64)part * L * n;
                 const u32 sf = conv_src_first_[moddown_plan_[t.level]];
                 for (u32 k = 0; k < np; ++k) intt.push_back({acc + (L + k) * n, y + k * n, nq + k, yf ? sf + k : 0});
-                jobs.push_back({y, z, moddown_plan_[t.level], yf ? 1u : 0u});
+                jobs.push_back({y, z, moddown_plan_[t.level], 0});
                 for (u32 i = 0; i < L; ++i) cols.push_back({z + i * n, z + i * n, i, 0});
             }
         }
@@ -894,6 +895,7 @@ void Context::prog_apply(Program& p, const std::vector<ApplyTask>& tasks) const 
         else
             p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size(), 0, fused_usable());
         p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_l, np);
+        if (!jobs.empty()) p.steps.back().yform = yf;
         if (conv_variant_md < 3 && !jobs.empty()) p.steps.back().sched = 1u | (conv_variant_md << 1) | (conv_tg << 8);
         p.step(S_NTT_COLS, p.add(cols), (u32)cols.size());
         p.step(S_ROWS_IP_FIN, p.add(rf), (u32)rf.size());
@@ -934,13 +936,13 @@ void Context::prog_mod_down(Program& p, const std::vector<MdTask>& tasks) const 
     std::vector<ConvJob> jobs;
     std::vector<FinishJob> fin;
     u32 max_l = 0;
-    const bool yf = yform_usable();
+    const bool yf = yform_usable(2);
     for (const MdTask& t : tasks) {
         const u32 L = t.level + 1;
         max_l = std::max(max_l, L);
         const u32 sf = conv_src_first_[moddown_plan_[t.level]];
         for (u32 k = 0; k < np; ++k) intt.push_back({t.acc + (L + k) * n, t.y + k * n, nq + k, yf ? sf + k : 0});
-        jobs.push_back({t.y, t.z, moddown_plan_[t.level], yf ? 1u : 0u});
+        jobs.push_back({t.y, t.z, moddown_plan_[t.level], 0});
         for (u32 i = 0; i < L; ++i) {
             u64* zi = t.z + i * n;
             nttr.push_back({zi, zi, i, 0});
@@ -954,6 +956,7 @@ void Context::prog_mod_down(Program& p, const std::vector<MdTask>& tasks) const 
     else
         p.step_ntt(S_NTT_INV, p.add(intt), (u32)intt.size(), 0, fused_usable());
     p.step(S_CONV, p.add(jobs), (u32)jobs.size(), 0, max_l, np);
+    if (!jobs.empty()) p.steps.back().yform = yf;
     if (conv_variant_md < 3 && !jobs.empty()) p.steps.back().sched = 1u | (conv_variant_md << 1) | (conv_tg << 8);
     const size_t o_ntt = p.add(nttr), o_fin = p.add(fin);
     p.step_ntt(S_NTT_FWD_FIN, o_ntt, (u32)nttr.size(), o_fin, fused_usable());
@@ -1017,7 +1020,7 @@ void Context::raise_level0(const std::vector<RaiseTask>& tasks, hipStream_t s) {
 
 void Context::decompose(const std::vector<DecTask>& tasks, hipStream_t s) {
     Program p;
-    prog_decompose(p, tasks);
+    prog_decompose(p, tasks, conv_variant);
     execute(p, s);
 }
 

@@ -161,9 +161,10 @@ struct Program {
         u32 a, b;
         u32 sched = 0;  // S_CONV: 0 = the context's schedule, else 1 | variant << 1 | tg << 8
         size_t ctl = 0;  // NTT steps: offset of the one-launch transform's zeroed control block
+        bool yform = false;  // S_CONV: the inputs are in y form (an S_NTT_INV_Y step wrote them)
         bool operator==(const Step& o) const {
             return kind == o.kind && count == o.count && off == o.off && off2 == o.off2 && a == o.a && b == o.b &&
-                   sched == o.sched && ctl == o.ctl;
+                   sched == o.sched && ctl == o.ctl && yform == o.yform;
         }
     };
     std::vector<uint8_t> bytes;
@@ -269,7 +270,9 @@ class Context : public std::enable_shared_from_this<Context> {
     u64 digit_words(u32 level) const { return (u64)(level + 1 + np) * n; }
 
     // The engine: build programs, run them (descriptors at d), or both (execute).
-    void prog_decompose(Program& p, const std::vector<DecTask>& tasks) const;
+    // variant: the conversion kernel variant the program's ModUp conversion will run with (it
+    // decides whether the inverse NTT leaves the y form, tuning key "conv_yform").
+    void prog_decompose(Program& p, const std::vector<DecTask>& tasks, u32 variant) const;
     void prog_apply(Program& p, const std::vector<ApplyTask>& tasks) const;
     void prog_mod_down(Program& p, const std::vector<MdTask>& tasks) const;
     void prog_rescale(Program& p, const std::vector<RescaleTask>& tasks) const;
@@ -339,10 +342,13 @@ class Context : public std::enable_shared_from_this<Context> {
     u32 fuse_ip = 1, fuse_fin = 1;
     bool fuse_ip_possible() const;
     // The inverse NTT before a ModUp or mod_down conversion leaves its limbs in the conversion's
-    // y form (YForm; step S_NTT_INV_Y, tuning key "conv_yform"), when the register-radix NTT
-    // applies: the conversion then skips its per-source product.
-    u32 conv_yform = 1;
-    bool yform_usable() const;
+    // y form (YForm; step S_NTT_INV_Y), when the register-radix NTT applies: the conversion then
+    // skips its per-source product. Tuning key "conv_yform", a mask: 1 = ModUp conversions on the
+    // vector kernel, 2 = mod_down conversions, 4 = ModUp conversions on the matrix-core kernel.
+    // Default 6: the vector kernel for six 50-bit sources holds 92 VGPRs in y form against 66
+    // (5 waves per SIMD against 7) and runs slower (3.14 against 2.67 ms per batched step).
+    u32 conv_yform = 6;
+    bool yform_usable(u32 mask_bit) const;
     std::atomic<u64> graph_captures{0}, graph_launches{0};
 
    private:

@@ -534,7 +534,8 @@ def test_graph_replays_equal_direct_launches():
 def test_schedules_give_the_same_switch(case):
     """Every conversion kernel variant with and without the row pass fused into the inner product
     (and with it, the mod_down finish fused or not; mod_down's conversions on each variant of
-    their own) gives the same batched switch, equal to the oracle."""
+    their own; the inverse NTTs in the conversions' y form or not) gives the same batched switch,
+    equal to the oracle."""
     ctx, ref = make(case)
     L = len(ref.q) - 1
     key = ctx.ksk_uniform(L, seed=61)
@@ -564,6 +565,18 @@ def test_schedules_give_the_same_switch(case):
                 assert (wb == gb).all() and (wa == ga).all(), ("fin", fin, "md", md)
     ctx.set_tuning("fuse_fin", 1)
     ctx.set_tuning("conv_variant_md", 1)
+    # the inverse NTT in the conversions' y form on every subset of the conversion kinds, on the
+    # vector and the matrix-core kernels
+    for var in (0, 2):
+        ctx.set_tuning("conv_variant", var)
+        for yf in range(8):
+            ctx.set_tuning("conv_yform", yf)
+            assert ctx.tuning("conv_yform") == yf
+            got = [(kb.download(), ka.download()) for kb, ka in ctx.key_switch_batch(polys, key)]
+            for (wb, wa), (gb, ga) in zip(want, got):
+                assert (wb == gb).all() and (wa == ga).all(), ("var", var, "conv_yform", yf)
+    ctx.set_tuning("conv_variant", 0)
+    ctx.set_tuning("conv_yform", 6)
     ob, oa = ref.key_switch(polys[-1].download().ravel(), lv[-1], ref.uniform_key(L, 61), L)
     assert (want[-1][0] == ob).all() and (want[-1][1] == oa).all()
     # the small-batch schedule: the whole batch (and each single switch) under every choice
