@@ -219,6 +219,11 @@ CKH_DEV void conv_coeff(const ConvSrc* srcs, const ConvTgt* tgts, const PrimeCon
                     carry += l < p ? 1u : 0u;
                     m += (u64)y0 * c1 + (u64)y1 * c0;
                     h += (u64)y1 * c1;
+                    // one term at a time: otherwise the compiler forms every term's products
+                    // first and holds them all (92 VGPRs instead of 66 with y-form inputs)
+                    CKH_OPAQUE(l);
+                    CKH_OPAQUE(m);
+                    CKH_OPAQUE(h);
                 }
                 {
                     const u64 p = (u64)(u32)e * (u32)g->nq;

@@ -345,9 +345,8 @@ class Context : public std::enable_shared_from_this<Context> {
     // y form (YForm; step S_NTT_INV_Y), when the register-radix NTT applies: the conversion then
     // skips its per-source product. Tuning key "conv_yform", a mask: 1 = ModUp conversions on the
     // vector kernel, 2 = mod_down conversions, 4 = ModUp conversions on the matrix-core kernel.
-    // Default 6: the vector kernel for six 50-bit sources holds 92 VGPRs in y form against 66
-    // (5 waves per SIMD against 7) and runs slower (3.14 against 2.67 ms per batched step).
-    u32 conv_yform = 6;
+    // Default 7 (every kind).
+    u32 conv_yform = 7;
     bool yform_usable(u32 mask_bit) const;
     std::atomic<u64> graph_captures{0}, graph_launches{0};
 

@@ -104,7 +104,7 @@ ckh_status ckh_autotune_ks(ckh_context *ctx, uint32_t n, const ckh_poly *const *
  * with fuse_ip, mod_down's finish of the Q limbs in the same kernel, so the inner-product sums
  * never reach memory); "conv_yform" (a mask of the conversions whose inverse NTT ends with the
  * conversion's first product, [x + h] (Q/q_j)^-1, folded into its last stage: 1 = ModUp on the
- * vector kernel, 2 = mod_down, 4 = ModUp on the matrix-core kernel; default 6); "small_limbs" (batched
+ * vector kernel, 2 = mod_down, 4 = ModUp on the matrix-core kernel; default 7); "small_limbs" (batched
  * switches of at most this many input limbs in all, a single switch for one, use their own
  * schedule: "conv_variant_small", "conv_tg_small", "fuse_ip_small"); "pool" (1 = the context's
  * block cache, the default: freed device blocks stay cached per size class and serve their own

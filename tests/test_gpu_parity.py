@@ -576,7 +576,7 @@ def test_schedules_give_the_same_switch(case):
             for (wb, wa), (gb, ga) in zip(want, got):
                 assert (wb == gb).all() and (wa == ga).all(), ("var", var, "conv_yform", yf)
     ctx.set_tuning("conv_variant", 0)
-    ctx.set_tuning("conv_yform", 6)
+    ctx.set_tuning("conv_yform", 7)
     ob, oa = ref.key_switch(polys[-1].download().ravel(), lv[-1], ref.uniform_key(L, 61), L)
     assert (want[-1][0] == ob).all() and (want[-1][1] == oa).all()
     # the small-batch schedule: the whole batch (and each single switch) under every choice
