@@ -36,35 +36,56 @@ CKH_DEV u32 galois_index(u32 i, u64 g, u32 log_n) {
 // order gives the reference's words). Primes below 2^50 sum f64 modular products (reduced every
 // four digits); larger primes sum exact 128-bit products and reduce once. Digit limbs may be read
 // through the Galois permutation (hoisted rotations, evaluator.rs:596-604).
+// Digits are taken IP_GROUP at a time, every word of a group loaded before any product, so the
+// loads of a group are in flight together.
+constexpr u32 IP_GROUP = 4;
+
 CKH_DEV void ip_coeff(const IpJob& J, const PrimeConst& C, u32 log_n, u32 c) {
     const u32 cs = J.galois ? galois_index(c, J.galois, log_n) : c;
+    const auto key = glb(J.key);
+    u64 dv[IP_GROUP], bv[IP_GROUP], av[IP_GROUP];
+    auto load_group = [&](u32 j0) {
+#pragma unroll
+        for (u32 u = 0; u < IP_GROUP; ++u) {
+            const u32 j = j0 + u;
+            if (j < J.ndig) {
+                dv[u] = glb((J.own && j == J.own_j) ? J.own : J.dig + j * J.dig_stride)[cs];
+                bv[u] = key[(2 * j) * J.key_poly + c];
+                av[u] = key[(2 * j + 1) * J.key_poly + c];
+            }
+        }
+    };
     if (C.q < FP_PRIME_BOUND) {
         const double q = C.qf, qinv = C.qinv;
         double sb = 0.0, sa = 0.0;
-        for (u32 j = 0; j < J.ndig; ++j) {
-            const auto src = glb((J.own && j == J.own_j) ? J.own : J.dig + j * J.dig_stride);
-            const auto key = glb(J.key);
-            const double d = u2d(src[cs]);
-            const double b = u2d(key[(2 * j) * J.key_poly + c]);
-            const double a = u2d(key[(2 * j + 1) * J.key_poly + c]);
-            sb += fmulmod(d, b, b * qinv, q);
-            sa += fmulmod(d, a, a * qinv, q);
-            if ((j & 3) == 3) {
-                sb = freduce(sb, q, qinv);
-                sa = freduce(sa, q, qinv);
+        for (u32 j0 = 0; j0 < J.ndig; j0 += IP_GROUP) {
+            load_group(j0);
+#pragma unroll
+            for (u32 u = 0; u < IP_GROUP; ++u) {
+                if (j0 + u < J.ndig) {
+                    const double d = u2d(dv[u]), b = u2d(bv[u]), a = u2d(av[u]);
+                    sb += fmulmod(d, b, b * qinv, q);
+                    sa += fmulmod(d, a, a * qinv, q);
+                }
             }
+            // four products (|.| < 0.7 q each) since the last reduction
+            sb = freduce(sb, q, qinv);
+            sa = freduce(sa, q, qinv);
         }
         glb(J.acc_b)[c] = d2u(fcanon(sb, q, qinv));
         glb(J.acc_a)[c] = d2u(fcanon(sa, q, qinv));
         return;
     }
     Dot60 sb, sa;
-    for (u32 j = 0; j < J.ndig; ++j) {
-        const auto src = glb((J.own && j == J.own_j) ? J.own : J.dig + j * J.dig_stride);
-        const auto key = glb(J.key);
-        const u64 d = src[cs];
-        sb.add(d, key[(2 * j) * J.key_poly + c]);
-        sa.add(d, key[(2 * j + 1) * J.key_poly + c]);
+    for (u32 j0 = 0; j0 < J.ndig; j0 += IP_GROUP) {
+        load_group(j0);
+#pragma unroll
+        for (u32 u = 0; u < IP_GROUP; ++u) {
+            if (j0 + u < J.ndig) {
+                sb.add(dv[u], bv[u]);
+                sa.add(dv[u], av[u]);
+            }
+        }
     }
     u64 hi, lo;
     sb.get(hi, lo);
