@@ -282,7 +282,7 @@ __global__ __launch_bounds__(64) void k_xcc_ids(u32* out) {
 // The fused row pass + inner product with mod_down's finish of both sums (RowIpFinJob); the same
 // block order as k_rows_ip.
 template <u32 M>
-__global__ __launch_bounds__(NT) void k_rows_ip_fin(const RowIpFinJob* __restrict__ jobs,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_rows_ip_fin(const RowIpFinJob* __restrict__ jobs,
                                                     const PrimeConst* __restrict__ pcs, const FTw* __restrict__ twf,
                                                     const FTw* __restrict__ rt, u32 log_n, u32 njobs) {
     __shared__ u64 lds[LDS_WORDS];
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(NT) void k_rows_ip_fin(const RowIpFinJob* __restric
 }
 
 template <u32 M>
-__global__ __launch_bounds__(NT) void k_rows_ip(const RowIpJob* __restrict__ jobs, const PrimeConst* __restrict__ pcs,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_rows_ip(const RowIpJob* __restrict__ jobs, const PrimeConst* __restrict__ pcs,
                                                 const FTw* __restrict__ twf, const FTw* __restrict__ rt, u32 log_n,
                                                 u32 njobs) {
     __shared__ u64 lds[LDS_WORDS];

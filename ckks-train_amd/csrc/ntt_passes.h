@@ -313,9 +313,14 @@ CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in_p, u64* __restric
 // The row pass of one tile, its canonical outputs left in LDS in natural order (word i of the
 // tile at pad(i)): as u64 residues, or (F64) as the f64 bits of the residue, or (RAW, f64 path)
 // as the f64 bits of the last stage's signed outputs (|.| <= 2.6 q), not reduced.
-template <u32 M, bool F64, class Ar, bool COH = false, class Lds, bool RAW = false>
+// pre() runs after the last LDS stores, before the closing barrier (the caller's loads issued
+// there are in flight while the workgroup meets).
+struct NoPre {
+    CKH_DEV void operator()() const {}
+};
+template <u32 M, bool F64, class Ar, bool COH = false, class Lds, bool RAW = false, class Pre = NoPre>
 CKH_DEV void fwd_rows_to_lds(const Ar& ar, const u64* __restrict__ in_p, Lds lds, u32 log_n, u32 tile,
-                             u32 tid = threadIdx.x) {
+                             u32 tid = threadIdx.x, Pre pre = Pre{}) {
     const auto in = glb(in_p);
     constexpr u32 R2 = M - 4, U = 4 - R2, TRL = TL - M, JM = (1u << R2) - 1;
     const u32 A = log_n - M;
@@ -345,6 +350,7 @@ CKH_DEV void fwd_rows_to_lds(const Ar& ar, const u64* __restrict__ in_p, Lds lds
         const u32 c = ((((o << U) | (e >> R2)) << R2) | (e & JM));
         lds[pad((rr << M) + c)] = RAW ? ar.to_mid(v[e]) : F64 ? ar.fwd_final_f(v[e]) : ar.fwd_final(v[e]);
     }
+    pre();
     __syncthreads();
 }
 
@@ -354,22 +360,24 @@ CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
     const auto out = glb(out_p);
     const u64 tbase = (u64)tile << TL;
     if (fin) {
-        // finish of rescale / mod_down: (a - t) * inv (+ add); the a (and add) words are loaded
-        // before the transform so their latency overlaps it
+        // finish of rescale / mod_down: (a - t) * inv (+ add)
         const auto fa = glb(fin->a), fadd = glb(fin->add);
         const auto fout = glb(fin->out);
         const bool has_add = fin->add != nullptr;
         const u64 inv = fin->inv, inv_s = fin->inv_s;
         u64 av[16], dv[16];
+        // a (and the addend) loaded once the row pass's registers are free, before its last barrier
+        auto pre = [&] {
 #pragma unroll
-        for (u32 k = 0; k < 16; ++k) {
-            av[k] = fa[tbase + tid + k * NT];
-            dv[k] = has_add ? (u64)fadd[tbase + tid + k * NT] : (u64)0;
-        }
+            for (u32 k = 0; k < 16; ++k) {
+                av[k] = fa[tbase + tid + k * NT];
+                dv[k] = has_add ? (u64)fadd[tbase + tid + k * NT] : (u64)0;
+            }
+        };
         if constexpr (Ar::F64) {
             // in f64: the transform left unreduced (|.| <= 2.6 q), a - t below 3.6 q, one modular
             // product (|.| <= 1.2 q), the addend, one reduction
-            fwd_rows_to_lds<M, false, Ar, COH, Lds, true>(ar, in, lds, log_n, tile, tid);
+            fwd_rows_to_lds<M, false, Ar, COH, Lds, true>(ar, in, lds, log_n, tile, tid, pre);
             const FTw iv = fin->inv_f;
 #pragma unroll
             for (u32 k = 0; k < 16; ++k) {
@@ -378,7 +386,7 @@ CKH_DEV void fwd_rows(const Ar& ar, const u64* __restrict__ in, u64* __restrict_
                 fout[tbase + tid + k * NT] = d2u(fcanon(v, ar.q, ar.qinv));
             }
         } else {
-            fwd_rows_to_lds<M, false, Ar, COH>(ar, in, lds, log_n, tile, tid);
+            fwd_rows_to_lds<M, false, Ar, COH, Lds, false>(ar, in, lds, log_n, tile, tid, pre);
 #pragma unroll
             for (u32 k = 0; k < 16; ++k) {
                 u64 v = shoup(sub_mod(av[k], lds[pad(tid + k * NT)], q), inv, inv_s, q);
@@ -519,12 +527,21 @@ CKH_DEV void rows_ip_sums(const FpAr& ar, const RowIpJob& J, Lds lds, u32 log_n,
                 sa[k] += fmulmod(x, a, a * qinv, q);
             }
         } else {
-            // the transform left unreduced (|x| <= 2.6 q: each product below q in magnitude)
-            fwd_rows_to_lds<M, true, FpAr, false, Lds, true>(ar, J.mid + j * J.mid_stride, lds, log_n, tile, tid);
+            // the transform left unreduced (|x| <= 2.6 q: each product below q in magnitude); the
+            // key words are loaded once the row pass's registers are free, before its last barrier
+            u64 kbw[16], kaw[16];
+            auto pre = [&] {
+#pragma unroll
+                for (u32 k = 0; k < 16; ++k) {
+                    kbw[k] = kb[tid + k * NT];
+                    kaw[k] = ka[tid + k * NT];
+                }
+            };
+            fwd_rows_to_lds<M, true, FpAr, false, Lds, true>(ar, J.mid + j * J.mid_stride, lds, log_n, tile, tid, pre);
 #pragma unroll
             for (u32 k = 0; k < 16; ++k) {
                 const double x = bits_f64(lds[pad(tid + k * NT)]);
-                const double b = u2d(kb[tid + k * NT]), a = u2d(ka[tid + k * NT]);
+                const double b = u2d(kbw[k]), a = u2d(kaw[k]);
                 sb[k] += fmulmod(x, b, b * qinv, q);
                 sa[k] += fmulmod(x, a, a * qinv, q);
             }
@@ -567,13 +584,21 @@ CKH_DEV void finish_sum(const FpAr& ar, const double (&s)[16], const u64* z, u64
     CKH_OPAQUE(tid);
     const u64 tbase = (u64)tile << TL;
     const double q = ar.q, qinv = ar.qinv;
-    fwd_rows_to_lds<M, false, FpAr, false, Lds, true>(ar, z, lds, log_n, tile, tid);
+    // the addend's words are loaded once the row pass's registers are free, before its last barrier
+    u64 dv[16];
+    auto pre = [&] {
+        if (add_p) {
+#pragma unroll
+            for (u32 k = 0; k < 16; ++k) dv[k] = glb(add_p)[tbase + tid + k * NT];
+        }
+    };
+    fwd_rows_to_lds<M, false, FpAr, false, Lds, true>(ar, z, lds, log_n, tile, tid, pre);
     const auto out = glb(out_p);
 #pragma unroll
     for (u32 k = 0; k < 16; ++k) {
         const double d = freduce(s[k], q, qinv) - bits_f64(lds[pad(tid + k * NT)]);
         double v = fmulmod(d, inv.w, inv.wq, q);
-        if (add_p) v = v + u2d(glb(add_p)[tbase + tid + k * NT]);
+        if (add_p) v = v + u2d(dv[k]);
         out[tbase + tid + k * NT] = d2u(fcanon(v, q, qinv));
     }
     __syncthreads();  // the next pass reuses the LDS
