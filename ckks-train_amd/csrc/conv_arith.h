@@ -265,7 +265,8 @@ struct MfPlan {
 };
 struct MfTgt {
     double rf, rinv, c32_f, c32_q, bias_f, nq_c;  // f64 path (nq_c: -Q mod r, centered)
-    u64 q, one_s, two64, two64_s, nq, corr;       // integer path (nq: -Q mod r)
+    u64 q, nq, corr;                              // integer path (nq: -Q mod r; corr = bias_f)
+    double nq_f, corr_f;                          // integer path: nq and corr as f64 (rounded)
     u32 limb;  // output limb; ~0u = padding
     u32 fp;    // 1: f64 recombination (r < 2^50)
 };
@@ -394,21 +395,20 @@ CKH_DEV void conv_mf_block(const ConvSrc* srcs, const ConvPlan& P, const MfPlan&
                     acc = __builtin_fma((double)e[nt], nq_c, acc);
                     o = d2u(fcanon(acc, r, rinv));
                 } else {
-                    // X + 2^45 + 2^77 = (lo + 2^45) + (hi + 2^45) 2^32 >= 0, below 2^79; e nq (below
-                    // 2^63) and corr (below 2^60) join the 128-bit value before its one reduction
-                    const u64 lu = d2u(lo + 0x1p45), hu = d2u(hi + 0x1p45);
+                    // V = X + A with X = lo + hi 2^32 (|lo|, |hi| < 2^45) and A = e nq + corr
+                    // (below 2^64). k = rint(V / r) from f64 estimates: their error (below 2^27 on
+                    // |V| < 2^79, then the reciprocal's) moves V / r by less than 2^-30, so
+                    // |V - k r| < r; V - k r is then computed exactly in wrapping 64-bit arithmetic
+                    // and one addition of r makes it non-negative.
+                    const double T = __builtin_fma(hi, 4294967296.0, lo) +
+                                     __builtin_fma((double)e[nt], tg.nq_f, tg.corr_f);
+                    const int64_t k = (int64_t)__builtin_rint(T * rinv);
+                    const u64 lo64 = (u64)((int64_t)l01 + (int64_t)l23 * 65536);
+                    const u64 hi64 = (u64)((int64_t)h01 + (int64_t)h23 * 65536);
                     const u32 ee = (u32)e[nt];
-                    const u64 add = (u64)ee * (u32)tg.nq + (((u64)ee * (u32)(tg.nq >> 32)) << 32) + tg.corr;
-                    const u64 xl0 = lu + (hu << 32);
-                    const u64 xh0 = (hu >> 32) + (xl0 < lu ? 1 : 0);
-                    const u64 xl = xl0 + add;
-                    const u64 xh = xh0 + (xl < add ? 1 : 0);
-                    PrimeConst C{};
-                    C.q = tg.q;
-                    C.one_s = tg.one_s;
-                    C.two64 = tg.two64;
-                    C.two64_s = tg.two64_s;
-                    o = reduce128(xh, xl, C);
+                    const u64 A = (u64)ee * (u32)tg.nq + (((u64)ee * (u32)(tg.nq >> 32)) << 32) + tg.corr;
+                    const u64 R = lo64 + (hi64 << 32) + A - (u64)k * tg.q;
+                    o = (int64_t)R < 0 ? R + tg.q : R;
                 }
                 outp[(u64)limb * n + c[nt]] = o;
             }

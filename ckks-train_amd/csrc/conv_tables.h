@@ -210,13 +210,10 @@ struct HostConvTables {
                         T.bias_f = (double)((bias + r - ct.hmod % r) % r);
                         T.nq_c = ct.nq_c;
                         T.q = r;
-                        T.one_s = hm::shoup(1, r);
-                        T.two64 = (u64)(((hm::u128)1 << 64) % r);
-                        T.two64_s = hm::shoup(T.two64, r);
                         T.nq = ct.nq;
-                        // integer path adds 2^45 + 2^77 to make the product non-negative
-                        const u64 off = (u64)((((hm::u128)1 << 45) + ((hm::u128)1 << 77)) % r);
-                        T.corr = (u64)(((hm::u128)bias + 2 * (hm::u128)r - ct.hmod % r - off) % r);
+                        T.corr = (u64)(((hm::u128)bias + r - ct.hmod % r) % r);
+                        T.nq_f = (double)T.nq;
+                        T.corr_f = (double)T.corr;
                     }
                     mt.push_back(T);
                 }
