@@ -18,8 +18,8 @@
 //    |x| <= q/2 by x - rint(x/q) q before x +- t, keeping every value within 1.5 q < 2^51.
 //    Inverse: |x|, |y| <= q throughout. Between the two passes a limb holds these signed
 //    values as f64 bits; the last stage writes canonical residues.
-//  * q >= 2^50 (IntAr): Harvey butterflies with 64-bit Shoup products, values in [0, 4q)
-//    (forward) or [0, 2q) (inverse), as in ntt.hip.
+//  * q >= 2^50 (IntAr): Harvey butterflies with 64-bit Shoup products, values in [0, 8q)
+//    (forward: x reduced on alternate stages, every pass ending below 6q) or [0, 2q) (inverse).
 //
 // The includer provides, besides arith.h: glb(p) (a global-memory view of a pointer),
 // load_coherent(p) (a load that sees other workgroups' completed stores), __syncthreads(),
@@ -86,6 +86,7 @@ struct FpAr {
     // |t| <= (1/2 + |y| / (4 q)) q the magnitudes stay below 1.7 q after a reduced stage,
     // 2.6 q after a lazy one, and 3.7 q where two lazy stages meet at the pass boundary; every
     // value and every fmulmod intermediate stays an exact integer below 2^53.
+    static constexpr bool REDUCE_LAST = false;
     CKH_DEV void ct_lazy(V& x, V& y, const W& w) const {
         const double t = fmulmod(y, w.w, w.wq, q);
         const double x0 = x;
@@ -133,16 +134,27 @@ struct IntAr {
 #pragma unroll
         for (u32 m = 0; m < (1u << D); ++m) g[m] = tab[full_base + m];
     }
+    // Forward butterflies over values below 8 q (q < 2^60, so below 2^63): the lazy product t is
+    // below 2 q for any 64-bit y. Stages alternate as in the f64 path, but the last stage of each
+    // pass always reduces (REDUCE_LAST): a pass takes inputs below 6 q, a lazy stage leaves them
+    // below 8 q, a reduced one (x taken below 4 q) below 6 q, so every pass ends below 6 q.
+    static constexpr bool REDUCE_LAST = true;
     CKH_DEV void ct(V& x, V& y, const W& w) const {
-        const u64 two_q = 2 * q;
-        const u64 xr = x >= two_q ? x - two_q : x;
+        const u64 four_q = 4 * q;
+        const u64 xr = x >= four_q ? x - four_q : x;
         const u64 t = shoup_lazy(y, w.w, w.ws, q);
         x = xr + t;
-        y = xr + two_q - t;
+        y = xr + 2 * q - t;
     }
-    CKH_DEV void ct_lazy(V& x, V& y, const W& w) const { ct(x, y, w); }
+    CKH_DEV void ct_lazy(V& x, V& y, const W& w) const {
+        const u64 t = shoup_lazy(y, w.w, w.ws, q);
+        const u64 x0 = x;
+        x = x0 + t;
+        y = x0 + 2 * q - t;
+    }
     CKH_DEV u64 fwd_final_f(V v) const { return fwd_final(v); }  // not used (f64 primes only)
     CKH_DEV u64 fwd_final(V v) const {
+        v = v >= 4 * q ? v - 4 * q : v;
         v = v >= 2 * q ? v - 2 * q : v;
         return v >= q ? v - q : v;
     }
@@ -172,10 +184,11 @@ struct IntAr {
 // In the row passes every thread needs, per stage, 2^D twiddles at consecutive table indices
 // starting at a multiple of 2^D; Ar::group builds them (for the f64 path from one load).
 
-// Forward butterfly of stage s of a pass (s is a constant after unrolling).
-template <class Ar>
+// Forward butterfly of stage s of a pass of M stages (s is a constant after unrolling; M = 0:
+// not the pass's last stage).
+template <class Ar, u32 M = 0>
 CKH_DEV void ct_stage(const Ar& ar, u32 s, typename Ar::V& x, typename Ar::V& y, const typename Ar::W& w) {
-    if (s & 1)
+    if ((s & 1) || (Ar::REDUCE_LAST && s + 1 == M))
         ar.ct(x, y, w);
     else
         ar.ct_lazy(x, y, w);
@@ -219,7 +232,7 @@ CKH_DEV void rows_round2_fwd(const Ar& ar, typename Ar::V (&v)[16], u32 A, u32 r
 #pragma unroll
         for (u32 e = 0; e < 16; ++e) {
             if (e & (1u << p)) continue;
-            ct_stage(ar, s, v[e], v[e + (1u << p)], g[e >> (p + 1)]);
+            ct_stage<Ar, M>(ar, s, v[e], v[e + (1u << p)], g[e >> (p + 1)]);
         }
     }
 }
@@ -300,7 +313,7 @@ CKH_DEV void fwd_cols(const Ar& ar, const u64* __restrict__ in_p, u64* __restric
         for (u32 e = 0; e < 16; ++e) {
             if (e & (1u << p)) continue;
             const u32 r = ((((o << U) | (e >> R2)) << R2) | (e & JM));
-            ct_stage(ar, s, v[e], v[e + (1u << p)], ar.tw((1u << s) + (r >> (M - s))));
+            ct_stage<Ar, M>(ar, s, v[e], v[e + (1u << p)], ar.tw((1u << s) + (r >> (M - s))));
         }
     }
 #pragma unroll
