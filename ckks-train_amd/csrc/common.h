@@ -53,6 +53,11 @@ template <class T>
 CKH_DEV __attribute__((address_space(1))) T* glb(T* p) {
     return (__attribute__((address_space(1))) T*)p;
 }
+// A streaming (non-temporal) store through a glb() view: word i of p.
+template <class T>
+CKH_DEV void st_nt(__attribute__((address_space(1))) T* p, u64 i, T v) {
+    __builtin_nontemporal_store(v, p + i);
+}
 
 // blockIdx.x remapped so that consecutive logical tiles of one launch land on one XCD
 // (workgroups are dealt round-robin over the 8 XCDs, each with its own L2).

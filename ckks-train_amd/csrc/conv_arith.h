@@ -245,7 +245,7 @@ CKH_DEV void conv_coeff(const ConvSrc* srcs, const ConvTgt* tgts, const PrimeCon
                 o = sub_mod(o, g->hmod, C.q);
             }
         }
-        outp[(u64)g->limb * n + c] = o;
+        st_nt(outp, (u64)g->limb * n + c, o);
     }
 }
 

@@ -258,6 +258,10 @@ emu::View<T> glb(T* p) {
     // a view of a null pointer (an absent optional operand) is allowed; accessing it is not
     return emu::View<T>{p, p ? emu::find(reinterpret_cast<const u64*>(p)) : nullptr};
 }
+template <class T>
+void st_nt(emu::View<T> p, u64 i, T v) {
+    p[i] = v;
+}
 namespace fast {
 inline u64 load_coherent(const u64* p) { return glb(p)[0]; }
 }  // namespace fast
