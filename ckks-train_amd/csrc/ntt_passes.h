@@ -455,7 +455,7 @@ CKH_DEV void inv_rows(const Ar& ar, const u64* __restrict__ in_p, u64* __restric
     rows_round2_inv<M, Ar>(ar, v, log_n - M, row);
     const u64 base = (u64)row << M;
 #pragma unroll
-    for (u32 k = 0; k < 16; ++k) out[base + o + (k << R2)] = ar.to_mid(v[k]);
+    for (u32 k = 0; k < 16; ++k) st_nt(out, base + o + (k << R2), ar.to_mid(v[k]));
 }
 
 // ADD: the last stage adds ar.add to every output (the y form of a conversion's source limb).
@@ -505,7 +505,7 @@ CKH_DEV void inv_cols(const Ar& ar, u64* __restrict__ io_p, Lds lds, u32 log_n, 
         }
     }
 #pragma unroll
-    for (u32 k = 0; k < 16; ++k) io[((u64)(o + (k << R2)) << B) + c0 + cc] = ar.canon_out(v[k]);
+    for (u32 k = 0; k < 16; ++k) st_nt(io, ((u64)(o + (k << R2)) << B) + c0 + cc, ar.canon_out(v[k]));
 }
 
 // ---------------------------------------------------------------- row pass + inner product
